@@ -1,0 +1,201 @@
+/*
+ * psc.h — C ABI of the MI355X strongly-connected-components path.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Every entry point returns PSC_OK (0) or a negative PSC_ERR_* code and never
+ * aborts; psc_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Reference interfaces replaced (paths are into ucrparlay/
+ * Parallel-Strong-Connectivity):
+ *   struct Graph                       src/graph.hpp:35-43      -> psc_graph
+ *   SCC::SCC(Graph&)                   src/scc.hpp:81-92        -> psc_create
+ *   SCC::scc(labels, beta, local_reach, local_scc)
+ *                                      src/scc.hpp:79,277-479   -> psc_run
+ *   globals tau (src/reach.hpp:19), _min_bag_size_ (src/scc.hpp:24), set by
+ *   src/scc.cpp:45-46                                           -> psc_params
+ *   SCC::breakdown_report              src/scc.hpp:106-114      -> psc_stats
+ *   SCC_status                         src/scc.cpp:13-29        -> psc_status
+ *   read_graph / read_large_graph      src/graph.hpp:248-315    -> psc_read_bin
+ *   make_directed                      src/graph.hpp:222-245    -> psc_make_directed
+ */
+#ifndef PSC_H
+#define PSC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSC_OK 0
+#define PSC_ERR_INVALID (-1) /* bad argument or malformed graph          */
+#define PSC_ERR_HIP (-2)     /* a HIP runtime call failed                */
+#define PSC_ERR_NOMEM (-3)   /* device or host allocation failed         */
+#define PSC_ERR_IO (-4)      /* file could not be read or written        */
+#define PSC_ERR_BUDGET (-5)  /* request exceeds the serving HBM budget   */
+
+/* Directed graph in CSR form, both directions (src/graph.hpp:35-43).
+ * offset/in_offset hold n+1 entries, edges/in_edges hold m entries.
+ * Vertex ids are < n < 2^32-1 (UINT_N_MAX is the empty key,
+ * src/utilities.h:26).  Duplicate edges and self-loops are tolerated. */
+typedef struct psc_graph {
+  uint64_t n;
+  uint64_t m;
+  const uint64_t *offset;
+  const uint32_t *edges;
+  const uint64_t *in_offset;
+  const uint32_t *in_edges;
+} psc_graph;
+
+/* Run parameters.  beta, local_reach, local_scc mirror the SCC::scc
+ * arguments; tau and min_bag the globals of src/scc.cpp:45-46; dense_div
+ * the divisor of the sparse/dense switch threshold m/20 (src/reach.hpp:49);
+ * seed drives the pivot permutation that replaces parlay::random_shuffle
+ * (src/scc.hpp:303); table_init, when non-zero, overrides the initial
+ * label-table slot count of src/scc.hpp:398-400,412-414 (used to force the
+ * overflow-and-double path of src/scc.hpp:256-275). */
+typedef struct psc_params {
+  double beta;          /* pivot batch growth, default 1.5          */
+  int32_t local_reach;  /* VGC in the first-SCC reach (-local_reach) */
+  int32_t local_scc;    /* VGC in multi-search (-local_scc)          */
+  uint64_t tau;         /* VGC local budget per seed, default 512    */
+  uint64_t min_bag;     /* -lambda, default 1024                     */
+  uint64_t dense_div;   /* default 20                                */
+  uint64_t seed;        /* pivot permutation seed, default 1         */
+  uint64_t table_init;  /* 0 = reference sizing                      */
+  int32_t canonical;    /* 1: labels_out = min vertex id of the SCC  */
+  int32_t timing;       /* 1: fill the phase times of psc_stats      */
+} psc_params;
+
+/* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).
+ * Times are device times between HIP events on the handle's stream, in ms. */
+typedef struct psc_stats {
+  double ms_total;     /* labels-init to labels-ready               */
+  double ms_init;      /* init + trim          (init_timer)         */
+  double ms_first;     /* first-SCC reaches    (first_round_timer)  */
+  double ms_multi;     /* multi-search rounds  (multi_search_timer) */
+  double ms_resize;    /* searches rerun after table overflow       */
+  double ms_finalize;  /* final label rewrite (+ canonicalisation)  */
+  uint64_t n_trimmed;
+  uint64_t first_scc_size;
+  uint64_t rounds;
+  uint64_t sub_rounds;
+  uint64_t levels_fwd;
+  uint64_t levels_bwd;
+  uint64_t resizes;
+  uint64_t host_syncs;
+  /* algorithmic traffic counters (bytes), counted on the device */
+  uint64_t bytes_csr;    /* offsets + adjacency read                 */
+  uint64_t bytes_label;  /* label / visited reads and writes         */
+  uint64_t bytes_table;  /* table probes and writes                  */
+  uint64_t bytes_front;  /* frontier / log appends and reads         */
+  uint64_t edges_scanned;
+  uint64_t table_inserts;
+} psc_stats;
+
+typedef struct psc_handle psc_handle;
+
+/* Fill p with the reference defaults (src/scc.cpp:40-46). */
+void psc_params_default(psc_params *p);
+
+/* Upload both CSRs once and allocate the device working set
+ * (SCC::SCC, src/scc.hpp:81-92).  Host arrays are only read. */
+int psc_create(const psc_graph *g, psc_handle **out);
+
+/* Same, for a graph whose four arrays already live in device memory of the
+ * current HIP device; the handle borrows them (caller keeps them alive). */
+int psc_create_device(const psc_graph *dev_g, psc_handle **out);
+
+/* SCC::scc (src/scc.hpp:277-479): labels_out[n] on the host receives
+ * (internal label & 0xffffffff) - 1, equal iff same SCC, or the minimum
+ * vertex id of the SCC when p->canonical.  stats may be NULL. */
+int psc_run(psc_handle *h, const psc_params *p, uint64_t *labels_out,
+            psc_stats *stats);
+
+/* Same, but labels stay in device memory: d_labels_out is a device pointer
+ * with room for n uint64 values. */
+int psc_run_device(psc_handle *h, const psc_params *p, uint64_t *d_labels_out,
+                   psc_stats *stats);
+
+/* SCC_status (src/scc.cpp:13-29) computed on the device from labels in
+ * [0, n] held in device memory: number of SCCs and the largest size. */
+int psc_status_device(psc_handle *h, const uint64_t *d_labels,
+                      uint64_t *n_scc, uint64_t *largest);
+
+void psc_destroy(psc_handle *h);
+
+/* Message of the last error on this thread ("" if none). */
+const char *psc_last_error(void);
+
+/* HIP stream a handle runs on (as void*, a hipStream_t). */
+void *psc_handle_stream(psc_handle *h);
+
+/* ------------------------------------------------------------------ I/O
+ * Host-side graph container (owned arrays). */
+typedef struct psc_host_graph {
+  psc_graph g;      /* views into the arrays below */
+  uint64_t *offset;
+  uint32_t *edges;
+  uint64_t *in_offset;
+  uint32_t *in_edges;
+} psc_host_graph;
+
+void psc_host_graph_free(psc_host_graph *hg);
+
+/* GBBS .bin reader: large=0 reads one CSR and builds the in-CSR by
+ * transposition (read_graph -> read_binary -> make_directed,
+ * src/graph.hpp:120-185,222-245,248-277); large=1 reads the out-CSR
+ * followed by the in-CSR (read_large_graph, src/graph.hpp:279-315). */
+int psc_read_bin(const char *path, int large, psc_host_graph **out);
+
+/* Writer for the same two layouts. */
+int psc_write_bin(const char *path, const psc_graph *g, int large);
+
+/* Build a host graph from an out-CSR, computing the in-CSR as
+ * make_directed does (src/graph.hpp:222-245): in-lists sorted by source. */
+int psc_make_directed(uint64_t n, uint64_t m, const uint64_t *offset,
+                      const uint32_t *edges, psc_host_graph **out);
+
+/* --------------------------------------------------------- generators
+ * Seeded synthetic inputs (no reference counterpart: the workloads the
+ * benchmark configurations name).  All remove self-loops and duplicate
+ * edges and return sorted adjacency lists in both directions.  The device
+ * versions write the identical graph straight into HBM. */
+
+/* RMAT: 2^scale vertices, `samples` edge samples with quadrant
+ * probabilities (a, b, c, 1-a-b-c); permute=1 relabels vertex ids by a
+ * seeded bijection. */
+int psc_gen_rmat(uint32_t scale, uint64_t samples, double a, double b,
+                 double c, uint64_t seed, int permute, psc_host_graph **out);
+
+/* rows x cols lattice, id = r*cols + c; each 4-neighbour pair is kept in
+ * both directions with probability p_both, else in one random direction. */
+int psc_gen_grid(uint32_t rows, uint32_t cols, double p_both, uint64_t seed,
+                 psc_host_graph **out);
+
+/* n points uniform in the unit cube (21-bit integer coordinates); edge
+ * i -> j for the k nearest j != i (ties broken by smaller id). */
+int psc_gen_knn3d(uint64_t n, uint32_t k, uint64_t seed, psc_host_graph **out);
+
+/* Device-resident graph produced by a device generator. */
+typedef struct psc_dev_graph {
+  psc_graph g; /* device pointers */
+} psc_dev_graph;
+
+int psc_dev_gen_rmat(uint32_t scale, uint64_t samples, double a, double b,
+                     double c, uint64_t seed, int permute,
+                     psc_dev_graph **out);
+int psc_dev_gen_grid(uint32_t rows, uint32_t cols, double p_both,
+                     uint64_t seed, psc_dev_graph **out);
+void psc_dev_graph_free(psc_dev_graph *dg);
+
+/* Copy a device graph back to host arrays (for checking). */
+int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSC_H */

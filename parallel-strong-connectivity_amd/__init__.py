@@ -1,0 +1,332 @@
+"""MI355X strongly-connected-components path, Python side.
+
+A thin ctypes layer over ``libpsc.so`` (C ABI in ``include/psc.h``).  It
+mirrors the reference's interface: ``Graph`` is ``struct Graph``
+(src/graph.hpp:35-43) with ``read_bin`` for ``read_graph`` /
+``read_large_graph`` (src/graph.hpp:248-315); ``SCC(graph).scc(...)`` is
+``SCC::SCC`` + ``SCC::scc(labels, beta, local_reach, local_scc)``
+(src/scc.hpp:79-92,277); ``status`` is ``SCC_status`` (src/scc.cpp:13-29).
+
+The GPU library is the only compute path: when ``libpsc.so`` is missing the
+import fails loudly instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsc.so")
+
+PSC_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory",
+          -4: "I/O error", -5: "over HBM budget"}
+
+
+class PscError(RuntimeError):
+    def __init__(self, where, code, msg):
+        super().__init__(f"{where} failed ({code}, {ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run parallel-strong-connectivity_amd/build.py "
+        "(or __graft_entry__.build()) first")
+
+lib = C.CDLL(LIB_PATH)
+
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+
+
+class CGraph(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("m", C.c_uint64),
+                ("offset", u64p), ("edges", u32p),
+                ("in_offset", u64p), ("in_edges", u32p)]
+
+
+class CParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("local_reach", C.c_int32),
+                ("local_scc", C.c_int32), ("tau", C.c_uint64),
+                ("min_bag", C.c_uint64), ("dense_div", C.c_uint64),
+                ("seed", C.c_uint64), ("table_init", C.c_uint64),
+                ("canonical", C.c_int32), ("timing", C.c_int32)]
+
+
+class CStats(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("ms_total", "ms_init", "ms_first", "ms_multi", "ms_resize",
+                 "ms_finalize")] + \
+               [(k, C.c_uint64) for k in
+                ("n_trimmed", "first_scc_size", "rounds", "sub_rounds",
+                 "levels_fwd", "levels_bwd", "resizes", "host_syncs",
+                 "bytes_csr", "bytes_label", "bytes_table", "bytes_front",
+                 "edges_scanned", "table_inserts")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CHostGraph(C.Structure):
+    _fields_ = [("g", CGraph), ("offset", u64p), ("edges", u32p),
+                ("in_offset", u64p), ("in_edges", u32p)]
+
+
+class CDevGraph(C.Structure):
+    _fields_ = [("g", CGraph)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("psc_params_default", None, C.POINTER(CParams))
+_sig("psc_last_error", C.c_char_p)
+_sig("psc_create", C.c_int, C.POINTER(CGraph), C.POINTER(C.c_void_p))
+_sig("psc_create_device", C.c_int, C.POINTER(CGraph), C.POINTER(C.c_void_p))
+_sig("psc_run", C.c_int, C.c_void_p, C.POINTER(CParams), u64p, C.POINTER(CStats))
+_sig("psc_run_device", C.c_int, C.c_void_p, C.POINTER(CParams), C.c_void_p,
+     C.POINTER(CStats))
+_sig("psc_status_device", C.c_int, C.c_void_p, C.c_void_p, u64p, u64p)
+_sig("psc_destroy", None, C.c_void_p)
+_sig("psc_handle_stream", C.c_void_p, C.c_void_p)
+_sig("psc_host_graph_free", None, C.POINTER(CHostGraph))
+_sig("psc_read_bin", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_write_bin", C.c_int, C.c_char_p, C.POINTER(CGraph), C.c_int)
+_sig("psc_make_directed", C.c_int, C.c_uint64, C.c_uint64, u64p, u32p,
+     C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_gen_rmat", C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.c_double,
+     C.c_double, C.c_uint64, C.c_int, C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64,
+     C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_gen_knn3d", C.c_int, C.c_uint64, C.c_uint32, C.c_uint64,
+     C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_dev_gen_rmat", C.c_int, C.c_uint32, C.c_uint64, C.c_double,
+     C.c_double, C.c_double, C.c_uint64, C.c_int, C.POINTER(C.POINTER(CDevGraph)))
+_sig("psc_dev_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double,
+     C.c_uint64, C.POINTER(C.POINTER(CDevGraph)))
+_sig("psc_dev_graph_free", None, C.POINTER(CDevGraph))
+_sig("psc_dev_graph_download", C.c_int, C.POINTER(CDevGraph),
+     C.POINTER(C.POINTER(CHostGraph)))
+
+
+def _check(where, rc):
+    if rc != PSC_OK:
+        raise PscError(where, rc, lib.psc_last_error().decode(errors="replace"))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None and a.size else None
+
+
+class Graph:
+    """Directed graph with out- and in-CSR (src/graph.hpp:35-43)."""
+
+    def __init__(self, n, offset, edges, in_offset, in_edges):
+        self.n = int(n)
+        self.offset = np.ascontiguousarray(offset, dtype=np.uint64)
+        self.edges = np.ascontiguousarray(edges, dtype=np.uint32)
+        self.in_offset = np.ascontiguousarray(in_offset, dtype=np.uint64)
+        self.in_edges = np.ascontiguousarray(in_edges, dtype=np.uint32)
+        self.m = int(self.edges.size)
+
+    def cgraph(self):
+        return CGraph(self.n, self.m, _ptr(self.offset, u64p), _ptr(self.edges, u32p),
+                      _ptr(self.in_offset, u64p), _ptr(self.in_edges, u32p))
+
+    @staticmethod
+    def _take(hgp):
+        hg = hgp.contents
+        n, m = hg.g.n, hg.g.m
+        off = np.ctypeslib.as_array(hg.offset, (n + 1,)).copy()
+        ioff = np.ctypeslib.as_array(hg.in_offset, (n + 1,)).copy()
+        e = np.ctypeslib.as_array(hg.edges, (max(m, 1),))[:m].copy()
+        ie = np.ctypeslib.as_array(hg.in_edges, (max(m, 1),))[:m].copy()
+        lib.psc_host_graph_free(hgp)
+        return Graph(n, off, e, ioff, ie)
+
+    @staticmethod
+    def from_out_csr(n, offset, edges):
+        """In-CSR by transposition, as make_directed (src/graph.hpp:222-245)."""
+        off = np.ascontiguousarray(offset, dtype=np.uint64)
+        e = np.ascontiguousarray(edges, dtype=np.uint32)
+        hgp = C.POINTER(CHostGraph)()
+        _check("psc_make_directed",
+               lib.psc_make_directed(n, e.size, _ptr(off, u64p), _ptr(e, u32p),
+                                     C.byref(hgp)))
+        return Graph._take(hgp)
+
+    @staticmethod
+    def from_edges(n, src, dst):
+        """Edge list -> CSR (kept as given: duplicates and self-loops stay)."""
+        src = np.asarray(src, dtype=np.uint64)
+        dst = np.asarray(dst, dtype=np.uint32)
+        order = np.lexsort((dst, src))
+        src, dst = src[order], dst[order]
+        off = np.zeros(n + 1, dtype=np.uint64)
+        np.add.at(off, src.astype(np.int64) + 1, 1)
+        off = np.cumsum(off).astype(np.uint64)
+        return Graph.from_out_csr(n, off, dst)
+
+    @staticmethod
+    def read_bin(path, large=False):
+        hgp = C.POINTER(CHostGraph)()
+        _check("psc_read_bin", lib.psc_read_bin(os.fsencode(path), int(large),
+                                                C.byref(hgp)))
+        return Graph._take(hgp)
+
+    def write_bin(self, path, large=True):
+        g = self.cgraph()
+        _check("psc_write_bin", lib.psc_write_bin(os.fsencode(path), C.byref(g),
+                                                  int(large)))
+
+
+def gen_rmat(scale, samples, a=0.57, b=0.19, c=0.19, seed=1, permute=True):
+    hgp = C.POINTER(CHostGraph)()
+    _check("psc_gen_rmat", lib.psc_gen_rmat(scale, samples, a, b, c, seed,
+                                            int(permute), C.byref(hgp)))
+    return Graph._take(hgp)
+
+
+def gen_grid(rows, cols, p_both=0.6, seed=2):
+    hgp = C.POINTER(CHostGraph)()
+    _check("psc_gen_grid", lib.psc_gen_grid(rows, cols, p_both, seed, C.byref(hgp)))
+    return Graph._take(hgp)
+
+
+def gen_knn3d(n, k=5, seed=4):
+    hgp = C.POINTER(CHostGraph)()
+    _check("psc_gen_knn3d", lib.psc_gen_knn3d(n, k, seed, C.byref(hgp)))
+    return Graph._take(hgp)
+
+
+class DeviceGraph:
+    """A graph generated straight into HBM by the device generators."""
+
+    def __init__(self, dgp):
+        self._p = dgp
+        self.n = dgp.contents.g.n
+        self.m = dgp.contents.g.m
+
+    @staticmethod
+    def rmat(scale, samples, a=0.57, b=0.19, c=0.19, seed=1, permute=True):
+        dgp = C.POINTER(CDevGraph)()
+        _check("psc_dev_gen_rmat", lib.psc_dev_gen_rmat(
+            scale, samples, a, b, c, seed, int(permute), C.byref(dgp)))
+        return DeviceGraph(dgp)
+
+    @staticmethod
+    def grid(rows, cols, p_both=0.6, seed=2):
+        dgp = C.POINTER(CDevGraph)()
+        _check("psc_dev_gen_grid", lib.psc_dev_gen_grid(rows, cols, p_both, seed,
+                                                        C.byref(dgp)))
+        return DeviceGraph(dgp)
+
+    def cgraph(self):
+        return self._p.contents.g
+
+    def to_host(self):
+        hgp = C.POINTER(CHostGraph)()
+        _check("psc_dev_graph_download",
+               lib.psc_dev_graph_download(self._p, C.byref(hgp)))
+        return Graph._take(hgp)
+
+    def free(self):
+        if self._p:
+            lib.psc_dev_graph_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def default_params(**kw):
+    p = CParams()
+    lib.psc_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class SCC:
+    """SCC(Graph&) + scc(labels, beta, local_reach, local_scc)
+    (src/scc.hpp:79-92,277).  The graph is uploaded once; every scc() call
+    reuses the device working set."""
+
+    def __init__(self, graph):
+        self.graph = graph
+        self.n = graph.n
+        self._g = graph.cgraph()
+        h = C.c_void_p()
+        if isinstance(graph, DeviceGraph):
+            _check("psc_create_device", lib.psc_create_device(C.byref(self._g),
+                                                              C.byref(h)))
+        else:
+            _check("psc_create", lib.psc_create(C.byref(self._g), C.byref(h)))
+        self._h = h
+        self.stats = None
+
+    def scc(self, labels=None, beta=1.5, local_reach=True, local_scc=True, **kw):
+        p = default_params(beta=beta, local_reach=int(local_reach),
+                           local_scc=int(local_scc), **kw)
+        if labels is None:
+            labels = np.empty(self.n, dtype=np.uint64)
+        st = CStats()
+        _check("psc_run", lib.psc_run(self._h, C.byref(p), _ptr(labels, u64p),
+                                      C.byref(st)))
+        self.stats = st.as_dict()
+        return labels
+
+    def scc_device(self, d_labels, beta=1.5, local_reach=True, local_scc=True, **kw):
+        """Labels stay in HBM at device address d_labels (n uint64)."""
+        p = default_params(beta=beta, local_reach=int(local_reach),
+                           local_scc=int(local_scc), **kw)
+        st = CStats()
+        _check("psc_run_device", lib.psc_run_device(self._h, C.byref(p),
+                                                    C.c_void_p(d_labels), C.byref(st)))
+        self.stats = st.as_dict()
+        return self.stats
+
+    def status_device(self, d_labels):
+        a, b = C.c_uint64(), C.c_uint64()
+        _check("psc_status_device", lib.psc_status_device(
+            self._h, C.c_void_p(d_labels), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self._h:
+            lib.psc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scc(graph, **kw):
+    """One-shot SCC labels (uint64, equal iff same SCC)."""
+    s = SCC(graph)
+    try:
+        return s.scc(**kw)
+    finally:
+        s.close()
+
+
+def status(labels):
+    """SCC_status (src/scc.cpp:13-29) on host labels: (n_scc, largest)."""
+    labels = np.asarray(labels)
+    if labels.size == 0:
+        return 0, 0
+    _, counts = np.unique(labels, return_counts=True)
+    return int(counts.size), int(counts.max())

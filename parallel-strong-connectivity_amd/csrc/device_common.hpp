@@ -1,0 +1,197 @@
+// Wave64 helpers shared by the gfx950 kernels: lane masks, wave scans,
+// wave-aggregated appends (one atomic per wave), visited bitmaps and the
+// (vertex, label) slot packing of the label table.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psc {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WPB = BLOCK / WAVE;
+
+// Internal label layout of src/scc.hpp:22-23: bit 32 marks "SCC found".
+constexpr uint64_t TOP_BIT = 1ull << 32;
+constexpr uint64_t VAL_MASK = 0xFFFFFFFFull;
+// Empty table slot {UINT_N_MAX, UINT_N_MAX} (src/scc.hpp:88-91,395).
+constexpr unsigned long long EMPTY_SLOT = ~0ull;
+
+// 32-bit integer mixer.  Placement only; results do not depend on it.
+__host__ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) {
+  return (uint64_t)__ballot(p);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int l = lane();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    T t = __shfl_up(x, d);
+    if (l >= d) x += t;
+  }
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+  return x;
+}
+
+// Wave-aggregated append: lanes with `want` get consecutive slots from
+// *ctr with one atomic for the whole wave.  Safe under partial masks.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *ctr, bool want) {
+  uint64_t m = ballot(want);
+  uint32_t base = 0;
+  if (m) {
+    int leader = __ffsll((unsigned long long)m) - 1;
+    if (lane() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+  }
+  return base + lanes_below(m);
+}
+
+__device__ __forceinline__ unsigned long long wave_append64(
+    unsigned long long *ctr, bool want) {
+  uint64_t m = ballot(want);
+  unsigned long long base = 0;
+  if (m) {
+    int leader = __ffsll((unsigned long long)m) - 1;
+    if (lane() == leader)
+      base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+  }
+  return base + lanes_below(m);
+}
+
+// Visited bitmap: test, and test-and-set with one atomicOr.
+__device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v) {
+  return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+__device__ __forceinline__ bool bit_set(uint32_t *bm, uint32_t v) {
+  const uint32_t b = 1u << (v & 31);
+  if (bm[v >> 5] & b) return false;
+  return (atomicOr(&bm[v >> 5], b) & b) == 0;
+}
+
+// Table slot / frontier pair: (vertex << 32) | label.
+__device__ __forceinline__ unsigned long long make_pair(uint32_t v,
+                                                        uint32_t lab) {
+  return ((unsigned long long)v << 32) | lab;
+}
+__device__ __forceinline__ uint32_t pair_vertex(unsigned long long p) {
+  return (uint32_t)(p >> 32);
+}
+__device__ __forceinline__ uint32_t pair_label(unsigned long long p) {
+  return (uint32_t)p;
+}
+
+// Label-table insert (src/resizable_table.h:214-240): linear probing from
+// hash32(k); one 64-bit CAS claims an empty slot.  Returns 1 if (k,v) is
+// new, 0 if present, -1 if the probe limit was hit (table treated as full).
+__device__ __forceinline__ int table_insert(unsigned long long *T,
+                                            uint64_t mask, uint32_t k,
+                                            uint32_t v, uint32_t limit,
+                                            uint32_t *probes) {
+  const unsigned long long kv = make_pair(k, v);
+  uint64_t h = hash32(k) & mask;
+  for (uint32_t i = 0; i < limit; ++i) {
+    unsigned long long s = __atomic_load_n(&T[h], __ATOMIC_RELAXED);
+    if (s == kv) {
+      *probes += i + 1;
+      return 0;
+    }
+    if (s == EMPTY_SLOT) {
+      unsigned long long old = atomicCAS(&T[h], EMPTY_SLOT, kv);
+      if (old == EMPTY_SLOT) {
+        *probes += i + 1;
+        return 1;
+      }
+      if (old == kv) {
+        *probes += i + 1;
+        return 0;
+      }
+    }
+    h = (h + 1) & mask;
+  }
+  *probes += limit;
+  return -1;
+}
+
+// contains(k, v) (src/resizable_table.h:283-294).
+__device__ __forceinline__ bool table_contains(const unsigned long long *T,
+                                               uint64_t mask, uint32_t k,
+                                               uint32_t v, uint32_t *probes) {
+  const unsigned long long kv = make_pair(k, v);
+  uint64_t h = hash32(k) & mask;
+  for (uint64_t i = 0; i <= mask; ++i) {
+    unsigned long long s = T[h];
+    if (s == kv) {
+      *probes += (uint32_t)i + 1;
+      return true;
+    }
+    if (s == EMPTY_SLOT) {
+      *probes += (uint32_t)i + 1;
+      return false;
+    }
+    h = (h + 1) & mask;
+  }
+  return false;
+}
+
+// Seeded bijection of [0, N) (cycle-walking a multiply/xorshift bijection
+// of the enclosing power of two); stands in for parlay::random_shuffle of
+// the non-trimmed vertices (src/scc.hpp:303).
+struct PivotPerm {
+  uint64_t N, mask, a, b;
+  int k;
+  __host__ __device__ PivotPerm() : N(0), mask(0), a(0), b(0), k(0) {}
+  __host__ __device__ PivotPerm(uint64_t n, uint64_t seed) : N(n) {
+    k = 0;
+    while (k < 63 && (1ull << k) < n) k++;
+    mask = (1ull << k) - 1;
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    a = (s ^ (s >> 31)) & mask;
+    s = s * 0xD1B54A32D192ED03ull + 1;
+    b = (s ^ (s >> 29)) & mask;
+  }
+  __host__ __device__ __forceinline__ uint64_t step(uint64_t x) const {
+    const int sh = k / 2 + 1;
+    x = (x * 0x9E3779B1ull + a) & mask;
+    x ^= x >> sh;
+    x = (x * 0x85EBCA77ull + b) & mask;
+    x ^= x >> sh;
+    return x;
+  }
+  __host__ __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+    uint64_t y = step(i);
+    while (y >= N) y = step(y);
+    return y;
+  }
+};
+
+}  // namespace psc

@@ -1,0 +1,70 @@
+// Host-side owner of one graph's device working set (the SCC object of
+// src/scc.hpp:38-115): CSR views, labels, reach bitmaps and frontiers,
+// pivot order, label tables, logs and pair frontiers.  Buffers grow on
+// demand and are reused across runs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/psc.h"
+
+namespace psc {
+
+struct ReachSearch;
+struct MSSearch;
+
+void set_error(const std::string &msg);
+extern thread_local std::string g_last_error;
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned staging layout (bytes): poll buffer, reach descriptors,
+// multi-search descriptors, flags.
+constexpr size_t PIN_POLL = 0, PIN_REACH = 1024, PIN_MS = 2048, PIN_FLAGS = 3072;
+
+struct Engine {
+  uint64_t n = 0, m = 0;
+  int device = 0, num_cu = 1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  void *h_pinned = nullptr;
+  bool owned_graph = false;
+  const uint64_t *d_off = nullptr, *d_ioff = nullptr;
+  const uint32_t *d_adj = nullptr, *d_iadj = nullptr;
+  DevBuf g_off, g_adj, g_ioff, g_iadj;   // owned copies (psc_create)
+  DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
+  DevBuf NZ, Z, P, centers, front, tiles, tile_off;
+  DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
+  DevBuf table[2], log[2], mq[2][2], mhp[2], mhc[2];
+  int grid_push = 1, grid_ms = 1, grid_pull = 1, grid_heavy = 1;
+  uint64_t log_n[2] = {0, 0};
+  uint64_t syncs = 0;
+  unsigned char h_reach[256];
+  unsigned char h_ms[256];
+
+  ~Engine();
+  std::vector<DevBuf *> all_bufs();
+  int init(const psc_graph *dg, bool owned);
+  int run(const psc_params &p, unsigned long long *d_out, psc_stats *st);
+  int status(const unsigned long long *d_labels, uint64_t *n_scc,
+             uint64_t *largest);
+
+  template <class F>
+  int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count);
+  int poll(void *dst, const void *src, size_t bytes);
+  int first_reach(const psc_params &p, psc_stats *st);
+  int ms_prepare(int s, uint64_t slots, MSSearch *d);
+  int multi_search(const psc_params &p, uint64_t nfront, uint64_t label_offset,
+                   uint64_t slots[2], psc_stats *st);
+};
+
+}  // namespace psc

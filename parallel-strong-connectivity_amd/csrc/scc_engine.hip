@@ -1,0 +1,1474 @@
+// MI355X (gfx950) SCC engine: the hot path of src/scc.hpp, src/reach.hpp,
+// src/hash_bag.h and src/resizable_table.h, re-designed for wave64 HIP.
+//
+//   trim + pivot order        src/scc.hpp:284-306   stable device select
+//   first SCC (fwd+bwd reach) src/scc.hpp:316-355   fused two-search levels
+//     sparse push + VGC       src/reach.hpp:114-171 k_reach_push/k_reach_heavy
+//     dense pull              src/reach.hpp:173-190 k_reach_pull
+//     direction switch        src/reach.hpp:86-112  k_reach_judge
+//   multi-search              src/scc.hpp:144-275   k_ms_seed/k_ms_expand/k_ms_heavy
+//   label table               src/resizable_table.h table_insert/table_contains
+//   partition                 src/scc.hpp:437-464   k_partition (atomicMax u64)
+//   finalize                  src/scc.hpp:475-476   k_finalize
+//
+// Frontiers are append buffers (wave ballot + prefix, one atomic per wave);
+// a multi-search frontier holds (vertex, label) pairs, i.e. exactly the
+// table entries created in the previous sub-round, so each pair is
+// propagated once.  All compactions are stable, so raw labels are a
+// deterministic function of (graph, seed).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "engine.hpp"
+#include "scc_kernels.hpp"
+
+namespace psc {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------ utilities
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n,
+                           unsigned long long v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Block-wide exclusive scan of one u32 per thread (BLOCK threads).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
+                                                    uint32_t *total) {
+  __shared__ uint32_t wsum[WPB];
+  const int w = threadIdx.x / WAVE, l = lane();
+  uint32_t inc = wave_incl_scan(x);
+  if (l == WAVE - 1) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < WPB; ++i) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
+}
+
+// ------------------------------------------------ stable select (3 kernels)
+constexpr int SEL_ITEMS = 8;
+constexpr int SEL_TILE = BLOCK * SEL_ITEMS;
+
+template <class F>
+__global__ __launch_bounds__(BLOCK) void k_sel_count(F f, uint64_t len,
+                                                     uint32_t *tile_cnt) {
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE + threadIdx.x * SEL_ITEMS;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j)
+    if (base + j < len && f.pred(base + j)) ++c;
+  uint32_t tot;
+  block_excl_scan(c, &tot);
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the tile counts in one block; total -> *out_total.
+__global__ __launch_bounds__(BLOCK) void k_sel_scan(uint32_t *tile_cnt,
+                                                    uint64_t ntiles,
+                                                    unsigned long long *tile_off,
+                                                    unsigned long long *out_total) {
+  __shared__ unsigned long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint64_t b = 0; b < ntiles; b += BLOCK) {
+    uint64_t i = b + threadIdx.x;
+    uint32_t x = i < ntiles ? tile_cnt[i] : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(x, &tot);
+    if (i < ntiles) tile_off[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+template <class F>
+__global__ __launch_bounds__(BLOCK) void k_sel_scatter(
+    F f, uint64_t len, const unsigned long long *tile_off, uint32_t *sel,
+    uint32_t *rej) {
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE + threadIdx.x * SEL_ITEMS;
+  bool p[SEL_ITEMS];
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    p[j] = base + j < len && f.pred(base + j);
+    c += p[j];
+  }
+  uint32_t tot;
+  uint64_t r = tile_off[blockIdx.x] + block_excl_scan(c, &tot);
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + j;
+    if (i >= len) break;
+    if (p[j]) {
+      sel[r++] = f.val(i);
+    } else if (rej) {
+      rej[i - r] = f.val(i);
+    }
+  }
+}
+
+// pred: vertex has non-zero in- and out-degree (src/scc.hpp:295-302)
+struct SelNonZero {
+  const uint64_t *off, *ioff;
+  __device__ bool pred(uint64_t v) const {
+    return off[v + 1] != off[v] && ioff[v + 1] != ioff[v];
+  }
+  __device__ uint32_t val(uint64_t v) const { return (uint32_t)v; }
+};
+// pred: not yet in a found SCC; select from a vertex list (src/scc.hpp:349,384-388)
+struct SelNotDone {
+  const uint32_t *src;
+  const unsigned long long *labels;
+  __device__ bool pred(uint64_t i) const {
+    return !(labels[src[i]] & TOP_BIT);
+  }
+  __device__ uint32_t val(uint64_t i) const { return src[i]; }
+};
+
+// ----------------------------------------------------------- trim kernels
+__global__ void k_trim_labels(const uint32_t *Z, uint64_t nz,
+                              unsigned long long *labels) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nz;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    labels[Z[i]] = 1 + (i | TOP_BIT);   // src/scc.hpp:304-305
+}
+__global__ void k_permute(const uint32_t *NZ, uint64_t nnz, PivotPerm perm,
+                          uint32_t *P) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    P[i] = NZ[perm(i)];
+}
+
+// ----------------------------------------------------------- reach kernels
+// Level 0 state: source visited, frontier {source}; both searches.
+__global__ void k_reach_init(ReachSearch *S, ReachCtl *C, int ns,
+                             const uint32_t *src_ptr, uint64_t words) {
+  const uint32_t src = *src_ptr;
+  for (int s = 0; s < ns; ++s) {
+    uint32_t *vis = S[s].vis;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+         i < words; i += (uint64_t)gridDim.x * blockDim.x)
+      vis[i] = (i == (src >> 5)) ? (1u << (src & 31)) : 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int s = 0; s < ns; ++s) {
+      S[s].q[0][0] = src;
+      ReachSlot z{};
+      C->slot[s][1] = z;
+      z.qsize = 1;
+      C->slot[s][0] = z;
+      C->done[s] = 0;
+      C->levels[s] = 0;
+      C->edges[s] = C->expanded[s] = C->visits[s] = 0;
+      C->pull_scans[s] = C->appended[s] = 0;
+    }
+  }
+}
+
+// judge() (src/reach.hpp:86-112): sum the frontier's out-degrees when the
+// frontier may be sparse; the consumers derive the mode from (nf, degsum).
+// Also clears the counters of the next level's slot.
+__global__ __launch_bounds__(BLOCK) void k_reach_judge(const ReachSearch *S,
+                                                       ReachCtl *C, int ns,
+                                                       uint32_t L) {
+  const int p = L & 1;
+  for (int s = 0; s < ns; ++s) {
+    ReachSlot *cur = &C->slot[s][p];
+    const uint32_t nf = cur->qsize;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ReachSlot z{};
+      C->slot[s][p ^ 1] = z;
+      if (nf == 0 && !C->done[s]) {
+        C->done[s] = 1;
+        C->levels[s] = L;
+      }
+    }
+    if (nf == 0 || nf >= S[s].thresh) continue;
+    const uint64_t *off = S[s].off;
+    const uint32_t *q = S[s].q[p];
+    unsigned long long d = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nf;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+      const uint32_t u = q[i];
+      d += off[u + 1] - off[u];
+    }
+    d = wave_sum(d);
+    if (lane() == 0 && d) atomicAdd(&cur->degsum, d);
+  }
+}
+
+// Sparse push step with VGC (src/reach.hpp:114-171).  A wave takes S seeds,
+// keeps a queue of up to REACH_RING vertices in LDS and expands it with
+// load-balanced edge scans (lane e of a 64-edge window finds its owner by
+// binary search over the wave's inclusive degree scan).  Newly visited
+// vertices stay in the queue while the VGC budget (tau per seed) lasts and
+// the ring has room; the rest spill to the next global frontier.  Vertices
+// with degree > HEAVY_DEG go to the heavy list, expanded by k_reach_heavy.
+__global__ __launch_bounds__(BLOCK) void k_reach_push(
+    const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
+    int vgc, uint32_t total_waves) {
+  __shared__ uint32_t ring_s[WPB][REACH_RING];
+  __shared__ uint32_t sc_s[WPB][WAVE];
+  __shared__ uint64_t st_s[WPB][WAVE];
+  const int w = threadIdx.x / WAVE, l = lane();
+  uint32_t *ring = ring_s[w];
+  uint32_t *sc = sc_s[w];
+  uint64_t *st = st_s[w];
+  const int p = L & 1;
+  for (int s = 0; s < ns; ++s) {
+    ReachSlot *cur = &C->slot[s][p];
+    ReachSlot *nxt = &C->slot[s][p ^ 1];
+    const uint32_t nf = cur->qsize;
+    const ReachSearch R = S[s];
+    if (reach_mode(nf, cur->degsum, R.thresh) != MODE_PUSH) continue;
+    uint32_t sd = 64;
+    if (vgc) sd = min(64u, max(1u, nf / total_waves));
+    unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0;
+    while (true) {
+      uint32_t g = 0;
+      if (l == 0) g = atomicAdd(&cur->grab, sd);
+      g = __shfl(g, 0);
+      if (g >= nf) break;
+      const uint32_t cnt = min(sd, nf - g);
+      if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
+      uint32_t head = 0, tail = cnt;
+      uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+      wave_sync();
+      while (head < tail) {
+        const uint32_t k = min(64u, tail - head);
+        const bool valid = (uint32_t)l < k;
+        const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
+        head += k;
+        uint64_t b = 0, deg = 0;
+        if (valid) {
+          b = R.off[u];
+          deg = R.off[u + 1] - b;
+          ++n_exp;
+        }
+        const bool heavy = deg > HEAVY_DEG;
+        const uint64_t hm = ballot(heavy);
+        if (hm) {
+          const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
+          const unsigned long long cinc = wave_incl_scan(nch);
+          const unsigned long long ctot = __shfl(cinc, WAVE - 1);
+          unsigned long long old = 0;
+          if (l == 0)
+            old = atomicAdd(&cur->heavy,
+                            ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
+          old = __shfl(old, 0);
+          if (heavy) {
+            const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
+            R.hv[idx] = u;
+            R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
+          }
+          if (heavy) deg = 0;
+        }
+        const uint32_t inc = wave_incl_scan((uint32_t)deg);
+        const uint32_t total = __shfl(inc, WAVE - 1);
+        sc[l] = inc;
+        st[l] = b;
+        wave_sync();
+        for (uint32_t base = 0; base < total; base += WAVE) {
+          const uint32_t e = base + l;
+          bool found = false;
+          uint32_t v = 0;
+          if (e < total) {
+            int lo = 0, hi = WAVE - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (sc[mid] > e) hi = mid; else lo = mid + 1;
+            }
+            const uint32_t ex = lo ? sc[lo - 1] : 0u;
+            v = R.adj[st[lo] + (e - ex)];
+            found = bit_set(R.vis, v);
+            ++n_edges;
+          }
+          const uint64_t fm = ballot(found);
+          if (!fm) continue;
+          const uint32_t nfound = (uint32_t)__popcll(fm);
+          n_vis += found;
+          const uint32_t room = REACH_RING - (tail - head);
+          const uint32_t take =
+              (uint32_t)min((uint64_t)min(nfound, room), budget);
+          const uint32_t rank = lanes_below(fm);
+          const bool loc = found && rank < take;
+          if (loc) ring[(tail + rank) & (REACH_RING - 1)] = v;
+          tail += take;
+          budget -= take;
+          const bool glob = found && !loc;
+          const uint32_t pos = wave_append(&nxt->qsize, glob);
+          if (glob) {
+            R.q[p ^ 1][pos] = v;
+            ++n_app;
+          }
+        }
+        wave_sync();
+      }
+    }
+    n_edges = wave_sum(n_edges);
+    n_exp = wave_sum(n_exp);
+    n_vis = wave_sum(n_vis);
+    n_app = wave_sum(n_app);
+    if (l == 0) {
+      if (n_edges) atomicAdd(&C->edges[s], n_edges);
+      if (n_exp) atomicAdd(&C->expanded[s], n_exp);
+      if (n_vis) atomicAdd(&C->visits[s], n_vis);
+      if (n_app) atomicAdd(&C->appended[s], n_app);
+    }
+  }
+}
+
+// Edge-parallel expansion of the heavy vertices of this level
+// (src/reach.hpp:155-167, granularity 2048): work items are CHUNK-edge
+// slices; a wave finds the owning vertex by binary search over hc.
+__global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
+                                                       ReachCtl *C, int ns,
+                                                       uint32_t L) {
+  const int l = lane();
+  const int p = L & 1;
+  for (int s = 0; s < ns; ++s) {
+    ReachSlot *cur = &C->slot[s][p];
+    ReachSlot *nxt = &C->slot[s][p ^ 1];
+    const ReachSearch R = S[s];
+    if (reach_mode(cur->qsize, cur->degsum, R.thresh) != MODE_PUSH) continue;
+    const unsigned long long hw = cur->heavy;
+    const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
+    if (H == 0) continue;
+    unsigned long long n_edges = 0, n_vis = 0, n_app = 0;
+    while (true) {
+      unsigned long long c = 0;
+      if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
+      c = __shfl(c, 0);
+      if (c >= Ct) break;
+      uint64_t lo = 0, hi = H - 1;
+      while (lo < hi) {   // last idx with hc[idx] <= c
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (R.hc[mid] <= c) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t u = R.hv[lo];
+      const uint64_t k = c - R.hc[lo];
+      const uint64_t eb = R.off[u] + k * CHUNK;
+      const uint64_t ee = min(R.off[u + 1], eb + CHUNK);
+      for (uint64_t j0 = eb; j0 < ee; j0 += WAVE) {
+        const uint64_t j = j0 + l;
+        bool found = false;
+        uint32_t v = 0;
+        if (j < ee) {
+          v = R.adj[j];
+          found = bit_set(R.vis, v);
+          ++n_edges;
+        }
+        n_vis += found;
+        const uint32_t pos = wave_append(&nxt->qsize, found);
+        if (found) {
+          R.q[p ^ 1][pos] = v;
+          ++n_app;
+        }
+      }
+    }
+    n_edges = wave_sum(n_edges);
+    n_vis = wave_sum(n_vis);
+    n_app = wave_sum(n_app);
+    if (l == 0) {
+      if (n_edges) atomicAdd(&C->edges[s], n_edges);
+      if (n_vis) atomicAdd(&C->visits[s], n_vis);
+      if (n_app) atomicAdd(&C->appended[s], n_app);
+    }
+  }
+}
+
+// Dense pull step (src/reach.hpp:173-190): every unvisited vertex scans its
+// in-edges until it meets a visited vertex.  Testing "visited" instead of
+// "in the previous frontier" keeps the reachable set identical and can only
+// stop earlier.  A wave covers 64 consecutive vertices (two bitmap words),
+// so the new bits are published with two atomicOr per wave.
+__global__ __launch_bounds__(BLOCK) void k_reach_pull(const ReachSearch *S,
+                                                      ReachCtl *C, int ns,
+                                                      uint32_t L) {
+  const int l = lane();
+  const int p = L & 1;
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (int s = 0; s < ns; ++s) {
+    ReachSlot *cur = &C->slot[s][p];
+    ReachSlot *nxt = &C->slot[s][p ^ 1];
+    const ReachSearch R = S[s];
+    if (reach_mode(cur->qsize, cur->degsum, R.thresh) != MODE_PULL) continue;
+    const uint64_t units = (R.n + 63) / 64;
+    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0;
+    for (uint64_t t = gw; t < units; t += nw) {
+      const uint64_t v = t * 64 + l;
+      bool found = false;
+      if (v < R.n && !bit_test(R.vis, (uint32_t)v)) {
+        ++n_scan;
+        const uint64_t b = R.roff[v], e = R.roff[v + 1];
+        for (uint64_t j = b; j < e; ++j) {
+          ++n_edges;
+          if (bit_test(R.vis, R.radj[j])) {
+            found = true;
+            break;
+          }
+        }
+      }
+      const uint64_t fm = ballot(found);
+      if (!fm) continue;
+      if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[t * 2], (uint32_t)fm);
+      if (l == 32 && (fm >> 32)) atomicOr(&R.vis[t * 2 + 1], (uint32_t)(fm >> 32));
+      n_vis += found;
+      const uint32_t pos = wave_append(&nxt->qsize, found);
+      if (found) R.q[p ^ 1][pos] = (uint32_t)v;
+    }
+    n_edges = wave_sum(n_edges);
+    n_scan = wave_sum(n_scan);
+    n_vis = wave_sum(n_vis);
+    if (l == 0) {
+      if (n_edges) atomicAdd(&C->edges[s], n_edges);
+      if (n_scan) atomicAdd(&C->pull_scans[s], n_scan);
+      if (n_vis) {
+        atomicAdd(&C->visits[s], n_vis);
+        atomicAdd(&C->appended[s], n_vis);
+      }
+    }
+  }
+}
+
+// First-SCC labelling (src/scc.hpp:340-347): both reaches -> found SCC,
+// one reach -> new subproblem label.
+__global__ void k_first_labels(const uint32_t *P, uint64_t np,
+                               const uint32_t *vf, const uint32_t *vb,
+                               unsigned long long *labels,
+                               unsigned long long label,
+                               unsigned long long *big) {
+  unsigned long long c = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < np;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = P[i];
+    const bool a = bit_test(vf, v), b = bit_test(vb, v);
+    if (a && b) {
+      labels[v] = label | TOP_BIT;
+      ++c;
+    } else if (a || b) {
+      labels[v] = label;
+    }
+  }
+  c = wave_sum(c);
+  if (lane() == 0 && c) atomicAdd(big, c);
+}
+
+// ----------------------------------------------------- multi-search kernels
+__global__ void k_ms_reset(MSCtl *C, int s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    MSSlot z{};
+    for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
+    C->log_n[s] = 0;
+    C->full[s] = 0;
+    C->edges[s] = C->expanded[s] = C->inserts[s] = 0;
+    C->probes[s] = C->appended[s] = 0;
+  }
+}
+
+// Seed (pivot_i, label_offset + i) into table, log and frontier
+// (src/scc.hpp:147-150).
+__global__ void k_ms_seed(MSSearch *S, MSCtl *C, int s, const uint32_t *front,
+                          uint64_t nfront, unsigned long long label_offset) {
+  const MSSearch M = S[s];
+  uint32_t probes = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nfront;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = front[i];
+    const uint32_t lab = (uint32_t)(label_offset + i);
+    const int r = table_insert(M.table, M.mask, v, lab, M.probe_limit, &probes);
+    if (r < 0) C->full[s] = 1;
+    M.log[i] = make_pair(v, lab);
+    M.q[0][i] = make_pair(v, lab);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    C->slot[s][0].qsize = nfront;
+    C->log_n[s] = nfront;
+    C->inserts[s] = nfront;
+  }
+}
+
+// Record a newly inserted pair in the log; flags overflow when the log
+// (load limit of the table) is exhausted.
+__device__ __forceinline__ bool ms_log(const MSSearch &M, MSCtl *C, int s,
+                                       bool ins, unsigned long long pr) {
+  const unsigned long long idx = wave_append64(&C->log_n[s], ins);
+  if (ins) {
+    if (idx < M.log_cap) {
+      M.log[idx] = pr;
+    } else {
+      C->full[s] = 1;
+      return false;
+    }
+  }
+  return ins;
+}
+
+// One sub-round of multi-search (src/scc.hpp:160-252) for both directions.
+// Frontier items are (v, l) pairs; for each edge v->w with
+// labels[w] == labels[v] and w not in a found SCC, insert (w, l); a new
+// pair is logged and either expanded by the same wave (VGC, budget tau per
+// seed, src/scc.hpp:177-219) or spilled to the next frontier.
+__global__ __launch_bounds__(BLOCK) void k_ms_expand(
+    const MSSearch *S, MSCtl *C, int ns, uint32_t L,
+    const unsigned long long *__restrict__ labels, uint32_t tau, int vgc,
+    uint32_t total_waves) {
+  __shared__ unsigned long long ring_s[WPB][MS_RING];
+  __shared__ uint32_t sc_s[WPB][WAVE];
+  __shared__ uint64_t st_s[WPB][WAVE];
+  __shared__ unsigned long long lab_s[WPB][WAVE];
+  __shared__ uint32_t el_s[WPB][WAVE];
+  const int w = threadIdx.x / WAVE, l = lane();
+  unsigned long long *ring = ring_s[w];
+  uint32_t *sc = sc_s[w];
+  uint64_t *st = st_s[w];
+  unsigned long long *lab = lab_s[w];
+  uint32_t *el = el_s[w];
+  const int ci = L % 3, ni = (L + 1) % 3, zi = (L + 2) % 3;
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)ns) {
+    MSSlot z{};
+    C->slot[threadIdx.x][zi] = z;
+  }
+  const int pq = L & 1;
+  for (int s = 0; s < ns; ++s) {
+    if (!C->active[s]) continue;
+    MSSlot *cur = &C->slot[s][ci];
+    MSSlot *nxt = &C->slot[s][ni];
+    const unsigned long long nf = cur->qsize;
+    if (nf == 0 || C->full[s]) continue;
+    const MSSearch M = S[s];
+    unsigned long long sd = 64;
+    if (vgc) sd = min(64ull, max(1ull, nf / total_waves));
+    unsigned long long n_edges = 0, n_exp = 0, n_ins = 0, n_app = 0;
+    uint32_t probes = 0;
+    while (true) {
+      unsigned long long g = 0;
+      if (l == 0) g = atomicAdd(&cur->grab, sd);
+      g = __shfl(g, 0);
+      if (g >= nf) break;
+      if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
+      const uint32_t cnt = (uint32_t)min(sd, nf - g);
+      if ((uint32_t)l < cnt) ring[l] = M.q[pq][g + l];
+      uint32_t head = 0, tail = cnt;
+      uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+      wave_sync();
+      while (head < tail) {
+        const uint32_t k = min(64u, tail - head);
+        const bool valid = (uint32_t)l < k;
+        const unsigned long long pr = valid ? ring[(head + l) & (MS_RING - 1)] : 0ull;
+        head += k;
+        const uint32_t v = pair_vertex(pr);
+        uint64_t b = 0, deg = 0;
+        unsigned long long lv = 0;
+        if (valid) {
+          b = M.off[v];
+          deg = M.off[v + 1] - b;
+          lv = labels[v];
+          ++n_exp;
+        }
+        const bool heavy = deg > HEAVY_DEG;
+        const uint64_t hm = ballot(heavy);
+        if (hm) {
+          const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
+          const unsigned long long cinc = wave_incl_scan(nch);
+          const unsigned long long ctot = __shfl(cinc, WAVE - 1);
+          unsigned long long old = 0;
+          if (l == 0)
+            old = atomicAdd(&cur->heavy,
+                            ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
+          old = __shfl(old, 0);
+          if (heavy) {
+            const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
+            M.hp[idx] = pr;
+            M.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
+            deg = 0;
+          }
+        }
+        const uint32_t inc = wave_incl_scan((uint32_t)deg);
+        const uint32_t total = __shfl(inc, WAVE - 1);
+        sc[l] = inc;
+        st[l] = b;
+        lab[l] = lv;
+        el[l] = pair_label(pr);
+        wave_sync();
+        for (uint32_t base = 0; base < total; base += WAVE) {
+          const uint32_t e = base + l;
+          bool ins = false;
+          unsigned long long np = 0;
+          if (e < total) {
+            int lo = 0, hi = WAVE - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (sc[mid] > e) hi = mid; else lo = mid + 1;
+            }
+            const uint32_t ex = lo ? sc[lo - 1] : 0u;
+            const uint32_t x = M.adj[st[lo] + (e - ex)];
+            ++n_edges;
+            const unsigned long long lx = labels[x];
+            if (!(lx & TOP_BIT) && lx == lab[lo]) {
+              const int r = table_insert(M.table, M.mask, x, el[lo],
+                                         M.probe_limit, &probes);
+              if (r < 0) C->full[s] = 1;
+              ins = r > 0;
+              np = make_pair(x, el[lo]);
+            }
+          }
+          ins = ms_log(M, C, s, ins, np);
+          const uint64_t fm = ballot(ins);
+          if (!fm) continue;
+          n_ins += ins;
+          const uint32_t nnew = (uint32_t)__popcll(fm);
+          const uint32_t room = MS_RING - (tail - head);
+          const uint32_t take = (uint32_t)min((uint64_t)min(nnew, room), budget);
+          const uint32_t rank = lanes_below(fm);
+          const bool loc = ins && rank < take;
+          if (loc) ring[(tail + rank) & (MS_RING - 1)] = np;
+          tail += take;
+          budget -= take;
+          const bool glob = ins && !loc;
+          const unsigned long long pos = wave_append64(&nxt->qsize, glob);
+          if (glob) {
+            if (pos < M.q_cap) M.q[pq ^ 1][pos] = np;
+            else C->full[s] = 1;
+            ++n_app;
+          }
+        }
+        wave_sync();
+      }
+    }
+    n_edges = wave_sum(n_edges);
+    n_exp = wave_sum(n_exp);
+    n_ins = wave_sum(n_ins);
+    n_app = wave_sum(n_app);
+    unsigned long long pb = wave_sum((unsigned long long)probes);
+    if (l == 0) {
+      if (n_edges) atomicAdd(&C->edges[s], n_edges);
+      if (n_exp) atomicAdd(&C->expanded[s], n_exp);
+      if (n_ins) atomicAdd(&C->inserts[s], n_ins);
+      if (n_app) atomicAdd(&C->appended[s], n_app);
+      if (pb) atomicAdd(&C->probes[s], pb);
+    }
+  }
+}
+
+// Heavy pairs of this sub-round: CHUNK-edge slices over all waves.
+__global__ __launch_bounds__(BLOCK) void k_ms_heavy(
+    const MSSearch *S, MSCtl *C, int ns, uint32_t L,
+    const unsigned long long *__restrict__ labels) {
+  const int l = lane();
+  const int ci = L % 3, ni = (L + 1) % 3;
+  const int pq = L & 1;
+  for (int s = 0; s < ns; ++s) {
+    if (!C->active[s]) continue;
+    MSSlot *cur = &C->slot[s][ci];
+    MSSlot *nxt = &C->slot[s][ni];
+    const unsigned long long hw = cur->heavy;
+    const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
+    if (H == 0 || C->full[s]) continue;
+    const MSSearch M = S[s];
+    unsigned long long n_edges = 0, n_ins = 0, n_app = 0;
+    uint32_t probes = 0;
+    while (true) {
+      unsigned long long c = 0;
+      if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
+      c = __shfl(c, 0);
+      if (c >= Ct) break;
+      uint64_t lo = 0, hi = H - 1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (M.hc[mid] <= c) lo = mid; else hi = mid - 1;
+      }
+      const unsigned long long pr = M.hp[lo];
+      const uint32_t v = pair_vertex(pr), ell = pair_label(pr);
+      const unsigned long long lv = labels[v];
+      const uint64_t k = c - M.hc[lo];
+      const uint64_t eb = M.off[v] + k * CHUNK;
+      const uint64_t ee = min(M.off[v + 1], eb + CHUNK);
+      for (uint64_t j0 = eb; j0 < ee; j0 += WAVE) {
+        const uint64_t j = j0 + l;
+        bool ins = false;
+        unsigned long long np = 0;
+        if (j < ee) {
+          const uint32_t x = M.adj[j];
+          ++n_edges;
+          const unsigned long long lx = labels[x];
+          if (!(lx & TOP_BIT) && lx == lv) {
+            const int r = table_insert(M.table, M.mask, x, ell, M.probe_limit,
+                                       &probes);
+            if (r < 0) C->full[s] = 1;
+            ins = r > 0;
+            np = make_pair(x, ell);
+          }
+        }
+        ins = ms_log(M, C, s, ins, np);
+        n_ins += ins;
+        const unsigned long long pos = wave_append64(&nxt->qsize, ins);
+        if (ins) {
+          if (pos < M.q_cap) M.q[pq ^ 1][pos] = np;
+          else C->full[s] = 1;
+          ++n_app;
+        }
+      }
+    }
+    n_edges = wave_sum(n_edges);
+    n_ins = wave_sum(n_ins);
+    n_app = wave_sum(n_app);
+    unsigned long long pb = wave_sum((unsigned long long)probes);
+    if (l == 0) {
+      if (n_edges) atomicAdd(&C->edges[s], n_edges);
+      if (n_ins) atomicAdd(&C->inserts[s], n_ins);
+      if (n_app) atomicAdd(&C->appended[s], n_app);
+      if (pb) atomicAdd(&C->probes[s], pb);
+    }
+  }
+}
+
+// Partition (src/scc.hpp:437-464): pairs of the smaller table get
+// label|TOP_BIT when the larger table holds them too, else label; pairs of
+// the larger table get label.  write_max is commutative, so both logs are
+// applied in one launch.
+__global__ __launch_bounds__(BLOCK) void k_partition(
+    const MSSearch *S, const MSCtl *C, int small, unsigned long long *labels,
+    unsigned long long *probe_ctr) {
+  const int large = small ^ 1;
+  const MSSearch A = S[small], B = S[large];
+  const uint64_t na = C->log_n[small], nb = C->log_n[large];
+  uint32_t probes = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+       i < na + nb; i += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long pr, val;
+    if (i < na) {
+      pr = A.log[i];
+      const uint32_t v = pair_vertex(pr), ell = pair_label(pr);
+      val = table_contains(B.table, B.mask, v, ell, &probes)
+                ? ((unsigned long long)ell | TOP_BIT)
+                : (unsigned long long)ell;
+    } else {
+      pr = B.log[i - na];
+      val = pair_label(pr);
+    }
+    unsigned long long *dst = &labels[pair_vertex(pr)];
+    if (*dst < val) atomicMax(dst, val);
+  }
+  unsigned long long pb = wave_sum((unsigned long long)probes);
+  if (lane() == 0 && pb) atomicAdd(probe_ctr, pb);
+}
+
+// -------------------------------------------------------- finalize, status
+__global__ void k_finalize(unsigned long long *labels, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    labels[i] = (labels[i] & VAL_MASK) - 1;   // src/scc.hpp:475-476
+}
+__global__ void k_canon_min(const unsigned long long *labels, uint64_t n,
+                            uint32_t *minv) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    atomicMin(&minv[labels[i]], (uint32_t)i);
+}
+__global__ void k_canon_apply(unsigned long long *labels, uint64_t n,
+                              const uint32_t *minv) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    labels[i] = minv[labels[i]];
+}
+__global__ void k_count_labels(const unsigned long long *labels, uint64_t n,
+                               uint32_t *cnt, unsigned long long *bad) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long x = labels[i];
+    if (x <= n) atomicAdd(&cnt[x], 1u);
+    else atomicAdd(bad, 1ull);
+  }
+}
+__global__ void k_status_reduce(const uint32_t *cnt, uint64_t len,
+                                unsigned long long *nscc, uint32_t *largest) {
+  unsigned long long c = 0;
+  uint32_t mx = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    c += cnt[i] != 0;
+    mx = max(mx, cnt[i]);
+  }
+  c = wave_sum(c);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d));
+  if (lane() == 0) {
+    if (c) atomicAdd(nscc, c);
+    atomicMax(largest, mx);
+  }
+}
+
+// ======================================================================
+//                              host side
+// ======================================================================
+#define PSC_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+      return PSC_ERR_HIP;                                                 \
+    }                                                                     \
+  } while (0)
+
+#define PSC_TRY(expr)            \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != PSC_OK) return r_; \
+  } while (0)
+
+static int grid_for(uint64_t n, int per_block = BLOCK, int cap = 65536) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > (uint64_t)cap) g = cap;
+  return (int)g;
+}
+
+int DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap) return PSC_OK;
+  if (ptr) {
+    hipError_t e = hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    if (e != hipSuccess) {
+      set_error(std::string("hipFree: ") + hipGetErrorString(e));
+      return PSC_ERR_HIP;
+    }
+  }
+  if (bytes == 0) return PSC_OK;
+  hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    set_error("hipMalloc(" + std::to_string(bytes) + " B): " +
+              hipGetErrorString(e));
+    return PSC_ERR_NOMEM;
+  }
+  cap = bytes;
+  return PSC_OK;
+}
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+Engine::~Engine() {
+  for (DevBuf *b : all_bufs()) b->release();
+  if (owned_graph)
+    for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
+  if (h_pinned) (void)hipHostFree(h_pinned);
+  for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+std::vector<DevBuf *> Engine::all_bufs() {
+  return {&labels, &vis[0], &vis[1], &rq[0][0], &rq[0][1], &rq[1][0],
+          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &Z, &P, &centers,
+          &front, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
+          &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
+          &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
+          &mhc[0], &mhc[1], &scratch32};
+}
+
+int Engine::init(const psc_graph *dg, bool owned) {
+  n = dg->n;
+  m = dg->m;
+  owned_graph = owned;
+  d_off = dg->offset;
+  d_adj = dg->edges;
+  d_ioff = dg->in_offset;
+  d_iadj = dg->in_edges;
+  PSC_HIP(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  PSC_HIP(hipGetDeviceProperties(&prop, device));
+  num_cu = prop.multiProcessorCount;
+  PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
+  PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  int occ = 0;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_push,
+                                                       BLOCK, 0));
+  grid_push = std::max(1, occ) * num_cu;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ms_expand,
+                                                       BLOCK, 0));
+  grid_ms = std::max(1, occ) * num_cu;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_pull,
+                                                       BLOCK, 0));
+  grid_pull = std::max(1, occ) * num_cu;
+  grid_heavy = grid_pull;
+
+  const uint64_t nn = std::max<uint64_t>(n, 1);
+  const uint64_t words = (nn + 31) / 32;
+  PSC_TRY(labels.ensure(nn * 8));
+  for (int s = 0; s < 2; ++s) {
+    PSC_TRY(vis[s].ensure(words * 4));
+    PSC_TRY(rq[s][0].ensure(nn * 4));
+    PSC_TRY(rq[s][1].ensure(nn * 4));
+    const uint64_t hcap = std::min<uint64_t>(nn, m / HEAVY_DEG + 1);
+    PSC_TRY(hv[s].ensure(hcap * 4));
+    PSC_TRY(hc[s].ensure(hcap * 8));
+  }
+  PSC_TRY(NZ.ensure(nn * 4));
+  PSC_TRY(Z.ensure(nn * 4));
+  PSC_TRY(P.ensure(nn * 4));
+  PSC_TRY(centers.ensure(nn * 4));
+  PSC_TRY(front.ensure(nn * 4));
+  const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
+  PSC_TRY(tiles.ensure(ntiles * 4));
+  PSC_TRY(tile_off.ensure(ntiles * 8));
+  PSC_TRY(reach_desc.ensure(sizeof(ReachSearch) * 2));
+  PSC_TRY(reach_ctl.ensure(sizeof(ReachCtl)));
+  PSC_TRY(ms_desc.ensure(sizeof(MSSearch) * 2));
+  PSC_TRY(ms_ctl.ensure(sizeof(MSCtl)));
+  PSC_TRY(misc.ensure(256));
+  PSC_TRY(scratch32.ensure((nn + 1) * 4));
+
+  ReachSearch rs[2];
+  for (int s = 0; s < 2; ++s) {
+    rs[s].off = s == 0 ? d_off : d_ioff;
+    rs[s].adj = s == 0 ? d_adj : d_iadj;
+    rs[s].roff = s == 0 ? d_ioff : d_off;
+    rs[s].radj = s == 0 ? d_iadj : d_adj;
+    rs[s].vis = vis[s].as<uint32_t>();
+    rs[s].q[0] = rq[s][0].as<uint32_t>();
+    rs[s].q[1] = rq[s][1].as<uint32_t>();
+    rs[s].hv = hv[s].as<uint32_t>();
+    rs[s].hc = hc[s].as<unsigned long long>();
+    rs[s].thresh = 0;
+    rs[s].n = (uint32_t)n;
+    rs[s].pad = 0;
+  }
+  std::memcpy(h_reach, rs, sizeof(rs));
+  PSC_HIP(hipStreamSynchronize(stream));
+  return PSC_OK;
+}
+
+// Stable select over [0, len) with functor f; count -> host.
+template <class F>
+int Engine::select(F f, uint64_t len, uint32_t *sel, uint32_t *rej,
+                   uint64_t *count) {
+  if (len == 0) {
+    *count = 0;
+    return PSC_OK;
+  }
+  const uint64_t nt = (len + SEL_TILE - 1) / SEL_TILE;
+  unsigned long long *tot = misc.as<unsigned long long>();
+  k_sel_count<F><<<(unsigned)nt, BLOCK, 0, stream>>>(f, len, tiles.as<uint32_t>());
+  k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
+                                       tile_off.as<unsigned long long>(), tot);
+  k_sel_scatter<F><<<(unsigned)nt, BLOCK, 0, stream>>>(
+      f, len, tile_off.as<unsigned long long>(), sel, rej);
+  PSC_HIP(hipGetLastError());
+  PSC_HIP(hipMemcpyAsync(h_pinned, tot, 8, hipMemcpyDeviceToHost, stream));
+  PSC_HIP(hipStreamSynchronize(stream));
+  ++syncs;
+  *count = *(uint64_t *)h_pinned;
+  return PSC_OK;
+}
+
+int Engine::poll(void *dst, const void *src, size_t bytes) {
+  char *buf = (char *)h_pinned + PIN_POLL;
+  PSC_HIP(hipMemcpyAsync(buf, src, bytes, hipMemcpyDeviceToHost, stream));
+  PSC_HIP(hipStreamSynchronize(stream));
+  ++syncs;
+  std::memcpy(dst, buf, bytes);
+  return PSC_OK;
+}
+
+// First round: fused forward + backward reach from P[0] (src/scc.hpp:316-339).
+int Engine::first_reach(const psc_params &p, psc_stats *st) {
+  ReachSearch rs[2];
+  std::memcpy(rs, h_reach, sizeof(rs));
+  const uint64_t thresh = m / (p.dense_div ? p.dense_div : 20);
+  rs[0].thresh = rs[1].thresh = thresh;
+  char *pin = (char *)h_pinned;
+  std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
+  PSC_HIP(hipMemcpyAsync(reach_desc.ptr, pin + PIN_REACH, sizeof(rs),
+                         hipMemcpyHostToDevice, stream));
+  ReachSearch *S = reach_desc.as<ReachSearch>();
+  ReachCtl *C = reach_ctl.as<ReachCtl>();
+  const uint64_t words = (n + 31) / 32;
+  k_reach_init<<<grid_for(words), BLOCK, 0, stream>>>(S, C, 2, P.as<uint32_t>(),
+                                                       words);
+  const uint32_t total_waves = (uint32_t)grid_push * WPB;
+  const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
+  uint32_t L = 0;
+  int K = 4;
+  ReachCtl hc_;
+  while (true) {
+    for (int k = 0; k < K; ++k, ++L) {
+      k_reach_judge<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L);
+      k_reach_push<<<grid_push, BLOCK, 0, stream>>>(S, C, 2, L, tau,
+                                                      p.local_reach, total_waves);
+      k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L);
+      k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L);
+    }
+    PSC_HIP(hipGetLastError());
+    PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
+    if (hc_.done[0] && hc_.done[1]) break;
+    K = std::min(K * 2, 64);
+  }
+  st->levels_fwd = hc_.levels[0];
+  st->levels_bwd = hc_.levels[1];
+  for (int s = 0; s < 2; ++s) {
+    st->edges_scanned += hc_.edges[s];
+    // B_csr: 16 B offsets per expanded vertex / pulled vertex + 4 B per edge
+    st->bytes_csr += 16 * (hc_.expanded[s] + hc_.pull_scans[s]) + 4 * hc_.edges[s];
+    // visited bitmap: 1 B-equivalent read per edge + write per new visit
+    st->bytes_label += hc_.edges[s] + hc_.visits[s];
+    st->bytes_front += 4 * hc_.appended[s] + 4 * hc_.expanded[s];
+  }
+  return PSC_OK;
+}
+
+// Allocate / size the multi-search buffers of search s for `slots` slots.
+int Engine::ms_prepare(int s, uint64_t slots, MSSearch *d) {
+  const uint64_t log_cap = std::max<uint64_t>(slots / 2, 64);
+  PSC_TRY(table[s].ensure(slots * 8));
+  PSC_TRY(log[s].ensure(log_cap * 8));
+  PSC_TRY(mq[s][0].ensure(log_cap * 8));
+  PSC_TRY(mq[s][1].ensure(log_cap * 8));
+  PSC_TRY(mhp[s].ensure(log_cap * 8));
+  PSC_TRY(mhc[s].ensure(log_cap * 8));
+  d->off = s == 0 ? d_off : d_ioff;
+  d->adj = s == 0 ? d_adj : d_iadj;
+  d->table = table[s].as<unsigned long long>();
+  d->mask = slots - 1;
+  d->log = log[s].as<unsigned long long>();
+  d->log_cap = log_cap;
+  d->q[0] = mq[s][0].as<unsigned long long>();
+  d->q[1] = mq[s][1].as<unsigned long long>();
+  d->q_cap = log_cap;
+  d->hp = mhp[s].as<unsigned long long>();
+  d->hc = mhc[s].as<unsigned long long>();
+  d->probe_limit = (uint32_t)std::min<uint64_t>(slots, 1u << 20);
+  d->pad = 0;
+  k_fill_u64<<<grid_for(slots, BLOCK, 8192), BLOCK, 0, stream>>>(
+      d->table, slots, EMPTY_SLOT);
+  PSC_HIP(hipGetLastError());
+  return PSC_OK;
+}
+
+static uint64_t pow2_at_least(uint64_t x) {
+  uint64_t m = 1;
+  while (m < x) m <<= 1;
+  return m;
+}
+
+// multi_search_safe for both directions (src/scc.hpp:256-275,395-423):
+// searches run fused; an overflowed search is rerun alone with a doubled
+// table.
+int Engine::multi_search(const psc_params &p, uint64_t nfront,
+                         uint64_t label_offset, uint64_t slots[2],
+                         psc_stats *st) {
+  MSSearch md[2];
+  std::memcpy(md, h_ms, sizeof(md));
+  MSSearch *S = ms_desc.as<MSSearch>();
+  MSCtl *C = ms_ctl.as<MSCtl>();
+  int want[2] = {1, 1};
+  bool first_try = true;
+  const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
+  const uint32_t total_waves = (uint32_t)grid_ms * WPB;
+  while (want[0] || want[1]) {
+    for (int s = 0; s < 2; ++s) {
+      if (!want[s]) continue;
+      while (slots[s] / 2 < nfront) {   // seeds alone exceed the load limit
+        slots[s] *= 2;
+        ++st->resizes;
+      }
+      PSC_TRY(ms_prepare(s, slots[s], &md[s]));
+    }
+    std::memcpy(h_ms, md, sizeof(md));
+    // descriptors and active flags go through pinned staging; the stream
+    // is synchronised (poll) before the staging area is written again
+    char *pin = (char *)h_pinned;
+    std::memcpy(pin + PIN_MS, md, sizeof(md));
+    PSC_HIP(hipMemcpyAsync(S, pin + PIN_MS, sizeof(md), hipMemcpyHostToDevice,
+                           stream));
+    for (int s = 0; s < 2; ++s) {
+      if (!want[s]) continue;
+      k_ms_reset<<<1, 64, 0, stream>>>(C, s);
+      k_ms_seed<<<grid_for(nfront), BLOCK, 0, stream>>>(
+          S, C, s, front.as<uint32_t>(), nfront, label_offset);
+    }
+    uint32_t act[2] = {(uint32_t)want[0], (uint32_t)want[1]};
+    std::memcpy(pin + PIN_FLAGS, act, sizeof(act));
+    PSC_HIP(hipMemcpyAsync(&C->active[0], pin + PIN_FLAGS, sizeof(act),
+                           hipMemcpyHostToDevice, stream));
+    PSC_HIP(hipGetLastError());
+    uint32_t L = 0;
+    int K = 2;
+    MSCtl hc_;
+    while (true) {
+      for (int k = 0; k < K; ++k, ++L) {
+        k_ms_expand<<<grid_ms, BLOCK, 0, stream>>>(
+            S, C, 2, L, labels.as<unsigned long long>(), tau, p.local_scc,
+            total_waves);
+        k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
+            S, C, 2, L, labels.as<unsigned long long>());
+      }
+      PSC_HIP(hipGetLastError());
+      PSC_TRY(poll(&hc_, C, sizeof(MSCtl)));
+      bool live = false;
+      for (int s = 0; s < 2; ++s)
+        if (want[s] && !hc_.full[s] && hc_.slot[s][L % 3].qsize) live = true;
+      if (!live) break;
+      K = std::min(K * 2, 32);
+    }
+    st->sub_rounds += L;
+    for (int s = 0; s < 2; ++s) {
+      if (!want[s]) continue;
+      st->edges_scanned += hc_.edges[s];
+      st->table_inserts += hc_.inserts[s];
+      st->bytes_csr += 16 * hc_.expanded[s] + 4 * hc_.edges[s];
+      st->bytes_label += 8 * hc_.expanded[s] + 8 * hc_.edges[s];
+      st->bytes_table += 8 * hc_.probes[s] + 8 * hc_.inserts[s];
+      st->bytes_front += 16 * hc_.appended[s] + 8 * hc_.inserts[s] + 8 * hc_.expanded[s];
+      if (hc_.full[s]) {
+        slots[s] *= 2;   // double_size (src/resizable_table.h:196-204)
+        ++st->resizes;
+        want[s] = 1;
+      } else {
+        log_n[s] = hc_.log_n[s];
+        want[s] = 0;
+      }
+    }
+    if (!first_try && !(want[0] || want[1])) break;
+    first_try = false;
+  }
+  return PSC_OK;
+}
+
+int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
+  *st = psc_stats{};
+  syncs = 0;
+  unsigned long long *lab = labels.as<unsigned long long>();
+  auto t_host0 = std::chrono::steady_clock::now();
+  if (p.timing) PSC_HIP(hipEventRecord(ev[0], stream));
+  // init + trim (src/scc.hpp:284-306)
+  k_fill_u64<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, 0ull);
+  uint64_t nnz = 0, nz = 0;
+  PSC_TRY(select(SelNonZero{d_off, d_ioff}, n, NZ.as<uint32_t>(),
+                 Z.as<uint32_t>(), &nnz));
+  nz = n - nnz;
+  st->n_trimmed = nz;
+  if (nz)
+    k_trim_labels<<<grid_for(nz, BLOCK, 16384), BLOCK, 0, stream>>>(
+        Z.as<uint32_t>(), nz, lab);
+  if (nnz)
+    k_permute<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
+        NZ.as<uint32_t>(), nnz, PivotPerm(nnz, p.seed), P.as<uint32_t>());
+  PSC_HIP(hipGetLastError());
+  if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
+  uint64_t label_offset = nz + 1;
+  uint64_t n_rem = 0;
+  if (nnz) {
+    PSC_TRY(first_reach(p, st));
+    unsigned long long *big = misc.as<unsigned long long>() + 1;
+    PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
+    k_first_labels<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
+        P.as<uint32_t>(), nnz, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(),
+        lab, label_offset, big);
+    PSC_HIP(hipGetLastError());
+    PSC_TRY(select(SelNotDone{P.as<uint32_t>(), lab}, nnz,
+                   centers.as<uint32_t>(), nullptr, &n_rem));
+    uint64_t bigh = 0;
+    PSC_TRY(poll(&bigh, big, 8));
+    st->first_scc_size = bigh;
+    st->bytes_label += 8 * nnz + 2 * nnz / 8;
+  }
+  if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
+  label_offset++;
+  // multi-pivot rounds (src/scc.hpp:369-473)
+  uint64_t step = 1, cur = 0, finished = 0;
+  uint64_t ne_prev[2] = {1, 1};
+  const double beta = p.beta > 0 ? p.beta : 1.5;
+  while (finished != n_rem) {
+    const uint64_t end = std::min(cur + step, n_rem);
+    const uint64_t vs = end - cur;
+    finished += vs;
+    step = (uint64_t)std::ceil((double)step * beta);
+    st->rounds++;
+    uint64_t nfront = 0;
+    PSC_TRY(select(SelNotDone{centers.as<uint32_t>() + cur, lab}, vs,
+                   front.as<uint32_t>(), nullptr, &nfront));
+    cur = end;
+    if (nfront == 0) continue;
+    uint64_t slots[2];
+    for (int s = 0; s < 2; ++s) {
+      const uint64_t base = std::max<uint64_t>(
+          std::min<uint64_t>((uint64_t)std::ceil(0.3 * (double)n_rem), 6000000),
+          (uint64_t)beta * ne_prev[s]);
+      slots[s] = pow2_at_least(p.table_init ? p.table_init : 2 * base);
+    }
+    PSC_TRY(multi_search(p, nfront, label_offset, slots, st));
+    label_offset += nfront;
+    const int small = (slots[0] <= slots[1]) ? 0 : 1;
+    k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
+        ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
+        misc.as<unsigned long long>() + 2);
+    PSC_HIP(hipGetLastError());
+    st->bytes_label += 16 * (log_n[0] + log_n[1]);
+    st->bytes_front += 8 * (log_n[0] + log_n[1]);
+    st->bytes_table += 8 * log_n[small];
+    ne_prev[0] = log_n[0];
+    ne_prev[1] = log_n[1];
+  }
+  if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
+  k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n);
+  if (p.canonical) {
+    uint32_t *minv = scratch32.as<uint32_t>();
+    k_fill_u32<<<grid_for(n + 1, BLOCK, 16384), BLOCK, 0, stream>>>(
+        minv, n + 1, 0xFFFFFFFFu);
+    k_canon_min<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
+    k_canon_apply<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
+  }
+  st->bytes_label += 16 * n * 2;
+  if (d_out != lab && n)
+    PSC_HIP(hipMemcpyAsync(d_out, lab, n * 8, hipMemcpyDeviceToDevice, stream));
+  PSC_HIP(hipGetLastError());
+  if (p.timing) PSC_HIP(hipEventRecord(ev[4], stream));
+  PSC_HIP(hipStreamSynchronize(stream));
+  ++syncs;
+  if (p.timing) {
+    float a = 0;
+    PSC_HIP(hipEventElapsedTime(&a, ev[0], ev[4]));
+    st->ms_total = a;
+    PSC_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    st->ms_init = a;
+    PSC_HIP(hipEventElapsedTime(&a, ev[1], ev[2]));
+    st->ms_first = a;
+    PSC_HIP(hipEventElapsedTime(&a, ev[2], ev[3]));
+    st->ms_multi = a;
+    PSC_HIP(hipEventElapsedTime(&a, ev[3], ev[4]));
+    st->ms_finalize = a;
+  } else {
+    st->ms_total = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t_host0).count();
+  }
+  st->host_syncs = syncs;
+  return PSC_OK;
+}
+
+int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
+                   uint64_t *largest) {
+  uint32_t *cnt = scratch32.as<uint32_t>();
+  unsigned long long *acc = misc.as<unsigned long long>() + 8;
+  PSC_HIP(hipMemsetAsync(acc, 0, 32, stream));
+  k_fill_u32<<<grid_for(n + 1, BLOCK, 16384), BLOCK, 0, stream>>>(cnt, n + 1, 0u);
+  k_count_labels<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
+      d_labels, n, cnt, acc + 2);
+  k_status_reduce<<<grid_for(n + 1, BLOCK, 4096), BLOCK, 0, stream>>>(
+      cnt, n + 1, acc, (uint32_t *)(acc + 1));
+  PSC_HIP(hipGetLastError());
+  unsigned long long h[3];
+  PSC_TRY(poll(h, acc, sizeof(h)));
+  if (h[2]) {
+    set_error("psc_status_device: labels outside [0, n]");
+    return PSC_ERR_INVALID;
+  }
+  *n_scc = h[0];
+  *largest = (uint32_t)h[1];
+  return PSC_OK;
+}
+
+}  // namespace psc
+
+// ======================================================================
+//                                C ABI
+// ======================================================================
+using psc::Engine;
+
+struct psc_handle {
+  Engine eng;
+};
+
+extern "C" {
+
+void psc_params_default(psc_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->beta = 1.5;           // src/scc.cpp:40
+  p->local_reach = 1;      // the experiments pass -local_reach -local_scc
+  p->local_scc = 1;        // (scripts/Experiment1.py:29)
+  p->tau = 512;            // src/scc.cpp:46
+  p->min_bag = 1024;       // src/scc.cpp:45
+  p->dense_div = 20;       // src/reach.hpp:49
+  p->seed = 1;
+  p->table_init = 0;
+  p->canonical = 0;
+  p->timing = 1;
+}
+
+const char *psc_last_error(void) { return psc::g_last_error.c_str(); }
+
+static int validate_host_graph(const psc_graph *g) {
+  if (!g) return PSC_ERR_INVALID;
+  if (g->n >= 0xFFFFFFFFull) {
+    psc::set_error("n must be < 2^32-1 (UINT_N_MAX is the empty key)");
+    return PSC_ERR_INVALID;
+  }
+  if (g->n && (!g->offset || !g->in_offset)) {
+    psc::set_error("missing offset arrays");
+    return PSC_ERR_INVALID;
+  }
+  if (g->m && (!g->edges || !g->in_edges)) {
+    psc::set_error("missing edge arrays");
+    return PSC_ERR_INVALID;
+  }
+  return PSC_OK;
+}
+
+int psc_create(const psc_graph *g, psc_handle **out) {
+  if (!out) return PSC_ERR_INVALID;
+  *out = nullptr;
+  int r = validate_host_graph(g);
+  if (r) return r;
+  if (g->n) {
+    if (g->offset[0] != 0 || g->offset[g->n] != g->m || g->in_offset[0] != 0 ||
+        g->in_offset[g->n] != g->m) {
+      psc::set_error("offsets must start at 0 and end at m");
+      return PSC_ERR_INVALID;
+    }
+    for (uint64_t i = 0; i < g->m; ++i)
+      if (g->edges[i] >= g->n || g->in_edges[i] >= g->n) {
+        psc::set_error("edge endpoint out of range");
+        return PSC_ERR_INVALID;
+      }
+  }
+  auto *h = new (std::nothrow) psc_handle;
+  if (!h) return PSC_ERR_NOMEM;
+  Engine &e = h->eng;
+  const uint64_t n1 = g->n + 1;
+  r = e.g_off.ensure(n1 * 8);
+  if (!r) r = e.g_ioff.ensure(n1 * 8);
+  if (!r) r = e.g_adj.ensure(std::max<uint64_t>(g->m, 1) * 4);
+  if (!r) r = e.g_iadj.ensure(std::max<uint64_t>(g->m, 1) * 4);
+  if (r) {
+    for (psc::DevBuf *b : {&e.g_off, &e.g_adj, &e.g_ioff, &e.g_iadj}) b->release();
+    delete h;
+    return r;
+  }
+  hipError_t he = hipSuccess;
+  if (g->n) {
+    he = hipMemcpy(e.g_off.ptr, g->offset, n1 * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy(e.g_ioff.ptr, g->in_offset, n1 * 8, hipMemcpyHostToDevice);
+  } else {
+    uint64_t z = 0;
+    he = hipMemcpy(e.g_off.ptr, &z, 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(e.g_ioff.ptr, &z, 8, hipMemcpyHostToDevice);
+  }
+  if (he == hipSuccess && g->m)
+    he = hipMemcpy(e.g_adj.ptr, g->edges, g->m * 4, hipMemcpyHostToDevice);
+  if (he == hipSuccess && g->m)
+    he = hipMemcpy(e.g_iadj.ptr, g->in_edges, g->m * 4, hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    psc::set_error(std::string("hipMemcpy graph: ") + hipGetErrorString(he));
+    delete h;
+    return PSC_ERR_HIP;
+  }
+  psc_graph dg{g->n, g->m, e.g_off.as<uint64_t>(), e.g_adj.as<uint32_t>(),
+               e.g_ioff.as<uint64_t>(), e.g_iadj.as<uint32_t>()};
+  r = e.init(&dg, true);
+  if (r) {
+    delete h;
+    return r;
+  }
+  *out = h;
+  return PSC_OK;
+}
+
+int psc_create_device(const psc_graph *dg, psc_handle **out) {
+  if (!out) return PSC_ERR_INVALID;
+  *out = nullptr;
+  int r = validate_host_graph(dg);
+  if (r) return r;
+  auto *h = new (std::nothrow) psc_handle;
+  if (!h) return PSC_ERR_NOMEM;
+  r = h->eng.init(dg, false);
+  if (r) {
+    delete h;
+    return r;
+  }
+  *out = h;
+  return PSC_OK;
+}
+
+static int run_common(psc_handle *h, const psc_params *p,
+                      unsigned long long *d_out, psc_stats *stats) {
+  psc_params def;
+  psc_params_default(&def);
+  if (!p) p = &def;
+  if (p->beta < 1.0) {
+    psc::set_error("beta must be >= 1");
+    return PSC_ERR_INVALID;
+  }
+  psc_stats tmp;
+  return h->eng.run(*p, d_out, stats ? stats : &tmp);
+}
+
+int psc_run(psc_handle *h, const psc_params *p, uint64_t *labels_out,
+            psc_stats *stats) {
+  if (!h) return PSC_ERR_INVALID;
+  Engine &e = h->eng;
+  int r = run_common(h, p, e.labels.as<unsigned long long>(), stats);
+  if (r) return r;
+  if (labels_out && e.n) {
+    hipError_t he = hipMemcpy(labels_out, e.labels.ptr, e.n * 8,
+                              hipMemcpyDeviceToHost);
+    if (he != hipSuccess) {
+      psc::set_error(std::string("hipMemcpy labels: ") + hipGetErrorString(he));
+      return PSC_ERR_HIP;
+    }
+  }
+  return PSC_OK;
+}
+
+int psc_run_device(psc_handle *h, const psc_params *p, uint64_t *d_labels_out,
+                   psc_stats *stats) {
+  if (!h || (!d_labels_out && h->eng.n)) return PSC_ERR_INVALID;
+  return run_common(h, p, (unsigned long long *)d_labels_out, stats);
+}
+
+int psc_status_device(psc_handle *h, const uint64_t *d_labels, uint64_t *n_scc,
+                      uint64_t *largest) {
+  if (!h || !n_scc || !largest) return PSC_ERR_INVALID;
+  if (h->eng.n == 0) {
+    *n_scc = *largest = 0;
+    return PSC_OK;
+  }
+  return h->eng.status((const unsigned long long *)d_labels, n_scc, largest);
+}
+
+void psc_destroy(psc_handle *h) { delete h; }
+
+void *psc_handle_stream(psc_handle *h) { return h ? (void *)h->eng.stream : nullptr; }
+
+}  // extern "C"

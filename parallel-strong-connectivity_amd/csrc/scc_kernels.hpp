@@ -1,0 +1,94 @@
+// Device-side state layouts and kernel declarations of the single-graph SCC
+// path.  One ReachSearch / MSSearch describes one direction of a search; the
+// forward and backward searches of src/scc.hpp run fused in the same
+// launches (search index 0 = out-CSR, 1 = in-CSR).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace psc {
+
+constexpr int REACH_RING = 1024;     // per-wave VGC queue (vertices, LDS)
+constexpr int MS_RING = 512;         // per-wave VGC queue (pairs, LDS)
+constexpr uint32_t HEAVY_DEG = 2048; // degree above which edges are chunked
+constexpr uint32_t CHUNK = 2048;     // edges per heavy work item
+constexpr unsigned long long H_CNT_SHIFT = 40;
+constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
+
+enum : int { MODE_NONE = 0, MODE_PUSH = 1, MODE_PULL = 2 };
+
+// ---------------------------------------------------------------- reach
+struct ReachSearch {
+  const uint64_t *off;   // push direction CSR
+  const uint32_t *adj;
+  const uint64_t *roff;  // pull direction CSR (transpose)
+  const uint32_t *radj;
+  uint32_t *vis;         // visited bitmap, ceil(n/32) words
+  uint32_t *q[2];        // frontier ping-pong, n entries each
+  uint32_t *hv;          // heavy vertices of the current level
+  unsigned long long *hc;  // exclusive chunk start of each heavy vertex
+  uint64_t thresh;       // m / dense_div (src/reach.hpp:49)
+  uint32_t n;
+  uint32_t pad;
+};
+
+struct ReachSlot {
+  uint32_t qsize;            // frontier size of this level
+  uint32_t grab;             // seeds handed out
+  unsigned long long degsum; // out-degree sum of the frontier (judge)
+  unsigned long long heavy;  // (count << 40) | chunks
+  unsigned long long hgrab;  // heavy chunks handed out
+};
+
+struct ReachCtl {
+  ReachSlot slot[2][2];      // [search][level parity]
+  uint32_t done[2];
+  uint32_t levels[2];
+  unsigned long long edges[2];      // edge ids read (push + pull)
+  unsigned long long expanded[2];   // vertices whose offsets were read
+  unsigned long long visits[2];     // newly visited vertices
+  unsigned long long pull_scans[2]; // vertices tested by pull steps
+  unsigned long long appended[2];   // frontier appends
+};
+
+__device__ __forceinline__ int reach_mode(uint32_t nf, unsigned long long ds,
+                                          uint64_t thresh) {
+  if (nf == 0) return MODE_NONE;
+  return ((uint64_t)nf + ds < thresh) ? MODE_PUSH : MODE_PULL;
+}
+
+// ------------------------------------------------------------ multi-search
+struct MSSearch {
+  const uint64_t *off;
+  const uint32_t *adj;
+  unsigned long long *table;    // (vertex << 32 | label) slots
+  uint64_t mask;                // logical size - 1
+  unsigned long long *log;      // every inserted pair, in insert order
+  unsigned long long log_cap;
+  unsigned long long *q[2];     // pair frontiers
+  unsigned long long q_cap;
+  unsigned long long *hp;       // heavy pairs of the current level
+  unsigned long long *hc;       // their exclusive chunk starts
+  uint32_t probe_limit;
+  uint32_t pad;
+};
+
+struct MSSlot {
+  unsigned long long qsize, grab, heavy, hgrab;
+};
+
+struct MSCtl {
+  MSSlot slot[2][3];           // [search][level % 3]
+  unsigned long long log_n[2];
+  uint32_t full[2];
+  uint32_t active[2];
+  unsigned long long edges[2];
+  unsigned long long expanded[2];
+  unsigned long long inserts[2];
+  unsigned long long probes[2];
+  unsigned long long appended[2];
+};
+
+}  // namespace psc

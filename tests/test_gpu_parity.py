@@ -1,0 +1,162 @@
+"""GPU parity: the HIP path against the oracle.
+
+Every check is bit-exact:
+  * canonical labels (min vertex id per SCC) == iterative Tarjan, which the
+    committed fixtures also pin against networkx and scipy;
+  * raw labels == the CPU restatement of the reference algorithm
+    (oracle/scc_ref.cpp) run with the same seed: all compactions of both
+    are stable, so the raw label values are a function of (graph, seed).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz")
+
+PARAM_SETS = [
+    {},
+    {"local_reach": False, "local_scc": False},
+    {"tau": 1},
+    {"tau": 64, "local_reach": False},
+    {"tau": 1 << 17},
+    {"beta": 2.0},
+    {"beta": 1.0},
+    {"table_init": 16},
+    {"dense_div": 1},
+    {"dense_div": 1 << 40},
+    {"seed": 7},
+]
+
+
+@pytest.fixture(scope="module")
+def fixtures(psc):
+    d = np.load(FIX)
+    out = []
+    for name in d["names"]:
+        name = str(name)
+        n = int(d[f"{name}__n"][0])
+        g = psc.Graph.from_out_csr(n, d[f"{name}__offset"], d[f"{name}__edges"])
+        out.append((name, g, d[f"{name}__canon"]))
+    return out
+
+
+def check(psc, oracle, g, expect=None, **kw):
+    labels = psc.scc(g, **kw)
+    canon = oracle.canonical(labels)
+    if expect is None:
+        expect = oracle.tarjan_canonical(g)
+    assert np.array_equal(canon, expect)
+    okw = {k: v for k, v in kw.items() if k in
+           ("beta", "local_reach", "local_scc", "tau", "dense_div", "seed",
+            "table_init")}
+    ref, _ = oracle.scc_ref(g, **okw)
+    assert np.array_equal(labels, ref), "raw labels differ from the oracle restatement"
+    if g.n:
+        assert labels.max() <= g.n
+    return labels
+
+
+def test_fixtures_default(psc, oracle, fixtures):
+    for name, g, canon in fixtures:
+        print("fixture", name, flush=True)
+        check(psc, oracle, g, canon)
+
+
+@pytest.mark.parametrize("kw", PARAM_SETS, ids=lambda k: ",".join(
+    f"{a}={b}" for a, b in k.items()) or "default")
+def test_fixtures_params(psc, oracle, fixtures, kw):
+    for name, g, canon in fixtures:
+        check(psc, oracle, g, canon, **kw)
+
+
+def test_canonical_flag(psc, oracle, fixtures):
+    for name, g, canon in fixtures:
+        labels = psc.scc(g, canonical=1)
+        assert np.array_equal(labels.astype(np.uint32), canon), name
+
+
+def test_repeat_runs_identical(psc, fixtures):
+    name, g, _ = [f for f in fixtures if f[0] == "rmat_12"][0]
+    s = psc.SCC(g)
+    a = s.scc().copy()
+    b = s.scc().copy()
+    c = s.scc(tau=3).copy()
+    s.close()
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_overflow_resize_path(psc, oracle, fixtures):
+    name, g, canon = [f for f in fixtures if f[0] == "random_2000_2600"][0]
+    s = psc.SCC(g)
+    labels = s.scc(table_init=16)
+    assert s.stats["resizes"] > 0
+    s.close()
+    assert np.array_equal(oracle.canonical(labels), canon)
+
+
+def test_c1_rmat16(psc, oracle):
+    """RMAT n=2^16, 2^20 samples (a,b,c,d)=(.57,.19,.19,.05), seed 1."""
+    g = psc.gen_rmat(16, 1 << 20, 0.57, 0.19, 0.19, seed=1)
+    print("C1 m =", g.m, flush=True)
+    labels = check(psc, oracle, g)
+    print("C1 n_scc, largest =", psc.status(labels), flush=True)
+
+
+@pytest.mark.parametrize("side", [64, 512])
+def test_grid_parity(psc, oracle, side):
+    g = psc.gen_grid(side, side, 0.6, seed=2)
+    check(psc, oracle, g)
+
+
+@pytest.mark.parametrize("tau", [1 << k for k in range(0, 18, 3)])
+def test_grid_tau_sweep(psc, oracle, tau):
+    g = psc.gen_grid(256, 256, 0.6, seed=2)
+    check(psc, oracle, g, tau=tau)
+
+
+def test_c2_grid_2048_full(psc, oracle):
+    """2048x2048 directed grid, p_both 0.6, seed 2: labels vs Tarjan."""
+    g = psc.gen_grid(2048, 2048, 0.6, seed=2)
+    labels = psc.scc(g)
+    assert np.array_equal(oracle.canonical(labels), oracle.tarjan_canonical(g))
+
+
+def test_knn_parity(psc, oracle):
+    g = psc.gen_knn3d(20000, 5, seed=4)
+    check(psc, oracle, g)
+
+
+def test_device_generators_match_host(psc):
+    for sc in (8, 12):
+        d = psc.DeviceGraph.rmat(sc, 8 << sc, seed=5)
+        h = psc.gen_rmat(sc, 8 << sc, seed=5)
+        hd = d.to_host()
+        d.free()
+        for f in ("offset", "edges", "in_offset", "in_edges"):
+            assert np.array_equal(getattr(hd, f), getattr(h, f)), (sc, f)
+    d = psc.DeviceGraph.grid(37, 53, 0.6, seed=3)
+    h = psc.gen_grid(37, 53, 0.6, seed=3)
+    hd = d.to_host()
+    d.free()
+    for f in ("offset", "edges", "in_offset", "in_edges"):
+        assert np.array_equal(getattr(hd, f), getattr(h, f)), f
+
+
+def test_device_graph_run_and_status(psc, oracle):
+    import torch
+    d = psc.DeviceGraph.rmat(14, 16 << 14, seed=8)
+    h = d.to_host()
+    s = psc.SCC(d)
+    out = torch.empty(d.n, dtype=torch.int64, device="cuda")
+    s.scc_device(out.data_ptr(), canonical=1)
+    torch.cuda.synchronize()
+    nscc, big = s.status_device(out.data_ptr())
+    labels = out.cpu().numpy().astype(np.uint64)
+    s.close()
+    d.free()
+    expect = oracle.tarjan_canonical(h)
+    assert np.array_equal(labels.astype(np.uint32), expect)
+    assert (nscc, big) == oracle.status(expect)

@@ -1,0 +1,242 @@
+"""Pins the oracle before it is trusted as the checker.
+
+* Tarjan (oracle/tarjan.c) against the committed fixtures, whose labels were
+  accepted only where networkx and scipy agreed, and live against networkx
+  on random digraphs and brute-force transitive closure on tiny ones;
+* closed-form families (DAGs, cycles, grids, stars);
+* the CPU restatement of the reference algorithm (oracle/scc_ref.cpp)
+  against Tarjan over every fixture and parameter corner, including the
+  forced table-overflow path;
+* the per-function pieces: label table vs. a Python set, hash bag vs. the
+  set of inserted ids, single reach vs. BFS closure;
+* planted mistakes: corrupted labellings must be rejected.
+"""
+import os
+
+import networkx as nx
+import numpy as np
+import pytest
+from hypothesis import given, settings, strategies as st
+
+from conftest import random_digraph
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz")
+
+
+@pytest.fixture(scope="module")
+def fixtures(psc):
+    d = np.load(FIX)
+    out = []
+    for name in d["names"]:
+        name = str(name)
+        n = int(d[f"{name}__n"][0])
+        g = psc.Graph.from_out_csr(n, d[f"{name}__offset"], d[f"{name}__edges"])
+        out.append((name, g, d[f"{name}__canon"]))
+    return out
+
+
+def nx_canon(g):
+    G = nx.DiGraph()
+    G.add_nodes_from(range(g.n))
+    src = np.repeat(np.arange(g.n), np.diff(g.offset).astype(np.int64))
+    G.add_edges_from(zip(src.tolist(), g.edges.tolist()))
+    out = np.zeros(g.n, dtype=np.uint32)
+    for comp in nx.strongly_connected_components(G):
+        c = sorted(comp)
+        out[c] = c[0]
+    return out
+
+
+def closure_canon(g):
+    """SCC = mutual reachability, by boolean transitive closure (n <= 64)."""
+    n = g.n
+    R = np.eye(n, dtype=bool)
+    for u in range(n):
+        R[u, g.edges[g.offset[u]:g.offset[u + 1]]] = True
+    for k in range(n):
+        R |= R[:, [k]] & R[[k], :]
+    M = R & R.T
+    return np.array([np.flatnonzero(M[v])[0] for v in range(n)], dtype=np.uint32)
+
+
+def test_fixture_count(fixtures):
+    assert len(fixtures) >= 30
+
+
+def test_tarjan_matches_fixtures(oracle, fixtures):
+    for name, g, canon in fixtures:
+        assert np.array_equal(oracle.tarjan_canonical(g), canon), name
+
+
+def test_tarjan_completion_order(oracle, psc):
+    # labels are the completion order 1..cnt (tarjan_scc.hpp:53-57):
+    # a path 0->1->2 completes 2 first
+    g = psc.Graph.from_edges(3, [0, 1], [1, 2])
+    comp, cnt = oracle.tarjan(g)
+    assert cnt == 3 and comp.tolist() == [3, 2, 1]
+
+
+@settings(max_examples=60, deadline=None)
+@given(n=st.integers(1, 60), density=st.floats(0.0, 3.0), seed=st.integers(0, 10**6))
+def test_tarjan_vs_networkx_random(psc, oracle, n, density, seed):
+    g = random_digraph(psc, n, int(n * density) + 1, seed, self_loops=True, dups=True)
+    assert np.array_equal(oracle.tarjan_canonical(g), nx_canon(g))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_tarjan_vs_closure(psc, oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 48))
+    g = random_digraph(psc, n, int(n * rng.uniform(0.5, 2.5)), seed)
+    assert np.array_equal(oracle.tarjan_canonical(g), closure_canon(g))
+
+
+def test_closed_forms(psc, oracle):
+    n = 500
+    # directed cycle: one SCC
+    g = psc.Graph.from_edges(n, np.arange(n), (np.arange(n) + 1) % n)
+    assert np.all(oracle.tarjan_canonical(g) == 0)
+    # path and random DAG: n singletons
+    g = psc.Graph.from_edges(n, np.arange(n - 1), np.arange(1, n))
+    assert np.array_equal(oracle.tarjan_canonical(g), np.arange(n))
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, n, 3000)
+    b = rng.integers(0, n, 3000)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    keep = lo != hi
+    perm = rng.permutation(n)
+    g = psc.Graph.from_edges(n, perm[lo[keep]], perm[hi[keep]])
+    assert np.array_equal(oracle.tarjan_canonical(g), np.arange(n))
+    # grids: all one-way -> singletons; both directions -> one SCC
+    r, c = np.divmod(np.arange(900), 30)
+    right = c + 1 < 30
+    down = r + 1 < 30
+    src = np.concatenate([np.arange(900)[right], np.arange(900)[down]])
+    dst = np.concatenate([np.arange(900)[right] + 1, np.arange(900)[down] + 30])
+    g = psc.Graph.from_edges(900, src, dst)
+    assert np.array_equal(oracle.tarjan_canonical(g), np.arange(900))
+    assert np.all(oracle.tarjan_canonical(psc.gen_grid(30, 30, 1.0, 1)) == 0)
+    # deep grid: recursion depth ~n would overflow a recursive Tarjan
+    g = psc.gen_grid(300, 300, 1.0, 1)
+    assert np.all(oracle.tarjan_canonical(g) == 0)
+
+
+def test_canonical_and_status(oracle):
+    lab = np.array([7, 3, 7, 9, 3, 3], dtype=np.uint64)
+    assert oracle.canonical(lab).tolist() == [0, 1, 0, 3, 1, 1]
+    assert oracle.status(oracle.canonical(lab)) == (3, 3)
+
+
+def test_planted_mistakes_are_caught(psc, oracle, fixtures):
+    """A labelling that merges two SCCs or splits one must not pass."""
+    name, g, canon = [f for f in fixtures if f[0] == "rmat_12"][0]
+    lab, _ = oracle.scc_ref(g)
+    good = oracle.canonical(lab)
+    assert np.array_equal(good, canon)
+    bad = lab.copy()
+    a, b = np.unique(lab)[:2]
+    bad[lab == b] = a                       # merge two SCCs
+    assert not np.array_equal(oracle.canonical(bad), canon)
+    big = np.bincount(canon).argmax()
+    bad = lab.copy()
+    v = np.flatnonzero(canon == big)[-1]
+    bad[v] = g.n + 5                        # split the giant SCC
+    assert not np.array_equal(oracle.canonical(bad), canon)
+
+
+REF_PARAMS = [
+    {},
+    {"local_reach": False, "local_scc": False},
+    {"tau": 1},
+    {"tau": 1 << 17},
+    {"beta": 1.0},
+    {"beta": 3.0},
+    {"table_init": 16},
+    {"min_bag": 16},
+    {"dense_div": 1},
+    {"seed": 99},
+]
+
+
+@pytest.mark.parametrize("kw", REF_PARAMS, ids=lambda k: ",".join(
+    f"{a}={b}" for a, b in k.items()) or "default")
+def test_restatement_matches_tarjan(oracle, fixtures, kw):
+    for name, g, canon in fixtures:
+        lab, st_ = oracle.scc_ref(g, **kw)
+        assert np.array_equal(oracle.canonical(lab), canon), (name, kw)
+        if g.n:
+            assert lab.max() <= g.n
+
+
+def test_restatement_resize_path_runs(oracle, fixtures):
+    name, g, canon = [f for f in fixtures if f[0] == "random_2000_2600"][0]
+    lab, st_ = oracle.scc_ref(g, table_init=16)
+    assert st_["resizes"] > 0
+    assert np.array_equal(oracle.canonical(lab), canon)
+
+
+def test_restatement_deterministic(oracle, fixtures):
+    name, g, canon = [f for f in fixtures if f[0] == "grid_40x40_p03"][0]
+    a, _ = oracle.scc_ref(g, threads=1)
+    b, _ = oracle.scc_ref(g, threads=4)
+    assert np.array_equal(a, b)
+
+
+def test_restatement_c1(psc, oracle):
+    g = psc.gen_rmat(16, 1 << 20, seed=1)
+    lab, st_ = oracle.scc_ref(g)
+    assert np.array_equal(oracle.canonical(lab), oracle.tarjan_canonical(g))
+    assert st_["first_scc_size"] == oracle.status(oracle.tarjan_canonical(g))[1]
+
+
+def test_table_semantics(oracle):
+    rng = np.random.default_rng(0)
+    keys = rng.integers(0, 300, 5000).astype(np.uint32)
+    vals = rng.integers(0, 40, 5000).astype(np.uint32)
+    t = oracle.Table(1 << 14)
+    fresh, full = t.insert(keys, vals)
+    assert not full
+    pairs = set(zip(keys.tolist(), vals.tolist()))
+    assert int(fresh.sum()) == len(pairs) == t.size()
+    for k in range(300):
+        want = sorted(v for (kk, v) in pairs if kk == k)
+        assert sorted(t.values(k).tolist()) == want
+    for k, v in list(pairs)[:200]:
+        assert t.contains(k, v)
+    assert not t.contains(1000, 1)
+    # re-inserting reports nothing new
+    fresh, _ = t.insert(keys, vals)
+    assert not fresh.any()
+    # a too-small table reports full
+    small = oracle.Table(64)
+    _, full = small.insert(np.arange(200, dtype=np.uint32), np.zeros(200, np.uint32))
+    assert full
+
+
+@pytest.mark.parametrize("stamped", [False, True])
+def test_hash_bag_is_a_set(oracle, stamped):
+    rng = np.random.default_rng(1)
+    n = 50000
+    items = rng.permutation(n)[:30000].astype(np.uint32)
+    out = oracle.bag_roundtrip(n, items, min_bag=64, stamped=stamped)
+    assert sorted(out.tolist()) == sorted(items.tolist())
+
+
+@pytest.mark.parametrize("local,tau", [(False, 512), (True, 1), (True, 8), (True, 512)])
+def test_reach_is_bfs_closure(psc, oracle, local, tau):
+    g = psc.gen_rmat(11, 6 << 11, seed=4)
+    G = nx.DiGraph()
+    src = np.repeat(np.arange(g.n), np.diff(g.offset).astype(np.int64))
+    G.add_edges_from(zip(src.tolist(), g.edges.tolist()))
+    s = int(src[0])
+    for direction, gg in ((0, G), (1, G.reverse())):
+        got, _ = oracle.reach(g, s, direction, local, tau)
+        want = np.zeros(g.n, dtype=bool)
+        want[list(nx.descendants(gg, s)) + [s]] = True
+        assert np.array_equal(got, want)
+
+
+def test_pivot_perm_is_bijection(oracle):
+    for N in (1, 2, 3, 7, 64, 1000, 4097):
+        vals = [oracle.pivot_perm(N, 5, i) for i in range(N)]
+        assert sorted(vals) == list(range(N))
