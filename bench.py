@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""SCC throughput on one MI355X: RMAT n=2^25, 2^29 edge samples.
+
+A step is one full SCC pass (trim, first-SCC reaches, multi-search rounds,
+partition, finalize) over the device-resident graph, i.e. the window the
+reference times around SCC::scc (src/scc.cpp:62-67).  The graph is
+generated on the device (seed 3, (a,b,c,d) = (.57,.19,.19,.05), ids
+scrambled, self-loops and duplicates removed) and is resident in HBM before
+the timed region.  Defaults follow the reference experiments: beta 1.5,
+tau 512, both VGC flags (scripts/Experiment1.py:29).
+
+Prints one JSON line: Gedges/s (m / t_scc), the roofline of the dominant
+kernel (algorithmic bytes per launch / average launch time, HIP events on
+the handle's stream), and the CPU baseline: the OpenMP restatement of the
+reference algorithm (oracle/scc_ref.cpp) on this host's cores, on the same
+graph, whose raw labels must equal the GPU's bit for bit.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_pmc(kernel, workload):
+    """HBM traffic per launch from committed rocprofv3 --pmc summaries."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return k.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=25)
+    ap.add_argument("--samples-log2", type=int, default=29)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--tau", type=int, default=512)
+    ap.add_argument("--no-vgc", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="warm-up + steps only (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (the multi-GPU sharding is a design, not a build)",
+              file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    psc = ge.load_package()
+
+    scale, samples = args.scale, 1 << args.samples_log2
+    workload = f"rmat{scale}_s{args.samples_log2}"
+    t = time.perf_counter()
+    dg = psc.DeviceGraph.rmat(scale, samples, 0.57, 0.19, 0.19, seed=args.seed,
+                              permute=True)
+    gen_s = time.perf_counter() - t
+    log(f"generated {workload}: n={dg.n} m={dg.m} in {gen_s:.2f}s")
+    s = psc.SCC(dg)
+    prm = dict(beta=1.5, local_reach=not args.no_vgc, local_scc=not args.no_vgc,
+               tau=args.tau, timing=1)
+    for _ in range(args.warmup):
+        s.run(**prm)
+    times = []
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        st = s.run(**prm)          # returns after the stream is synchronised
+        times.append(time.perf_counter() - a)
+        dev_ms.append(st["ms_total"])
+    t1 = time.perf_counter()
+    total = t1 - t0
+    if args.profile_only:
+        log(f"profile-only: {args.steps} steps, {total / args.steps * 1e3:.3f} ms/step")
+        return
+    ms_step = total / args.steps * 1e3
+    gedges = dg.m * args.steps / total / 1e9
+    log(f"{args.steps} steps: {ms_step:.3f} ms/step (min {min(times)*1e3:.3f}), "
+        f"device {sum(dev_ms)/len(dev_ms):.3f} ms, {gedges:.3f} Gedges/s")
+    log("stats:", json.dumps(st))
+
+    # per-kernel profile: one extra pass with an event pair per launch
+    s.run(**dict(prm, timing=2))
+    prof_st = s.stats
+    kp = [k for k in s.kernel_profile() if k["launches"]]
+    for k in kp:
+        log(f"  {k['name']:22s} launches {k['launches']:6d}  {k['total_ms']:9.3f} ms  "
+            f"{k['bytes'] / 1e9:8.3f} GB  "
+            f"{k['bytes'] / max(k['total_ms'], 1e-9) / 1e6:8.1f} GB/s")
+    dom = max(kp, key=lambda k: k["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bpl = dom["bytes"] / dom["launches"]
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    traffic = load_pmc(dom["name"], workload)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": traffic, "kernel": dom["name"],
+                "launches": dom["launches"], "avg_launch_us": round(avg_ms * 1e3, 3),
+                "bytes_per_launch": round(bpl)}
+    path_bytes = sum(prof_st[k] for k in ("bytes_csr", "bytes_label",
+                                          "bytes_table", "bytes_front"))
+    mean_step_s = total / args.steps
+    path = {"algorithmic_bytes": path_bytes,
+            "achieved_GBs": round(path_bytes / mean_step_s / 1e9, 2),
+            "frac": round(path_bytes / mean_step_s / 1e9 / PEAK_HBM_GBS, 4),
+            "floor_bytes": 2 * (8 * (dg.n + 1) + 4 * dg.m) + 16 * dg.n}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import numpy as np
+        import oracle
+        labels = s.scc(**prm)
+        t = time.perf_counter()
+        g = dg.to_host()
+        log(f"graph downloaded in {time.perf_counter() - t:.1f}s")
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) \
+            or len(os.sched_getaffinity(0))
+        ref = oracle.RefSCC(g, beta=1.5, local_reach=not args.no_vgc,
+                            local_scc=not args.no_vgc, tau=args.tau, threads=threads)
+        t = time.perf_counter()
+        rl, rst = ref.run()
+        cpu_s = time.perf_counter() - t
+        ref.close()
+        parity = bool(np.array_equal(rl, labels))
+        log(f"cpu restatement: {cpu_s:.2f}s on {threads} threads; raw labels "
+            f"{'identical' if parity else 'DIFFER'}")
+        cpu = {"value": round(dg.m / cpu_s / 1e9, 4), "unit": "Gedges/s",
+               "cores": threads, "kind": "port",
+               "sample": f"one scc() pass of the OpenMP restatement of the "
+                         f"reference (oracle/scc_ref.cpp) on the full {workload} "
+                         f"graph, {cpu_s:.2f} s",
+               "seconds": round(cpu_s, 3)}
+        if not parity:
+            log("ERROR: GPU labels differ from the oracle restatement")
+    out = {
+        "metric": "SCC Gedges/s + %HBM roofline on RMAT 2^25",
+        "value": round(gedges, 4),
+        "unit": "Gedges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64",
+        "data": "synthetic",
+        "config": {"workload": f"RMAT n=2^{scale}, 2^{args.samples_log2} samples, "
+                               f"(a,b,c,d)=(.57,.19,.19,.05), seed {args.seed}, "
+                               "dedup, ids scrambled",
+                   "n": dg.n, "m": dg.m, "beta": 1.5, "tau": args.tau,
+                   "vgc": not args.no_vgc},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "path_roofline": path,
+        "parity_vs_oracle": parity,
+        "phases_ms": {k: round(prof_st[k], 3) for k in
+                      ("ms_init", "ms_first", "ms_multi", "ms_resize", "ms_finalize")},
+        "gen_seconds": round(gen_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+    dg.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -124,7 +124,27 @@ int psc_run_device(psc_handle *h, const psc_params *p, uint64_t *d_labels_out,
 int psc_status_device(psc_handle *h, const uint64_t *d_labels,
                       uint64_t *n_scc, uint64_t *largest);
 
+/* Per-kernel breakdown of the last psc_run made with timing >= 2: one HIP
+ * event pair around every launch, summed per kernel, plus the algorithmic
+ * bytes each kernel moved (CSR, labels, table, frontier; counted on the
+ * device).  Returns the number of entries written. */
+typedef struct psc_kernel_prof {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+  double bytes;
+} psc_kernel_prof;
+
+int psc_kernel_profile(psc_handle *h, psc_kernel_prof *out, int cap);
+
 void psc_destroy(psc_handle *h);
+
+/* Device-memory plumbing for callers without a HIP runtime of their own
+ * (no reference counterpart): allocate / free device memory of the current
+ * device and copy in either direction (hipMemcpyDefault). */
+int psc_device_alloc(size_t bytes, void **ptr);
+int psc_device_free(void *ptr);
+int psc_device_copy(void *dst, const void *src, size_t bytes);
 
 /* Message of the last error on this thread ("" if none). */
 const char *psc_last_error(void);

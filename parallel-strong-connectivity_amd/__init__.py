@@ -68,6 +68,11 @@ class CStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CKernelProf(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64),
+                ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
 class CHostGraph(C.Structure):
     _fields_ = [("g", CGraph), ("offset", u64p), ("edges", u32p),
                 ("in_offset", u64p), ("in_edges", u32p)]
@@ -93,7 +98,11 @@ _sig("psc_run_device", C.c_int, C.c_void_p, C.POINTER(CParams), C.c_void_p,
      C.POINTER(CStats))
 _sig("psc_status_device", C.c_int, C.c_void_p, C.c_void_p, u64p, u64p)
 _sig("psc_destroy", None, C.c_void_p)
+_sig("psc_kernel_profile", C.c_int, C.c_void_p, C.POINTER(CKernelProf), C.c_int)
 _sig("psc_handle_stream", C.c_void_p, C.c_void_p)
+_sig("psc_device_alloc", C.c_int, C.c_size_t, C.POINTER(C.c_void_p))
+_sig("psc_device_free", C.c_int, C.c_void_p)
+_sig("psc_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("psc_host_graph_free", None, C.POINTER(CHostGraph))
 _sig("psc_read_bin", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.POINTER(CHostGraph)))
 _sig("psc_write_bin", C.c_int, C.c_char_p, C.POINTER(CGraph), C.c_int)
@@ -247,6 +256,35 @@ class DeviceGraph:
             pass
 
 
+class DeviceArray:
+    """A uint64 buffer in device memory (plumbing for psc_run_device)."""
+
+    def __init__(self, count):
+        self.count = int(count)
+        p = C.c_void_p()
+        _check("psc_device_alloc", lib.psc_device_alloc(max(self.count, 1) * 8,
+                                                        C.byref(p)))
+        self.ptr = p.value
+
+    def numpy(self):
+        out = np.empty(self.count, dtype=np.uint64)
+        if self.count:
+            _check("psc_device_copy", lib.psc_device_copy(
+                out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.count * 8))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.psc_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def default_params(**kw):
     p = CParams()
     lib.psc_params_default(C.byref(p))
@@ -295,6 +333,26 @@ class SCC:
                                                     C.c_void_p(d_labels), C.byref(st)))
         self.stats = st.as_dict()
         return self.stats
+
+    def run(self, beta=1.5, local_reach=True, local_scc=True, **kw):
+        """One SCC pass with labels left in the handle's device buffer."""
+        p = default_params(beta=beta, local_reach=int(local_reach),
+                           local_scc=int(local_scc), **kw)
+        st = CStats()
+        _check("psc_run", lib.psc_run(self._h, C.byref(p), None, C.byref(st)))
+        self.stats = st.as_dict()
+        return self.stats
+
+    def kernel_profile(self):
+        """Per-kernel launches / time / algorithmic bytes of the last run
+        made with timing=2."""
+        buf = (CKernelProf * 16)()
+        k = lib.psc_kernel_profile(self._h, buf, 16)
+        if k < 0:
+            _check("psc_kernel_profile", k)
+        return [{"name": buf[i].name.decode(), "launches": buf[i].launches,
+                 "total_ms": buf[i].total_ms, "bytes": buf[i].bytes}
+                for i in range(k)]
 
     def status_device(self, d_labels):
         a, b = C.c_uint64(), C.c_uint64()

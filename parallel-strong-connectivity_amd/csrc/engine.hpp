@@ -31,6 +31,19 @@ struct DevBuf {
 // multi-search descriptors, flags.
 constexpr size_t PIN_POLL = 0, PIN_REACH = 1024, PIN_MS = 2048, PIN_FLAGS = 3072;
 
+// kernel kinds timed by the per-launch profiler (timing >= 2)
+enum : int {
+  KK_STEP, KK_HEAVY, KK_MS_EXPAND, KK_MS_HEAVY, KK_PARTITION, KK_SELECT,
+  KK_SEED, KK_MISC, KK_N
+};
+extern const char *const KK_NAMES[KK_N];
+
+struct KProf {
+  uint64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;
+};
+
 struct Engine {
   uint64_t n = 0, m = 0;
   int device = 0, num_cu = 1;
@@ -45,7 +58,13 @@ struct Engine {
   DevBuf NZ, Z, P, centers, front, tiles, tile_off;
   DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
   DevBuf table[2], log[2], mq[2][2], mhp[2], mhc[2];
-  int grid_push = 1, grid_ms = 1, grid_pull = 1, grid_heavy = 1;
+  int grid_step = 1, grid_ms = 1, grid_heavy = 1;
+  int prof_level = 0;
+  std::vector<hipEvent_t> pev;
+  size_t pev_used = 0;
+  std::vector<int> pkind;
+  KProf kprof[KK_N];
+  double kbytes[KK_N] = {};
   uint64_t log_n[2] = {0, 0};
   uint64_t syncs = 0;
   unsigned char h_reach[256];
@@ -61,6 +80,10 @@ struct Engine {
   template <class F>
   int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count);
   int poll(void *dst, const void *src, size_t bytes);
+  int prof_reset(int level);
+  void prof_begin(int kind);
+  void prof_end();
+  int prof_collect();
   int first_reach(const psc_params &p, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
   int multi_search(const psc_params &p, uint64_t nfront, uint64_t label_offset,

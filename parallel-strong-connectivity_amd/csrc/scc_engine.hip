@@ -180,56 +180,46 @@ __global__ void k_reach_init(ReachSearch *S, ReachCtl *C, int ns,
       S[s].q[0][0] = src;
       ReachSlot z{};
       C->slot[s][1] = z;
+      C->slot[s][2] = z;
       z.qsize = 1;
+      z.degsum = S[s].off[src + 1] - S[s].off[src];
       C->slot[s][0] = z;
       C->done[s] = 0;
       C->levels[s] = 0;
-      C->edges[s] = C->expanded[s] = C->visits[s] = 0;
-      C->pull_scans[s] = C->appended[s] = 0;
+      for (int c = 0; c < NRC; ++c) C->cnt[s][c] = 0;
     }
   }
 }
 
-// judge() (src/reach.hpp:86-112): sum the frontier's out-degrees when the
-// frontier may be sparse; the consumers derive the mode from (nf, degsum).
-// Also clears the counters of the next level's slot.
-__global__ __launch_bounds__(BLOCK) void k_reach_judge(const ReachSearch *S,
-                                                       ReachCtl *C, int ns,
-                                                       uint32_t L) {
-  const int p = L & 1;
-  for (int s = 0; s < ns; ++s) {
-    ReachSlot *cur = &C->slot[s][p];
-    const uint32_t nf = cur->qsize;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      ReachSlot z{};
-      C->slot[s][p ^ 1] = z;
-      if (nf == 0 && !C->done[s]) {
-        C->done[s] = 1;
-        C->levels[s] = L;
-      }
-    }
-    if (nf == 0 || nf >= S[s].thresh) continue;
-    const uint64_t *off = S[s].off;
-    const uint32_t *q = S[s].q[p];
-    unsigned long long d = 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nf;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-      const uint32_t u = q[i];
-      d += off[u + 1] - off[u];
-    }
-    d = wave_sum(d);
-    if (lane() == 0 && d) atomicAdd(&cur->degsum, d);
-  }
+// Direction choice of judge() (src/reach.hpp:86-112): sparse iff
+// n_front + out_edges < m / dense_div.  out_edges is exact after a push
+// level (summed at append time) and estimated as n_front * m / n after a
+// pull level, where the reference samples (dense_sample, :192-206).
+__device__ __forceinline__ int reach_mode(const ReachSlot *c,
+                                          const ReachSearch &R) {
+  const uint32_t nf = c->qsize;
+  if (nf == 0) return MODE_NONE;
+  unsigned long long ds = c->degsum;
+  if (c->from_pull) ds = (unsigned long long)((double)nf * R.avg_deg);
+  return ((uint64_t)nf + ds < R.thresh) ? MODE_PUSH : MODE_PULL;
 }
 
-// Sparse push step with VGC (src/reach.hpp:114-171).  A wave takes S seeds,
-// keeps a queue of up to REACH_RING vertices in LDS and expands it with
-// load-balanced edge scans (lane e of a 64-edge window finds its owner by
-// binary search over the wave's inclusive degree scan).  Newly visited
-// vertices stay in the queue while the VGC budget (tau per seed) lasts and
-// the ring has room; the rest spill to the next global frontier.  Vertices
-// with degree > HEAVY_DEG go to the heavy list, expanded by k_reach_heavy.
-__global__ __launch_bounds__(BLOCK) void k_reach_push(
+__device__ __forceinline__ void add_cnt(unsigned long long *c, unsigned long long v) {
+  v = wave_sum(v);
+  if (lane() == 0 && v) atomicAdd(c, v);
+}
+
+// One BFS level of both searches.  Push levels (src/reach.hpp:114-171):
+// a wave takes S seeds, keeps a queue of up to REACH_RING vertices in LDS
+// and expands it with load-balanced edge scans (lane e of a 64-edge window
+// finds its owner by binary search over the wave's inclusive degree scan).
+// Newly visited vertices stay in the queue while the VGC budget (tau per
+// seed) lasts and the ring has room; the rest spill to the next frontier.
+// Vertices with degree > HEAVY_DEG go to the heavy list (k_reach_heavy).
+// Pull levels (src/reach.hpp:173-190): every unvisited vertex scans its
+// in-edges until it meets a visited vertex; testing "visited" instead of
+// "in the previous frontier" keeps the reachable set identical.
+__global__ __launch_bounds__(BLOCK) void k_reach_step(
     const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
@@ -239,132 +229,173 @@ __global__ __launch_bounds__(BLOCK) void k_reach_push(
   uint32_t *ring = ring_s[w];
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
+  const int ci = L % 3, ni = (L + 1) % 3, zi = (L + 2) % 3;
   const int p = L & 1;
-  for (int s = 0; s < ns; ++s) {
-    ReachSlot *cur = &C->slot[s][p];
-    ReachSlot *nxt = &C->slot[s][p ^ 1];
-    const uint32_t nf = cur->qsize;
-    const ReachSearch R = S[s];
-    if (reach_mode(nf, cur->degsum, R.thresh) != MODE_PUSH) continue;
-    uint32_t sd = 64;
-    if (vgc) sd = min(64u, max(1u, nf / total_waves));
-    unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0;
-    while (true) {
-      uint32_t g = 0;
-      if (l == 0) g = atomicAdd(&cur->grab, sd);
-      g = __shfl(g, 0);
-      if (g >= nf) break;
-      const uint32_t cnt = min(sd, nf - g);
-      if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
-      uint32_t head = 0, tail = cnt;
-      uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
-      wave_sync();
-      while (head < tail) {
-        const uint32_t k = min(64u, tail - head);
-        const bool valid = (uint32_t)l < k;
-        const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
-        head += k;
-        uint64_t b = 0, deg = 0;
-        if (valid) {
-          b = R.off[u];
-          deg = R.off[u + 1] - b;
-          ++n_exp;
-        }
-        const bool heavy = deg > HEAVY_DEG;
-        const uint64_t hm = ballot(heavy);
-        if (hm) {
-          const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
-          const unsigned long long cinc = wave_incl_scan(nch);
-          const unsigned long long ctot = __shfl(cinc, WAVE - 1);
-          unsigned long long old = 0;
-          if (l == 0)
-            old = atomicAdd(&cur->heavy,
-                            ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
-          old = __shfl(old, 0);
-          if (heavy) {
-            const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
-            R.hv[idx] = u;
-            R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
-          }
-          if (heavy) deg = 0;
-        }
-        const uint32_t inc = wave_incl_scan((uint32_t)deg);
-        const uint32_t total = __shfl(inc, WAVE - 1);
-        sc[l] = inc;
-        st[l] = b;
-        wave_sync();
-        for (uint32_t base = 0; base < total; base += WAVE) {
-          const uint32_t e = base + l;
-          bool found = false;
-          uint32_t v = 0;
-          if (e < total) {
-            int lo = 0, hi = WAVE - 1;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (sc[mid] > e) hi = mid; else lo = mid + 1;
-            }
-            const uint32_t ex = lo ? sc[lo - 1] : 0u;
-            v = R.adj[st[lo] + (e - ex)];
-            found = bit_set(R.vis, v);
-            ++n_edges;
-          }
-          const uint64_t fm = ballot(found);
-          if (!fm) continue;
-          const uint32_t nfound = (uint32_t)__popcll(fm);
-          n_vis += found;
-          const uint32_t room = REACH_RING - (tail - head);
-          const uint32_t take =
-              (uint32_t)min((uint64_t)min(nfound, room), budget);
-          const uint32_t rank = lanes_below(fm);
-          const bool loc = found && rank < take;
-          if (loc) ring[(tail + rank) & (REACH_RING - 1)] = v;
-          tail += take;
-          budget -= take;
-          const bool glob = found && !loc;
-          const uint32_t pos = wave_append(&nxt->qsize, glob);
-          if (glob) {
-            R.q[p ^ 1][pos] = v;
-            ++n_app;
-          }
-        }
-        wave_sync();
-      }
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)ns) {
+    const int s = threadIdx.x;
+    ReachSlot z{};
+    C->slot[s][zi] = z;
+    if (C->slot[s][ci].qsize == 0 && !C->done[s]) {
+      C->done[s] = 1;
+      C->levels[s] = L;
     }
-    n_edges = wave_sum(n_edges);
-    n_exp = wave_sum(n_exp);
-    n_vis = wave_sum(n_vis);
-    n_app = wave_sum(n_app);
-    if (l == 0) {
-      if (n_edges) atomicAdd(&C->edges[s], n_edges);
-      if (n_exp) atomicAdd(&C->expanded[s], n_exp);
-      if (n_vis) atomicAdd(&C->visits[s], n_vis);
-      if (n_app) atomicAdd(&C->appended[s], n_app);
+  }
+  for (int s = 0; s < ns; ++s) {
+    ReachSlot *cur = &C->slot[s][ci];
+    ReachSlot *nxt = &C->slot[s][ni];
+    const ReachSearch R = S[s];
+    const int mode = reach_mode(cur, R);
+    const uint32_t nf = cur->qsize;
+    if (mode == MODE_PUSH) {
+      uint32_t sd = 64;
+      if (vgc) sd = min(64u, max(1u, nf / total_waves));
+      unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
+      while (true) {
+        uint32_t g = 0;
+        if (l == 0) g = atomicAdd(&cur->grab, sd);
+        g = __shfl(g, 0);
+        if (g >= nf) break;
+        const uint32_t cnt = min(sd, nf - g);
+        if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
+        uint32_t head = 0, tail = cnt;
+        uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+        wave_sync();
+        while (head < tail) {
+          const uint32_t k = min(64u, tail - head);
+          const bool valid = (uint32_t)l < k;
+          const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
+          head += k;
+          uint64_t b = 0, deg = 0;
+          if (valid) {
+            b = R.off[u];
+            deg = R.off[u + 1] - b;
+            ++n_exp;
+          }
+          const bool heavy = deg > HEAVY_DEG;
+          const uint64_t hm = ballot(heavy);
+          if (hm) {
+            const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
+            const unsigned long long cinc = wave_incl_scan(nch);
+            const unsigned long long ctot = __shfl(cinc, WAVE - 1);
+            unsigned long long old = 0;
+            if (l == 0)
+              old = atomicAdd(&cur->heavy,
+                              ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
+            old = __shfl(old, 0);
+            if (heavy) {
+              const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
+              R.hv[idx] = u;
+              R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
+              deg = 0;
+            }
+          }
+          const uint32_t inc = wave_incl_scan((uint32_t)deg);
+          const uint32_t total = __shfl(inc, WAVE - 1);
+          sc[l] = inc;
+          st[l] = b;
+          wave_sync();
+          for (uint32_t base = 0; base < total; base += WAVE) {
+            const uint32_t e = base + l;
+            bool found = false;
+            uint32_t v = 0;
+            if (e < total) {
+              int lo = 0, hi = WAVE - 1;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sc[mid] > e) hi = mid; else lo = mid + 1;
+              }
+              const uint32_t ex = lo ? sc[lo - 1] : 0u;
+              v = R.adj[st[lo] + (e - ex)];
+              found = bit_set(R.vis, v);
+              ++n_edges;
+            }
+            const uint64_t fm = ballot(found);
+            if (!fm) continue;
+            const uint32_t nfound = (uint32_t)__popcll(fm);
+            n_vis += found;
+            const uint32_t room = REACH_RING - (tail - head);
+            const uint32_t take = (uint32_t)min((uint64_t)min(nfound, room), budget);
+            const uint32_t rank = lanes_below(fm);
+            const bool loc = found && rank < take;
+            if (loc) ring[(tail + rank) & (REACH_RING - 1)] = v;
+            tail += take;
+            budget -= take;
+            const bool glob = found && !loc;
+            const uint32_t pos = wave_append(&nxt->qsize, glob);
+            if (glob) {
+              R.q[p ^ 1][pos] = v;
+              dsum += R.off[v + 1] - R.off[v];
+              ++n_app;
+            }
+          }
+          wave_sync();
+        }
+      }
+      add_cnt(&nxt->degsum, dsum);
+      add_cnt(&C->cnt[s][R_PUSH_EDGES], n_edges);
+      add_cnt(&C->cnt[s][R_PUSH_EXP], n_exp);
+      add_cnt(&C->cnt[s][R_PUSH_VIS], n_vis);
+      add_cnt(&C->cnt[s][R_PUSH_APP], n_app);
+    } else if (mode == MODE_PULL) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) nxt->from_pull = 1;
+      const uint64_t units = (R.n + 63) / 64;
+      unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_units = 0;
+      for (uint64_t t = gw; t < units; t += nw) {
+        const uint64_t v = t * 64 + l;
+        bool found = false;
+        ++n_units;
+        if (v < R.n && !bit_test(R.vis, (uint32_t)v)) {
+          ++n_scan;
+          const uint64_t b = R.roff[v], e = R.roff[v + 1];
+          for (uint64_t j = b; j < e; ++j) {
+            ++n_edges;
+            if (bit_test(R.vis, R.radj[j])) {
+              found = true;
+              break;
+            }
+          }
+        }
+        const uint64_t fm = ballot(found);
+        if (!fm) continue;
+        if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[t * 2], (uint32_t)fm);
+        if (l == 32 && (fm >> 32)) atomicOr(&R.vis[t * 2 + 1], (uint32_t)(fm >> 32));
+        n_vis += found;
+        const uint32_t pos = wave_append(&nxt->qsize, found);
+        if (found) R.q[p ^ 1][pos] = (uint32_t)v;
+      }
+      add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
+      add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
+      add_cnt(&C->cnt[s][R_PULL_VIS], n_vis);
+      add_cnt(&C->cnt[s][R_PULL_UNITS], l == 0 ? n_units : 0);
     }
   }
 }
 
-// Edge-parallel expansion of the heavy vertices of this level
+// Edge-parallel expansion of the heavy vertices of a push level
 // (src/reach.hpp:155-167, granularity 2048): work items are CHUNK-edge
 // slices; a wave finds the owning vertex by binary search over hc.
 __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
                                                        ReachCtl *C, int ns,
                                                        uint32_t L) {
   const int l = lane();
+  const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
   for (int s = 0; s < ns; ++s) {
-    ReachSlot *cur = &C->slot[s][p];
-    ReachSlot *nxt = &C->slot[s][p ^ 1];
-    const ReachSearch R = S[s];
-    if (reach_mode(cur->qsize, cur->degsum, R.thresh) != MODE_PUSH) continue;
+    ReachSlot *cur = &C->slot[s][ci];
+    ReachSlot *nxt = &C->slot[s][ni];
     const unsigned long long hw = cur->heavy;
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0) continue;
-    unsigned long long n_edges = 0, n_vis = 0, n_app = 0;
+    const ReachSearch R = S[s];
+    unsigned long long n_edges = 0, n_vis = 0, n_app = 0, n_items = 0, dsum = 0;
     while (true) {
       unsigned long long c = 0;
       if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
       c = __shfl(c, 0);
       if (c >= Ct) break;
+      if (l == 0) ++n_items;
       uint64_t lo = 0, hi = H - 1;
       while (lo < hi) {   // last idx with hc[idx] <= c
         const uint64_t mid = (lo + hi + 1) >> 1;
@@ -387,73 +418,16 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
         const uint32_t pos = wave_append(&nxt->qsize, found);
         if (found) {
           R.q[p ^ 1][pos] = v;
+          dsum += R.off[v + 1] - R.off[v];
           ++n_app;
         }
       }
     }
-    n_edges = wave_sum(n_edges);
-    n_vis = wave_sum(n_vis);
-    n_app = wave_sum(n_app);
-    if (l == 0) {
-      if (n_edges) atomicAdd(&C->edges[s], n_edges);
-      if (n_vis) atomicAdd(&C->visits[s], n_vis);
-      if (n_app) atomicAdd(&C->appended[s], n_app);
-    }
-  }
-}
-
-// Dense pull step (src/reach.hpp:173-190): every unvisited vertex scans its
-// in-edges until it meets a visited vertex.  Testing "visited" instead of
-// "in the previous frontier" keeps the reachable set identical and can only
-// stop earlier.  A wave covers 64 consecutive vertices (two bitmap words),
-// so the new bits are published with two atomicOr per wave.
-__global__ __launch_bounds__(BLOCK) void k_reach_pull(const ReachSearch *S,
-                                                      ReachCtl *C, int ns,
-                                                      uint32_t L) {
-  const int l = lane();
-  const int p = L & 1;
-  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (int s = 0; s < ns; ++s) {
-    ReachSlot *cur = &C->slot[s][p];
-    ReachSlot *nxt = &C->slot[s][p ^ 1];
-    const ReachSearch R = S[s];
-    if (reach_mode(cur->qsize, cur->degsum, R.thresh) != MODE_PULL) continue;
-    const uint64_t units = (R.n + 63) / 64;
-    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0;
-    for (uint64_t t = gw; t < units; t += nw) {
-      const uint64_t v = t * 64 + l;
-      bool found = false;
-      if (v < R.n && !bit_test(R.vis, (uint32_t)v)) {
-        ++n_scan;
-        const uint64_t b = R.roff[v], e = R.roff[v + 1];
-        for (uint64_t j = b; j < e; ++j) {
-          ++n_edges;
-          if (bit_test(R.vis, R.radj[j])) {
-            found = true;
-            break;
-          }
-        }
-      }
-      const uint64_t fm = ballot(found);
-      if (!fm) continue;
-      if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[t * 2], (uint32_t)fm);
-      if (l == 32 && (fm >> 32)) atomicOr(&R.vis[t * 2 + 1], (uint32_t)(fm >> 32));
-      n_vis += found;
-      const uint32_t pos = wave_append(&nxt->qsize, found);
-      if (found) R.q[p ^ 1][pos] = (uint32_t)v;
-    }
-    n_edges = wave_sum(n_edges);
-    n_scan = wave_sum(n_scan);
-    n_vis = wave_sum(n_vis);
-    if (l == 0) {
-      if (n_edges) atomicAdd(&C->edges[s], n_edges);
-      if (n_scan) atomicAdd(&C->pull_scans[s], n_scan);
-      if (n_vis) {
-        atomicAdd(&C->visits[s], n_vis);
-        atomicAdd(&C->appended[s], n_vis);
-      }
-    }
+    add_cnt(&nxt->degsum, dsum);
+    add_cnt(&C->cnt[s][R_HEAVY_EDGES], n_edges);
+    add_cnt(&C->cnt[s][R_HEAVY_VIS], n_vis);
+    add_cnt(&C->cnt[s][R_HEAVY_APP], n_app);
+    add_cnt(&C->cnt[s][R_HEAVY_ITEMS], n_items);
   }
 }
 
@@ -487,8 +461,7 @@ __global__ void k_ms_reset(MSCtl *C, int s) {
     for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
     C->log_n[s] = 0;
     C->full[s] = 0;
-    C->edges[s] = C->expanded[s] = C->inserts[s] = 0;
-    C->probes[s] = C->appended[s] = 0;
+    for (int c = 0; c < NMC; ++c) C->cnt[s][c] = 0;
   }
 }
 
@@ -510,7 +483,7 @@ __global__ void k_ms_seed(MSSearch *S, MSCtl *C, int s, const uint32_t *front,
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     C->slot[s][0].qsize = nfront;
     C->log_n[s] = nfront;
-    C->inserts[s] = nfront;
+    C->cnt[s][M_INS] = nfront;
   }
 }
 
@@ -662,18 +635,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
         wave_sync();
       }
     }
-    n_edges = wave_sum(n_edges);
-    n_exp = wave_sum(n_exp);
-    n_ins = wave_sum(n_ins);
-    n_app = wave_sum(n_app);
-    unsigned long long pb = wave_sum((unsigned long long)probes);
-    if (l == 0) {
-      if (n_edges) atomicAdd(&C->edges[s], n_edges);
-      if (n_exp) atomicAdd(&C->expanded[s], n_exp);
-      if (n_ins) atomicAdd(&C->inserts[s], n_ins);
-      if (n_app) atomicAdd(&C->appended[s], n_app);
-      if (pb) atomicAdd(&C->probes[s], pb);
-    }
+    add_cnt(&C->cnt[s][M_EDGES], n_edges);
+    add_cnt(&C->cnt[s][M_EXP], n_exp);
+    add_cnt(&C->cnt[s][M_INS], n_ins);
+    add_cnt(&C->cnt[s][M_APP], n_app);
+    add_cnt(&C->cnt[s][M_PROBES], (unsigned long long)probes);
   }
 }
 
@@ -692,13 +658,14 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0 || C->full[s]) continue;
     const MSSearch M = S[s];
-    unsigned long long n_edges = 0, n_ins = 0, n_app = 0;
+    unsigned long long n_edges = 0, n_ins = 0, n_app = 0, n_items = 0;
     uint32_t probes = 0;
     while (true) {
       unsigned long long c = 0;
       if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
       c = __shfl(c, 0);
       if (c >= Ct) break;
+      if (l == 0) ++n_items;
       uint64_t lo = 0, hi = H - 1;
       while (lo < hi) {
         const uint64_t mid = (lo + hi + 1) >> 1;
@@ -736,16 +703,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
         }
       }
     }
-    n_edges = wave_sum(n_edges);
-    n_ins = wave_sum(n_ins);
-    n_app = wave_sum(n_app);
-    unsigned long long pb = wave_sum((unsigned long long)probes);
-    if (l == 0) {
-      if (n_edges) atomicAdd(&C->edges[s], n_edges);
-      if (n_ins) atomicAdd(&C->inserts[s], n_ins);
-      if (n_app) atomicAdd(&C->appended[s], n_app);
-      if (pb) atomicAdd(&C->probes[s], pb);
-    }
+    add_cnt(&C->cnt[s][M_HEAVY_EDGES], n_edges);
+    add_cnt(&C->cnt[s][M_HEAVY_INS], n_ins);
+    add_cnt(&C->cnt[s][M_HEAVY_APP], n_app);
+    add_cnt(&C->cnt[s][M_HEAVY_PROBES], (unsigned long long)probes);
+    add_cnt(&C->cnt[s][M_HEAVY_ITEMS], n_items);
   }
 }
 
@@ -884,6 +846,7 @@ Engine::~Engine() {
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
   if (h_pinned) (void)hipHostFree(h_pinned);
   for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+  for (auto &e : pev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -912,16 +875,15 @@ int Engine::init(const psc_graph *dg, bool owned) {
   for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
   PSC_HIP(hipHostMalloc(&h_pinned, 4096));
   int occ = 0;
-  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_push,
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
                                                        BLOCK, 0));
-  grid_push = std::max(1, occ) * num_cu;
+  grid_step = std::max(1, occ) * num_cu;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ms_expand,
                                                        BLOCK, 0));
   grid_ms = std::max(1, occ) * num_cu;
-  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_pull,
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_heavy,
                                                        BLOCK, 0));
-  grid_pull = std::max(1, occ) * num_cu;
-  grid_heavy = grid_pull;
+  grid_heavy = std::max(1, occ) * num_cu;
 
   const uint64_t nn = std::max<uint64_t>(n, 1);
   const uint64_t words = (nn + 31) / 32;
@@ -979,11 +941,13 @@ int Engine::select(F f, uint64_t len, uint32_t *sel, uint32_t *rej,
   }
   const uint64_t nt = (len + SEL_TILE - 1) / SEL_TILE;
   unsigned long long *tot = misc.as<unsigned long long>();
+  prof_begin(KK_SELECT);
   k_sel_count<F><<<(unsigned)nt, BLOCK, 0, stream>>>(f, len, tiles.as<uint32_t>());
   k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
                                        tile_off.as<unsigned long long>(), tot);
   k_sel_scatter<F><<<(unsigned)nt, BLOCK, 0, stream>>>(
       f, len, tile_off.as<unsigned long long>(), sel, rej);
+  prof_end();
   PSC_HIP(hipGetLastError());
   PSC_HIP(hipMemcpyAsync(h_pinned, tot, 8, hipMemcpyDeviceToHost, stream));
   PSC_HIP(hipStreamSynchronize(stream));
@@ -1002,11 +966,17 @@ int Engine::poll(void *dst, const void *src, size_t bytes) {
 }
 
 // First round: fused forward + backward reach from P[0] (src/scc.hpp:316-339).
+// Levels are enqueued in batches of K without host round trips; kernels of
+// levels past the end exit at once.  The host reads the done flags after
+// each batch.
 int Engine::first_reach(const psc_params &p, psc_stats *st) {
   ReachSearch rs[2];
   std::memcpy(rs, h_reach, sizeof(rs));
   const uint64_t thresh = m / (p.dense_div ? p.dense_div : 20);
-  rs[0].thresh = rs[1].thresh = thresh;
+  for (int s = 0; s < 2; ++s) {
+    rs[s].thresh = thresh;
+    rs[s].avg_deg = n ? (double)m / (double)n : 0.0;
+  }
   char *pin = (char *)h_pinned;
   std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
   PSC_HIP(hipMemcpyAsync(reach_desc.ptr, pin + PIN_REACH, sizeof(rs),
@@ -1014,20 +984,24 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   ReachSearch *S = reach_desc.as<ReachSearch>();
   ReachCtl *C = reach_ctl.as<ReachCtl>();
   const uint64_t words = (n + 31) / 32;
+  prof_begin(KK_MISC);
   k_reach_init<<<grid_for(words), BLOCK, 0, stream>>>(S, C, 2, P.as<uint32_t>(),
                                                        words);
-  const uint32_t total_waves = (uint32_t)grid_push * WPB;
+  prof_end();
+  const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   uint32_t L = 0;
   int K = 4;
   ReachCtl hc_;
   while (true) {
     for (int k = 0; k < K; ++k, ++L) {
-      k_reach_judge<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L);
-      k_reach_push<<<grid_push, BLOCK, 0, stream>>>(S, C, 2, L, tau,
+      prof_begin(KK_STEP);
+      k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, L, tau,
                                                       p.local_reach, total_waves);
+      prof_end();
+      prof_begin(KK_HEAVY);
       k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L);
-      k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L);
+      prof_end();
     }
     PSC_HIP(hipGetLastError());
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
@@ -1037,12 +1011,28 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   st->levels_fwd = hc_.levels[0];
   st->levels_bwd = hc_.levels[1];
   for (int s = 0; s < 2; ++s) {
-    st->edges_scanned += hc_.edges[s];
-    // B_csr: 16 B offsets per expanded vertex / pulled vertex + 4 B per edge
-    st->bytes_csr += 16 * (hc_.expanded[s] + hc_.pull_scans[s]) + 4 * hc_.edges[s];
-    // visited bitmap: 1 B-equivalent read per edge + write per new visit
-    st->bytes_label += hc_.edges[s] + hc_.visits[s];
-    st->bytes_front += 4 * hc_.appended[s] + 4 * hc_.expanded[s];
+    const unsigned long long *c = hc_.cnt[s];
+    st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES];
+    // push: offsets of expanded vertices, edge ids, visited test per edge,
+    // visited write per new vertex, frontier read/append, degree of appended
+    const double push = 16.0 * c[R_PUSH_EXP] + 5.0 * c[R_PUSH_EDGES] +
+                        c[R_PUSH_VIS] + 4.0 * c[R_PUSH_EXP] + 20.0 * c[R_PUSH_APP];
+    const double heavy = 5.0 * c[R_HEAVY_EDGES] + c[R_HEAVY_VIS] +
+                         20.0 * c[R_HEAVY_APP] + 32.0 * c[R_HEAVY_ITEMS];
+    // pull: bitmap words of every 64-vertex unit, in-offsets of unvisited
+    // vertices, in-edge ids + visited test per scanned edge, new vertices
+    const double pull = 8.0 * c[R_PULL_UNITS] + 16.0 * c[R_PULL_SCANS] +
+                        5.0 * c[R_PULL_EDGES] + 5.0 * c[R_PULL_VIS];
+    kbytes[KK_STEP] += push + pull;
+    kbytes[KK_HEAVY] += heavy;
+    st->bytes_csr += 16 * (c[R_PUSH_EXP] + c[R_PULL_SCANS]) +
+                     4 * (c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES]);
+    st->bytes_label += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
+                       c[R_PUSH_VIS] + c[R_HEAVY_VIS] + c[R_PULL_VIS] +
+                       8 * c[R_PULL_UNITS];
+    st->bytes_front += 4 * (c[R_PUSH_APP] + c[R_HEAVY_APP] + c[R_PULL_VIS] +
+                            c[R_PUSH_EXP]) +
+                       16 * (c[R_PUSH_APP] + c[R_HEAVY_APP]);
   }
   return PSC_OK;
 }
@@ -1069,8 +1059,11 @@ int Engine::ms_prepare(int s, uint64_t slots, MSSearch *d) {
   d->hc = mhc[s].as<unsigned long long>();
   d->probe_limit = (uint32_t)std::min<uint64_t>(slots, 1u << 20);
   d->pad = 0;
+  prof_begin(KK_MISC);
   k_fill_u64<<<grid_for(slots, BLOCK, 8192), BLOCK, 0, stream>>>(
       d->table, slots, EMPTY_SLOT);
+  prof_end();
+  kbytes[KK_MISC] += 8.0 * slots;
   PSC_HIP(hipGetLastError());
   return PSC_OK;
 }
@@ -1092,10 +1085,11 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
   MSSearch *S = ms_desc.as<MSSearch>();
   MSCtl *C = ms_ctl.as<MSCtl>();
   int want[2] = {1, 1};
-  bool first_try = true;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   const uint32_t total_waves = (uint32_t)grid_ms * WPB;
+  int attempt = 0;
   while (want[0] || want[1]) {
+    const auto t_attempt = std::chrono::steady_clock::now();
     for (int s = 0; s < 2; ++s) {
       if (!want[s]) continue;
       while (slots[s] / 2 < nfront) {   // seeds alone exceed the load limit
@@ -1113,9 +1107,12 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
                            stream));
     for (int s = 0; s < 2; ++s) {
       if (!want[s]) continue;
+      prof_begin(KK_SEED);
       k_ms_reset<<<1, 64, 0, stream>>>(C, s);
       k_ms_seed<<<grid_for(nfront), BLOCK, 0, stream>>>(
           S, C, s, front.as<uint32_t>(), nfront, label_offset);
+      prof_end();
+      kbytes[KK_SEED] += 4.0 * nfront + 32.0 * nfront;
     }
     uint32_t act[2] = {(uint32_t)want[0], (uint32_t)want[1]};
     std::memcpy(pin + PIN_FLAGS, act, sizeof(act));
@@ -1127,11 +1124,15 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
     MSCtl hc_;
     while (true) {
       for (int k = 0; k < K; ++k, ++L) {
+        prof_begin(KK_MS_EXPAND);
         k_ms_expand<<<grid_ms, BLOCK, 0, stream>>>(
             S, C, 2, L, labels.as<unsigned long long>(), tau, p.local_scc,
             total_waves);
+        prof_end();
+        prof_begin(KK_MS_HEAVY);
         k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
             S, C, 2, L, labels.as<unsigned long long>());
+        prof_end();
       }
       PSC_HIP(hipGetLastError());
       PSC_TRY(poll(&hc_, C, sizeof(MSCtl)));
@@ -1144,12 +1145,25 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
     st->sub_rounds += L;
     for (int s = 0; s < 2; ++s) {
       if (!want[s]) continue;
-      st->edges_scanned += hc_.edges[s];
-      st->table_inserts += hc_.inserts[s];
-      st->bytes_csr += 16 * hc_.expanded[s] + 4 * hc_.edges[s];
-      st->bytes_label += 8 * hc_.expanded[s] + 8 * hc_.edges[s];
-      st->bytes_table += 8 * hc_.probes[s] + 8 * hc_.inserts[s];
-      st->bytes_front += 16 * hc_.appended[s] + 8 * hc_.inserts[s] + 8 * hc_.expanded[s];
+      const unsigned long long *c = hc_.cnt[s];
+      st->edges_scanned += c[M_EDGES] + c[M_HEAVY_EDGES];
+      st->table_inserts += c[M_INS] + c[M_HEAVY_INS];
+      // expand: pair read + offsets + labels[v] per expanded pair; edge id
+      // + labels[w] per edge; 8 B per probed slot; CAS write, log append
+      // and frontier append per new pair
+      const double ex = 32.0 * c[M_EXP] + 12.0 * c[M_EDGES] +
+                        8.0 * c[M_PROBES] + 16.0 * c[M_INS] + 8.0 * c[M_APP];
+      const double hv = 12.0 * c[M_HEAVY_EDGES] + 8.0 * c[M_HEAVY_PROBES] +
+                        16.0 * c[M_HEAVY_INS] + 8.0 * c[M_HEAVY_APP] +
+                        40.0 * c[M_HEAVY_ITEMS];
+      kbytes[KK_MS_EXPAND] += ex;
+      kbytes[KK_MS_HEAVY] += hv;
+      st->bytes_csr += 16 * c[M_EXP] + 4 * (c[M_EDGES] + c[M_HEAVY_EDGES]);
+      st->bytes_label += 8 * (c[M_EXP] + c[M_EDGES] + c[M_HEAVY_EDGES]);
+      st->bytes_table += 8 * (c[M_PROBES] + c[M_HEAVY_PROBES] + c[M_INS] +
+                              c[M_HEAVY_INS]);
+      st->bytes_front += 8 * (c[M_EXP] + c[M_INS] + c[M_HEAVY_INS] + c[M_APP] +
+                              c[M_HEAVY_APP]);
       if (hc_.full[s]) {
         slots[s] *= 2;   // double_size (src/resizable_table.h:196-204)
         ++st->resizes;
@@ -1159,8 +1173,11 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
         want[s] = 0;
       }
     }
-    if (!first_try && !(want[0] || want[1])) break;
-    first_try = false;
+    // reruns after an overflow are the "hash table resize time" of
+    // src/scc.hpp:112-113
+    if (attempt++ > 0)
+      st->ms_resize += std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - t_attempt).count();
   }
   return PSC_OK;
 }
@@ -1168,40 +1185,54 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
 int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   *st = psc_stats{};
   syncs = 0;
+  PSC_TRY(prof_reset(p.timing));
   unsigned long long *lab = labels.as<unsigned long long>();
   auto t_host0 = std::chrono::steady_clock::now();
   if (p.timing) PSC_HIP(hipEventRecord(ev[0], stream));
   // init + trim (src/scc.hpp:284-306)
+  prof_begin(KK_MISC);
   k_fill_u64<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, 0ull);
+  prof_end();
+  kbytes[KK_MISC] += 8.0 * n;
   uint64_t nnz = 0, nz = 0;
   PSC_TRY(select(SelNonZero{d_off, d_ioff}, n, NZ.as<uint32_t>(),
                  Z.as<uint32_t>(), &nnz));
+  kbytes[KK_SELECT] += 2 * 32.0 * n + 4.0 * n;
   nz = n - nnz;
   st->n_trimmed = nz;
+  prof_begin(KK_MISC);
   if (nz)
     k_trim_labels<<<grid_for(nz, BLOCK, 16384), BLOCK, 0, stream>>>(
         Z.as<uint32_t>(), nz, lab);
   if (nnz)
     k_permute<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
         NZ.as<uint32_t>(), nnz, PivotPerm(nnz, p.seed), P.as<uint32_t>());
+  prof_end();
+  kbytes[KK_MISC] += 12.0 * nz + 8.0 * nnz;
   PSC_HIP(hipGetLastError());
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
+  st->bytes_label += 8 * n + 8 * nz;
+  st->bytes_csr += 32 * n;
   uint64_t label_offset = nz + 1;
   uint64_t n_rem = 0;
   if (nnz) {
     PSC_TRY(first_reach(p, st));
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
+    prof_begin(KK_MISC);
     k_first_labels<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
         P.as<uint32_t>(), nnz, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(),
         lab, label_offset, big);
+    prof_end();
+    kbytes[KK_MISC] += 4.0 * nnz + 8.0 * nnz + nnz / 4.0;
     PSC_HIP(hipGetLastError());
     PSC_TRY(select(SelNotDone{P.as<uint32_t>(), lab}, nnz,
                    centers.as<uint32_t>(), nullptr, &n_rem));
+    kbytes[KK_SELECT] += 2 * 12.0 * nnz + 4.0 * n_rem;
     uint64_t bigh = 0;
     PSC_TRY(poll(&bigh, big, 8));
     st->first_scc_size = bigh;
-    st->bytes_label += 8 * nnz + 2 * nnz / 8;
+    st->bytes_label += 8 * nnz + nnz / 4;
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
   label_offset++;
@@ -1218,6 +1249,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     uint64_t nfront = 0;
     PSC_TRY(select(SelNotDone{centers.as<uint32_t>() + cur, lab}, vs,
                    front.as<uint32_t>(), nullptr, &nfront));
+    kbytes[KK_SELECT] += 2 * 12.0 * vs + 4.0 * nfront;
     cur = end;
     if (nfront == 0) continue;
     uint64_t slots[2];
@@ -1230,32 +1262,46 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(multi_search(p, nfront, label_offset, slots, st));
     label_offset += nfront;
     const int small = (slots[0] <= slots[1]) ? 0 : 1;
+    prof_begin(KK_PARTITION);
     k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
         ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
         misc.as<unsigned long long>() + 2);
+    prof_end();
     PSC_HIP(hipGetLastError());
+    // log reads + label read-modify-write per pair; contains() probes of
+    // the small log's pairs are added after the run from misc[2]
+    kbytes[KK_PARTITION] += 24.0 * (log_n[0] + log_n[1]);
     st->bytes_label += 16 * (log_n[0] + log_n[1]);
     st->bytes_front += 8 * (log_n[0] + log_n[1]);
-    st->bytes_table += 8 * log_n[small];
     ne_prev[0] = log_n[0];
     ne_prev[1] = log_n[1];
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
+  prof_begin(KK_MISC);
   k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n);
+  kbytes[KK_MISC] += 16.0 * n;
   if (p.canonical) {
     uint32_t *minv = scratch32.as<uint32_t>();
     k_fill_u32<<<grid_for(n + 1, BLOCK, 16384), BLOCK, 0, stream>>>(
         minv, n + 1, 0xFFFFFFFFu);
     k_canon_min<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
     k_canon_apply<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
+    kbytes[KK_MISC] += 4.0 * n + 12.0 * n + 20.0 * n;
   }
-  st->bytes_label += 16 * n * 2;
+  prof_end();
+  st->bytes_label += 16 * n;
   if (d_out != lab && n)
     PSC_HIP(hipMemcpyAsync(d_out, lab, n * 8, hipMemcpyDeviceToDevice, stream));
   PSC_HIP(hipGetLastError());
   if (p.timing) PSC_HIP(hipEventRecord(ev[4], stream));
   PSC_HIP(hipStreamSynchronize(stream));
   ++syncs;
+  {
+    unsigned long long pprobe = 0;
+    PSC_TRY(poll(&pprobe, misc.as<unsigned long long>() + 2, 8));
+    kbytes[KK_PARTITION] += 8.0 * pprobe;
+    st->bytes_table += 8 * pprobe;
+  }
   if (p.timing) {
     float a = 0;
     PSC_HIP(hipEventElapsedTime(&a, ev[0], ev[4]));
@@ -1272,9 +1318,60 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     st->ms_total = std::chrono::duration<double, std::milli>(
                        std::chrono::steady_clock::now() - t_host0).count();
   }
+  PSC_TRY(prof_collect());
   st->host_syncs = syncs;
   return PSC_OK;
 }
+
+// ---------------------------------------------------------------- profiler
+// timing >= 2: one event pair around every launch of the path, summed per
+// kernel kind after the run (the per-launch average rocprofv3 reports).
+int Engine::prof_reset(int level) {
+  prof_level = level;
+  pev_used = 0;
+  pkind.clear();
+  for (int k = 0; k < KK_N; ++k) {
+    kprof[k] = KProf{};
+    kbytes[k] = 0;
+  }
+  (void)hipMemsetAsync(misc.as<unsigned long long>() + 2, 0, 8, stream);
+  return PSC_OK;
+}
+
+void Engine::prof_begin(int kind) {
+  if (prof_level < 2) return;
+  if (pev_used + 2 > pev.size()) {
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pev.push_back(e);
+    }
+  }
+  (void)hipEventRecord(pev[pev_used], stream);
+  pkind.push_back(kind);
+}
+
+void Engine::prof_end() {
+  if (prof_level < 2 || pkind.size() * 2 != pev_used + 2) return;
+  (void)hipEventRecord(pev[pev_used + 1], stream);
+  pev_used += 2;
+}
+
+int Engine::prof_collect() {
+  for (int k = 0; k < KK_N; ++k) kprof[k].bytes = kbytes[k];
+  if (prof_level < 2) return PSC_OK;
+  for (size_t i = 0; i < pkind.size() && 2 * i + 1 < pev_used; ++i) {
+    float a = 0;
+    PSC_HIP(hipEventElapsedTime(&a, pev[2 * i], pev[2 * i + 1]));
+    kprof[pkind[i]].ms += a;
+    kprof[pkind[i]].launches += 1;
+  }
+  return PSC_OK;
+}
+
+const char *const KK_NAMES[KK_N] = {
+    "k_reach_step", "k_reach_heavy", "k_ms_expand", "k_ms_heavy",
+    "k_partition", "select (3 kernels)", "k_ms_seed", "init/trim/finalize"};
 
 int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
                    uint64_t *largest) {
@@ -1467,7 +1564,49 @@ int psc_status_device(psc_handle *h, const uint64_t *d_labels, uint64_t *n_scc,
   return h->eng.status((const unsigned long long *)d_labels, n_scc, largest);
 }
 
+int psc_kernel_profile(psc_handle *h, psc_kernel_prof *out, int cap) {
+  if (!h || (!out && cap > 0)) return PSC_ERR_INVALID;
+  int k = 0;
+  for (; k < psc::KK_N && k < cap; ++k) {
+    std::memset(&out[k], 0, sizeof(out[k]));
+    std::strncpy(out[k].name, psc::KK_NAMES[k], sizeof(out[k].name) - 1);
+    out[k].launches = h->eng.kprof[k].launches;
+    out[k].total_ms = h->eng.kprof[k].ms;
+    out[k].bytes = h->eng.kprof[k].bytes;
+  }
+  return k;
+}
+
 void psc_destroy(psc_handle *h) { delete h; }
+
+int psc_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return PSC_ERR_INVALID;
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    psc::set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return PSC_ERR_NOMEM;
+  }
+  return PSC_OK;
+}
+
+int psc_device_free(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) {
+    psc::set_error(std::string("hipFree: ") + hipGetErrorString(e));
+    return PSC_ERR_HIP;
+  }
+  return PSC_OK;
+}
+
+int psc_device_copy(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+  if (e != hipSuccess) {
+    psc::set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+    return PSC_ERR_HIP;
+  }
+  return PSC_OK;
+}
 
 void *psc_handle_stream(psc_handle *h) { return h ? (void *)h->eng.stream : nullptr; }
 

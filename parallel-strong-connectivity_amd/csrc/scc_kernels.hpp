@@ -30,6 +30,7 @@ struct ReachSearch {
   uint32_t *hv;          // heavy vertices of the current level
   unsigned long long *hc;  // exclusive chunk start of each heavy vertex
   uint64_t thresh;       // m / dense_div (src/reach.hpp:49)
+  double avg_deg;        // m / n, out-edge estimate after a pull level
   uint32_t n;
   uint32_t pad;
 };
@@ -37,27 +38,27 @@ struct ReachSearch {
 struct ReachSlot {
   uint32_t qsize;            // frontier size of this level
   uint32_t grab;             // seeds handed out
-  unsigned long long degsum; // out-degree sum of the frontier (judge)
+  unsigned long long degsum; // out-degree sum of the frontier
   unsigned long long heavy;  // (count << 40) | chunks
   unsigned long long hgrab;  // heavy chunks handed out
+  uint32_t from_pull;        // frontier produced by a pull level
+  uint32_t pad;
+};
+
+// per-search counters (algorithmic traffic accounting)
+enum : int {
+  R_PUSH_EXP, R_PUSH_EDGES, R_PUSH_VIS, R_PUSH_APP,
+  R_HEAVY_EDGES, R_HEAVY_VIS, R_HEAVY_APP, R_HEAVY_ITEMS,
+  R_PULL_SCANS, R_PULL_EDGES, R_PULL_VIS, R_PULL_UNITS,
+  NRC
 };
 
 struct ReachCtl {
-  ReachSlot slot[2][2];      // [search][level parity]
+  ReachSlot slot[2][3];      // [search][level % 3]
   uint32_t done[2];
   uint32_t levels[2];
-  unsigned long long edges[2];      // edge ids read (push + pull)
-  unsigned long long expanded[2];   // vertices whose offsets were read
-  unsigned long long visits[2];     // newly visited vertices
-  unsigned long long pull_scans[2]; // vertices tested by pull steps
-  unsigned long long appended[2];   // frontier appends
+  unsigned long long cnt[2][NRC];
 };
-
-__device__ __forceinline__ int reach_mode(uint32_t nf, unsigned long long ds,
-                                          uint64_t thresh) {
-  if (nf == 0) return MODE_NONE;
-  return ((uint64_t)nf + ds < thresh) ? MODE_PUSH : MODE_PULL;
-}
 
 // ------------------------------------------------------------ multi-search
 struct MSSearch {
@@ -79,16 +80,18 @@ struct MSSlot {
   unsigned long long qsize, grab, heavy, hgrab;
 };
 
+enum : int {
+  M_EXP, M_EDGES, M_INS, M_APP, M_PROBES,
+  M_HEAVY_EDGES, M_HEAVY_INS, M_HEAVY_APP, M_HEAVY_PROBES, M_HEAVY_ITEMS,
+  NMC
+};
+
 struct MSCtl {
   MSSlot slot[2][3];           // [search][level % 3]
   unsigned long long log_n[2];
   uint32_t full[2];
   uint32_t active[2];
-  unsigned long long edges[2];
-  unsigned long long expanded[2];
-  unsigned long long inserts[2];
-  unsigned long long probes[2];
-  unsigned long long appended[2];
+  unsigned long long cnt[2][NMC];
 };
 
 }  // namespace psc

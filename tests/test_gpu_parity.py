@@ -146,15 +146,14 @@ def test_device_generators_match_host(psc):
 
 
 def test_device_graph_run_and_status(psc, oracle):
-    import torch
     d = psc.DeviceGraph.rmat(14, 16 << 14, seed=8)
     h = d.to_host()
     s = psc.SCC(d)
-    out = torch.empty(d.n, dtype=torch.int64, device="cuda")
-    s.scc_device(out.data_ptr(), canonical=1)
-    torch.cuda.synchronize()
-    nscc, big = s.status_device(out.data_ptr())
-    labels = out.cpu().numpy().astype(np.uint64)
+    out = psc.DeviceArray(d.n)
+    s.scc_device(out.ptr, canonical=1)
+    nscc, big = s.status_device(out.ptr)
+    labels = out.numpy()
+    out.free()
     s.close()
     d.free()
     expect = oracle.tarjan_canonical(h)
