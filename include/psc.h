@@ -214,6 +214,70 @@ void psc_dev_graph_free(psc_dev_graph *dg);
 /* Copy a device graph back to host arrays (for checking). */
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
 
+/* ------------------------------------------------------------- batches
+ * Many graphs in one handle (no reference counterpart; the reference runs
+ * one graph per process).  Packed form: graph g has n_g = vbase[g+1] -
+ * vbase[g] vertices and m_g = ebase[g+1] - ebase[g] edges; its n_g + 1
+ * local offsets (0..m_g) sit at offset[vbase[g] + g], its uint32 local
+ * neighbour ids at edges[ebase[g]]; the in-CSR likewise.  The batch is held
+ * on the device as one disjoint union; each graph keeps its own trim
+ * numbering, pivot permutation (seed), label namespace and pivot-batch
+ * schedule, so its labels equal a stand-alone psc_run of that graph. */
+typedef struct psc_batch_desc {
+  uint64_t num_graphs;
+  const uint64_t *vbase;      /* num_graphs + 1 */
+  const uint64_t *ebase;      /* num_graphs + 1 */
+  const uint64_t *offset;     /* vbase[G] + G   */
+  const uint32_t *edges;      /* ebase[G]       */
+  const uint64_t *in_offset;  /* vbase[G] + G   */
+  const uint32_t *in_edges;   /* ebase[G]       */
+} psc_batch_desc;
+
+/* on_device = 1: the desc arrays are device pointers. */
+int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out);
+
+/* Number of graphs of a handle (1 for psc_create) and their vertex prefix
+ * (num_graphs + 1 entries). */
+uint64_t psc_handle_graphs(psc_handle *h);
+int psc_handle_vbase(psc_handle *h, uint64_t *vbase_out);
+
+/* SCC of every graph of the batch.  labels_out (host, vbase[G] entries,
+ * may be NULL) receives each graph's labels over its local ids, laid out
+ * like the vertices; beta_per_graph (may be NULL) overrides p->beta per
+ * graph.  tau and dense_div of p apply to the whole batch. */
+int psc_batch_run(psc_handle *h, const psc_params *p,
+                  const double *beta_per_graph, uint64_t *labels_out,
+                  psc_stats *stats);
+
+/* Seeded description of a mixed batch: kind_out[g] in {0 RMAT, 1 grid,
+ * 2 3D k-NN} among the kinds set in kinds_mask (bit k = kind k), vertex
+ * count log-uniform in [2^lo_log2, 2^hi_log2] (RMAT rounded to a power of
+ * two, grids to a square). */
+int psc_batch_spec(uint64_t num_graphs, double lo_log2, double hi_log2,
+                   uint64_t seed, uint32_t kinds_mask, uint32_t *kind_out,
+                   uint64_t *n_out);
+
+typedef struct psc_dev_batch psc_dev_batch;
+
+/* Generate a batch straight into HBM as a disjoint union: RMAT graphs get
+ * 16 n samples, (a,b,c,d) = (.57,.19,.19,.05), scrambled ids; grids keep a
+ * neighbour pair in both directions with probability 0.6; k-NN uses k = 5.
+ * Graph g uses seeds[g], or splitmix64(seed * 0x9E3779B97F4A7C15 + g + 1)
+ * when seeds is NULL; with that seed it equals psc_gen_rmat(log2 n, 16 n,
+ * .57, .19, .19, seed_g, 1), psc_gen_grid(side, side, 0.6, seed_g) or
+ * psc_gen_knn3d(n, 5, seed_g). */
+int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
+                      const uint64_t *n, uint64_t seed, const uint64_t *seeds,
+                      psc_dev_batch **out);
+void psc_dev_batch_free(psc_dev_batch *db);
+int psc_dev_batch_info(const psc_dev_batch *db, uint64_t *num_graphs,
+                       uint64_t *vbase, uint64_t *ebase, uint32_t *kind);
+/* Handle over a device batch (borrowed; keep the batch alive). */
+int psc_batch_create_dev(const psc_dev_batch *db, psc_handle **out);
+/* Graph g of a device batch as its own device graph with local ids. */
+int psc_dev_batch_extract(const psc_dev_batch *db, uint64_t g,
+                          psc_dev_graph **out);
+
 #ifdef __cplusplus
 }
 #endif

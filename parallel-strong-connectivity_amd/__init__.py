@@ -123,6 +123,29 @@ _sig("psc_dev_graph_download", C.c_int, C.POINTER(CDevGraph),
      C.POINTER(C.POINTER(CHostGraph)))
 
 
+class CBatchDesc(C.Structure):
+    _fields_ = [("num_graphs", C.c_uint64), ("vbase", u64p), ("ebase", u64p),
+                ("offset", u64p), ("edges", u32p), ("in_offset", u64p),
+                ("in_edges", u32p)]
+
+
+_sig("psc_batch_create", C.c_int, C.POINTER(CBatchDesc), C.c_int,
+     C.POINTER(C.c_void_p))
+_sig("psc_handle_graphs", C.c_uint64, C.c_void_p)
+_sig("psc_handle_vbase", C.c_int, C.c_void_p, u64p)
+_sig("psc_batch_run", C.c_int, C.c_void_p, C.POINTER(CParams),
+     C.POINTER(C.c_double), u64p, C.POINTER(CStats))
+_sig("psc_batch_spec", C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_uint64,
+     C.c_uint32, u32p, u64p)
+_sig("psc_dev_gen_batch", C.c_int, C.c_uint64, u32p, u64p, C.c_uint64, u64p,
+     C.POINTER(C.c_void_p))
+_sig("psc_dev_batch_free", None, C.c_void_p)
+_sig("psc_dev_batch_info", C.c_int, C.c_void_p, u64p, u64p, u64p, u32p)
+_sig("psc_batch_create_dev", C.c_int, C.c_void_p, C.POINTER(C.c_void_p))
+_sig("psc_dev_batch_extract", C.c_int, C.c_void_p, C.c_uint64,
+     C.POINTER(C.POINTER(CDevGraph)))
+
+
 def _check(where, rc):
     if rc != PSC_OK:
         raise PscError(where, rc, lib.psc_last_error().decode(errors="replace"))
@@ -388,3 +411,129 @@ def status(labels):
         return 0, 0
     _, counts = np.unique(labels, return_counts=True)
     return int(counts.size), int(counts.max())
+
+
+M64 = (1 << 64) - 1
+
+
+def graph_seed(seed, g):
+    """Seed of graph g of a generated batch: mix(seed * phi + g + 1)."""
+    z = (seed * 0x9E3779B97F4A7C15 + g + 1) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def batch_spec(num_graphs, lo_log2=10, hi_log2=20, seed=4, kinds=(0, 1, 2)):
+    """Seeded mix of graph kinds (0 RMAT, 1 grid, 2 3D k-NN) and sizes."""
+    kind = np.zeros(num_graphs, dtype=np.uint32)
+    n = np.zeros(num_graphs, dtype=np.uint64)
+    mask = sum(1 << k for k in kinds)
+    _check("psc_batch_spec", lib.psc_batch_spec(
+        num_graphs, lo_log2, hi_log2, seed, mask, _ptr(kind, u32p), _ptr(n, u64p)))
+    return kind, n
+
+
+class DeviceBatch:
+    """A batch of graphs generated straight into HBM as one disjoint union."""
+
+    def __init__(self, kind, n, seed=4, seeds=None):
+        kind = np.ascontiguousarray(kind, dtype=np.uint32)
+        n = np.ascontiguousarray(n, dtype=np.uint64)
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
+        p = C.c_void_p()
+        _check("psc_dev_gen_batch", lib.psc_dev_gen_batch(
+            kind.size, _ptr(kind, u32p), _ptr(n, u64p), seed,
+            _ptr(sd, u64p) if sd is not None else None, C.byref(p)))
+        self._p = p
+        self.seed = seed
+        G = kind.size
+        self.num_graphs = G
+        self.vbase = np.zeros(G + 1, dtype=np.uint64)
+        self.ebase = np.zeros(G + 1, dtype=np.uint64)
+        self.kind = np.zeros(G, dtype=np.uint32)
+        cnt = C.c_uint64()
+        _check("psc_dev_batch_info", lib.psc_dev_batch_info(
+            self._p, C.byref(cnt), _ptr(self.vbase, u64p), _ptr(self.ebase, u64p),
+            _ptr(self.kind, u32p)))
+        self.n = int(self.vbase[-1])
+        self.m = int(self.ebase[-1])
+
+    def extract(self, g):
+        dgp = C.POINTER(CDevGraph)()
+        _check("psc_dev_batch_extract", lib.psc_dev_batch_extract(self._p, g,
+                                                                  C.byref(dgp)))
+        return DeviceGraph(dgp)
+
+    def free(self):
+        if self._p:
+            lib.psc_dev_batch_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BatchSCC:
+    """SCC of many graphs in one handle; labels per graph equal a
+    stand-alone SCC(graph).scc() with the same beta and seed."""
+
+    def __init__(self, graphs=None, device_batch=None):
+        h = C.c_void_p()
+        if device_batch is not None:
+            _check("psc_batch_create_dev", lib.psc_batch_create_dev(
+                device_batch._p, C.byref(h)))
+            self._keep = device_batch
+        else:
+            G = len(graphs)
+            vbase = np.zeros(G + 1, dtype=np.uint64)
+            ebase = np.zeros(G + 1, dtype=np.uint64)
+            vbase[1:] = np.cumsum([g.n for g in graphs])
+            ebase[1:] = np.cumsum([g.m for g in graphs])
+            cat = lambda xs, dt: np.ascontiguousarray(
+                np.concatenate(xs) if xs else np.zeros(0), dtype=dt)
+            off = cat([g.offset for g in graphs], np.uint64)
+            ioff = cat([g.in_offset for g in graphs], np.uint64)
+            e = cat([g.edges for g in graphs], np.uint32)
+            ie = cat([g.in_edges for g in graphs], np.uint32)
+            d = CBatchDesc(G, _ptr(vbase, u64p), _ptr(ebase, u64p), _ptr(off, u64p),
+                           _ptr(e, u32p), _ptr(ioff, u64p), _ptr(ie, u32p))
+            _check("psc_batch_create", lib.psc_batch_create(C.byref(d), 0, C.byref(h)))
+            self._keep = None
+        self._h = h
+        G = lib.psc_handle_graphs(h)
+        self.num_graphs = int(G)
+        self.vbase = np.zeros(G + 1, dtype=np.uint64)
+        _check("psc_handle_vbase", lib.psc_handle_vbase(h, _ptr(self.vbase, u64p)))
+        self.n = int(self.vbase[-1])
+        self.stats = None
+
+    def run(self, beta=1.5, beta_per_graph=None, labels=True, **kw):
+        p = default_params(beta=beta, **kw)
+        out = np.empty(self.n, dtype=np.uint64) if labels else None
+        bp = None
+        if beta_per_graph is not None:
+            bpg = np.ascontiguousarray(beta_per_graph, dtype=np.float64)
+            bp = bpg.ctypes.data_as(C.POINTER(C.c_double))
+        st = CStats()
+        _check("psc_batch_run", lib.psc_batch_run(
+            self._h, C.byref(p), bp, _ptr(out, u64p) if labels else None, C.byref(st)))
+        self.stats = st.as_dict()
+        return out
+
+    def split(self, labels):
+        return [labels[self.vbase[g]:self.vbase[g + 1]] for g in range(self.num_graphs)]
+
+    def close(self):
+        if self._h:
+            lib.psc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

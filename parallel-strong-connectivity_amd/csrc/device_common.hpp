@@ -163,6 +163,17 @@ __device__ __forceinline__ bool table_contains(const unsigned long long *T,
   return false;
 }
 
+// largest i in [0, cnt) with pre[i] <= x (pre non-decreasing, pre[0] <= x)
+__device__ __forceinline__ uint32_t seg_of(const unsigned long long *pre,
+                                           uint32_t cnt, uint64_t x) {
+  uint32_t lo = 0, hi = cnt - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Seeded bijection of [0, N) (cycle-walking a multiply/xorshift bijection
 // of the enclosing power of two); stands in for parlay::random_shuffle of
 // the non-trimmed vertices (src/scc.hpp:303).

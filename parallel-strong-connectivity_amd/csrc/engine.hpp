@@ -65,20 +65,35 @@ struct Engine {
   std::vector<int> pkind;
   KProf kprof[KK_N];
   double kbytes[KK_N] = {};
+  // graphs held by this handle: graph g owns vertices [voff[g], voff[g+1])
+  uint32_t G = 1;
+  std::vector<uint64_t> voff_h;
+  std::vector<double> beta_g;     // per-graph pivot-batch growth (batches)
+  DevBuf voff_d, gA, gB, gC, gD, gid_d;
+  unsigned long long *h_gpin = nullptr;   // pinned per-graph staging
+  size_t h_gpin_cap = 0;
   uint64_t log_n[2] = {0, 0};
+  uint32_t ms_levels_hint = 2;   // sub-rounds of the previous multi-search
   uint64_t syncs = 0;
   unsigned char h_reach[256];
   unsigned char h_ms[256];
 
   ~Engine();
   std::vector<DevBuf *> all_bufs();
-  int init(const psc_graph *dg, bool owned);
+  int init(const psc_graph *dg, bool owned, uint32_t ngraphs = 1,
+           const uint64_t *voff = nullptr);
+  static constexpr int NREG = 6;
+  int gpin(size_t count);
+  unsigned long long *region(int r);
+  int upload(DevBuf &dst, const uint64_t *src, size_t count, int reg);
+  int upload32(DevBuf &dst, const uint32_t *src, size_t count, int reg);
   int run(const psc_params &p, unsigned long long *d_out, psc_stats *st);
   int status(const unsigned long long *d_labels, uint64_t *n_scc,
              uint64_t *largest);
 
   template <class F>
-  int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count);
+  int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count,
+             unsigned long long *count_dev = nullptr);
   int poll(void *dst, const void *src, size_t bytes);
   int prof_reset(int level);
   void prof_begin(int kind);
@@ -86,8 +101,13 @@ struct Engine {
   int prof_collect();
   int first_reach(const psc_params &p, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
-  int multi_search(const psc_params &p, uint64_t nfront, uint64_t label_offset,
+  int multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
                    uint64_t slots[2], psc_stats *st);
 };
 
 }  // namespace psc
+
+// The opaque handle of the C ABI.
+struct psc_handle {
+  psc::Engine eng;
+};

@@ -140,6 +140,20 @@ struct SelNonZero {
   }
   __device__ uint32_t val(uint64_t v) const { return (uint32_t)v; }
 };
+// pred: not yet in a found SCC, over the concatenation of per-graph windows
+// of the centers list (src/scc.hpp:384-388): window a covers
+// centers[wsrc[a], wsrc[a] + wpre[a+1] - wpre[a]).
+struct SelWindow {
+  const uint32_t *centers;
+  const unsigned long long *wpre, *wsrc;
+  uint32_t A;
+  const unsigned long long *labels;
+  __device__ uint32_t val(uint64_t t) const {
+    const uint32_t a = seg_of(wpre, A, t);
+    return centers[wsrc[a] + (t - wpre[a])];
+  }
+  __device__ bool pred(uint64_t t) const { return !(labels[val(t)] & TOP_BIT); }
+};
 // pred: not yet in a found SCC; select from a vertex list (src/scc.hpp:349,384-388)
 struct SelNotDone {
   const uint32_t *src;
@@ -150,43 +164,120 @@ struct SelNotDone {
   __device__ uint32_t val(uint64_t i) const { return src[i]; }
 };
 
-// ----------------------------------------------------------- trim kernels
+// ------------------------------------------------------- per-graph helpers
+// A handle holds G graphs as one disjoint union: graph g owns the vertex id
+// range [voff[g], voff[g+1]).  Per-graph lists (trimmed vertices, pivot
+// order, SCC centers, round fronts) are stored graph by graph.
+
+// start[a] = first index i of `list` (grouped by graph, graphs ascending)
+// whose vertex belongs to graph gid[a] or a later one; start[A] = len.
+__global__ void k_segment_starts(const uint32_t *list, uint64_t len,
+                                 const unsigned long long *len_ptr,
+                                 const unsigned long long *voff, uint32_t G,
+                                 const uint32_t *gid, uint32_t A,
+                                 unsigned long long *start) {
+  if (len_ptr) len = *len_ptr;
+  for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a <= A;
+       a += (uint64_t)gridDim.x * blockDim.x) {
+    if (a == A) {
+      start[a] = len;
+      continue;
+    }
+    const uint32_t g = gid ? gid[a] : (uint32_t)a;
+    uint64_t lo = 0, hi = len;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (seg_of(voff, G, list[mid]) < g) lo = mid + 1; else hi = mid;
+    }
+    start[a] = lo;
+  }
+}
+
+// Trimmed vertices get 1 + (rank within their graph | TOP_BIT)
+// (src/scc.hpp:304-305).
 __global__ void k_trim_labels(const uint32_t *Z, uint64_t nz,
+                              const unsigned long long *voff, uint32_t G,
+                              const unsigned long long *zstart,
                               unsigned long long *labels) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nz;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    labels[Z[i]] = 1 + (i | TOP_BIT);   // src/scc.hpp:304-305
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = Z[i];
+    const uint32_t g = seg_of(voff, G, v);
+    labels[v] = 1 + ((i - zstart[g]) | TOP_BIT);
+  }
 }
-__global__ void k_permute(const uint32_t *NZ, uint64_t nnz, PivotPerm perm,
-                          uint32_t *P) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nnz;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    P[i] = NZ[perm(i)];
+
+// P = per-graph seeded permutation of the non-trimmed vertices
+// (parlay::random_shuffle, src/scc.hpp:303).
+__global__ void k_permute(const uint32_t *NZ, uint64_t nnz,
+                          const unsigned long long *nstart, uint32_t G,
+                          uint64_t seed, uint32_t *P) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nnz;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = seg_of(nstart, G, j);
+    const uint64_t b = nstart[g];
+    const PivotPerm perm(nstart[g + 1] - b, seed);
+    P[j] = NZ[b + perm(j - b)];
+  }
 }
 
 // ----------------------------------------------------------- reach kernels
-// Level 0 state: source visited, frontier {source}; both searches.
-__global__ void k_reach_init(ReachSearch *S, ReachCtl *C, int ns,
-                             const uint32_t *src_ptr, uint64_t words) {
-  const uint32_t src = *src_ptr;
+// Dead ends are pre-marked visited: in a search that pushes along CSR
+// `off`, a vertex with no edges in `off` (a sink of the forward search, a
+// source of the backward one) is a trimmed vertex that cannot lead anywhere,
+// so it is neither appended nor scanned.  It also never appears as a pull
+// neighbour (pull follows the opposite CSR, where it would need an edge in
+// `off`), so "visited" keeps meaning "reached" wherever a pull step reads
+// it.  First-round labels only concern the non-trimmed vertices
+// (src/scc.hpp:341-347), whose reach sets this does not change.
+// k_reach_reset starts each visited bitmap from its dead-end set and clears
+// the level control of both searches.
+__global__ void k_reach_reset(ReachSearch *S, ReachCtl *C, int ns,
+                              uint64_t words) {
   for (int s = 0; s < ns; ++s) {
-    uint32_t *vis = S[s].vis;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-         i < words; i += (uint64_t)gridDim.x * blockDim.x)
-      vis[i] = (i == (src >> 5)) ? (1u << (src & 31)) : 0u;
+    const ReachSearch R = S[s];
+    const uint64_t n = R.n;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+         w < words; w += (uint64_t)gridDim.x * blockDim.x) {
+      uint32_t bits = 0;
+      const uint64_t v0 = w * 32;
+      const uint64_t cnt = min((uint64_t)32, n - v0);
+      uint64_t o = R.off[v0];
+      for (uint64_t k = 0; k < cnt; ++k) {
+        const uint64_t o2 = R.off[v0 + k + 1];
+        if (o2 == o) bits |= 1u << k;
+        o = o2;
+      }
+      R.vis[w] = bits;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (int s = 0; s < ns; ++s) {
-      S[s].q[0][0] = src;
       ReachSlot z{};
-      C->slot[s][1] = z;
-      C->slot[s][2] = z;
-      z.qsize = 1;
-      z.degsum = S[s].off[src + 1] - S[s].off[src];
-      C->slot[s][0] = z;
+      for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
       C->done[s] = 0;
       C->levels[s] = 0;
       for (int c = 0; c < NRC; ++c) C->cnt[s][c] = 0;
+    }
+  }
+}
+
+// Level 0: the first pivot P_g[0] of every graph with non-trimmed vertices
+// (src/scc.hpp:317); the graphs are disjoint, so one multi-source reach
+// gives each graph's single reach from its own pivot.
+__global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
+                                const uint32_t *P,
+                                const unsigned long long *nstart, uint32_t G) {
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    if (nstart[g + 1] == nstart[g]) continue;
+    const uint32_t src = P[nstart[g]];
+    for (int s = 0; s < ns; ++s) {
+      atomicOr(&S[s].vis[src >> 5], 1u << (src & 31));
+      const uint32_t idx = atomicAdd(&C->slot[s][0].qsize, 1u);
+      S[s].q[0][idx] = src;
+      atomicAdd(&C->slot[s][0].degsum,
+                (unsigned long long)(S[s].off[src + 1] - S[s].off[src]));
     }
   }
 }
@@ -225,6 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
   __shared__ uint32_t ring_s[WPB][REACH_RING];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
+  __shared__ uint32_t fl_s[WPB][WAVE];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *ring = ring_s[w];
   uint32_t *sc = sc_s[w];
@@ -252,12 +344,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       uint32_t sd = 64;
       if (vgc) sd = min(64u, max(1u, nf / total_waves));
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
+      // first chunk of each wave is static (no atomic), the rest dynamic
+      uint64_t g = gw * sd;
       while (true) {
-        uint32_t g = 0;
-        if (l == 0) g = atomicAdd(&cur->grab, sd);
-        g = __shfl(g, 0);
         if (g >= nf) break;
-        const uint32_t cnt = min(sd, nf - g);
+        const uint32_t cnt = (uint32_t)min((uint64_t)sd, nf - g);
         if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
         uint32_t head = 0, tail = cnt;
         uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
@@ -332,6 +423,9 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
           }
           wave_sync();
         }
+        uint32_t gd = 0;
+        if (l == 0) gd = atomicAdd(&cur->grab, sd);
+        g = (uint64_t)__shfl(gd, 0) + (uint64_t)total_waves * sd;
       }
       add_cnt(&nxt->degsum, dsum);
       add_cnt(&C->cnt[s][R_PUSH_EDGES], n_edges);
@@ -341,21 +435,77 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
     } else if (mode == MODE_PULL) {
       if (blockIdx.x == 0 && threadIdx.x == 0) nxt->from_pull = 1;
       const uint64_t units = (R.n + 63) / 64;
+      uint32_t *fl = fl_s[w];   // per-owner "resolved" flags
       unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_units = 0;
       for (uint64_t t = gw; t < units; t += nw) {
         const uint64_t v = t * 64 + l;
-        bool found = false;
         ++n_units;
+        bool found = false;
+        uint64_t b = 0, deg = 0;
         if (v < R.n && !bit_test(R.vis, (uint32_t)v)) {
           ++n_scan;
-          const uint64_t b = R.roff[v], e = R.roff[v + 1];
-          for (uint64_t j = b; j < e; ++j) {
+          b = R.roff[v];
+          deg = R.roff[v + 1] - b;
+          // most vertices of a dense level meet a visited in-neighbour
+          // among their first few in-edges
+          const uint64_t k = min(deg, (uint64_t)PULL_PROBE);
+          for (uint64_t j = 0; j < k; ++j) {
             ++n_edges;
-            if (bit_test(R.vis, R.radj[j])) {
+            if (bit_test(R.vis, R.radj[b + j])) {
               found = true;
               break;
             }
           }
+          b += k;
+          deg = found ? 0 : deg - k;
+        }
+        // long remainders go to the heavy list (k_reach_heavy, chunked)
+        const bool heavy = deg > HEAVY_DEG;
+        const uint64_t hm = ballot(heavy);
+        if (hm) {
+          const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
+          const unsigned long long cinc = wave_incl_scan(nch);
+          const unsigned long long ctot = __shfl(cinc, WAVE - 1);
+          unsigned long long old = 0;
+          if (l == 0)
+            old = atomicAdd(&cur->heavy,
+                            ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
+          old = __shfl(old, 0);
+          if (heavy) {
+            const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
+            R.hv[idx] = (uint32_t)v;
+            R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
+            deg = 0;
+          }
+        }
+        // the rest: load-balanced 64-edge windows over the remainders of
+        // the wave's 64 vertices, skipping owners already resolved
+        const uint32_t inc = wave_incl_scan((uint32_t)deg);
+        const uint32_t total = __shfl(inc, WAVE - 1);
+        if (total) {
+          sc[l] = inc;
+          st[l] = b;
+          fl[l] = 0;
+          wave_sync();
+          for (uint32_t base = 0; base < total; base += WAVE) {
+            const uint32_t e = base + l;
+            if (e < total) {
+              int lo = 0, hi = WAVE - 1;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sc[mid] > e) hi = mid; else lo = mid + 1;
+              }
+              const uint32_t ex = lo ? sc[lo - 1] : 0u;
+              if (!fl[lo]) {
+                ++n_edges;
+                if (bit_test(R.vis, R.radj[st[lo] + (e - ex)])) fl[lo] = 1;
+              }
+            }
+            wave_sync();
+            if (!ballot(deg > 0 && !fl[l])) break;
+          }
+          if (deg > 0 && fl[l]) found = true;
+          wave_sync();
         }
         const uint64_t fm = ballot(found);
         if (!fm) continue;
@@ -373,15 +523,20 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
   }
 }
 
-// Edge-parallel expansion of the heavy vertices of a push level
-// (src/reach.hpp:155-167, granularity 2048): work items are CHUNK-edge
-// slices; a wave finds the owning vertex by binary search over hc.
+// Heavy vertices of a level, CHUNK-edge work items spread over all waves
+// (a wave finds the owning vertex by binary search over hc).
+// Push level: edge-parallel expansion (src/reach.hpp:155-167, granularity
+// 2048).  Pull level: the in-edge remainder of an unvisited vertex with a
+// long in-list; the first chunk to meet a visited in-neighbour sets the
+// vertex's bit and appends it, the other chunks of that vertex stop early.
 __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
                                                        ReachCtl *C, int ns,
-                                                       uint32_t L) {
+                                                       uint32_t L,
+                                                       uint32_t total_waves) {
   const int l = lane();
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   for (int s = 0; s < ns; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
@@ -389,12 +544,13 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0) continue;
     const ReachSearch R = S[s];
+    const bool pull = reach_mode(cur, R) == MODE_PULL;
+    const uint64_t *off = pull ? R.roff : R.off;
+    const uint32_t *adj = pull ? R.radj : R.adj;
+    const uint64_t skip = pull ? PULL_PROBE : 0;
     unsigned long long n_edges = 0, n_vis = 0, n_app = 0, n_items = 0, dsum = 0;
-    while (true) {
-      unsigned long long c = 0;
-      if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
-      c = __shfl(c, 0);
-      if (c >= Ct) break;
+    uint64_t c = gw;   // first item static, then dynamic
+    while (c < Ct) {
       if (l == 0) ++n_items;
       uint64_t lo = 0, hi = H - 1;
       while (lo < hi) {   // last idx with hc[idx] <= c
@@ -403,25 +559,50 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
       }
       const uint32_t u = R.hv[lo];
       const uint64_t k = c - R.hc[lo];
-      const uint64_t eb = R.off[u] + k * CHUNK;
-      const uint64_t ee = min(R.off[u + 1], eb + CHUNK);
-      for (uint64_t j0 = eb; j0 < ee; j0 += WAVE) {
-        const uint64_t j = j0 + l;
-        bool found = false;
-        uint32_t v = 0;
-        if (j < ee) {
-          v = R.adj[j];
-          found = bit_set(R.vis, v);
-          ++n_edges;
+      const uint64_t eb = off[u] + skip + k * CHUNK;
+      const uint64_t ee = min(off[u + 1], eb + CHUNK);
+      if (!pull) {
+        for (uint64_t j0 = eb; j0 < ee; j0 += WAVE) {
+          const uint64_t j = j0 + l;
+          bool found = false;
+          uint32_t v = 0;
+          if (j < ee) {
+            v = adj[j];
+            found = bit_set(R.vis, v);
+            ++n_edges;
+          }
+          n_vis += found;
+          const uint32_t pos = wave_append(&nxt->qsize, found);
+          if (found) {
+            R.q[p ^ 1][pos] = v;
+            dsum += R.off[v + 1] - R.off[v];
+            ++n_app;
+          }
         }
-        n_vis += found;
-        const uint32_t pos = wave_append(&nxt->qsize, found);
-        if (found) {
-          R.q[p ^ 1][pos] = v;
-          dsum += R.off[v + 1] - R.off[v];
-          ++n_app;
+      } else if (!bit_test(R.vis, u)) {
+        bool hit = false;
+        for (uint64_t j0 = eb; j0 < ee && !hit; j0 += WAVE) {
+          const uint64_t j = j0 + l;
+          bool h = false;
+          if (j < ee) {
+            ++n_edges;
+            h = bit_test(R.vis, adj[j]);
+          }
+          hit = ballot(h) != 0;
+        }
+        if (hit && l == 0) {
+          const uint32_t bit = 1u << (u & 31);
+          if (!(atomicOr(&R.vis[u >> 5], bit) & bit)) {
+            const uint32_t pos = atomicAdd(&nxt->qsize, 1u);
+            R.q[p ^ 1][pos] = u;
+            ++n_vis;
+            ++n_app;
+          }
         }
       }
+      unsigned long long cd = 0;
+      if (l == 0) cd = atomicAdd(&cur->hgrab, 1ull);
+      c = __shfl(cd, 0) + total_waves;
     }
     add_cnt(&nxt->degsum, dsum);
     add_cnt(&C->cnt[s][R_HEAVY_EDGES], n_edges);
@@ -432,22 +613,26 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
 }
 
 // First-SCC labelling (src/scc.hpp:340-347): both reaches -> found SCC,
-// one reach -> new subproblem label.
+// one reach -> new subproblem label; lab1[g] = |trimmed of g| + 1.
 __global__ void k_first_labels(const uint32_t *P, uint64_t np,
+                               const unsigned long long *nstart, uint32_t G,
                                const uint32_t *vf, const uint32_t *vb,
                                unsigned long long *labels,
-                               unsigned long long label,
+                               const unsigned long long *lab1,
                                unsigned long long *big) {
   unsigned long long c = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < np;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = P[i];
     const bool a = bit_test(vf, v), b = bit_test(vb, v);
-    if (a && b) {
-      labels[v] = label | TOP_BIT;
-      ++c;
-    } else if (a || b) {
-      labels[v] = label;
+    if (a || b) {
+      const unsigned long long label = lab1[seg_of(nstart, G, i)];
+      if (a && b) {
+        labels[v] = label | TOP_BIT;
+        ++c;
+      } else {
+        labels[v] = label;
+      }
     }
   }
   c = wave_sum(c);
@@ -465,16 +650,23 @@ __global__ void k_ms_reset(MSCtl *C, int s) {
   }
 }
 
-// Seed (pivot_i, label_offset + i) into table, log and frontier
-// (src/scc.hpp:147-150).
+// Seed (pivot_i, label_offset_g + i) into table, log and frontier
+// (src/scc.hpp:147-150); i counts the pivots of graph g in this round, whose
+// front segment starts at fstart[a] (a = the graph's window, g = wgid[a]).
+// The front size is read on the device.
 __global__ void k_ms_seed(MSSearch *S, MSCtl *C, int s, const uint32_t *front,
-                          uint64_t nfront, unsigned long long label_offset) {
+                          const unsigned long long *nfront_ptr,
+                          const unsigned long long *fstart, uint32_t A,
+                          const uint32_t *wgid,
+                          const unsigned long long *lab_off) {
   const MSSearch M = S[s];
+  const uint64_t nfront = *nfront_ptr;
   uint32_t probes = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nfront;
        i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = seg_of(fstart, A, i);
     const uint32_t v = front[i];
-    const uint32_t lab = (uint32_t)(label_offset + i);
+    const uint32_t lab = (uint32_t)(lab_off[wgid[a]] + (i - fstart[a]));
     const int r = table_insert(M.table, M.mask, v, lab, M.probe_limit, &probes);
     if (r < 0) C->full[s] = 1;
     M.log[i] = make_pair(v, lab);
@@ -485,6 +677,14 @@ __global__ void k_ms_seed(MSSearch *S, MSCtl *C, int s, const uint32_t *front,
     C->log_n[s] = nfront;
     C->cnt[s][M_INS] = nfront;
   }
+}
+
+// label_offset_g += pivots of g in this round (src/scc.hpp:425)
+__global__ void k_label_advance(const unsigned long long *fstart, uint32_t A,
+                                const uint32_t *wgid, unsigned long long *lab_off) {
+  for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < A;
+       a += (uint64_t)gridDim.x * blockDim.x)
+    lab_off[wgid[a]] += fstart[a + 1] - fstart[a];
 }
 
 // Record a newly inserted pair in the log; flags overflow when the log
@@ -540,10 +740,9 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     if (vgc) sd = min(64ull, max(1ull, nf / total_waves));
     unsigned long long n_edges = 0, n_exp = 0, n_ins = 0, n_app = 0;
     uint32_t probes = 0;
+    const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+    unsigned long long g = gw * sd;   // first chunk static, then dynamic
     while (true) {
-      unsigned long long g = 0;
-      if (l == 0) g = atomicAdd(&cur->grab, sd);
-      g = __shfl(g, 0);
       if (g >= nf) break;
       if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
       const uint32_t cnt = (uint32_t)min(sd, nf - g);
@@ -634,6 +833,9 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
         }
         wave_sync();
       }
+      unsigned long long gd = 0;
+      if (l == 0) gd = atomicAdd(&cur->grab, sd);
+      g = __shfl(gd, 0) + (unsigned long long)total_waves * sd;
     }
     add_cnt(&C->cnt[s][M_EDGES], n_edges);
     add_cnt(&C->cnt[s][M_EXP], n_exp);
@@ -646,7 +848,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
 // Heavy pairs of this sub-round: CHUNK-edge slices over all waves.
 __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const MSSearch *S, MSCtl *C, int ns, uint32_t L,
-    const unsigned long long *__restrict__ labels) {
+    const unsigned long long *__restrict__ labels, uint32_t total_waves) {
   const int l = lane();
   const int ci = L % 3, ni = (L + 1) % 3;
   const int pq = L & 1;
@@ -660,11 +862,9 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const MSSearch M = S[s];
     unsigned long long n_edges = 0, n_ins = 0, n_app = 0, n_items = 0;
     uint32_t probes = 0;
-    while (true) {
-      unsigned long long c = 0;
-      if (l == 0) c = atomicAdd(&cur->hgrab, 1ull);
-      c = __shfl(c, 0);
-      if (c >= Ct) break;
+    const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+    unsigned long long c = gw;   // first item static, then dynamic
+    while (c < Ct) {
       if (l == 0) ++n_items;
       uint64_t lo = 0, hi = H - 1;
       while (lo < hi) {
@@ -702,6 +902,9 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
           ++n_app;
         }
       }
+      unsigned long long cd = 0;
+      if (l == 0) cd = atomicAdd(&cur->hgrab, 1ull);
+      c = __shfl(cd, 0) + total_waves;
     }
     add_cnt(&C->cnt[s][M_HEAVY_EDGES], n_edges);
     add_cnt(&C->cnt[s][M_HEAVY_INS], n_ins);
@@ -748,17 +951,25 @@ __global__ void k_finalize(unsigned long long *labels, uint64_t n) {
        i += (uint64_t)gridDim.x * blockDim.x)
     labels[i] = (labels[i] & VAL_MASK) - 1;   // src/scc.hpp:475-476
 }
+// Canonical labels: minimum local vertex id per (graph, label); labels
+// are in [0, n_g], so (voff[g] + g + label) indexes a table of n + G slots.
 __global__ void k_canon_min(const unsigned long long *labels, uint64_t n,
+                            const unsigned long long *voff, uint32_t G,
                             uint32_t *minv) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    atomicMin(&minv[labels[i]], (uint32_t)i);
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = seg_of(voff, G, i);
+    atomicMin(&minv[voff[g] + g + labels[i]], (uint32_t)(i - voff[g]));
+  }
 }
 __global__ void k_canon_apply(unsigned long long *labels, uint64_t n,
+                              const unsigned long long *voff, uint32_t G,
                               const uint32_t *minv) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    labels[i] = minv[labels[i]];
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = seg_of(voff, G, i);
+    labels[i] = minv[voff[g] + g + labels[i]];
+  }
 }
 __global__ void k_count_labels(const unsigned long long *labels, uint64_t n,
                                uint32_t *cnt, unsigned long long *bad) {
@@ -845,6 +1056,7 @@ Engine::~Engine() {
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
   if (h_pinned) (void)hipHostFree(h_pinned);
+  if (h_gpin) (void)hipHostFree(h_gpin);
   for (auto &e : ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : pev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -856,12 +1068,21 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &front, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
-          &mhc[0], &mhc[1], &scratch32};
+          &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
+          &gid_d};
 }
 
-int Engine::init(const psc_graph *dg, bool owned) {
+int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
+                 const uint64_t *voff) {
   n = dg->n;
   m = dg->m;
+  G = ngraphs ? ngraphs : 1;
+  voff_h.assign(G + 1, 0);
+  if (voff) {
+    for (uint32_t g = 0; g <= G; ++g) voff_h[g] = voff[g];
+  } else {
+    voff_h[G] = n;
+  }
   owned_graph = owned;
   d_off = dg->offset;
   d_adj = dg->edges;
@@ -909,7 +1130,11 @@ int Engine::init(const psc_graph *dg, bool owned) {
   PSC_TRY(ms_desc.ensure(sizeof(MSSearch) * 2));
   PSC_TRY(ms_ctl.ensure(sizeof(MSCtl)));
   PSC_TRY(misc.ensure(256));
-  PSC_TRY(scratch32.ensure((nn + 1) * 4));
+  PSC_TRY(scratch32.ensure((nn + G) * 4));
+  for (DevBuf *b : {&voff_d, &gA, &gB, &gC, &gD}) PSC_TRY(b->ensure((G + 1) * 8));
+  PSC_TRY(gid_d.ensure((G + 1) * 4));
+  PSC_TRY(gpin(G + 2));
+  PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
 
   ReachSearch rs[2];
   for (int s = 0; s < 2; ++s) {
@@ -934,13 +1159,15 @@ int Engine::init(const psc_graph *dg, bool owned) {
 // Stable select over [0, len) with functor f; count -> host.
 template <class F>
 int Engine::select(F f, uint64_t len, uint32_t *sel, uint32_t *rej,
-                   uint64_t *count) {
+                   uint64_t *count, unsigned long long *count_dev) {
   if (len == 0) {
-    *count = 0;
+    if (count) *count = 0;
+    if (count_dev)
+      k_fill_u64<<<1, 64, 0, stream>>>(count_dev, 1, 0ull);
     return PSC_OK;
   }
   const uint64_t nt = (len + SEL_TILE - 1) / SEL_TILE;
-  unsigned long long *tot = misc.as<unsigned long long>();
+  unsigned long long *tot = count_dev ? count_dev : misc.as<unsigned long long>();
   prof_begin(KK_SELECT);
   k_sel_count<F><<<(unsigned)nt, BLOCK, 0, stream>>>(f, len, tiles.as<uint32_t>());
   k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
@@ -949,15 +1176,54 @@ int Engine::select(F f, uint64_t len, uint32_t *sel, uint32_t *rej,
       f, len, tile_off.as<unsigned long long>(), sel, rej);
   prof_end();
   PSC_HIP(hipGetLastError());
-  PSC_HIP(hipMemcpyAsync(h_pinned, tot, 8, hipMemcpyDeviceToHost, stream));
+  if (count_dev) return PSC_OK;   // count stays on the device, no sync
+  return poll(count, tot, 8);
+}
+
+// Pinned staging for per-graph arrays: NREG regions of (G + 2) u64.
+// Region 0 serves polls; each uploaded array of a round has its own region.
+// Every round polls the device after its uploads, so a region is never
+// rewritten while its previous copy is still pending.
+int Engine::gpin(size_t count) {
+  const size_t want = count * NREG;
+  if (want <= h_gpin_cap) return PSC_OK;
   PSC_HIP(hipStreamSynchronize(stream));
-  ++syncs;
-  *count = *(uint64_t *)h_pinned;
+  if (h_gpin) (void)hipHostFree(h_gpin);
+  h_gpin = nullptr;
+  h_gpin_cap = 0;
+  PSC_HIP(hipHostMalloc((void **)&h_gpin, want * 8));
+  h_gpin_cap = want;
+  return PSC_OK;
+}
+
+unsigned long long *Engine::region(int r) {
+  return h_gpin + (size_t)r * (h_gpin_cap / NREG);
+}
+
+int Engine::upload(DevBuf &dst, const uint64_t *src, size_t count, int reg) {
+  PSC_TRY(dst.ensure(std::max<size_t>(count, 1) * 8));
+  PSC_TRY(gpin(count));
+  std::memcpy(region(reg), src, count * 8);
+  PSC_HIP(hipMemcpyAsync(dst.ptr, region(reg), count * 8, hipMemcpyHostToDevice,
+                         stream));
+  return PSC_OK;
+}
+
+int Engine::upload32(DevBuf &dst, const uint32_t *src, size_t count, int reg) {
+  PSC_TRY(dst.ensure(std::max<size_t>(count, 1) * 4));
+  PSC_TRY(gpin((count + 1) / 2));
+  std::memcpy(region(reg), src, count * 4);
+  PSC_HIP(hipMemcpyAsync(dst.ptr, region(reg), count * 4, hipMemcpyHostToDevice,
+                         stream));
   return PSC_OK;
 }
 
 int Engine::poll(void *dst, const void *src, size_t bytes) {
   char *buf = (char *)h_pinned + PIN_POLL;
+  if (bytes > PIN_REACH - PIN_POLL) {
+    PSC_TRY(gpin((bytes + 7) / 8));
+    buf = (char *)region(0);
+  }
   PSC_HIP(hipMemcpyAsync(buf, src, bytes, hipMemcpyDeviceToHost, stream));
   PSC_HIP(hipStreamSynchronize(stream));
   ++syncs;
@@ -985,8 +1251,9 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   ReachCtl *C = reach_ctl.as<ReachCtl>();
   const uint64_t words = (n + 31) / 32;
   prof_begin(KK_MISC);
-  k_reach_init<<<grid_for(words), BLOCK, 0, stream>>>(S, C, 2, P.as<uint32_t>(),
-                                                       words);
+  k_reach_reset<<<grid_for(words, BLOCK, 8192), BLOCK, 0, stream>>>(S, C, 2, words);
+  k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
+      S, C, 2, P.as<uint32_t>(), gA.as<unsigned long long>(), G);
   prof_end();
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
@@ -1000,7 +1267,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
                                                       p.local_reach, total_waves);
       prof_end();
       prof_begin(KK_HEAVY);
-      k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L);
+      k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L,
+                                                       (uint32_t)grid_heavy * WPB);
       prof_end();
     }
     PSC_HIP(hipGetLastError());
@@ -1077,9 +1345,8 @@ static uint64_t pow2_at_least(uint64_t x) {
 // multi_search_safe for both directions (src/scc.hpp:256-275,395-423):
 // searches run fused; an overflowed search is rerun alone with a doubled
 // table.
-int Engine::multi_search(const psc_params &p, uint64_t nfront,
-                         uint64_t label_offset, uint64_t slots[2],
-                         psc_stats *st) {
+int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
+                         uint64_t slots[2], psc_stats *st) {
   MSSearch md[2];
   std::memcpy(md, h_ms, sizeof(md));
   MSSearch *S = ms_desc.as<MSSearch>();
@@ -1110,7 +1377,9 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
       prof_begin(KK_SEED);
       k_ms_reset<<<1, 64, 0, stream>>>(C, s);
       k_ms_seed<<<grid_for(nfront), BLOCK, 0, stream>>>(
-          S, C, s, front.as<uint32_t>(), nfront, label_offset);
+          S, C, s, front.as<uint32_t>(), misc.as<unsigned long long>() + 3,
+          gC.as<unsigned long long>(), A, gid_d.as<uint32_t>(),
+          gD.as<unsigned long long>());
       prof_end();
       kbytes[KK_SEED] += 4.0 * nfront + 32.0 * nfront;
     }
@@ -1120,7 +1389,7 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
                            hipMemcpyHostToDevice, stream));
     PSC_HIP(hipGetLastError());
     uint32_t L = 0;
-    int K = 2;
+    int K = (int)std::min<uint32_t>(std::max<uint32_t>(ms_levels_hint, 2), 32);
     MSCtl hc_;
     while (true) {
       for (int k = 0; k < K; ++k, ++L) {
@@ -1131,7 +1400,8 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
         prof_end();
         prof_begin(KK_MS_HEAVY);
         k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
-            S, C, 2, L, labels.as<unsigned long long>());
+            S, C, 2, L, labels.as<unsigned long long>(),
+            (uint32_t)grid_heavy * WPB);
         prof_end();
       }
       PSC_HIP(hipGetLastError());
@@ -1143,6 +1413,7 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront,
       K = std::min(K * 2, 32);
     }
     st->sub_rounds += L;
+    ms_levels_hint = L;
     for (int s = 0; s < 2; ++s) {
       if (!want[s]) continue;
       const unsigned long long *c = hc_.cnt[s];
@@ -1187,9 +1458,10 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   syncs = 0;
   PSC_TRY(prof_reset(p.timing));
   unsigned long long *lab = labels.as<unsigned long long>();
+  unsigned long long *voffd = voff_d.as<unsigned long long>();
   auto t_host0 = std::chrono::steady_clock::now();
   if (p.timing) PSC_HIP(hipEventRecord(ev[0], stream));
-  // init + trim (src/scc.hpp:284-306)
+  // ---- init + trim, per graph (src/scc.hpp:284-306)
   prof_begin(KK_MISC);
   k_fill_u64<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, 0ull);
   prof_end();
@@ -1200,67 +1472,122 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   kbytes[KK_SELECT] += 2 * 32.0 * n + 4.0 * n;
   nz = n - nnz;
   st->n_trimmed = nz;
+  // per-graph segment starts of NZ (gA) and Z (gB)
+  unsigned long long *nstart = gA.as<unsigned long long>();
+  unsigned long long *zstart = gB.as<unsigned long long>();
   prof_begin(KK_MISC);
+  k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
+      NZ.as<uint32_t>(), nnz, nullptr, voffd, G, nullptr, G, nstart);
+  k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
+      Z.as<uint32_t>(), nz, nullptr, voffd, G, nullptr, G, zstart);
   if (nz)
     k_trim_labels<<<grid_for(nz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        Z.as<uint32_t>(), nz, lab);
+        Z.as<uint32_t>(), nz, voffd, G, zstart, lab);
   if (nnz)
     k_permute<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        NZ.as<uint32_t>(), nnz, PivotPerm(nnz, p.seed), P.as<uint32_t>());
+        NZ.as<uint32_t>(), nnz, nstart, G, p.seed, P.as<uint32_t>());
   prof_end();
   kbytes[KK_MISC] += 12.0 * nz + 8.0 * nnz;
   PSC_HIP(hipGetLastError());
+  std::vector<uint64_t> h_nstart(G + 1), h_zstart(G + 1);
+  PSC_TRY(poll(h_nstart.data(), nstart, 8 * (G + 1)));
+  PSC_TRY(poll(h_zstart.data(), zstart, 8 * (G + 1)));
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
   st->bytes_label += 8 * n + 8 * nz;
   st->bytes_csr += 32 * n;
-  uint64_t label_offset = nz + 1;
+  // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1
+  std::vector<uint64_t> label_off(G);
+  for (uint32_t g = 0; g < G; ++g) label_off[g] = h_zstart[g + 1] - h_zstart[g] + 1;
   uint64_t n_rem = 0;
+  std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
+    // ---- first SCC of every graph (src/scc.hpp:316-355)
     PSC_TRY(first_reach(p, st));
+    PSC_TRY(upload(gC, label_off.data(), G, 5));
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
     prof_begin(KK_MISC);
     k_first_labels<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        P.as<uint32_t>(), nnz, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(),
-        lab, label_offset, big);
+        P.as<uint32_t>(), nnz, nstart, G, vis[0].as<uint32_t>(),
+        vis[1].as<uint32_t>(), lab, gC.as<unsigned long long>(), big);
     prof_end();
     kbytes[KK_MISC] += 4.0 * nnz + 8.0 * nnz + nnz / 4.0;
     PSC_HIP(hipGetLastError());
     PSC_TRY(select(SelNotDone{P.as<uint32_t>(), lab}, nnz,
                    centers.as<uint32_t>(), nullptr, &n_rem));
     kbytes[KK_SELECT] += 2 * 12.0 * nnz + 4.0 * n_rem;
+    prof_begin(KK_MISC);
+    k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
+        centers.as<uint32_t>(), n_rem, nullptr, voffd, G, nullptr, G,
+        gB.as<unsigned long long>());
+    prof_end();
+    PSC_TRY(poll(h_cstart.data(), gB.ptr, 8 * (G + 1)));
     uint64_t bigh = 0;
     PSC_TRY(poll(&bigh, big, 8));
     st->first_scc_size = bigh;
     st->bytes_label += 8 * nnz + nnz / 4;
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
-  label_offset++;
-  // multi-pivot rounds (src/scc.hpp:369-473)
-  uint64_t step = 1, cur = 0, finished = 0;
+  for (uint32_t g = 0; g < G; ++g) label_off[g]++;
+  PSC_TRY(upload(gD, label_off.data(), G, 4));
+  // ---- multi-pivot rounds (src/scc.hpp:369-473), every graph on its own
+  // pivot-batch schedule; the rounds of all graphs advance together
+  std::vector<uint64_t> cur(G, 0), stepg(G, 1);
+  std::vector<uint64_t> wpre, wsrc;
+  std::vector<uint32_t> wgid;
   uint64_t ne_prev[2] = {1, 1};
-  const double beta = p.beta > 0 ? p.beta : 1.5;
-  while (finished != n_rem) {
-    const uint64_t end = std::min(cur + step, n_rem);
-    const uint64_t vs = end - cur;
-    finished += vs;
-    step = (uint64_t)std::ceil((double)step * beta);
+  const double beta_def = p.beta > 0 ? p.beta : 1.5;
+  uint64_t live = 0;
+  for (uint32_t g = 0; g < G; ++g) live += (h_cstart[g + 1] > h_cstart[g]);
+  while (live) {
+    wpre.assign(1, 0);
+    wsrc.clear();
+    wgid.clear();
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint64_t rem = h_cstart[g + 1] - h_cstart[g];
+      if (cur[g] == rem) continue;
+      const uint64_t end = std::min(cur[g] + stepg[g], rem);
+      const double b = beta_g.empty() ? beta_def : beta_g[g];
+      stepg[g] = (uint64_t)std::ceil((double)stepg[g] * b);
+      wsrc.push_back(h_cstart[g] + cur[g]);
+      wpre.push_back(wpre.back() + (end - cur[g]));
+      wgid.push_back(g);
+      cur[g] = end;
+      if (cur[g] == rem) --live;
+    }
+    const uint32_t A = (uint32_t)wgid.size();
     st->rounds++;
-    uint64_t nfront = 0;
-    PSC_TRY(select(SelNotDone{centers.as<uint32_t>() + cur, lab}, vs,
-                   front.as<uint32_t>(), nullptr, &nfront));
-    kbytes[KK_SELECT] += 2 * 12.0 * vs + 4.0 * nfront;
-    cur = end;
-    if (nfront == 0) continue;
+    PSC_TRY(upload(gA, wpre.data(), A + 1, 1));
+    PSC_TRY(upload(gB, wsrc.data(), A, 2));
+    PSC_TRY(upload32(gid_d, wgid.data(), A, 3));
+    // front = not-done pivots of every window, grouped by graph; its size
+    // stays on the device (misc[3]) -- one host sync per round, in
+    // multi_search
+    unsigned long long *nfront_d = misc.as<unsigned long long>() + 3;
+    const uint64_t V = wpre[A];
+    PSC_TRY(select(SelWindow{centers.as<uint32_t>(), gA.as<unsigned long long>(),
+                             gB.as<unsigned long long>(), A, lab},
+                   V, front.as<uint32_t>(), nullptr, nullptr, nfront_d));
+    kbytes[KK_SELECT] += 2 * 12.0 * V;
+    prof_begin(KK_MISC);
+    k_segment_starts<<<grid_for(A + 1), BLOCK, 0, stream>>>(
+        front.as<uint32_t>(), 0, nfront_d, voffd, G, gid_d.as<uint32_t>(), A,
+        gC.as<unsigned long long>());
+    prof_end();
     uint64_t slots[2];
     for (int s = 0; s < 2; ++s) {
+      const uint64_t cap = 6000000ull * G;
       const uint64_t base = std::max<uint64_t>(
-          std::min<uint64_t>((uint64_t)std::ceil(0.3 * (double)n_rem), 6000000),
-          (uint64_t)beta * ne_prev[s]);
+          std::min<uint64_t>((uint64_t)std::ceil(0.3 * (double)n_rem), cap),
+          (uint64_t)beta_def * ne_prev[s]);
       slots[s] = pow2_at_least(p.table_init ? p.table_init : 2 * base);
     }
-    PSC_TRY(multi_search(p, nfront, label_offset, slots, st));
-    label_offset += nfront;
+    PSC_TRY(multi_search(p, V, A, slots, st));
+    prof_begin(KK_MISC);
+    k_label_advance<<<grid_for(A), BLOCK, 0, stream>>>(
+        gC.as<unsigned long long>(), A, gid_d.as<uint32_t>(),
+        gD.as<unsigned long long>());
+    prof_end();
     const int small = (slots[0] <= slots[1]) ? 0 : 1;
     prof_begin(KK_PARTITION);
     k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
@@ -1277,15 +1604,16 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     ne_prev[1] = log_n[1];
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
+  // ---- finalize (src/scc.hpp:475-476)
   prof_begin(KK_MISC);
   k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n);
   kbytes[KK_MISC] += 16.0 * n;
   if (p.canonical) {
     uint32_t *minv = scratch32.as<uint32_t>();
-    k_fill_u32<<<grid_for(n + 1, BLOCK, 16384), BLOCK, 0, stream>>>(
-        minv, n + 1, 0xFFFFFFFFu);
-    k_canon_min<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
-    k_canon_apply<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, minv);
+    k_fill_u32<<<grid_for(n + G, BLOCK, 16384), BLOCK, 0, stream>>>(
+        minv, n + G, 0xFFFFFFFFu);
+    k_canon_min<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, voffd, G, minv);
+    k_canon_apply<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, voffd, G, minv);
     kbytes[KK_MISC] += 4.0 * n + 12.0 * n + 20.0 * n;
   }
   prof_end();
@@ -1401,10 +1729,6 @@ int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
 //                                C ABI
 // ======================================================================
 using psc::Engine;
-
-struct psc_handle {
-  Engine eng;
-};
 
 extern "C" {
 

@@ -14,6 +14,7 @@ constexpr int REACH_RING = 1024;     // per-wave VGC queue (vertices, LDS)
 constexpr int MS_RING = 512;         // per-wave VGC queue (pairs, LDS)
 constexpr uint32_t HEAVY_DEG = 2048; // degree above which edges are chunked
 constexpr uint32_t CHUNK = 2048;     // edges per heavy work item
+constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests serially
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 
