@@ -44,6 +44,144 @@ def load_pmc(kernel, workload):
     return k.get("hbm_bytes_per_launch")
 
 
+def bench_grid(args, psc):
+    """C2: 2048x2048 directed grid (p_both 0.6, seed 2), tau swept
+    2^0..2^17 as scripts/Experiment4.py:13 does; labels checked vs Tarjan."""
+    import numpy as np
+    import oracle
+    side = args.grid_side
+    dg = psc.DeviceGraph.grid(side, side, 0.6, seed=2)
+    s = psc.SCC(dg)
+    rows = []
+    taus = [1 << k for k in range(18)] if args.taus is None else args.taus
+    for tau in taus:
+        prm = dict(beta=1.5, local_reach=True, local_scc=True, tau=tau, timing=1)
+        s.run(**prm)
+        ts = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            st = s.run(**prm)
+            ts.append(time.perf_counter() - t)
+        ms = float(np.median(ts)) * 1e3
+        rows.append({"tau": tau, "ms": round(ms, 3),
+                     "gedges_s": round(dg.m / (ms * 1e-3) / 1e9, 4),
+                     "levels": [st["levels_fwd"], st["levels_bwd"]],
+                     "ms_first": round(st["ms_first"], 3),
+                     "ms_multi": round(st["ms_multi"], 3)})
+        log(f"tau {tau:7d}: {ms:9.3f} ms  levels {st['levels_fwd']}/{st['levels_bwd']}")
+    labels = s.scc(beta=1.5, tau=512)
+    g = dg.to_host()
+    ok = bool(np.array_equal(oracle.canonical(labels), oracle.tarjan_canonical(g)))
+    best = min(rows, key=lambda r: r["ms"])
+    print(json.dumps({"metric": "SCC time on the 2048x2048 directed grid vs tau",
+                      "value": best["ms"], "unit": "ms", "higher_is_better": False,
+                      "best_tau": best["tau"], "n": dg.n, "m": dg.m,
+                      "labels_equal_tarjan": ok, "sweep": rows}), flush=True)
+    s.close()
+    dg.free()
+
+
+def bench_batch(args, psc):
+    """C4: 2048 graphs (RMAT avg out-degree 16 / directed grid / 3D 5-NN,
+    chosen by seed), n log-uniform in [2^10, 2^20]; batched graphs/s against
+    the one-graph-per-launch path on the same GPU, identical labels."""
+    import numpy as np
+    import oracle
+    import psc_amd.autotune as at
+    G = args.graphs
+    kind, n = psc.batch_spec(G, 10, args.hi_log2, seed=4)
+    seeds = np.array([psc.graph_seed(4, g) for g in range(G)], dtype=np.uint64)
+    table = at.load_table()
+    groups = at.group_by_class(n, table, max_groups=args.lanes)
+    t = time.perf_counter()
+    dbs = [psc.DeviceBatch(kind[ix], n[ix], seeds=seeds[ix]) for ix, _, _ in groups]
+    gen_s = time.perf_counter() - t
+    n_tot = sum(db.n for db in dbs)
+    m_tot = sum(db.m for db in dbs)
+    log(f"generated {G} graphs in {len(groups)} class groups: n={n_tot} m={m_tot} "
+        f"in {gen_s:.1f}s")
+    bss = [psc.BatchSCC(device_batch=db) for db in dbs]
+    prms = [p for _, p, _ in groups]
+    at.run_groups(bss, prms)                       # warm-up (allocations)
+    walls = [at.run_groups(bss, prms)[0] for _ in range(args.steps)]
+    wall = float(np.median(walls))
+    _, labs = at.run_groups(bss, prms, labels=True)
+    log(f"batched: {wall * 1e3:.2f} ms for {G} graphs ({G / wall:.1f} graphs/s)")
+    # one graph per launch, same parameters; only psc_run is timed
+    plain_s, mism, checked = 0.0, 0, 0
+    rng = np.random.default_rng(0)
+    check = set(rng.choice(G, min(G, args.tarjan_checks), replace=False).tolist())
+    for gi, (ix, prm, _) in enumerate(groups):
+        parts = bss[gi].split(labs[gi])
+        for j, g in enumerate(ix):
+            dg = dbs[gi].extract(j)
+            s = psc.SCC(dg)
+            kw = dict(beta=float(prm["betas"][j]), tau=prm["tau"],
+                      dense_div=prm["dense_div"], timing=0)
+            s.run(**kw)                            # warm-up (allocations)
+            t = time.perf_counter()
+            s.run(**kw)
+            plain_s += time.perf_counter() - t
+            lab = s.scc(**kw)
+            if not np.array_equal(lab, parts[j]):
+                mism += 1
+            if int(g) in check:
+                h = dg.to_host()
+                if not np.array_equal(oracle.canonical(lab), oracle.tarjan_canonical(h)):
+                    mism += 1
+                checked += 1
+            s.close()
+            dg.free()
+    log(f"plain: {plain_s * 1e3:.2f} ms ({G / plain_s:.1f} graphs/s); "
+        f"mismatches {mism}; Tarjan-checked {checked}")
+    out = {"metric": "batched SCC graphs/s vs one-graph-per-launch",
+           "value": round(G / wall, 3), "unit": "graphs/s", "higher_is_better": True,
+           "plain_graphs_per_s": round(G / plain_s, 3),
+           "speedup_vs_plain": round(plain_s / wall, 3),
+           "batched_ms": round(wall * 1e3, 3), "plain_ms": round(plain_s * 1e3, 3),
+           "graphs": G, "n_total": n_tot, "m_total": m_tot,
+           "batched_gedges_s": round(m_tot / wall / 1e9, 3),
+           "labels_identical": mism == 0, "tarjan_checked": checked,
+           "groups": [{"classes": cs, "graphs": int(len(ix)), "tau": p["tau"],
+                       "dense_div": p["dense_div"]} for ix, p, cs in groups],
+           "gen_seconds": round(gen_s, 2)}
+    print(json.dumps(out), flush=True)
+    for b in bss:
+        b.close()
+    for db in dbs:
+        db.free()
+
+
+def bench_serve(args, psc):
+    """C5: Poisson SCC requests (config-4 graph mix, n log-uniform in
+    [2^10, 2^hi]) under an HBM budget, continuous batching vs one request
+    per launch; p50/p99 latency and graphs/s per arrival rate."""
+    import psc_amd.autotune as at
+    import psc_amd.serve as sv
+    table = at.load_table()
+    rows = []
+    for rate in args.rates:
+        for mode in ("batched", "plain"):
+            reqs = sv.poisson_requests(psc, args.requests, rate, 10, args.hi_log2,
+                                       seed=11, table=table)
+            srv = sv.Server(psc, budget_bytes=int(args.budget_gb * 2**30),
+                            lanes=args.lanes, mode=mode)
+            summ = srv.serve(reqs)
+            summ["rate"] = rate
+            rows.append(summ)
+            log(json.dumps(summ))
+    best = max((r for r in rows if r["mode"] == "batched"), key=lambda r: r["graphs_per_s"])
+    plain = [r for r in rows if r["mode"] == "plain" and r["rate"] == best["rate"]][0]
+    print(json.dumps({"metric": "SCC serving graphs/s and p50/p99 latency",
+                      "value": best["graphs_per_s"], "unit": "graphs/s",
+                      "higher_is_better": True, "p50_ms": best["p50_ms"],
+                      "p99_ms": best["p99_ms"],
+                      "plain_graphs_per_s": plain["graphs_per_s"],
+                      "plain_p50_ms": plain["p50_ms"], "plain_p99_ms": plain["p99_ms"],
+                      "requests": args.requests, "hi_log2": args.hi_log2,
+                      "budget_GB": args.budget_gb, "sweep": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +196,16 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="warm-up + steps only (for rocprofv3 runs)")
+    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve"), default="c3")
+    ap.add_argument("--grid-side", type=int, default=2048)
+    ap.add_argument("--taus", type=int, nargs="*", default=None)
+    ap.add_argument("--graphs", type=int, default=2048)
+    ap.add_argument("--hi-log2", type=float, default=None)
+    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--tarjan-checks", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0])
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
@@ -67,6 +215,14 @@ def main():
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
     psc = ge.load_package()
+    if args.mode == "grid":
+        return bench_grid(args, psc)
+    if args.mode == "batch":
+        args.hi_log2 = args.hi_log2 or 20
+        return bench_batch(args, psc)
+    if args.mode == "serve":
+        args.hi_log2 = args.hi_log2 or 24
+        return bench_serve(args, psc)
 
     scale, samples = args.scale, 1 << args.samples_log2
     workload = f"rmat{scale}_s{args.samples_log2}"

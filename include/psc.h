@@ -236,6 +236,16 @@ typedef struct psc_batch_desc {
 /* on_device = 1: the desc arrays are device pointers. */
 int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out);
 
+/* Fold device-resident graphs (arrays in device memory, local ids) into one
+ * batch handle; the graphs are copied, the caller may free them after. */
+int psc_batch_create_from_graphs(const psc_graph *dev_graphs, uint64_t count,
+                                 psc_handle **out);
+
+/* Re-bind a handle made by psc_batch_create_from_graphs (or psc_create) to
+ * a new set of device graphs, reusing its stream and device buffers (they
+ * only grow).  Used by serving lanes. */
+int psc_batch_rebind(psc_handle *h, const psc_graph *dev_graphs, uint64_t count);
+
 /* Number of graphs of a handle (1 for psc_create) and their vertex prefix
  * (num_graphs + 1 entries). */
 uint64_t psc_handle_graphs(psc_handle *h);
@@ -270,6 +280,9 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
                       const uint64_t *n, uint64_t seed, const uint64_t *seeds,
                       psc_dev_batch **out);
 void psc_dev_batch_free(psc_dev_batch *db);
+/* The union graph of a device batch (device pointers, global ids; for a
+ * one-graph batch these are the graph's local ids). */
+int psc_dev_batch_union(const psc_dev_batch *db, psc_graph *out);
 int psc_dev_batch_info(const psc_dev_batch *db, uint64_t *num_graphs,
                        uint64_t *vbase, uint64_t *ebase, uint32_t *kind);
 /* Handle over a device batch (borrowed; keep the batch alive). */

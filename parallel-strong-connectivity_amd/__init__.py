@@ -132,6 +132,10 @@ class CBatchDesc(C.Structure):
 _sig("psc_batch_create", C.c_int, C.POINTER(CBatchDesc), C.c_int,
      C.POINTER(C.c_void_p))
 _sig("psc_handle_graphs", C.c_uint64, C.c_void_p)
+_sig("psc_batch_create_from_graphs", C.c_int, C.POINTER(CGraph), C.c_uint64,
+     C.POINTER(C.c_void_p))
+_sig("psc_batch_rebind", C.c_int, C.c_void_p, C.POINTER(CGraph), C.c_uint64)
+_sig("psc_dev_batch_union", C.c_int, C.c_void_p, C.POINTER(CGraph))
 _sig("psc_handle_vbase", C.c_int, C.c_void_p, u64p)
 _sig("psc_batch_run", C.c_int, C.c_void_p, C.POINTER(CParams),
      C.POINTER(C.c_double), u64p, C.POINTER(CStats))
@@ -459,6 +463,12 @@ class DeviceBatch:
         self.n = int(self.vbase[-1])
         self.m = int(self.ebase[-1])
 
+    def union_graph(self):
+        """The batch's union as a CGraph of device pointers."""
+        g = CGraph()
+        _check("psc_dev_batch_union", lib.psc_dev_batch_union(self._p, C.byref(g)))
+        return g
+
     def extract(self, g):
         dgp = C.POINTER(CDevGraph)()
         _check("psc_dev_batch_extract", lib.psc_dev_batch_extract(self._p, g,
@@ -481,9 +491,14 @@ class BatchSCC:
     """SCC of many graphs in one handle; labels per graph equal a
     stand-alone SCC(graph).scc() with the same beta and seed."""
 
-    def __init__(self, graphs=None, device_batch=None):
+    def __init__(self, graphs=None, device_batch=None, device_graphs=None):
         h = C.c_void_p()
-        if device_batch is not None:
+        if device_graphs is not None:
+            arr = (CGraph * len(device_graphs))(*[d.cgraph() for d in device_graphs])
+            _check("psc_batch_create_from_graphs", lib.psc_batch_create_from_graphs(
+                arr, len(device_graphs), C.byref(h)))
+            self._keep = None
+        elif device_batch is not None:
             _check("psc_batch_create_dev", lib.psc_batch_create_dev(
                 device_batch._p, C.byref(h)))
             self._keep = device_batch
@@ -504,12 +519,25 @@ class BatchSCC:
             _check("psc_batch_create", lib.psc_batch_create(C.byref(d), 0, C.byref(h)))
             self._keep = None
         self._h = h
-        G = lib.psc_handle_graphs(h)
+        self._refresh()
+        self.stats = None
+
+    @classmethod
+    def _wrap(cls, h):
+        """Adopt a handle created through the C ABI."""
+        obj = cls.__new__(cls)
+        obj._h = h
+        obj._keep = None
+        obj.stats = None
+        obj._refresh()
+        return obj
+
+    def _refresh(self):
+        G = lib.psc_handle_graphs(self._h)
         self.num_graphs = int(G)
         self.vbase = np.zeros(G + 1, dtype=np.uint64)
-        _check("psc_handle_vbase", lib.psc_handle_vbase(h, _ptr(self.vbase, u64p)))
+        _check("psc_handle_vbase", lib.psc_handle_vbase(self._h, _ptr(self.vbase, u64p)))
         self.n = int(self.vbase[-1])
-        self.stats = None
 
     def run(self, beta=1.5, beta_per_graph=None, labels=True, **kw):
         p = default_params(beta=beta, **kw)

@@ -85,6 +85,20 @@ __global__ void k_extract(const uint64_t *goff, const uint32_t *gadj,
     ladj[j] = (uint32_t)(gadj[e0 + j] - v0);
 }
 
+__global__ void k_fill_one(uint64_t *p, uint64_t v) { *p = v; }
+
+// one device graph (local ids) into its slot of a union
+__global__ void k_rebase_one(const uint64_t *loff, const uint32_t *ladj,
+                             uint64_t n, uint64_t m, uint64_t v0, uint64_t e0,
+                             uint64_t *goff, uint32_t *gadj) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    goff[i] = e0 + loff[i];
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m;
+       j += (uint64_t)gridDim.x * blockDim.x)
+    gadj[j] = (uint32_t)(v0 + ladj[j]);
+}
+
 template <class T>
 int dev_copy_in(const T *src, size_t count, int on_device, T **dst) {
   *dst = nullptr;
@@ -391,6 +405,92 @@ done:
     return rc;
   }
   *out = h;
+  return PSC_OK;
+}
+
+// Gather device-resident graphs (local ids, as made by the device
+// generators) into the union buffers of handle h: offsets rebased by the
+// edge prefix, ids by the vertex prefix.  The handle's buffers are reused
+// and only grow, so a serving lane re-binds the same handle batch after
+// batch.
+static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count) {
+  const uint32_t G = (uint32_t)count;
+  std::vector<uint64_t> vb(G + 1, 0), eb(G + 1, 0);
+  for (uint32_t g = 0; g < G; ++g) {
+    vb[g + 1] = vb[g] + graphs[g].n;
+    eb[g + 1] = eb[g] + graphs[g].m;
+  }
+  if (vb[G] >= 0xFFFFFFFFull) {
+    psc::set_error("batch too large: total vertices must be < 2^32-1");
+    return PSC_ERR_INVALID;
+  }
+  psc::Engine &e = h->eng;
+  const uint64_t ntot = vb[G], mtot = eb[G];
+  int rc = e.g_off.ensure((ntot + 1) * 8);
+  if (!rc) rc = e.g_ioff.ensure((ntot + 1) * 8);
+  if (!rc) rc = e.g_adj.ensure(std::max<uint64_t>(mtot, 1) * 4);
+  if (!rc) rc = e.g_iadj.ensure(std::max<uint64_t>(mtot, 1) * 4);
+  if (rc) return rc;
+  hipStream_t st = e.stream;
+  hipError_t he = hipSuccess;
+  if (!st) he = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  for (uint32_t g = 0; g < G && he == hipSuccess; ++g) {
+    const psc_graph &x = graphs[g];
+    if (x.n == 0) continue;
+    k_rebase_one<<<blocks_for(std::max(x.n + 1, x.m)), 256, 0, st>>>(
+        x.offset, x.edges, x.n, x.m, vb[g], eb[g], e.g_off.as<uint64_t>() + vb[g],
+        e.g_adj.as<uint32_t>() + eb[g]);
+    k_rebase_one<<<blocks_for(std::max(x.n + 1, x.m)), 256, 0, st>>>(
+        x.in_offset, x.in_edges, x.n, x.m, vb[g], eb[g],
+        e.g_ioff.as<uint64_t>() + vb[g], e.g_iadj.as<uint32_t>() + eb[g]);
+    he = hipGetLastError();
+  }
+  // an empty graph owns no offset slot of its own: its neighbours' rebased
+  // offsets already hold the right value; only an all-empty batch needs
+  // its single offset written
+  if (he == hipSuccess && ntot == 0) {
+    k_fill_one<<<1, 1, 0, st>>>(e.g_off.as<uint64_t>(), 0);
+    k_fill_one<<<1, 1, 0, st>>>(e.g_ioff.as<uint64_t>(), 0);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipStreamSynchronize(st);
+  if (!e.stream && st) (void)hipStreamDestroy(st);
+  if (he != hipSuccess) {
+    psc::set_error(std::string("batch gather: ") + hipGetErrorString(he));
+    return PSC_ERR_HIP;
+  }
+  psc_graph ug{ntot, mtot, e.g_off.as<uint64_t>(), e.g_adj.as<uint32_t>(),
+               e.g_ioff.as<uint64_t>(), e.g_iadj.as<uint32_t>()};
+  return e.init(&ug, true, G, vb.data());
+}
+
+int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
+                                 psc_handle **out) {
+  if (!graphs || !out || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
+  *out = nullptr;
+  auto *h = new (std::nothrow) psc_handle;
+  if (!h) return PSC_ERR_NOMEM;
+  int rc = gather_graphs(h, graphs, count);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PSC_OK;
+}
+
+int psc_batch_rebind(psc_handle *h, const psc_graph *graphs, uint64_t count) {
+  if (!h || !graphs || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
+  if (!h->eng.owned_graph && h->eng.stream) {
+    psc::set_error("psc_batch_rebind: handle borrows its graph");
+    return PSC_ERR_INVALID;
+  }
+  return gather_graphs(h, graphs, count);
+}
+
+int psc_dev_batch_union(const psc_dev_batch *db, psc_graph *out) {
+  if (!db || !out) return PSC_ERR_INVALID;
+  *out = db->g;
   return PSC_OK;
 }
 

@@ -88,6 +88,45 @@ __device__ __forceinline__ unsigned long long wave_append64(
   return base + lanes_below(m);
 }
 
+// Per-wave output buffer in LDS.  Appends are staged in the buffer and
+// written to the global list in bulk: one atomic per flush instead of one
+// per 64-lane step, and the global stores of a flush are coalesced.  `n` is
+// wave-uniform.  Entries at or beyond `cap` are dropped and reported.
+template <class T, int CAP>
+struct WaveOut {
+  T *buf;
+  uint32_t n;
+  __device__ __forceinline__ void init(T *b) {
+    buf = b;
+    n = 0;
+  }
+  // returns false if the global list overflowed `cap`
+  template <class Ctr>
+  __device__ __forceinline__ bool flush(Ctr *ctr, T *dst, unsigned long long cap) {
+    if (n == 0) return true;
+    wave_sync();
+    Ctr base = 0;
+    if (lane() == 0) base = atomicAdd(ctr, (Ctr)n);
+    base = __shfl(base, 0);
+    for (uint32_t i = lane(); i < n; i += WAVE)
+      if ((unsigned long long)base + i < cap) dst[base + i] = buf[i];
+    const bool ok = (unsigned long long)base + n <= cap;
+    n = 0;
+    wave_sync();
+    return ok;
+  }
+  template <class Ctr>
+  __device__ __forceinline__ bool put(bool want, T v, Ctr *ctr, T *dst,
+                                      unsigned long long cap) {
+    const uint64_t m = ballot(want);
+    if (!m) return true;
+    if (want) buf[n + lanes_below(m)] = v;
+    n += (uint32_t)__popcll(m);
+    if (n > (uint32_t)(CAP - WAVE)) return flush(ctr, dst, cap);
+    return true;
+  }
+};
+
 // Visited bitmap: test, and test-and-set with one atomicOr.
 __device__ __forceinline__ bool bit_test(const uint32_t *bm, uint32_t v) {
   return (bm[v >> 5] >> (v & 31)) & 1u;

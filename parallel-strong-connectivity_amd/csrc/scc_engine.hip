@@ -72,17 +72,28 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
 constexpr int SEL_ITEMS = 8;
 constexpr int SEL_TILE = BLOCK * SEL_ITEMS;
 
+// A tile is SEL_ITEMS rounds of BLOCK consecutive items; in each round
+// thread t handles item round*BLOCK + t, so predicate loads are coalesced.
+// Ranks come from wave ballots and a per-round block prefix, which keeps
+// the output in index order (stable).
 template <class F>
 __global__ __launch_bounds__(BLOCK) void k_sel_count(F f, uint64_t len,
                                                      uint32_t *tile_cnt) {
-  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE + threadIdx.x * SEL_ITEMS;
+  __shared__ uint32_t wc[WPB];
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   uint32_t c = 0;
 #pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j)
-    if (base + j < len && f.pred(base + j)) ++c;
-  uint32_t tot;
-  block_excl_scan(c, &tot);
-  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    c += (uint32_t)__popcll(ballot(i < len && f.pred(i)));
+  }
+  if (lane() == 0) wc[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WPB; ++w) t += wc[w];
+    tile_cnt[blockIdx.x] = t;
+  }
 }
 
 // Exclusive scan of the tile counts in one block; total -> *out_total.
@@ -110,25 +121,36 @@ template <class F>
 __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
     F f, uint64_t len, const unsigned long long *tile_off, uint32_t *sel,
     uint32_t *rej) {
-  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE + threadIdx.x * SEL_ITEMS;
+  __shared__ uint32_t wc[SEL_ITEMS][WPB];
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
+  const int w = threadIdx.x / WAVE;
   bool p[SEL_ITEMS];
-  uint32_t c = 0;
+  uint32_t rk[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
-    p[j] = base + j < len && f.pred(base + j);
-    c += p[j];
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    p[j] = i < len && f.pred(i);
+    const uint64_t m = ballot(p[j]);
+    rk[j] = lanes_below(m);
+    if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
   }
-  uint32_t tot;
-  uint64_t r = tile_off[blockIdx.x] + block_excl_scan(c, &tot);
+  __syncthreads();
+  uint64_t r0 = tile_off[blockIdx.x];   // selected items before this round
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + j;
-    if (i >= len) break;
-    if (p[j]) {
-      sel[r++] = f.val(i);
-    } else if (rej) {
-      rej[i - r] = f.val(i);
+    uint32_t before = 0, round_tot = 0;
+#pragma unroll
+    for (int x = 0; x < WPB; ++x) {
+      before += x < w ? wc[j][x] : 0u;
+      round_tot += wc[j][x];
     }
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    if (i < len) {
+      const uint64_t r = r0 + before + rk[j];   // rank among selected
+      if (p[j]) sel[r] = f.val(i);
+      else if (rej) rej[i - r] = f.val(i);
+    }
+    r0 += round_tot;
   }
 }
 
@@ -153,6 +175,16 @@ struct SelWindow {
     return centers[wsrc[a] + (t - wpre[a])];
   }
   __device__ bool pred(uint64_t t) const { return !(labels[val(t)] & TOP_BIT); }
+};
+// pred: not in the first-round SCC, i.e. not reached both ways
+// (src/scc.hpp:349); reads the two visited bitmaps instead of the labels.
+struct SelNotBoth {
+  const uint32_t *src, *vf, *vb;
+  __device__ uint32_t val(uint64_t i) const { return src[i]; }
+  __device__ bool pred(uint64_t i) const {
+    const uint32_t v = src[i];
+    return !(bit_test(vf, v) && bit_test(vb, v));
+  }
 };
 // pred: not yet in a found SCC; select from a vertex list (src/scc.hpp:349,384-388)
 struct SelNotDone {
@@ -234,21 +266,18 @@ __global__ void k_permute(const uint32_t *NZ, uint64_t nnz,
 // the level control of both searches.
 __global__ void k_reach_reset(ReachSearch *S, ReachCtl *C, int ns,
                               uint64_t words) {
+  // one lane per vertex, two bitmap words per wave (coalesced offsets)
+  const uint64_t units = (words + 1) / 2;
   for (int s = 0; s < ns; ++s) {
     const ReachSearch R = S[s];
     const uint64_t n = R.n;
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-         w < words; w += (uint64_t)gridDim.x * blockDim.x) {
-      uint32_t bits = 0;
-      const uint64_t v0 = w * 32;
-      const uint64_t cnt = min((uint64_t)32, n - v0);
-      uint64_t o = R.off[v0];
-      for (uint64_t k = 0; k < cnt; ++k) {
-        const uint64_t o2 = R.off[v0 + k + 1];
-        if (o2 == o) bits |= 1u << k;
-        o = o2;
-      }
-      R.vis[w] = bits;
+    for (uint64_t t = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+         t < units; t += (uint64_t)gridDim.x * blockDim.x / WAVE) {
+      const uint64_t v = t * 64 + lane();
+      const bool dead = v < n && R.off[v + 1] == R.off[v];
+      const uint64_t m = ballot(dead);
+      if (lane() == 0) R.vis[t * 2] = (uint32_t)m;
+      if (lane() == 32 && t * 2 + 1 < words) R.vis[t * 2 + 1] = (uint32_t)(m >> 32);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -300,20 +329,48 @@ __device__ __forceinline__ void add_cnt(unsigned long long *c, unsigned long lon
   if (lane() == 0 && v) atomicAdd(c, v);
 }
 
+// Heavy-list append shared by push and pull levels: vertex u with `deg`
+// edges still to scan becomes ceil(deg / CHUNK) work items; one packed
+// atomic per wave reserves both the list slots and the chunk numbers.
+__device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cur,
+                                             bool heavy, uint32_t u, uint64_t deg) {
+  const uint64_t hm = ballot(heavy);
+  if (!hm) return;
+  const int l = lane();
+  const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
+  const unsigned long long cinc = wave_incl_scan(nch);
+  const unsigned long long ctot = __shfl(cinc, WAVE - 1);
+  unsigned long long old = 0;
+  if (l == 0)
+    old = atomicAdd(&cur->heavy,
+                    ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
+  old = __shfl(old, 0);
+  if (heavy) {
+    const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
+    R.hv[idx] = u;
+    R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
+  }
+}
+
 // One BFS level of both searches.  Push levels (src/reach.hpp:114-171):
-// a wave takes S seeds, keeps a queue of up to REACH_RING vertices in LDS
-// and expands it with load-balanced edge scans (lane e of a 64-edge window
-// finds its owner by binary search over the wave's inclusive degree scan).
-// Newly visited vertices stay in the queue while the VGC budget (tau per
-// seed) lasts and the ring has room; the rest spill to the next frontier.
-// Vertices with degree > HEAVY_DEG go to the heavy list (k_reach_heavy).
+// wave w takes a contiguous share of the frontier, S seeds at a time, keeps
+// a queue of up to REACH_RING vertices in LDS and expands it with
+// load-balanced edge scans (lane e of a 64-edge window finds its owner by
+// binary search over the wave's inclusive degree scan).  Newly visited
+// sparse vertices stay in the queue while the VGC budget (tau per seed)
+// lasts and the ring has room; the rest are staged in LDS and spilled to
+// the next frontier in bulk.  Vertices with more than HEAVY_DEG edges go to
+// the heavy list (k_reach_heavy).
 // Pull levels (src/reach.hpp:173-190): every unvisited vertex scans its
 // in-edges until it meets a visited vertex; testing "visited" instead of
-// "in the previous frontier" keeps the reachable set identical.
+// "in the previous frontier" keeps the reachable set identical.  Long
+// remainders (> PULL_HEAVY edges after the first probes) are chunked onto
+// the heavy list.
 __global__ __launch_bounds__(BLOCK) void k_reach_step(
     const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
+  __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
   __shared__ uint32_t fl_s[WPB][WAVE];
@@ -334,22 +391,26 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       C->levels[s] = L;
     }
   }
+  WaveOut<uint32_t, OUT_CAP> out;
+  out.init(out_s[w]);
   for (int s = 0; s < ns; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
     const ReachSearch R = S[s];
     const int mode = reach_mode(cur, R);
     const uint32_t nf = cur->qsize;
+    uint32_t *qn = R.q[p ^ 1];
     if (mode == MODE_PUSH) {
-      uint32_t sd = 64;
-      if (vgc) sd = min(64u, max(1u, nf / total_waves));
+      // contiguous share of the frontier, S seeds at a time
+      const uint64_t per = (nf + (uint64_t)total_waves - 1) / total_waves;
+      const uint32_t sd = (uint32_t)min((uint64_t)64, per);
+      uint64_t g = gw * per;
+      const uint64_t gend = min((uint64_t)nf, g + per);
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
-      // first chunk of each wave is static (no atomic), the rest dynamic
-      uint64_t g = gw * sd;
-      while (true) {
-        if (g >= nf) break;
-        const uint32_t cnt = (uint32_t)min((uint64_t)sd, nf - g);
+      while (g < gend) {
+        const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
         if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
+        g += cnt;
         uint32_t head = 0, tail = cnt;
         uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
         wave_sync();
@@ -365,23 +426,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
             ++n_exp;
           }
           const bool heavy = deg > HEAVY_DEG;
-          const uint64_t hm = ballot(heavy);
-          if (hm) {
-            const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
-            const unsigned long long cinc = wave_incl_scan(nch);
-            const unsigned long long ctot = __shfl(cinc, WAVE - 1);
-            unsigned long long old = 0;
-            if (l == 0)
-              old = atomicAdd(&cur->heavy,
-                              ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
-            old = __shfl(old, 0);
-            if (heavy) {
-              const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
-              R.hv[idx] = u;
-              R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
-              deg = 0;
-            }
-          }
+          heavy_append(R, cur, heavy, u, deg);
+          if (heavy) deg = 0;
           const uint32_t inc = wave_incl_scan((uint32_t)deg);
           const uint32_t total = __shfl(inc, WAVE - 1);
           sc[l] = inc;
@@ -404,29 +450,31 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
             }
             const uint64_t fm = ballot(found);
             if (!fm) continue;
-            const uint32_t nfound = (uint32_t)__popcll(fm);
             n_vis += found;
+            uint64_t dv = 0;
+            if (found) dv = R.off[v + 1] - R.off[v];
+            // VGC (src/reach.hpp:125-154): sparse new vertices stay in the
+            // wave's queue while the budget lasts and the ring has room
+            const bool want = found && vgc && dv <= VGC_MAX_DEG;
+            const uint64_t lm = ballot(want);
+            const uint32_t nwant = (uint32_t)__popcll(lm);
             const uint32_t room = REACH_RING - (tail - head);
-            const uint32_t take = (uint32_t)min((uint64_t)min(nfound, room), budget);
-            const uint32_t rank = lanes_below(fm);
-            const bool loc = found && rank < take;
-            if (loc) ring[(tail + rank) & (REACH_RING - 1)] = v;
+            const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
+            const bool loc = want && lanes_below(lm) < take;
+            if (loc) ring[(tail + lanes_below(lm)) & (REACH_RING - 1)] = v;
             tail += take;
             budget -= take;
             const bool glob = found && !loc;
-            const uint32_t pos = wave_append(&nxt->qsize, glob);
             if (glob) {
-              R.q[p ^ 1][pos] = v;
-              dsum += R.off[v + 1] - R.off[v];
+              dsum += dv;
               ++n_app;
             }
+            out.put(glob, v, &nxt->qsize, qn, R.n);
           }
           wave_sync();
         }
-        uint32_t gd = 0;
-        if (l == 0) gd = atomicAdd(&cur->grab, sd);
-        g = (uint64_t)__shfl(gd, 0) + (uint64_t)total_waves * sd;
       }
+      out.flush(&nxt->qsize, qn, R.n);
       add_cnt(&nxt->degsum, dsum);
       add_cnt(&C->cnt[s][R_PUSH_EDGES], n_edges);
       add_cnt(&C->cnt[s][R_PUSH_EXP], n_exp);
@@ -447,37 +495,27 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
           b = R.roff[v];
           deg = R.roff[v + 1] - b;
           // most vertices of a dense level meet a visited in-neighbour
-          // among their first few in-edges
+          // among their first few in-edges; their loads are issued together
           const uint64_t k = min(deg, (uint64_t)PULL_PROBE);
-          for (uint64_t j = 0; j < k; ++j) {
-            ++n_edges;
-            if (bit_test(R.vis, R.radj[b + j])) {
-              found = true;
-              break;
-            }
-          }
+          uint32_t nb[PULL_PROBE];
+#pragma unroll
+          for (uint32_t j = 0; j < PULL_PROBE; ++j)
+            nb[j] = j < k ? R.radj[b + j] : 0u;
+          uint32_t wd[PULL_PROBE];
+#pragma unroll
+          for (uint32_t j = 0; j < PULL_PROBE; ++j)
+            wd[j] = j < k ? R.vis[nb[j] >> 5] : 0u;
+#pragma unroll
+          for (uint32_t j = 0; j < PULL_PROBE; ++j)
+            found |= j < k && ((wd[j] >> (nb[j] & 31)) & 1u);
+          n_edges += k;
           b += k;
           deg = found ? 0 : deg - k;
         }
         // long remainders go to the heavy list (k_reach_heavy, chunked)
-        const bool heavy = deg > HEAVY_DEG;
-        const uint64_t hm = ballot(heavy);
-        if (hm) {
-          const unsigned long long nch = heavy ? (deg + CHUNK - 1) / CHUNK : 0;
-          const unsigned long long cinc = wave_incl_scan(nch);
-          const unsigned long long ctot = __shfl(cinc, WAVE - 1);
-          unsigned long long old = 0;
-          if (l == 0)
-            old = atomicAdd(&cur->heavy,
-                            ((unsigned long long)__popcll(hm) << H_CNT_SHIFT) | ctot);
-          old = __shfl(old, 0);
-          if (heavy) {
-            const uint64_t idx = (old >> H_CNT_SHIFT) + lanes_below(hm);
-            R.hv[idx] = (uint32_t)v;
-            R.hc[idx] = (old & H_CHUNK_MASK) + cinc - nch;
-            deg = 0;
-          }
-        }
+        const bool heavy = deg > PULL_HEAVY;
+        heavy_append(R, cur, heavy, (uint32_t)v, deg);
+        if (heavy) deg = 0;
         // the rest: load-balanced 64-edge windows over the remainders of
         // the wave's 64 vertices, skipping owners already resolved
         const uint32_t inc = wave_incl_scan((uint32_t)deg);
@@ -512,9 +550,9 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
         if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[t * 2], (uint32_t)fm);
         if (l == 32 && (fm >> 32)) atomicOr(&R.vis[t * 2 + 1], (uint32_t)(fm >> 32));
         n_vis += found;
-        const uint32_t pos = wave_append(&nxt->qsize, found);
-        if (found) R.q[p ^ 1][pos] = (uint32_t)v;
+        out.put(found, (uint32_t)v, &nxt->qsize, qn, R.n);
       }
+      out.flush(&nxt->qsize, qn, R.n);
       add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
       add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
       add_cnt(&C->cnt[s][R_PULL_VIS], n_vis);
@@ -523,8 +561,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
   }
 }
 
-// Heavy vertices of a level, CHUNK-edge work items spread over all waves
-// (a wave finds the owning vertex by binary search over hc).
+// Heavy vertices of a level, CHUNK-edge work items; wave w takes a
+// contiguous range of items and walks the owner list forward.
 // Push level: edge-parallel expansion (src/reach.hpp:155-167, granularity
 // 2048).  Pull level: the in-edge remainder of an unvisited vertex with a
 // long in-list; the first chunk to meet a visited in-neighbour sets the
@@ -533,10 +571,13 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
                                                        ReachCtl *C, int ns,
                                                        uint32_t L,
                                                        uint32_t total_waves) {
+  __shared__ uint32_t out_s[WPB][OUT_CAP];
   const int l = lane();
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
   const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  WaveOut<uint32_t, OUT_CAP> out;
+  out.init(out_s[threadIdx.x / WAVE]);
   for (int s = 0; s < ns; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
@@ -544,21 +585,29 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0) continue;
     const ReachSearch R = S[s];
+    uint32_t *qn = R.q[p ^ 1];
     const bool pull = reach_mode(cur, R) == MODE_PULL;
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
     const uint64_t skip = pull ? PULL_PROBE : 0;
     unsigned long long n_edges = 0, n_vis = 0, n_app = 0, n_items = 0, dsum = 0;
-    uint64_t c = gw;   // first item static, then dynamic
-    while (c < Ct) {
-      if (l == 0) ++n_items;
+    const uint64_t per = (Ct + total_waves - 1) / total_waves;
+    uint64_t c = gw * per;
+    const uint64_t cend = min(Ct, c + per);
+    uint64_t own = 0;
+    if (c < cend) {   // owner of the first item: last idx with hc[idx] <= c
       uint64_t lo = 0, hi = H - 1;
-      while (lo < hi) {   // last idx with hc[idx] <= c
+      while (lo < hi) {
         const uint64_t mid = (lo + hi + 1) >> 1;
         if (R.hc[mid] <= c) lo = mid; else hi = mid - 1;
       }
-      const uint32_t u = R.hv[lo];
-      const uint64_t k = c - R.hc[lo];
+      own = lo;
+    }
+    for (; c < cend; ++c) {
+      while (own + 1 < H && R.hc[own + 1] <= c) ++own;
+      if (l == 0) ++n_items;
+      const uint32_t u = R.hv[own];
+      const uint64_t k = c - R.hc[own];
       const uint64_t eb = off[u] + skip + k * CHUNK;
       const uint64_t ee = min(off[u + 1], eb + CHUNK);
       if (!pull) {
@@ -571,13 +620,12 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
             found = bit_set(R.vis, v);
             ++n_edges;
           }
-          n_vis += found;
-          const uint32_t pos = wave_append(&nxt->qsize, found);
           if (found) {
-            R.q[p ^ 1][pos] = v;
             dsum += R.off[v + 1] - R.off[v];
+            ++n_vis;
             ++n_app;
           }
+          out.put(found, v, &nxt->qsize, qn, R.n);
         }
       } else if (!bit_test(R.vis, u)) {
         bool hit = false;
@@ -590,20 +638,19 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
           }
           hit = ballot(h) != 0;
         }
+        bool mine = false;
         if (hit && l == 0) {
           const uint32_t bit = 1u << (u & 31);
-          if (!(atomicOr(&R.vis[u >> 5], bit) & bit)) {
-            const uint32_t pos = atomicAdd(&nxt->qsize, 1u);
-            R.q[p ^ 1][pos] = u;
+          mine = !(atomicOr(&R.vis[u >> 5], bit) & bit);
+          if (mine) {
             ++n_vis;
             ++n_app;
           }
         }
+        out.put(mine, u, &nxt->qsize, qn, R.n);
       }
-      unsigned long long cd = 0;
-      if (l == 0) cd = atomicAdd(&cur->hgrab, 1ull);
-      c = __shfl(cd, 0) + total_waves;
     }
+    out.flush(&nxt->qsize, qn, R.n);
     add_cnt(&nxt->degsum, dsum);
     add_cnt(&C->cnt[s][R_HEAVY_EDGES], n_edges);
     add_cnt(&C->cnt[s][R_HEAVY_VIS], n_vis);
@@ -613,20 +660,21 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
 }
 
 // First-SCC labelling (src/scc.hpp:340-347): both reaches -> found SCC,
-// one reach -> new subproblem label; lab1[g] = |trimmed of g| + 1.
-__global__ void k_first_labels(const uint32_t *P, uint64_t np,
-                               const unsigned long long *nstart, uint32_t G,
-                               const uint32_t *vf, const uint32_t *vb,
+// one reach -> new subproblem label; lab1[g] = |trimmed of g| + 1.  Runs in
+// vertex order (coalesced label writes); trimmed vertices already hold a
+// non-zero label and are skipped.
+__global__ void k_first_labels(uint64_t n, const unsigned long long *voff,
+                               uint32_t G, const uint32_t *vf, const uint32_t *vb,
                                unsigned long long *labels,
                                const unsigned long long *lab1,
                                unsigned long long *big) {
   unsigned long long c = 0;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < np;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = P[i];
-    const bool a = bit_test(vf, v), b = bit_test(vb, v);
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    if (labels[v]) continue;
+    const bool a = bit_test(vf, (uint32_t)v), b = bit_test(vb, (uint32_t)v);
     if (a || b) {
-      const unsigned long long label = lab1[seg_of(nstart, G, i)];
+      const unsigned long long label = lab1[seg_of(voff, G, v)];
       if (a && b) {
         labels[v] = label | TOP_BIT;
         ++c;
@@ -687,32 +735,38 @@ __global__ void k_label_advance(const unsigned long long *fstart, uint32_t A,
     lab_off[wgid[a]] += fstart[a + 1] - fstart[a];
 }
 
-// Record a newly inserted pair in the log; flags overflow when the log
-// (load limit of the table) is exhausted.
-__device__ __forceinline__ bool ms_log(const MSSearch &M, MSCtl *C, int s,
-                                       bool ins, unsigned long long pr) {
-  const unsigned long long idx = wave_append64(&C->log_n[s], ins);
-  if (ins) {
-    if (idx < M.log_cap) {
-      M.log[idx] = pr;
-    } else {
-      C->full[s] = 1;
-      return false;
-    }
+// Staged log / frontier appends of one wave; an overflow of either list
+// (the table's load limit) marks the search full.
+struct MSOut {
+  WaveOut<unsigned long long, OUT_CAP64> log, q;
+  __device__ __forceinline__ void put(const MSSearch &M, MSCtl *C, int s,
+                                      unsigned long long *qn, MSSlot *nxt,
+                                      bool logit, bool spill,
+                                      unsigned long long pr) {
+    if (!log.put(logit, pr, &C->log_n[s], M.log, M.log_cap)) C->full[s] = 1;
+    if (!q.put(spill, pr, &nxt->qsize, qn, M.q_cap)) C->full[s] = 1;
   }
-  return ins;
-}
+  __device__ __forceinline__ void flush(const MSSearch &M, MSCtl *C, int s,
+                                        unsigned long long *qn, MSSlot *nxt) {
+    if (!log.flush(&C->log_n[s], M.log, M.log_cap)) C->full[s] = 1;
+    if (!q.flush(&nxt->qsize, qn, M.q_cap)) C->full[s] = 1;
+  }
+};
 
 // One sub-round of multi-search (src/scc.hpp:160-252) for both directions.
 // Frontier items are (v, l) pairs; for each edge v->w with
 // labels[w] == labels[v] and w not in a found SCC, insert (w, l); a new
 // pair is logged and either expanded by the same wave (VGC, budget tau per
-// seed, src/scc.hpp:177-219) or spilled to the next frontier.
+// seed, src/scc.hpp:177-219) or spilled to the next frontier.  Wave w takes
+// a contiguous share of the frontier; log and frontier appends are staged
+// in LDS.
 __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     const MSSearch *S, MSCtl *C, int ns, uint32_t L,
     const unsigned long long *__restrict__ labels, uint32_t tau, int vgc,
     uint32_t total_waves) {
   __shared__ unsigned long long ring_s[WPB][MS_RING];
+  __shared__ unsigned long long olog_s[WPB][OUT_CAP64];
+  __shared__ unsigned long long oq_s[WPB][OUT_CAP64];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
   __shared__ unsigned long long lab_s[WPB][WAVE];
@@ -729,6 +783,10 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     C->slot[threadIdx.x][zi] = z;
   }
   const int pq = L & 1;
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  MSOut out;
+  out.log.init(olog_s[w]);
+  out.q.init(oq_s[w]);
   for (int s = 0; s < ns; ++s) {
     if (!C->active[s]) continue;
     MSSlot *cur = &C->slot[s][ci];
@@ -736,17 +794,18 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     const unsigned long long nf = cur->qsize;
     if (nf == 0 || C->full[s]) continue;
     const MSSearch M = S[s];
-    unsigned long long sd = 64;
-    if (vgc) sd = min(64ull, max(1ull, nf / total_waves));
+    unsigned long long *qn = M.q[pq ^ 1];
+    const uint64_t per = (nf + total_waves - 1) / total_waves;
+    const uint32_t sd = (uint32_t)min((uint64_t)64, per);
+    uint64_t g = gw * per;
+    const uint64_t gend = min((uint64_t)nf, g + per);
     unsigned long long n_edges = 0, n_exp = 0, n_ins = 0, n_app = 0;
     uint32_t probes = 0;
-    const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-    unsigned long long g = gw * sd;   // first chunk static, then dynamic
-    while (true) {
-      if (g >= nf) break;
+    while (g < gend) {
       if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
-      const uint32_t cnt = (uint32_t)min(sd, nf - g);
+      const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
       if ((uint32_t)l < cnt) ring[l] = M.q[pq][g + l];
+      g += cnt;
       uint32_t head = 0, tail = cnt;
       uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
       wave_sync();
@@ -811,32 +870,32 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
               np = make_pair(x, el[lo]);
             }
           }
-          ins = ms_log(M, C, s, ins, np);
           const uint64_t fm = ballot(ins);
           if (!fm) continue;
           n_ins += ins;
-          const uint32_t nnew = (uint32_t)__popcll(fm);
+          // VGC (src/scc.hpp:177-219): sparse new pairs stay in the wave's
+          // queue while the budget lasts and the ring has room
+          bool want = false;
+          if (ins && vgc) {
+            const uint32_t x = pair_vertex(np);
+            want = M.off[x + 1] - M.off[x] <= VGC_MAX_DEG;
+          }
+          const uint64_t lm = ballot(want);
+          const uint32_t nwant = (uint32_t)__popcll(lm);
           const uint32_t room = MS_RING - (tail - head);
-          const uint32_t take = (uint32_t)min((uint64_t)min(nnew, room), budget);
-          const uint32_t rank = lanes_below(fm);
-          const bool loc = ins && rank < take;
-          if (loc) ring[(tail + rank) & (MS_RING - 1)] = np;
+          const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
+          const bool loc = want && lanes_below(lm) < take;
+          if (loc) ring[(tail + lanes_below(lm)) & (MS_RING - 1)] = np;
           tail += take;
           budget -= take;
           const bool glob = ins && !loc;
-          const unsigned long long pos = wave_append64(&nxt->qsize, glob);
-          if (glob) {
-            if (pos < M.q_cap) M.q[pq ^ 1][pos] = np;
-            else C->full[s] = 1;
-            ++n_app;
-          }
+          n_app += glob;
+          out.put(M, C, s, qn, nxt, ins, glob, np);
         }
         wave_sync();
       }
-      unsigned long long gd = 0;
-      if (l == 0) gd = atomicAdd(&cur->grab, sd);
-      g = __shfl(gd, 0) + (unsigned long long)total_waves * sd;
     }
+    out.flush(M, C, s, qn, nxt);
     add_cnt(&C->cnt[s][M_EDGES], n_edges);
     add_cnt(&C->cnt[s][M_EXP], n_exp);
     add_cnt(&C->cnt[s][M_INS], n_ins);
@@ -845,13 +904,20 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
   }
 }
 
-// Heavy pairs of this sub-round: CHUNK-edge slices over all waves.
+// Heavy pairs of this sub-round: CHUNK-edge items, a contiguous range per
+// wave.
 __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const MSSearch *S, MSCtl *C, int ns, uint32_t L,
     const unsigned long long *__restrict__ labels, uint32_t total_waves) {
+  __shared__ unsigned long long olog_s[WPB][OUT_CAP64];
+  __shared__ unsigned long long oq_s[WPB][OUT_CAP64];
   const int l = lane();
   const int ci = L % 3, ni = (L + 1) % 3;
   const int pq = L & 1;
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  MSOut out;
+  out.log.init(olog_s[threadIdx.x / WAVE]);
+  out.q.init(oq_s[threadIdx.x / WAVE]);
   for (int s = 0; s < ns; ++s) {
     if (!C->active[s]) continue;
     MSSlot *cur = &C->slot[s][ci];
@@ -860,21 +926,28 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0 || C->full[s]) continue;
     const MSSearch M = S[s];
+    unsigned long long *qn = M.q[pq ^ 1];
     unsigned long long n_edges = 0, n_ins = 0, n_app = 0, n_items = 0;
     uint32_t probes = 0;
-    const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-    unsigned long long c = gw;   // first item static, then dynamic
-    while (c < Ct) {
-      if (l == 0) ++n_items;
+    const uint64_t per = (Ct + total_waves - 1) / total_waves;
+    uint64_t c = gw * per;
+    const uint64_t cend = min(Ct, c + per);
+    uint64_t own = 0;
+    if (c < cend) {
       uint64_t lo = 0, hi = H - 1;
       while (lo < hi) {
         const uint64_t mid = (lo + hi + 1) >> 1;
         if (M.hc[mid] <= c) lo = mid; else hi = mid - 1;
       }
-      const unsigned long long pr = M.hp[lo];
+      own = lo;
+    }
+    for (; c < cend; ++c) {
+      while (own + 1 < H && M.hc[own + 1] <= c) ++own;
+      if (l == 0) ++n_items;
+      const unsigned long long pr = M.hp[own];
       const uint32_t v = pair_vertex(pr), ell = pair_label(pr);
       const unsigned long long lv = labels[v];
-      const uint64_t k = c - M.hc[lo];
+      const uint64_t k = c - M.hc[own];
       const uint64_t eb = M.off[v] + k * CHUNK;
       const uint64_t ee = min(M.off[v + 1], eb + CHUNK);
       for (uint64_t j0 = eb; j0 < ee; j0 += WAVE) {
@@ -893,19 +966,12 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
             np = make_pair(x, ell);
           }
         }
-        ins = ms_log(M, C, s, ins, np);
         n_ins += ins;
-        const unsigned long long pos = wave_append64(&nxt->qsize, ins);
-        if (ins) {
-          if (pos < M.q_cap) M.q[pq ^ 1][pos] = np;
-          else C->full[s] = 1;
-          ++n_app;
-        }
+        n_app += ins;
+        out.put(M, C, s, qn, nxt, ins, ins, np);
       }
-      unsigned long long cd = 0;
-      if (l == 0) cd = atomicAdd(&cur->hgrab, 1ull);
-      c = __shfl(cd, 0) + total_waves;
     }
+    out.flush(M, C, s, qn, nxt);
     add_cnt(&C->cnt[s][M_HEAVY_EDGES], n_edges);
     add_cnt(&C->cnt[s][M_HEAVY_INS], n_ins);
     add_cnt(&C->cnt[s][M_HEAVY_APP], n_app);
@@ -1092,9 +1158,13 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   hipDeviceProp_t prop;
   PSC_HIP(hipGetDeviceProperties(&prop, device));
   num_cu = prop.multiProcessorCount;
-  PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
-  PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  // a handle may be re-bound to new graphs (serving lanes): the stream,
+  // events, pinned staging and device buffers persist and only grow
+  if (!stream) {
+    PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
+    PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  }
   int occ = 0;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
                                                        BLOCK, 0));
@@ -1113,7 +1183,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     PSC_TRY(vis[s].ensure(words * 4));
     PSC_TRY(rq[s][0].ensure(nn * 4));
     PSC_TRY(rq[s][1].ensure(nn * 4));
-    const uint64_t hcap = std::min<uint64_t>(nn, m / HEAVY_DEG + 1);
+    const uint64_t hcap = std::min<uint64_t>(nn, m / std::min(HEAVY_DEG, PULL_HEAVY) + 1);
     PSC_TRY(hv[s].ensure(hcap * 4));
     PSC_TRY(hc[s].ensure(hcap * 8));
   }
@@ -1507,14 +1577,15 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
     prof_begin(KK_MISC);
-    k_first_labels<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        P.as<uint32_t>(), nnz, nstart, G, vis[0].as<uint32_t>(),
-        vis[1].as<uint32_t>(), lab, gC.as<unsigned long long>(), big);
+    k_first_labels<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
+        n, voffd, G, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
+        gC.as<unsigned long long>(), big);
     prof_end();
-    kbytes[KK_MISC] += 4.0 * nnz + 8.0 * nnz + nnz / 4.0;
+    kbytes[KK_MISC] += 8.0 * n + 8.0 * nnz + nnz / 4.0;
     PSC_HIP(hipGetLastError());
-    PSC_TRY(select(SelNotDone{P.as<uint32_t>(), lab}, nnz,
-                   centers.as<uint32_t>(), nullptr, &n_rem));
+    PSC_TRY(select(SelNotBoth{P.as<uint32_t>(), vis[0].as<uint32_t>(),
+                              vis[1].as<uint32_t>()},
+                   nnz, centers.as<uint32_t>(), nullptr, &n_rem));
     kbytes[KK_SELECT] += 2 * 12.0 * nnz + 4.0 * n_rem;
     prof_begin(KK_MISC);
     k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(

@@ -10,11 +10,15 @@
 
 namespace psc {
 
-constexpr int REACH_RING = 1024;     // per-wave VGC queue (vertices, LDS)
+constexpr int REACH_RING = 512;      // per-wave VGC queue (vertices, LDS)
 constexpr int MS_RING = 512;         // per-wave VGC queue (pairs, LDS)
-constexpr uint32_t HEAVY_DEG = 2048; // degree above which edges are chunked
-constexpr uint32_t CHUNK = 2048;     // edges per heavy work item
-constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests serially
+constexpr int OUT_CAP = 512;         // per-wave staged appends (vertices)
+constexpr int OUT_CAP64 = 256;       // per-wave staged appends (pairs)
+constexpr uint32_t PULL_HEAVY = 64;  // pull remainder above which it is chunked
+constexpr uint32_t HEAVY_DEG = 512;  // degree above which edges are chunked
+constexpr uint32_t CHUNK = 1024;     // edges per heavy work item
+constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests first
+constexpr uint32_t VGC_MAX_DEG = 32; // only vertices this sparse stay local
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 

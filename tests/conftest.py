@@ -16,7 +16,8 @@ def load_psc():
     if "psc_amd" in sys.modules:
         return sys.modules["psc_amd"]
     spec = importlib.util.spec_from_file_location(
-        "psc_amd", os.path.join(PKG_DIR, "__init__.py"))
+        "psc_amd", os.path.join(PKG_DIR, "__init__.py"),
+        submodule_search_locations=[PKG_DIR])
     mod = importlib.util.module_from_spec(spec)
     sys.modules["psc_amd"] = mod
     spec.loader.exec_module(mod)

@@ -200,3 +200,40 @@ def test_cli_bad_file_fails_cleanly(tmp_path):
                          text=True, timeout=60)
     assert out.returncode == 1
     assert "cannot open" in out.stderr
+
+
+def test_serve_rejects_over_budget_without_gpu(psc):
+    """Admission control runs on the host: a request larger than the HBM
+    budget is refused before any device work."""
+    import psc_amd.serve as sv
+    reqs = sv.poisson_requests(psc, 3, 0.0, lo_log2=12, hi_log2=13, seed=1)
+    srv = sv.Server(psc, budget_bytes=1 << 16, lanes=2)
+    with pytest.raises(RuntimeError, match="over the"):
+        srv.serve(reqs)
+
+
+def test_autotune_grouping(psc):
+    import psc_amd.autotune as at
+    kind, n = psc.batch_spec(500, 10, 20, seed=4)
+    table = {c: {"tau": 64, "beta": 2.0 if c < 15 else 1.5, "dense_div": 20}
+             for c in range(10, 21)}
+    groups = at.group_by_class(n, table, max_groups=4)
+    idx = np.sort(np.concatenate([g[0] for g in groups]))
+    assert np.array_equal(idx, np.arange(500))
+    assert len(groups) <= 4
+    for ix, prm, cs in groups:
+        cls = [at.size_class(x) for x in n[ix]]
+        assert set(cls) <= set(cs)
+        assert np.allclose(prm["betas"], [2.0 if c < 15 else 1.5 for c in cls])
+
+
+def test_batch_spec_distribution(psc):
+    kind, n = psc.batch_spec(4000, 10, 20, seed=4)
+    assert set(np.unique(kind).tolist()) == {0, 1, 2}
+    assert n.min() >= 2 ** 10 * 0.9 and n.max() <= 2 ** 20 * 1.01
+    ln = np.log2(n.astype(np.float64))
+    assert abs(np.median(ln) - 15) < 0.5     # log-uniform in [10, 20]
+    rm = n[kind == 0]
+    assert np.all((rm & (rm - 1)) == 0)      # RMAT sizes are powers of two
+    side = np.sqrt(n[kind == 1].astype(np.float64))
+    assert np.all(side == np.round(side))    # grids are squares
