@@ -68,7 +68,9 @@ def bench_grid(args, psc):
                      "levels": [st["levels_fwd"], st["levels_bwd"]],
                      "ms_first": round(st["ms_first"], 3),
                      "ms_multi": round(st["ms_multi"], 3)})
-        log(f"tau {tau:7d}: {ms:9.3f} ms  levels {st['levels_fwd']}/{st['levels_bwd']}")
+        log(f"tau {tau:7d}: {ms:9.3f} ms  levels {st['levels_fwd']}/{st['levels_bwd']} "
+            f"first {st['ms_first']:.2f} multi {st['ms_multi']:.2f} init {st['ms_init']:.2f} "
+            f"rounds {st['rounds']} sub {st['sub_rounds']} syncs {st['host_syncs']}")
     labels = s.scc(beta=1.5, tau=512)
     g = dg.to_host()
     ok = bool(np.array_equal(oracle.canonical(labels), oracle.tarjan_canonical(g)))
@@ -105,8 +107,17 @@ def bench_batch(args, psc):
     at.run_groups(bss, prms)                       # warm-up (allocations)
     walls = [at.run_groups(bss, prms)[0] for _ in range(args.steps)]
     wall = float(np.median(walls))
-    _, labs = at.run_groups(bss, prms, labels=True)
+    _, labs = at.run_groups(bss, prms, labels=True, timing=1)
     log(f"batched: {wall * 1e3:.2f} ms for {G} graphs ({G / wall:.1f} graphs/s)")
+    for (ix, prm, cs), b in zip(groups, bss):
+        st = b.stats
+        log(f"  group {cs[0]}..{cs[-1]}: {len(ix)} graphs n={b.n} "
+            f"first {st['ms_first']:.1f} ms multi {st['ms_multi']:.1f} ms "
+            f"init {st['ms_init']:.1f} rounds {st['rounds']} sub-rounds "
+            f"{st['sub_rounds']} levels {st['levels_fwd']}/{st['levels_bwd']} "
+            f"syncs {st['host_syncs']} resizes {st['resizes']}")
+    if args.no_plain:
+        return
     # one graph per launch, same parameters; only psc_run is timed
     plain_s, mism, checked = 0.0, 0, 0
     rng = np.random.default_rng(0)
@@ -203,6 +214,7 @@ def main():
     ap.add_argument("--hi-log2", type=float, default=None)
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--tarjan-checks", type=int, default=64)
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--requests", type=int, default=1000)
     ap.add_argument("--rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0])
     ap.add_argument("--budget-gb", type=float, default=200.0)

@@ -127,7 +127,7 @@ def group_by_class(n, table, max_groups=4):
     return out
 
 
-def run_groups(batches, params, labels=False):
+def run_groups(batches, params, labels=False, timing=0):
     """Run BatchSCC objects concurrently, one host thread each.  Returns
     (wall seconds, per-batch label arrays or None)."""
     results = [None] * len(batches)
@@ -138,7 +138,7 @@ def run_groups(batches, params, labels=False):
             p = params[i]
             results[i] = batches[i].run(labels=labels, beta=p["beta"],
                                         beta_per_graph=p.get("betas"), tau=p["tau"],
-                                        dense_div=p["dense_div"], timing=0)
+                                        dense_div=p["dense_div"], timing=timing)
         except Exception as e:   # surfaced to the caller below
             errors.append(e)
 

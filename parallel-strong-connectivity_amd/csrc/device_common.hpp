@@ -149,15 +149,31 @@ __device__ __forceinline__ uint32_t pair_label(unsigned long long p) {
   return (uint32_t)p;
 }
 
+// Slot of a (vertex, label) pair.  The reference hashes the vertex alone
+// (src/resizable_table.h:113) because it walks all labels of a vertex along
+// one probe chain (iter_kv, :49-97); this path never walks per-vertex
+// chains (frontiers and the partition read logs of pairs), so it hashes
+// the whole pair and a vertex with many labels does not form one long
+// cluster.  Placement does not affect any result.
+__device__ __forceinline__ uint64_t pair_slot(unsigned long long kv) {
+  kv ^= kv >> 33;
+  kv *= 0xFF51AFD7ED558CCDull;
+  kv ^= kv >> 33;
+  kv *= 0xC4CEB9FE1A85EC53ull;
+  kv ^= kv >> 33;
+  return kv;
+}
+
 // Label-table insert (src/resizable_table.h:214-240): linear probing from
-// hash32(k); one 64-bit CAS claims an empty slot.  Returns 1 if (k,v) is
-// new, 0 if present, -1 if the probe limit was hit (table treated as full).
+// the pair's slot; one 64-bit CAS claims an empty slot.  Returns 1 if (k,v)
+// is new, 0 if present, -1 if the probe limit was hit (table treated as
+// full: it is doubled and the search rerun).
 __device__ __forceinline__ int table_insert(unsigned long long *T,
                                             uint64_t mask, uint32_t k,
                                             uint32_t v, uint32_t limit,
                                             uint32_t *probes) {
   const unsigned long long kv = make_pair(k, v);
-  uint64_t h = hash32(k) & mask;
+  uint64_t h = pair_slot(kv) & mask;
   for (uint32_t i = 0; i < limit; ++i) {
     unsigned long long s = __atomic_load_n(&T[h], __ATOMIC_RELAXED);
     if (s == kv) {
@@ -186,7 +202,7 @@ __device__ __forceinline__ bool table_contains(const unsigned long long *T,
                                                uint64_t mask, uint32_t k,
                                                uint32_t v, uint32_t *probes) {
   const unsigned long long kv = make_pair(k, v);
-  uint64_t h = hash32(k) & mask;
+  uint64_t h = pair_slot(kv) & mask;
   for (uint64_t i = 0; i <= mask; ++i) {
     unsigned long long s = T[h];
     if (s == kv) {

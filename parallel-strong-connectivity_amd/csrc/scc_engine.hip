@@ -1395,7 +1395,9 @@ int Engine::ms_prepare(int s, uint64_t slots, MSSearch *d) {
   d->q_cap = log_cap;
   d->hp = mhp[s].as<unsigned long long>();
   d->hc = mhc[s].as<unsigned long long>();
-  d->probe_limit = (uint32_t)std::min<uint64_t>(slots, 1u << 20);
+  // at load <= 1/2 linear probes are short; a run this long means the
+  // table overflowed (staged log appends may overshoot the load limit)
+  d->probe_limit = (uint32_t)std::min<uint64_t>(slots, 4096);
   d->pad = 0;
   prof_begin(KK_MISC);
   k_fill_u64<<<grid_for(slots, BLOCK, 8192), BLOCK, 0, stream>>>(
