@@ -145,6 +145,10 @@ void psc_destroy(psc_handle *h);
 int psc_device_alloc(size_t bytes, void **ptr);
 int psc_device_free(void *ptr);
 int psc_device_copy(void *dst, const void *src, size_t bytes);
+/* Page-locked host memory (hipHostMalloc): label buffers that downloads
+ * fill at full link speed. */
+int psc_host_alloc(size_t bytes, void **ptr);
+int psc_host_free(void *ptr);
 
 /* Message of the last error on this thread ("" if none). */
 const char *psc_last_error(void);
@@ -236,15 +240,21 @@ typedef struct psc_batch_desc {
 /* on_device = 1: the desc arrays are device pointers. */
 int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out);
 
-/* Fold device-resident graphs (arrays in device memory, local ids) into one
- * batch handle; the graphs are copied, the caller may free them after. */
+/* Fold device-resident graphs (arrays in device memory) into one batch
+ * handle; the graphs are copied, the caller may free them after.  Graph g's
+ * ids and offsets are shifted by id_base[g] / edge_base[g] (NULL means 0:
+ * local ids). */
 int psc_batch_create_from_graphs(const psc_graph *dev_graphs, uint64_t count,
-                                 psc_handle **out);
+                                 const uint64_t *id_base,
+                                 const uint64_t *edge_base, psc_handle **out);
 
 /* Re-bind a handle made by psc_batch_create_from_graphs (or psc_create) to
  * a new set of device graphs, reusing its stream and device buffers (they
- * only grow).  Used by serving lanes. */
-int psc_batch_rebind(psc_handle *h, const psc_graph *dev_graphs, uint64_t count);
+ * only grow).  Used by serving lanes.  Graph g's ids and offsets may be
+ * shifted by id_base[g] / edge_base[g] (views into a device batch, see
+ * psc_dev_batch_view); NULL means 0. */
+int psc_batch_rebind(psc_handle *h, const psc_graph *dev_graphs, uint64_t count,
+                     const uint64_t *id_base, const uint64_t *edge_base);
 
 /* Number of graphs of a handle (1 for psc_create) and their vertex prefix
  * (num_graphs + 1 entries). */
@@ -287,6 +297,10 @@ int psc_dev_batch_info(const psc_dev_batch *db, uint64_t *num_graphs,
                        uint64_t *vbase, uint64_t *ebase, uint32_t *kind);
 /* Handle over a device batch (borrowed; keep the batch alive). */
 int psc_batch_create_dev(const psc_dev_batch *db, psc_handle **out);
+/* Graph g of a device batch as a view into the union (no copy) and the
+ * id / offset bases to pass to psc_batch_rebind. */
+int psc_dev_batch_view(const psc_dev_batch *db, uint64_t g, psc_graph *out,
+                       uint64_t *id_base, uint64_t *edge_base);
 /* Graph g of a device batch as its own device graph with local ids. */
 int psc_dev_batch_extract(const psc_dev_batch *db, uint64_t g,
                           psc_dev_graph **out);

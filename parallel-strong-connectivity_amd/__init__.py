@@ -103,6 +103,8 @@ _sig("psc_handle_stream", C.c_void_p, C.c_void_p)
 _sig("psc_device_alloc", C.c_int, C.c_size_t, C.POINTER(C.c_void_p))
 _sig("psc_device_free", C.c_int, C.c_void_p)
 _sig("psc_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+_sig("psc_host_alloc", C.c_int, C.c_size_t, C.POINTER(C.c_void_p))
+_sig("psc_host_free", C.c_int, C.c_void_p)
 _sig("psc_host_graph_free", None, C.POINTER(CHostGraph))
 _sig("psc_read_bin", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.POINTER(CHostGraph)))
 _sig("psc_write_bin", C.c_int, C.c_char_p, C.POINTER(CGraph), C.c_int)
@@ -133,8 +135,11 @@ _sig("psc_batch_create", C.c_int, C.POINTER(CBatchDesc), C.c_int,
      C.POINTER(C.c_void_p))
 _sig("psc_handle_graphs", C.c_uint64, C.c_void_p)
 _sig("psc_batch_create_from_graphs", C.c_int, C.POINTER(CGraph), C.c_uint64,
-     C.POINTER(C.c_void_p))
-_sig("psc_batch_rebind", C.c_int, C.c_void_p, C.POINTER(CGraph), C.c_uint64)
+     u64p, u64p, C.POINTER(C.c_void_p))
+_sig("psc_batch_rebind", C.c_int, C.c_void_p, C.POINTER(CGraph), C.c_uint64,
+     u64p, u64p)
+_sig("psc_dev_batch_view", C.c_int, C.c_void_p, C.c_uint64, C.POINTER(CGraph),
+     u64p, u64p)
 _sig("psc_dev_batch_union", C.c_int, C.c_void_p, C.POINTER(CGraph))
 _sig("psc_handle_vbase", C.c_int, C.c_void_p, u64p)
 _sig("psc_batch_run", C.c_int, C.c_void_p, C.POINTER(CParams),
@@ -312,6 +317,41 @@ class DeviceArray:
             pass
 
 
+class PinnedLabels:
+    """A growable page-locked uint64 host buffer; labels downloaded into it
+    move at full link speed.  view(n) is valid until the next grow."""
+
+    def __init__(self, count=0):
+        self.ptr, self.cap = None, 0
+        self.grow(count)
+
+    def grow(self, count):
+        count = int(count)
+        if count <= self.cap:
+            return
+        self.free()
+        cap = max(count, 1 << 20)
+        p = C.c_void_p()
+        _check("psc_host_alloc", lib.psc_host_alloc(cap * 8, C.byref(p)))
+        self.ptr, self.cap = p.value, cap
+
+    def view(self, count):
+        self.grow(count)
+        buf = (C.c_uint64 * self.cap).from_address(self.ptr)
+        return np.ctypeslib.as_array(buf)[:count]
+
+    def free(self):
+        if self.ptr:
+            lib.psc_host_free(C.c_void_p(self.ptr))
+            self.ptr, self.cap = None, 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def default_params(**kw):
     p = CParams()
     lib.psc_params_default(C.byref(p))
@@ -469,6 +509,13 @@ class DeviceBatch:
         _check("psc_dev_batch_union", lib.psc_dev_batch_union(self._p, C.byref(g)))
         return g
 
+    def view(self, g):
+        """(CGraph view of graph g inside the union, id base, edge base)."""
+        cg, vb, eb = CGraph(), C.c_uint64(), C.c_uint64()
+        _check("psc_dev_batch_view", lib.psc_dev_batch_view(
+            self._p, g, C.byref(cg), C.byref(vb), C.byref(eb)))
+        return cg, vb.value, eb.value
+
     def extract(self, g):
         dgp = C.POINTER(CDevGraph)()
         _check("psc_dev_batch_extract", lib.psc_dev_batch_extract(self._p, g,
@@ -496,7 +543,7 @@ class BatchSCC:
         if device_graphs is not None:
             arr = (CGraph * len(device_graphs))(*[d.cgraph() for d in device_graphs])
             _check("psc_batch_create_from_graphs", lib.psc_batch_create_from_graphs(
-                arr, len(device_graphs), C.byref(h)))
+                arr, len(device_graphs), None, None, C.byref(h)))
             self._keep = None
         elif device_batch is not None:
             _check("psc_batch_create_dev", lib.psc_batch_create_dev(
@@ -539,9 +586,15 @@ class BatchSCC:
         _check("psc_handle_vbase", lib.psc_handle_vbase(self._h, _ptr(self.vbase, u64p)))
         self.n = int(self.vbase[-1])
 
-    def run(self, beta=1.5, beta_per_graph=None, labels=True, **kw):
+    def run(self, beta=1.5, beta_per_graph=None, labels=True, out=None, **kw):
+        """labels: download the labels (into `out` when given, e.g. a
+        PinnedLabels view of n entries)."""
         p = default_params(beta=beta, **kw)
-        out = np.empty(self.n, dtype=np.uint64) if labels else None
+        if labels and out is None:
+            out = np.empty(self.n, dtype=np.uint64)
+        if labels and (out.dtype != np.uint64 or out.size != self.n or
+                       not out.flags.c_contiguous):
+            raise ValueError("out must be a contiguous uint64 array of n entries")
         bp = None
         if beta_per_graph is not None:
             bpg = np.ascontiguousarray(beta_per_graph, dtype=np.float64)

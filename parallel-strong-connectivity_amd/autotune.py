@@ -152,3 +152,23 @@ def run_groups(batches, params, labels=False, timing=0):
     if errors:
         raise errors[0]
     return wall, results
+
+
+def main():
+    """python parallel-strong-connectivity_amd/autotune.py [lo hi [out.json]]"""
+    import importlib.util
+    import sys
+    spec = importlib.util.spec_from_file_location(
+        "psc_amd", os.path.join(HERE, "__init__.py"),
+        submodule_search_locations=[HERE])
+    psc = importlib.util.module_from_spec(spec)
+    sys.modules["psc_amd"] = psc
+    spec.loader.exec_module(psc)
+    lo = int(sys.argv[1]) if len(sys.argv) > 1 else 10
+    hi = int(sys.argv[2]) if len(sys.argv) > 2 else 24
+    path = sys.argv[3] if len(sys.argv) > 3 else TABLE_PATH
+    tune(psc, lo, hi, log=lambda s: print(s, flush=True), path=path)
+
+
+if __name__ == "__main__":
+    main()

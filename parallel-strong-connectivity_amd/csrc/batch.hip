@@ -87,16 +87,19 @@ __global__ void k_extract(const uint64_t *goff, const uint32_t *gadj,
 
 __global__ void k_fill_one(uint64_t *p, uint64_t v) { *p = v; }
 
-// one device graph (local ids) into its slot of a union
+// one device graph into its slot of a union: source offsets start at
+// `se0` and source ids at `sv0` (0 for a stand-alone graph, the graph's
+// prefixes for a view into another union)
 __global__ void k_rebase_one(const uint64_t *loff, const uint32_t *ladj,
                              uint64_t n, uint64_t m, uint64_t v0, uint64_t e0,
-                             uint64_t *goff, uint32_t *gadj) {
+                             uint64_t sv0, uint64_t se0, uint64_t *goff,
+                             uint32_t *gadj) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    goff[i] = e0 + loff[i];
+    goff[i] = e0 + (loff[i] - se0);
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m;
        j += (uint64_t)gridDim.x * blockDim.x)
-    gadj[j] = (uint32_t)(v0 + ladj[j]);
+    gadj[j] = (uint32_t)(v0 + (ladj[j] - sv0));
 }
 
 template <class T>
@@ -114,6 +117,26 @@ int dev_copy_in(const T *src, size_t count, int on_device, T **dst) {
     return PSC_ERR_HIP;
   }
   return PSC_OK;
+}
+
+// Device batches live in the device's stream-ordered memory pool (the
+// engine's buffers do too): a plain hipFree waits for all work on the
+// device, which would stall every other handle's stream each time a batch
+// is released.  Frees are ordered on a library stream.
+template <class T>
+static hipError_t pool_alloc(T **p, size_t bytes, hipStream_t st) {
+  return hipMallocAsync(reinterpret_cast<void **>(p), std::max<size_t>(bytes, 8), st);
+}
+
+static void pool_free(void *p) {
+  if (p) (void)hipFreeAsync(p, psc::pool_free_stream());
+}
+
+__global__ void k_gather_at(const uint64_t *__restrict__ src,
+                            const uint64_t *__restrict__ idx, uint32_t count,
+                            uint64_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = src[idx[i]];
 }
 
 // ------------------------------------------------------ batch generator
@@ -368,7 +391,8 @@ int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out) {
   auto *h = new (std::nothrow) psc_handle;
   psc::Engine *e = h ? &h->eng : nullptr;
   if (!h) return PSC_ERR_NOMEM;
-  rc = dev_copy_in(vb.data(), G + 1, 0, &d_vb);
+  rc = e->setup();
+  if (!rc) rc = dev_copy_in(vb.data(), G + 1, 0, &d_vb);
   if (!rc) rc = dev_copy_in(eb.data(), G + 1, 0, &d_eb);
   if (!rc) rc = dev_copy_in(b->offset, ntot + G, on_device, &d_lo);
   if (!rc) rc = dev_copy_in(b->in_offset, ntot + G, on_device, &d_lio);
@@ -379,18 +403,18 @@ int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out) {
   if (!rc) rc = e->g_adj.ensure(std::max<uint64_t>(mtot, 1) * 4);
   if (!rc) rc = e->g_iadj.ensure(std::max<uint64_t>(mtot, 1) * 4);
   if (rc) goto done;
-  k_rebase_offsets<<<blocks_for(ntot + 1), 256>>>(d_lo, d_vb, d_eb, G, ntot,
-                                                  e->g_off.as<uint64_t>());
-  k_rebase_offsets<<<blocks_for(ntot + 1), 256>>>(d_lio, d_vb, d_eb, G, ntot,
-                                                  e->g_ioff.as<uint64_t>());
+  k_rebase_offsets<<<blocks_for(ntot + 1), 256, 0, e->stream>>>(
+      d_lo, d_vb, d_eb, G, ntot, e->g_off.as<uint64_t>());
+  k_rebase_offsets<<<blocks_for(ntot + 1), 256, 0, e->stream>>>(
+      d_lio, d_vb, d_eb, G, ntot, e->g_ioff.as<uint64_t>());
   if (mtot) {
-    k_rebase_edges<<<blocks_for(mtot), 256>>>(d_le, d_vb, d_eb, G, mtot,
-                                              e->g_adj.as<uint32_t>());
-    k_rebase_edges<<<blocks_for(mtot), 256>>>(d_lie, d_vb, d_eb, G, mtot,
-                                              e->g_iadj.as<uint32_t>());
+    k_rebase_edges<<<blocks_for(mtot), 256, 0, e->stream>>>(d_le, d_vb, d_eb, G, mtot,
+                                                            e->g_adj.as<uint32_t>());
+    k_rebase_edges<<<blocks_for(mtot), 256, 0, e->stream>>>(d_lie, d_vb, d_eb, G, mtot,
+                                                            e->g_iadj.as<uint32_t>());
   }
   B_HIP(hipGetLastError());
-  B_HIP(hipDeviceSynchronize());
+  B_HIP(hipStreamSynchronize(e->stream));
   {
     psc_graph ug{ntot, mtot, e->g_off.as<uint64_t>(), e->g_adj.as<uint32_t>(),
                  e->g_ioff.as<uint64_t>(), e->g_iadj.as<uint32_t>()};
@@ -413,7 +437,8 @@ done:
 // edge prefix, ids by the vertex prefix.  The handle's buffers are reused
 // and only grow, so a serving lane re-binds the same handle batch after
 // batch.
-static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count) {
+static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count,
+                         const uint64_t *id_base, const uint64_t *edge_base) {
   const uint32_t G = (uint32_t)count;
   std::vector<uint64_t> vb(G + 1, 0), eb(G + 1, 0);
   for (uint32_t g = 0; g < G; ++g) {
@@ -426,22 +451,23 @@ static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count)
   }
   psc::Engine &e = h->eng;
   const uint64_t ntot = vb[G], mtot = eb[G];
-  int rc = e.g_off.ensure((ntot + 1) * 8);
+  int rc = e.setup();
+  if (!rc) rc = e.g_off.ensure((ntot + 1) * 8);
   if (!rc) rc = e.g_ioff.ensure((ntot + 1) * 8);
   if (!rc) rc = e.g_adj.ensure(std::max<uint64_t>(mtot, 1) * 4);
   if (!rc) rc = e.g_iadj.ensure(std::max<uint64_t>(mtot, 1) * 4);
   if (rc) return rc;
   hipStream_t st = e.stream;
   hipError_t he = hipSuccess;
-  if (!st) he = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   for (uint32_t g = 0; g < G && he == hipSuccess; ++g) {
     const psc_graph &x = graphs[g];
     if (x.n == 0) continue;
+    const uint64_t sv0 = id_base ? id_base[g] : 0, se0 = edge_base ? edge_base[g] : 0;
     k_rebase_one<<<blocks_for(std::max(x.n + 1, x.m)), 256, 0, st>>>(
-        x.offset, x.edges, x.n, x.m, vb[g], eb[g], e.g_off.as<uint64_t>() + vb[g],
-        e.g_adj.as<uint32_t>() + eb[g]);
+        x.offset, x.edges, x.n, x.m, vb[g], eb[g], sv0, se0,
+        e.g_off.as<uint64_t>() + vb[g], e.g_adj.as<uint32_t>() + eb[g]);
     k_rebase_one<<<blocks_for(std::max(x.n + 1, x.m)), 256, 0, st>>>(
-        x.in_offset, x.in_edges, x.n, x.m, vb[g], eb[g],
+        x.in_offset, x.in_edges, x.n, x.m, vb[g], eb[g], sv0, se0,
         e.g_ioff.as<uint64_t>() + vb[g], e.g_iadj.as<uint32_t>() + eb[g]);
     he = hipGetLastError();
   }
@@ -454,7 +480,6 @@ static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count)
     he = hipGetLastError();
   }
   if (he == hipSuccess) he = hipStreamSynchronize(st);
-  if (!e.stream && st) (void)hipStreamDestroy(st);
   if (he != hipSuccess) {
     psc::set_error(std::string("batch gather: ") + hipGetErrorString(he));
     return PSC_ERR_HIP;
@@ -465,12 +490,13 @@ static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count)
 }
 
 int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
+                                 const uint64_t *id_base, const uint64_t *edge_base,
                                  psc_handle **out) {
   if (!graphs || !out || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
   *out = nullptr;
   auto *h = new (std::nothrow) psc_handle;
   if (!h) return PSC_ERR_NOMEM;
-  int rc = gather_graphs(h, graphs, count);
+  int rc = gather_graphs(h, graphs, count, id_base, edge_base);
   if (rc) {
     delete h;
     return rc;
@@ -479,13 +505,29 @@ int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
   return PSC_OK;
 }
 
-int psc_batch_rebind(psc_handle *h, const psc_graph *graphs, uint64_t count) {
+int psc_batch_rebind(psc_handle *h, const psc_graph *graphs, uint64_t count,
+                     const uint64_t *id_base, const uint64_t *edge_base) {
   if (!h || !graphs || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
   if (!h->eng.owned_graph && h->eng.stream) {
     psc::set_error("psc_batch_rebind: handle borrows its graph");
     return PSC_ERR_INVALID;
   }
-  return gather_graphs(h, graphs, count);
+  return gather_graphs(h, graphs, count, id_base, edge_base);
+}
+
+// Graph g of a device batch as a view into its union (global ids from
+// vbase[g], offsets from ebase[g]); with id_base = vbase[g] and edge_base
+// = ebase[g] it can be gathered by psc_batch_rebind.
+int psc_dev_batch_view(const psc_dev_batch *db, uint64_t g, psc_graph *out,
+                       uint64_t *id_base, uint64_t *edge_base) {
+  if (!db || !out || g >= db->num_graphs) return PSC_ERR_INVALID;
+  const uint64_t v0 = db->vbase[g], e0 = db->ebase[g];
+  *out = psc_graph{db->vbase[g + 1] - v0, db->ebase[g + 1] - e0,
+                   db->g.offset + v0, db->g.edges + e0, db->g.in_offset + v0,
+                   db->g.in_edges + e0};
+  if (id_base) *id_base = v0;
+  if (edge_base) *edge_base = e0;
+  return PSC_OK;
 }
 
 int psc_dev_batch_union(const psc_dev_batch *db, psc_graph *out) {
@@ -522,11 +564,7 @@ int psc_batch_run(psc_handle *h, const psc_params *p, const double *beta_per_gra
   int rc = e.run(*p, e.labels.as<unsigned long long>(), stats ? stats : &tmp);
   e.beta_g.clear();
   if (rc) return rc;
-  if (labels_out && e.n &&
-      hipMemcpy(labels_out, e.labels.ptr, e.n * 8, hipMemcpyDeviceToHost) != hipSuccess) {
-    psc::set_error("hipMemcpy labels");
-    return PSC_ERR_HIP;
-  }
+  if (labels_out && e.n) return e.download_labels(labels_out);
   return PSC_OK;
 }
 
@@ -639,21 +677,26 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
   uint64_t m_done = 0;
   psc_dev_batch *db = nullptr;
   std::vector<uint64_t> eb(G + 1, 0);
+  uint64_t *d_vb = nullptr, *d_eb = nullptr;
+  psc::pool_setup();
   B_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  B_HIP(hipMalloc(&keys, std::max<uint64_t>(max_group, 1) * 8));
-  B_HIP(hipMalloc(&tmp, std::max<uint64_t>(max_group, 1) * 8));
-  B_HIP(hipMalloc(&cnt, 8));
-  B_HIP(hipMalloc(&d_g, G * sizeof(GenGraph)));
-  B_HIP(hipMalloc(&off, (ntot + 1) * 8));
-  B_HIP(hipMalloc(&ioff, (ntot + 1) * 8));
-  B_HIP(hipMalloc(&E, std::max<uint64_t>(total_keys, 1) * 4));
-  B_HIP(hipMalloc(&iE, std::max<uint64_t>(total_keys, 1) * 4));
+  B_HIP(pool_alloc(&keys, std::max<uint64_t>(max_group, 1) * 8, st));
+  B_HIP(pool_alloc(&tmp, std::max<uint64_t>(max_group, 1) * 8, st));
+  B_HIP(pool_alloc(&cnt, 8, st));
+  B_HIP(pool_alloc(&d_g, G * sizeof(GenGraph), st));
+  B_HIP(pool_alloc(&d_vb, (G + 1) * 8, st));
+  B_HIP(pool_alloc(&d_eb, (G + 1) * 8, st));
+  B_HIP(pool_alloc(&off, (ntot + 1) * 8, st));
+  B_HIP(pool_alloc(&ioff, (ntot + 1) * 8, st));
+  B_HIP(pool_alloc(&E, std::max<uint64_t>(total_keys, 1) * 4, st));
+  B_HIP(pool_alloc(&iE, std::max<uint64_t>(total_keys, 1) * 4, st));
+  B_HIP(hipMemcpyAsync(d_vb, vb.data(), (G + 1) * 8, hipMemcpyHostToDevice, st));
   B_HIP(rocprim::radix_sort_keys(nullptr, tb, keys, tmp, std::max<uint64_t>(max_group, 1),
                                  0, 64, st));
   B_HIP(rocprim::unique(nullptr, tb2, tmp, keys, cnt, std::max<uint64_t>(max_group, 1),
                         rocprim::equal_to<unsigned long long>(), st));
   tb = std::max(tb, tb2);
-  B_HIP(hipMalloc(&temp, std::max<size_t>(tb, 8)));
+  B_HIP(pool_alloc(&temp, std::max<size_t>(tb, 8), st));
   for (uint32_t g0 = 0; g0 < G;) {
     // ---- one group of graphs [g0, g1)
     uint32_t g1 = g0;
@@ -682,7 +725,8 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
       kgv.push_back(x);
     }
     if (!rg.empty()) {
-      B_HIP(hipMemcpy(d_g, rg.data(), rg.size() * sizeof(GenGraph), hipMemcpyHostToDevice));
+      B_HIP(hipMemcpyAsync(d_g, rg.data(), rg.size() * sizeof(GenGraph),
+                           hipMemcpyHostToDevice, st));
       k_batch_keys<<<blocks_for(items), 256, 0, st>>>(
           d_g, (uint32_t)rg.size(), items, psc::prob_threshold(0.57),
           psc::prob_threshold(0.76), psc::prob_threshold(0.95),
@@ -690,10 +734,11 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
       B_HIP(hipStreamSynchronize(st));
     }
     if (!kgv.empty()) {
-      B_HIP(hipMemcpy(d_g, kgv.data(), kgv.size() * sizeof(GenGraph), hipMemcpyHostToDevice));
-      if (cstart) (void)hipFree(cstart);
+      B_HIP(hipMemcpyAsync(d_g, kgv.data(), kgv.size() * sizeof(GenGraph),
+                           hipMemcpyHostToDevice, st));
+      if (cstart) (void)hipFreeAsync(cstart, st);
       cstart = nullptr;
-      B_HIP(hipMalloc(&cstart, (ncells + 1) * 8));
+      B_HIP(pool_alloc(&cstart, (ncells + 1) * 8, st));
       // cell keys in the k-NN key region (it holds 5 slots per point)
       unsigned long long *ck = keys + kgv[0].kbase;
       k_knn_cellkeys<<<blocks_for(kitems), 256, 0, st>>>(d_g, (uint32_t)kgv.size(),
@@ -709,7 +754,11 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
     // sort + unique the group's keys; DROP sorts last
     unsigned long long hcnt = 0, last = 0;
     uint64_t m = 0;
-    B_HIP(rocprim::radix_sort_keys(temp, tb, keys, tmp, gkeys, 0, 64, st));
+    // ids of the group are < vb[g1]: keys only use bits [0, 32 + idb)
+    // (DROP keeps all of them set, so it still sorts last)
+    unsigned idb = 1;
+    while (idb < 32 && (1ull << idb) < vb[g1]) ++idb;
+    B_HIP(rocprim::radix_sort_keys(temp, tb, keys, tmp, gkeys, 0, 32 + idb, st));
     B_HIP(rocprim::unique(temp, tb, tmp, keys, cnt, gkeys,
                           rocprim::equal_to<unsigned long long>(), st));
     B_HIP(hipMemcpyAsync(&hcnt, cnt, 8, hipMemcpyDeviceToHost, st));
@@ -724,20 +773,20 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
     k_low32<<<blocks_for(m), 256, 0, st>>>(keys, E + m_done, m);
     k_offsets_from_keys<<<blocks_for(gn + 1), 256, 0, st>>>(keys, m, gn, off + v0,
                                                             v0, m_done);
+    // in-CSR: the keys are in (src, dst) order, so a stable sort of the
+    // swapped keys on the dst bits alone yields (dst, src) order
     k_swap32<<<blocks_for(m), 256, 0, st>>>(keys, tmp, m);
-    if (m) B_HIP(rocprim::radix_sort_keys(temp, tb, tmp, keys, m, 0, 64, st));
+    if (m) B_HIP(rocprim::radix_sort_keys(temp, tb, tmp, keys, m, 32, 32 + idb, st));
     k_low32<<<blocks_for(m), 256, 0, st>>>(keys, iE + m_done, m);
     k_offsets_from_keys<<<blocks_for(gn + 1), 256, 0, st>>>(keys, m, gn, ioff + v0,
                                                             v0, m_done);
+    // per-graph edge prefix of the group: off at each graph's first vertex
+    k_gather_at<<<(g1 - g0 + 255) / 256, 256, 0, st>>>(off, d_vb + g0, g1 - g0,
+                                                       d_eb + g0);
     B_HIP(hipGetLastError());
+    B_HIP(hipMemcpyAsync(eb.data() + g0, d_eb + g0, (g1 - g0) * 8,
+                         hipMemcpyDeviceToHost, st));
     B_HIP(hipStreamSynchronize(st));
-    // per-graph edge prefix of the group
-    for (uint32_t g = g0; g <= g1 && g <= G; ++g) {
-      if (g == g1 && g1 < G) break;
-      uint64_t x = 0;
-      B_HIP(hipMemcpy(&x, off + vb[g], 8, hipMemcpyDeviceToHost));
-      eb[g] = x;
-    }
     m_done += m;
     g0 = g1;
   }
@@ -754,19 +803,26 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
   E = iE = nullptr;
 done:
   delete db;
+  if (st) (void)hipStreamSynchronize(st);
   for (void *p : {(void *)d_g, (void *)keys, (void *)tmp, (void *)cnt, (void *)cstart,
-                  (void *)off, (void *)ioff, (void *)E, (void *)iE, temp})
-    if (p) (void)hipFree(p);
-  if (st) (void)hipStreamDestroy(st);
+                  (void *)d_vb, (void *)d_eb, (void *)off, (void *)ioff, (void *)E,
+                  (void *)iE, temp})
+    if (p) (void)hipFreeAsync(p, st);
+  if (st) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
   return rc;
 }
 
+// The caller guarantees that no work using the batch is still in flight
+// (every handle that read it has returned from its run / rebind).
 void psc_dev_batch_free(psc_dev_batch *db) {
   if (!db) return;
-  (void)hipFree((void *)db->g.offset);
-  (void)hipFree((void *)db->g.edges);
-  (void)hipFree((void *)db->g.in_offset);
-  (void)hipFree((void *)db->g.in_edges);
+  pool_free((void *)db->g.offset);
+  pool_free((void *)db->g.edges);
+  pool_free((void *)db->g.in_offset);
+  pool_free((void *)db->g.in_edges);
   delete db;
 }
 

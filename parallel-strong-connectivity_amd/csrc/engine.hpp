@@ -19,9 +19,16 @@ struct MSSearch;
 void set_error(const std::string &msg);
 extern thread_local std::string g_last_error;
 
+// Device memory comes from the device's stream-ordered pool, allocated
+// and freed in the order of the owning handle's stream: growing a buffer
+// never waits for other streams (a plain hipFree waits for the device).
+void pool_setup();
+hipStream_t pool_free_stream();
+
 struct DevBuf {
   void *ptr = nullptr;
   size_t cap = 0;
+  hipStream_t *owner = nullptr;   // the stream that orders its (re)allocation
   int ensure(size_t bytes);
   void release();
   template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -79,7 +86,10 @@ struct Engine {
   unsigned char h_ms[256];
 
   ~Engine();
+  Engine();
   std::vector<DevBuf *> all_bufs();
+  // one-time device set-up: stream, events, pinned staging, launch shapes
+  int setup();
   int init(const psc_graph *dg, bool owned, uint32_t ngraphs = 1,
            const uint64_t *voff = nullptr);
   static constexpr int NREG = 6;
@@ -95,6 +105,8 @@ struct Engine {
   int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count,
              unsigned long long *count_dev = nullptr);
   int poll(void *dst, const void *src, size_t bytes);
+  // copy the n labels to host memory on this handle's stream
+  int download_labels(uint64_t *dst);
   int prof_reset(int level);
   void prof_begin(int kind);
   void prof_end();

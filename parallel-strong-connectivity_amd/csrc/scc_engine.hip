@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
@@ -1089,38 +1090,70 @@ static int grid_for(uint64_t n, int per_block = BLOCK, int cap = 65536) {
   return (int)g;
 }
 
+static std::once_flag g_pool_once;
+static hipStream_t g_free_stream = nullptr;
+
+void pool_setup() {
+  std::call_once(g_pool_once, [] {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = ~0ull;   // keep freed memory in the pool for reuse
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    (void)hipStreamCreateWithFlags(&g_free_stream, hipStreamNonBlocking);
+  });
+}
+
+hipStream_t pool_free_stream() {
+  pool_setup();
+  return g_free_stream;
+}
+
 int DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return PSC_OK;
+  pool_setup();
+  hipStream_t st = owner ? *owner : nullptr;
   if (ptr) {
-    hipError_t e = hipFree(ptr);
+    hipError_t e = hipFreeAsync(ptr, st);
     ptr = nullptr;
+    // a buffer that has to grow again gets headroom
+    bytes = std::max(bytes, cap + cap / 2);
     cap = 0;
     if (e != hipSuccess) {
-      set_error(std::string("hipFree: ") + hipGetErrorString(e));
+      set_error(std::string("hipFreeAsync: ") + hipGetErrorString(e));
       return PSC_ERR_HIP;
     }
   }
   if (bytes == 0) return PSC_OK;
-  hipError_t e = hipMalloc(&ptr, bytes);
+  hipError_t e = hipMallocAsync(&ptr, bytes, st);
   if (e != hipSuccess) {
     ptr = nullptr;
-    set_error("hipMalloc(" + std::to_string(bytes) + " B): " +
+    set_error("hipMallocAsync(" + std::to_string(bytes) + " B): " +
               hipGetErrorString(e));
     return PSC_ERR_NOMEM;
   }
   cap = bytes;
   return PSC_OK;
 }
+
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
   ptr = nullptr;
   cap = 0;
+}
+
+Engine::Engine() {
+  for (DevBuf *b : all_bufs()) b->owner = &stream;
+  for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->owner = &stream;
 }
 
 Engine::~Engine() {
   for (DevBuf *b : all_bufs()) b->release();
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
+  if (stream) (void)hipStreamSynchronize(stream);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (h_gpin) (void)hipHostFree(h_gpin);
   for (auto &e : ev) if (e) (void)hipEventDestroy(e);
@@ -1136,6 +1169,39 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
           &gid_d};
+}
+
+int Engine::download_labels(uint64_t *dst) {
+  hipError_t he = hipMemcpyAsync(dst, labels.ptr, n * 8, hipMemcpyDeviceToHost, stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(stream);
+  if (he != hipSuccess) {
+    psc::set_error(std::string("labels download: ") + hipGetErrorString(he));
+    return PSC_ERR_HIP;
+  }
+  return PSC_OK;
+}
+
+int Engine::setup() {
+  if (stream) return PSC_OK;
+  pool_setup();
+  PSC_HIP(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  PSC_HIP(hipGetDeviceProperties(&prop, device));
+  num_cu = prop.multiProcessorCount;
+  PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
+  PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  int occ = 0;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
+                                                       BLOCK, 0));
+  grid_step = std::max(1, occ) * num_cu;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ms_expand,
+                                                       BLOCK, 0));
+  grid_ms = std::max(1, occ) * num_cu;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_heavy,
+                                                       BLOCK, 0));
+  grid_heavy = std::max(1, occ) * num_cu;
+  return PSC_OK;
 }
 
 int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
@@ -1154,27 +1220,10 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   d_adj = dg->edges;
   d_ioff = dg->in_offset;
   d_iadj = dg->in_edges;
-  PSC_HIP(hipGetDevice(&device));
-  hipDeviceProp_t prop;
-  PSC_HIP(hipGetDeviceProperties(&prop, device));
-  num_cu = prop.multiProcessorCount;
   // a handle may be re-bound to new graphs (serving lanes): the stream,
-  // events, pinned staging and device buffers persist and only grow
-  if (!stream) {
-    PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
-    PSC_HIP(hipHostMalloc(&h_pinned, 4096));
-  }
-  int occ = 0;
-  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
-                                                       BLOCK, 0));
-  grid_step = std::max(1, occ) * num_cu;
-  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ms_expand,
-                                                       BLOCK, 0));
-  grid_ms = std::max(1, occ) * num_cu;
-  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_heavy,
-                                                       BLOCK, 0));
-  grid_heavy = std::max(1, occ) * num_cu;
+  // events, pinned staging, launch shapes and device buffers persist and
+  // buffers only grow
+  PSC_TRY(setup());
 
   const uint64_t nn = std::max<uint64_t>(n, 1);
   const uint64_t words = (nn + 31) / 32;
@@ -1860,7 +1909,8 @@ int psc_create(const psc_graph *g, psc_handle **out) {
   if (!h) return PSC_ERR_NOMEM;
   Engine &e = h->eng;
   const uint64_t n1 = g->n + 1;
-  r = e.g_off.ensure(n1 * 8);
+  r = e.setup();
+  if (!r) r = e.g_off.ensure(n1 * 8);
   if (!r) r = e.g_ioff.ensure(n1 * 8);
   if (!r) r = e.g_adj.ensure(std::max<uint64_t>(g->m, 1) * 4);
   if (!r) r = e.g_iadj.ensure(std::max<uint64_t>(g->m, 1) * 4);
@@ -1870,19 +1920,18 @@ int psc_create(const psc_graph *g, psc_handle **out) {
     return r;
   }
   hipError_t he = hipSuccess;
-  if (g->n) {
-    he = hipMemcpy(e.g_off.ptr, g->offset, n1 * 8, hipMemcpyHostToDevice);
-    if (he == hipSuccess)
-      he = hipMemcpy(e.g_ioff.ptr, g->in_offset, n1 * 8, hipMemcpyHostToDevice);
-  } else {
-    uint64_t z = 0;
-    he = hipMemcpy(e.g_off.ptr, &z, 8, hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(e.g_ioff.ptr, &z, 8, hipMemcpyHostToDevice);
-  }
+  const uint64_t z = 0;
+  const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+  he = hipMemcpyAsync(e.g_off.ptr, g->n ? g->offset : &z, (g->n ? n1 : 1) * 8, h2d,
+                      e.stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(e.g_ioff.ptr, g->n ? g->in_offset : &z, (g->n ? n1 : 1) * 8,
+                        h2d, e.stream);
   if (he == hipSuccess && g->m)
-    he = hipMemcpy(e.g_adj.ptr, g->edges, g->m * 4, hipMemcpyHostToDevice);
+    he = hipMemcpyAsync(e.g_adj.ptr, g->edges, g->m * 4, h2d, e.stream);
   if (he == hipSuccess && g->m)
-    he = hipMemcpy(e.g_iadj.ptr, g->in_edges, g->m * 4, hipMemcpyHostToDevice);
+    he = hipMemcpyAsync(e.g_iadj.ptr, g->in_edges, g->m * 4, h2d, e.stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e.stream);
   if (he != hipSuccess) {
     psc::set_error(std::string("hipMemcpy graph: ") + hipGetErrorString(he));
     delete h;
@@ -1934,14 +1983,7 @@ int psc_run(psc_handle *h, const psc_params *p, uint64_t *labels_out,
   Engine &e = h->eng;
   int r = run_common(h, p, e.labels.as<unsigned long long>(), stats);
   if (r) return r;
-  if (labels_out && e.n) {
-    hipError_t he = hipMemcpy(labels_out, e.labels.ptr, e.n * 8,
-                              hipMemcpyDeviceToHost);
-    if (he != hipSuccess) {
-      psc::set_error(std::string("hipMemcpy labels: ") + hipGetErrorString(he));
-      return PSC_ERR_HIP;
-    }
-  }
+  if (labels_out && e.n) return e.download_labels(labels_out);
   return PSC_OK;
 }
 
@@ -1991,6 +2033,27 @@ int psc_device_free(void *ptr) {
   hipError_t e = hipFree(ptr);
   if (e != hipSuccess) {
     psc::set_error(std::string("hipFree: ") + hipGetErrorString(e));
+    return PSC_ERR_HIP;
+  }
+  return PSC_OK;
+}
+
+int psc_host_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return PSC_ERR_INVALID;
+  *ptr = nullptr;
+  hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    psc::set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    return PSC_ERR_NOMEM;
+  }
+  return PSC_OK;
+}
+
+int psc_host_free(void *ptr) {
+  hipError_t e = hipHostFree(ptr);
+  if (e != hipSuccess) {
+    psc::set_error(std::string("hipHostFree: ") + hipGetErrorString(e));
     return PSC_ERR_HIP;
   }
   return PSC_OK;

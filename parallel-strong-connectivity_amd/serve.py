@@ -69,11 +69,30 @@ def est_bytes(kind, n):
     return int(2 * csr + 96 * n + (1 << 20))
 
 
+class _Pool:
+    """A device batch shared by the requests admitted together; freed when
+    the last of them completes."""
+
+    def __init__(self, db, users):
+        self.db = db
+        self.users = users
+        self.lock = threading.Lock()
+
+    def release(self):
+        with self.lock:
+            self.users -= 1
+            last = self.users == 0
+        if last:
+            self.db.free()
+
+
 class Server:
     def __init__(self, psc, budget_bytes=200 << 30, lanes=4, mode="batched",
                  max_batch_vertices=1 << 27, tau=512, dense_div=20,
-                 keep_labels=False, target_batch=16, max_wait_s=0.002):
+                 keep_labels=False, target_batch=16, max_wait_s=0.002,
+                 max_admit=256):
         self.psc = psc
+        self.max_admit = max_admit
         self.target_batch = target_batch
         self.max_wait_s = max_wait_s
         self.budget = int(budget_bytes)
@@ -84,45 +103,73 @@ class Server:
         self.dense_div = dense_div
         self.keep_labels = keep_labels
         self.cv = threading.Condition()
-        self.ready = []          # (request, DeviceBatch of one graph)
+        self.ready = []          # (request, _Pool, index in the pool, bytes)
         self.in_use = 0
         self.peak_in_use = 0
         self.finished = 0
         self.errors = []
+        # seconds spent per phase (summed over threads)
+        self.phase = {"generate": 0.0, "gather": 0.0, "scc": 0.0, "labels": 0.0}
 
     # ---------------------------------------------------------- admission
     def _admit_loop(self, reqs, t0):
-        for r in reqs:
-            delay = t0 + r.arrival - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            need = est_bytes(r.kind, r.n)
-            if need > self.budget:
-                self.errors.append(RuntimeError(
-                    f"request {r.rid} needs {need} B, over the {self.budget} B budget"))
-                with self.cv:
-                    self.finished += 1
-                    self.cv.notify_all()
+        """Admit every request that has arrived and fits the budget; the
+        graphs of one admission tick are generated together as one device
+        batch (one generator call), and lanes gather per-request views."""
+        i = 0
+        total = len(reqs)
+        while i < total:
+            now = time.perf_counter() - t0
+            if reqs[i].arrival > now:
+                time.sleep(reqs[i].arrival - now)
                 continue
+            group = []
             with self.cv:
-                while self.in_use + need > self.budget:
-                    self.cv.wait()
-                self.in_use += need
-                self.peak_in_use = max(self.peak_in_use, self.in_use)
+                while i < total and reqs[i].arrival <= time.perf_counter() - t0:
+                    r = reqs[i]
+                    need = est_bytes(r.kind, r.n)
+                    if need > self.budget:
+                        self.errors.append(RuntimeError(
+                            f"request {r.rid} needs {need} B, over the "
+                            f"{self.budget} B budget"))
+                        self.finished += 1
+                        i += 1
+                        self.cv.notify_all()
+                        continue
+                    if self.in_use + need > self.budget:
+                        if group:
+                            break
+                        self.cv.wait()      # wait for memory to free up
+                        continue
+                    self.in_use += need
+                    self.peak_in_use = max(self.peak_in_use, self.in_use)
+                    group.append((r, need))
+                    i += 1
+                    if len(group) >= self.max_admit:
+                        break
+            if not group:
+                continue
+            tg = time.perf_counter()
             try:
-                db = self.psc.DeviceBatch(np.array([r.kind], np.uint32),
-                                          np.array([r.n], np.uint64),
-                                          seeds=np.array([r.seed], np.uint64))
+                db = self.psc.DeviceBatch(
+                    np.array([r.kind for r, _ in group], np.uint32),
+                    np.array([r.n for r, _ in group], np.uint64),
+                    seeds=np.array([r.seed for r, _ in group], np.uint64))
             except Exception as e:
                 self.errors.append(e)
                 with self.cv:
-                    self.in_use -= need
-                    self.finished += 1
+                    for _, need in group:
+                        self.in_use -= need
+                    self.finished += len(group)
                     self.cv.notify_all()
                 continue
-            r.admitted = time.perf_counter() - t0
+            shared = _Pool(db, len(group))
+            adm = time.perf_counter() - t0
+            self.phase["generate"] += adm + t0 - tg
             with self.cv:
-                self.ready.append((r, db, need))
+                for j, (r, need) in enumerate(group):
+                    r.admitted = adm
+                    self.ready.append((r, shared, j, need))
                 self.cv.notify_all()
 
     # --------------------------------------------------------------- lanes
@@ -131,15 +178,17 @@ class Server:
         if self.mode == "plain":
             return [self.ready.pop(0)]
         take, nv = [], 0
-        while self.ready and (not take or nv + self.ready[0][1].n <= self.max_batch_vertices):
+        while self.ready and (not take or
+                              nv + self.ready[0][0].n <= self.max_batch_vertices):
             item = self.ready.pop(0)
-            nv += item[1].n
+            nv += item[0].n
             take.append(item)
         return take
 
     def _lane_loop(self, total, t0):
         psc = self.psc
         handle = None
+        pinned = psc.PinnedLabels()
         try:
             while True:
                 with self.cv:
@@ -164,30 +213,40 @@ class Server:
                             continue
                     items = self._take()
                 start = time.perf_counter() - t0
-                graphs = [db.union_graph() for _, db, _ in items]
-                arr = (psc.CGraph * len(graphs))(*graphs)
+                views = [pool.db.view(j) for _, pool, j, _ in items]
+                arr = (psc.CGraph * len(views))(*[v[0] for v in views])
+                vb = np.array([v[1] for v in views], dtype=np.uint64)
+                eb = np.array([v[2] for v in views], dtype=np.uint64)
+                vbp, ebp = psc._ptr(vb, psc.u64p), psc._ptr(eb, psc.u64p)
                 if handle is None:
                     h = C.c_void_p()
                     psc._check("psc_batch_create_from_graphs",
-                               psc.lib.psc_batch_create_from_graphs(arr, len(graphs),
-                                                                    C.byref(h)))
+                               psc.lib.psc_batch_create_from_graphs(
+                                   arr, len(views), vbp, ebp, C.byref(h)))
                     handle = psc.BatchSCC._wrap(h)
                 else:
                     psc._check("psc_batch_rebind",
-                               psc.lib.psc_batch_rebind(handle._h, arr, len(graphs)))
+                               psc.lib.psc_batch_rebind(handle._h, arr, len(views),
+                                                        vbp, ebp))
                     handle._refresh()
-                betas = np.array([r.beta for r, _, _ in items], dtype=np.float64)
+                t1 = time.perf_counter()
+                betas = np.array([r.beta for r, _, _, _ in items], dtype=np.float64)
                 labels = handle.run(beta_per_graph=betas, tau=self.tau,
-                                    dense_div=self.dense_div, timing=0)
+                                    dense_div=self.dense_div, timing=0,
+                                    out=pinned.view(handle.n))
                 done = time.perf_counter() - t0
+                scc_s = handle.stats["ms_total"] / 1e3
+                self.phase["gather"] += t1 - t0 - start
+                self.phase["scc"] += scc_s
+                self.phase["labels"] += done + t0 - t1 - scc_s
                 parts = handle.split(labels)
-                for (r, db, need), lab in zip(items, parts):
+                for (r, pool, j, need), lab in zip(items, parts):
                     r.started, r.done, r.batch = start, done, len(items)
                     if self.keep_labels:
                         r.labels = lab.copy()
-                    db.free()
+                    pool.release()
                 with self.cv:
-                    for _, _, need in items:
+                    for _, _, _, need in items:
                         self.in_use -= need
                     self.finished += len(items)
                     self.cv.notify_all()
@@ -199,6 +258,7 @@ class Server:
         finally:
             if handle is not None:
                 handle.close()
+            pinned.free()
 
     def serve(self, reqs):
         """Run the request stream; returns a summary dict."""
@@ -225,4 +285,5 @@ class Server:
             "mean_batch": round(float(np.mean([r.batch for r in reqs])), 2),
             "peak_pool_GB": round(self.peak_in_use / 2**30, 2),
             "budget_GB": round(self.budget / 2**30, 2),
+            "phase_s": {k: round(v, 3) for k, v in self.phase.items()},
         }
