@@ -193,6 +193,27 @@ def bench_serve(args, psc):
                       "budget_GB": args.budget_gb, "sweep": rows}), flush=True)
 
 
+def bench_copy(args, psc):
+    """Calibration for the HBM counters: device-to-device copies of a known
+    size (the runtime's plain copy kernel); FETCH_SIZE / WRITE_SIZE of these
+    dispatches against the bytes copied give the counters' scale."""
+    nbytes = args.copy_mb << 20
+    a = psc.DeviceArray(nbytes // 8)
+    b = psc.DeviceArray(nbytes // 8)
+    for _ in range(args.warmup):
+        psc.lib.psc_device_copy(psc.C.c_void_p(b.ptr), psc.C.c_void_p(a.ptr), nbytes)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        psc._check("psc_device_copy", psc.lib.psc_device_copy(
+            psc.C.c_void_p(b.ptr), psc.C.c_void_p(a.ptr), nbytes))
+    dt = (time.perf_counter() - t) / args.steps
+    print(json.dumps({"metric": "device copy", "bytes_per_copy": nbytes,
+                      "ms_per_copy": round(dt * 1e3, 4),
+                      "GBs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+    a.free()
+    b.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,7 +228,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="warm-up + steps only (for rocprofv3 runs)")
-    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve"), default="c3")
+    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve", "copy"),
+                    default="c3")
+    ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--grid-side", type=int, default=2048)
     ap.add_argument("--taus", type=int, nargs="*", default=None)
     ap.add_argument("--graphs", type=int, default=2048)
@@ -227,6 +250,8 @@ def main():
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
     psc = ge.load_package()
+    if args.mode == "copy":
+        return bench_copy(args, psc)
     if args.mode == "grid":
         return bench_grid(args, psc)
     if args.mode == "batch":

@@ -77,13 +77,17 @@ struct Engine {
   std::vector<uint64_t> voff_h;
   std::vector<double> beta_g;     // per-graph pivot-batch growth (batches)
   DevBuf voff_d, gA, gB, gC, gD, gid_d;
+  DevBuf rnd;   // per-round windows: prefix (A+1), sources (A), graphs (A)
   unsigned long long *h_gpin = nullptr;   // pinned per-graph staging
   size_t h_gpin_cap = 0;
   uint64_t log_n[2] = {0, 0};
   uint32_t ms_levels_hint = 2;   // sub-rounds of the previous multi-search
+  uint32_t reach_levels_hint = 0; // levels of the previous first reach
   uint64_t syncs = 0;
   unsigned char h_reach[256];
   unsigned char h_ms[256];
+  unsigned char h_ms_dev[256];   // descriptors last copied to the device
+  bool ms_dev_valid = false;
 
   ~Engine();
   Engine();
@@ -97,6 +101,8 @@ struct Engine {
   unsigned long long *region(int r);
   int upload(DevBuf &dst, const uint64_t *src, size_t count, int reg);
   int upload32(DevBuf &dst, const uint32_t *src, size_t count, int reg);
+  int upload_round(const std::vector<uint64_t> &wpre, const std::vector<uint64_t> &wsrc,
+                   const std::vector<uint32_t> &wgid);
   int run(const psc_params &p, unsigned long long *d_out, psc_stats *st);
   int status(const unsigned long long *d_labels, uint64_t *n_scc,
              uint64_t *largest);
@@ -114,7 +120,7 @@ struct Engine {
   int first_reach(const psc_params &p, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
   int multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
-                   uint64_t slots[2], psc_stats *st);
+                   const uint32_t *wgid_d, uint64_t slots[2], psc_stats *st);
 };
 
 }  // namespace psc

@@ -155,6 +155,59 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
   }
 }
 
+// Stable select of a short list in one block of SMALL_BLOCK threads (one
+// launch instead of three); each thread issues the predicates of
+// SMALL_ITEMS items before ranking them.  With wpre given (the list is the
+// concatenation of A windows), fstart[a] receives the number of selected
+// items before window a, fstart[A] the total.
+constexpr int SMALL_BLOCK = 1024, SMALL_WAVES = SMALL_BLOCK / WAVE, SMALL_ITEMS = 4;
+constexpr uint64_t SMALL_SELECT = (uint64_t)SMALL_BLOCK * SMALL_ITEMS * 4;
+template <class F>
+__global__ __launch_bounds__(SMALL_BLOCK) void k_sel_small(
+    F f, uint64_t len, uint32_t *sel, unsigned long long *count_dev,
+    const unsigned long long *wpre, uint32_t A, unsigned long long *fstart) {
+  __shared__ uint32_t wc[SMALL_ITEMS][SMALL_WAVES];
+  const int w = threadIdx.x / WAVE;
+  unsigned long long carry = 0;
+  for (uint64_t b = 0; b < len; b += (uint64_t)SMALL_BLOCK * SMALL_ITEMS) {
+    bool p[SMALL_ITEMS];
+    uint64_t m[SMALL_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SMALL_ITEMS; ++j) {
+      const uint64_t t = b + (uint64_t)j * SMALL_BLOCK + threadIdx.x;
+      p[j] = t < len && f.pred(t);
+    }
+#pragma unroll
+    for (int j = 0; j < SMALL_ITEMS; ++j) {
+      m[j] = ballot(p[j]);
+      if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m[j]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SMALL_ITEMS; ++j) {
+      uint32_t before = 0, tot = 0;
+#pragma unroll
+      for (int x = 0; x < SMALL_WAVES; ++x) {
+        before += x < w ? wc[j][x] : 0u;
+        tot += wc[j][x];
+      }
+      const uint64_t t = b + (uint64_t)j * SMALL_BLOCK + threadIdx.x;
+      const unsigned long long r = carry + before + lanes_below(m[j]);
+      if (p[j]) sel[r] = f.val(t);
+      if (fstart && t < len) {
+        const uint32_t a = seg_of(wpre, A, t);
+        if (wpre[a] == t) fstart[a] = r;
+      }
+      carry += tot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *count_dev = carry;
+    if (fstart) fstart[A] = carry;
+  }
+}
+
 // pred: vertex has non-zero in- and out-degree (src/scc.hpp:295-302)
 struct SelNonZero {
   const uint64_t *off, *ioff;
@@ -400,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
     const ReachSearch R = S[s];
     const int mode = reach_mode(cur, R);
     const uint32_t nf = cur->qsize;
-    uint32_t *qn = R.q[p ^ 1];
+    uint32_t *qn = (p ? R.q[0] : R.q[1]);
     if (mode == MODE_PUSH) {
       // contiguous share of the frontier, S seeds at a time
       const uint64_t per = (nf + (uint64_t)total_waves - 1) / total_waves;
@@ -410,10 +463,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
       while (g < gend) {
         const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
-        if ((uint32_t)l < cnt) ring[l] = R.q[p][g + l];
+        if ((uint32_t)l < cnt) ring[l] = (p ? R.q[1] : R.q[0])[g + l];
         g += cnt;
         uint32_t head = 0, tail = cnt;
         uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+        uint64_t ebudget = budget * VGC_EDGE_MULT;
         wave_sync();
         while (head < tail) {
           const uint32_t k = min(64u, tail - head);
@@ -433,6 +487,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
           const uint32_t total = __shfl(inc, WAVE - 1);
           sc[l] = inc;
           st[l] = b;
+          ebudget = ebudget > total ? ebudget - total : 0;
+          if (!ebudget) budget = 0;
           wave_sync();
           for (uint32_t base = 0; base < total; base += WAVE) {
             const uint32_t e = base + l;
@@ -586,7 +642,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0) continue;
     const ReachSearch R = S[s];
-    uint32_t *qn = R.q[p ^ 1];
+    uint32_t *qn = (p ? R.q[0] : R.q[1]);
     const bool pull = reach_mode(cur, R) == MODE_PULL;
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
@@ -663,77 +719,108 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
 // First-SCC labelling (src/scc.hpp:340-347): both reaches -> found SCC,
 // one reach -> new subproblem label; lab1[g] = |trimmed of g| + 1.  Runs in
 // vertex order (coalesced label writes); trimmed vertices already hold a
-// non-zero label and are skipped.
-__global__ void k_first_labels(uint64_t n, const unsigned long long *voff,
-                               uint32_t G, const uint32_t *vf, const uint32_t *vb,
-                               unsigned long long *labels,
-                               const unsigned long long *lab1,
-                               unsigned long long *big) {
+// non-zero label and are skipped.  Each thread issues the loads of
+// FL_UNROLL vertices before using any of them, and the found-SCC count is
+// reduced per block (one atomic per block).
+constexpr int FL_UNROLL = 4;
+__global__ __launch_bounds__(BLOCK) void k_first_labels(
+    uint64_t n, const unsigned long long *voff, uint32_t G, const uint32_t *vf,
+    const uint32_t *vb, unsigned long long *labels,
+    const unsigned long long *lab1, unsigned long long *big) {
+  __shared__ unsigned long long red[WPB];
   unsigned long long c = 0;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
-       v += (uint64_t)gridDim.x * blockDim.x) {
-    if (labels[v]) continue;
-    const bool a = bit_test(vf, (uint32_t)v), b = bit_test(vb, (uint32_t)v);
-    if (a || b) {
-      const unsigned long long label = lab1[seg_of(voff, G, v)];
-      if (a && b) {
-        labels[v] = label | TOP_BIT;
-        ++c;
-      } else {
-        labels[v] = label;
-      }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const unsigned long long lab_one = lab1[0];
+  for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n;
+       base += stride * FL_UNROLL) {
+    unsigned long long lv[FL_UNROLL];
+    uint32_t wf[FL_UNROLL], wb[FL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < FL_UNROLL; ++u) {
+      const uint64_t v = base + u * stride;
+      const bool in = v < n;
+      lv[u] = in ? labels[v] : 1ull;
+      wf[u] = in ? vf[v >> 5] : 0u;
+      wb[u] = in ? vb[v >> 5] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < FL_UNROLL; ++u) {
+      const uint64_t v = base + u * stride;
+      const bool a = (wf[u] >> (v & 31)) & 1u, b = (wb[u] >> (v & 31)) & 1u;
+      if (lv[u] || !(a || b)) continue;
+      const unsigned long long label = G == 1 ? lab_one : lab1[seg_of(voff, G, v)];
+      labels[v] = (a && b) ? (label | TOP_BIT) : label;
+      c += a && b;
     }
   }
   c = wave_sum(c);
-  if (lane() == 0 && c) atomicAdd(big, c);
+  if (lane() == 0) red[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < WPB; ++w) t += red[w];
+    if (t) atomicAdd(big, t);
+  }
 }
 
 // ----------------------------------------------------- multi-search kernels
-__global__ void k_ms_reset(MSCtl *C, int s) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    MSSlot z{};
-    for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
-    C->log_n[s] = 0;
-    C->full[s] = 0;
-    for (int c = 0; c < NMC; ++c) C->cnt[s][c] = 0;
-  }
-}
-
-// Seed (pivot_i, label_offset_g + i) into table, log and frontier
-// (src/scc.hpp:147-150); i counts the pivots of graph g in this round, whose
-// front segment starts at fstart[a] (a = the graph's window, g = wgid[a]).
-// The front size is read on the device.
-__global__ void k_ms_seed(MSSearch *S, MSCtl *C, int s, const uint32_t *front,
-                          const unsigned long long *nfront_ptr,
-                          const unsigned long long *fstart, uint32_t A,
-                          const uint32_t *wgid,
-                          const unsigned long long *lab_off) {
-  const MSSearch M = S[s];
-  const uint64_t nfront = *nfront_ptr;
-  uint32_t probes = 0;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nfront;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t a = seg_of(fstart, A, i);
-    const uint32_t v = front[i];
-    const uint32_t lab = (uint32_t)(lab_off[wgid[a]] + (i - fstart[a]));
-    const int r = table_insert(M.table, M.mask, v, lab, M.probe_limit, &probes);
-    if (r < 0) C->full[s] = 1;
-    M.log[i] = make_pair(v, lab);
-    M.q[0][i] = make_pair(v, lab);
-  }
+// Start of a multi-search attempt for the searches in `want` (bit s):
+// empty their tables (slots of EMPTY_SLOT, src/resizable_table.h:44-50)
+// and reset their level control; a search not in `want` keeps its table and
+// log from the previous attempt (the other one overflowed) and is inactive.
+__global__ void k_ms_clear(MSCtl *C, uint32_t want, unsigned long long *t0,
+                           uint64_t s0, unsigned long long *t1, uint64_t s1) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (want & 1)
+    for (uint64_t i = i0; i < s0; i += stride) t0[i] = EMPTY_SLOT;
+  if (want & 2)
+    for (uint64_t i = i0; i < s1; i += stride) t1[i] = EMPTY_SLOT;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    C->slot[s][0].qsize = nfront;
-    C->log_n[s] = nfront;
-    C->cnt[s][M_INS] = nfront;
+    for (int s = 0; s < 2; ++s) {
+      const bool w = (want >> s) & 1;
+      C->active[s] = w;
+      if (!w) continue;
+      MSSlot z{};
+      for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
+      C->log_n[s] = 0;
+      C->full[s] = 0;
+      for (int c = 0; c < NMC; ++c) C->cnt[s][c] = 0;
+    }
   }
 }
 
-// label_offset_g += pivots of g in this round (src/scc.hpp:425)
-__global__ void k_label_advance(const unsigned long long *fstart, uint32_t A,
-                                const uint32_t *wgid, unsigned long long *lab_off) {
-  for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < A;
-       a += (uint64_t)gridDim.x * blockDim.x)
-    lab_off[wgid[a]] += fstart[a + 1] - fstart[a];
+// Seed (pivot_i, label_offset_g + i) into table, log and frontier of both
+// searches (src/scc.hpp:147-150); i counts the pivots of graph g in this
+// round, whose front segment starts at fstart[a] (a = the graph's window,
+// g = wgid[a]).  The front size is read on the device.
+__global__ void k_ms_begin(const MSSearch *S, MSCtl *C, uint32_t want,
+                           const uint32_t *front,
+                           const unsigned long long *nfront_ptr,
+                           const unsigned long long *fstart, uint32_t A,
+                           const uint32_t *wgid,
+                           const unsigned long long *lab_off) {
+  const uint64_t nfront = *nfront_ptr;
+  for (int s = 0; s < 2; ++s) {
+    if (!((want >> s) & 1)) continue;
+    const MSSearch M = S[s];
+    uint32_t probes = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nfront;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+      const uint32_t a = seg_of(fstart, A, i);
+      const uint32_t v = front[i];
+      const uint32_t lab = (uint32_t)(lab_off[wgid[a]] + (i - fstart[a]));
+      const int r = table_insert(M.table, M.mask, v, lab, M.probe_limit, &probes);
+      if (r < 0) C->full[s] = 1;
+      M.log[i] = make_pair(v, lab);
+      M.q[0][i] = make_pair(v, lab);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      C->slot[s][0].qsize = nfront;
+      C->log_n[s] = nfront;
+      C->cnt[s][M_INS] = nfront;
+    }
+  }
 }
 
 // Staged log / frontier appends of one wave; an overflow of either list
@@ -795,7 +882,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     const unsigned long long nf = cur->qsize;
     if (nf == 0 || C->full[s]) continue;
     const MSSearch M = S[s];
-    unsigned long long *qn = M.q[pq ^ 1];
+    unsigned long long *qn = (pq ? M.q[0] : M.q[1]);
     const uint64_t per = (nf + total_waves - 1) / total_waves;
     const uint32_t sd = (uint32_t)min((uint64_t)64, per);
     uint64_t g = gw * per;
@@ -805,10 +892,11 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     while (g < gend) {
       if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
       const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
-      if ((uint32_t)l < cnt) ring[l] = M.q[pq][g + l];
+      if ((uint32_t)l < cnt) ring[l] = (pq ? M.q[1] : M.q[0])[g + l];
       g += cnt;
       uint32_t head = 0, tail = cnt;
       uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+      uint64_t ebudget = budget * VGC_EDGE_MULT;
       wave_sync();
       while (head < tail) {
         const uint32_t k = min(64u, tail - head);
@@ -848,6 +936,8 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
         st[l] = b;
         lab[l] = lv;
         el[l] = pair_label(pr);
+        ebudget = ebudget > total ? ebudget - total : 0;
+        if (!ebudget) budget = 0;
         wave_sync();
         for (uint32_t base = 0; base < total; base += WAVE) {
           const uint32_t e = base + l;
@@ -927,7 +1017,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
     const uint64_t H = hw >> H_CNT_SHIFT, Ct = hw & H_CHUNK_MASK;
     if (H == 0 || C->full[s]) continue;
     const MSSearch M = S[s];
-    unsigned long long *qn = M.q[pq ^ 1];
+    unsigned long long *qn = (pq ? M.q[0] : M.q[1]);
     unsigned long long n_edges = 0, n_ins = 0, n_app = 0, n_items = 0;
     uint32_t probes = 0;
     const uint64_t per = (Ct + total_waves - 1) / total_waves;
@@ -987,7 +1077,12 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
 // applied in one launch.
 __global__ __launch_bounds__(BLOCK) void k_partition(
     const MSSearch *S, const MSCtl *C, int small, unsigned long long *labels,
-    unsigned long long *probe_ctr) {
+    unsigned long long *probe_ctr, const unsigned long long *fstart, uint32_t nwin,
+    const uint32_t *wgid, unsigned long long *lab_off) {
+  // label_offset_g += pivots of g in this round (src/scc.hpp:425)
+  if (blockIdx.x == 0)
+    for (uint32_t a = threadIdx.x; a < nwin; a += blockDim.x)
+      lab_off[wgid[a]] += fstart[a + 1] - fstart[a];
   const int large = small ^ 1;
   const MSSearch A = S[small], B = S[large];
   const uint64_t na = C->log_n[small], nb = C->log_n[large];
@@ -1168,7 +1263,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d};
+          &gid_d, &rnd};
 }
 
 int Engine::download_labels(uint64_t *dst) {
@@ -1337,6 +1432,24 @@ int Engine::upload32(DevBuf &dst, const uint32_t *src, size_t count, int reg) {
   return PSC_OK;
 }
 
+// The windows of a round in one copy: prefix (A+1) u64, sources (A) u64,
+// graph ids (A) u32, staged in pinned regions 1..3 (3 (G + 2) u64 >= what
+// A <= G windows need).
+int Engine::upload_round(const std::vector<uint64_t> &wpre,
+                         const std::vector<uint64_t> &wsrc,
+                         const std::vector<uint32_t> &wgid) {
+  const size_t A = wgid.size();
+  const size_t words = (A + 1) + A + (A + 1) / 2;
+  PSC_TRY(rnd.ensure(std::max<size_t>(words, 1) * 8));
+  PSC_TRY(gpin(G + 2));
+  unsigned long long *h = region(1);
+  std::memcpy(h, wpre.data(), (A + 1) * 8);
+  std::memcpy(h + A + 1, wsrc.data(), A * 8);
+  std::memcpy(h + 2 * A + 1, wgid.data(), A * 4);
+  PSC_HIP(hipMemcpyAsync(rnd.ptr, h, words * 8, hipMemcpyHostToDevice, stream));
+  return PSC_OK;
+}
+
 int Engine::poll(void *dst, const void *src, size_t bytes) {
   char *buf = (char *)h_pinned + PIN_POLL;
   if (bytes > PIN_REACH - PIN_POLL) {
@@ -1377,7 +1490,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   uint32_t L = 0;
-  int K = 4;
+  // the previous run's level count (same graph: one poll, no idle levels)
+  int K = reach_levels_hint ? (int)std::min<uint32_t>(reach_levels_hint + 1, 64) : 4;
   ReachCtl hc_;
   while (true) {
     for (int k = 0; k < K; ++k, ++L) {
@@ -1397,6 +1511,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   }
   st->levels_fwd = hc_.levels[0];
   st->levels_bwd = hc_.levels[1];
+  reach_levels_hint = std::max(hc_.levels[0], hc_.levels[1]);
   for (int s = 0; s < 2; ++s) {
     const unsigned long long *c = hc_.cnt[s];
     st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES];
@@ -1448,12 +1563,6 @@ int Engine::ms_prepare(int s, uint64_t slots, MSSearch *d) {
   // table overflowed (staged log appends may overshoot the load limit)
   d->probe_limit = (uint32_t)std::min<uint64_t>(slots, 4096);
   d->pad = 0;
-  prof_begin(KK_MISC);
-  k_fill_u64<<<grid_for(slots, BLOCK, 8192), BLOCK, 0, stream>>>(
-      d->table, slots, EMPTY_SLOT);
-  prof_end();
-  kbytes[KK_MISC] += 8.0 * slots;
-  PSC_HIP(hipGetLastError());
   return PSC_OK;
 }
 
@@ -1467,7 +1576,7 @@ static uint64_t pow2_at_least(uint64_t x) {
 // searches run fused; an overflowed search is rerun alone with a doubled
 // table.
 int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
-                         uint64_t slots[2], psc_stats *st) {
+                         const uint32_t *wgid_d, uint64_t slots[2], psc_stats *st) {
   MSSearch md[2];
   std::memcpy(md, h_ms, sizeof(md));
   MSSearch *S = ms_desc.as<MSSearch>();
@@ -1487,27 +1596,27 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
       PSC_TRY(ms_prepare(s, slots[s], &md[s]));
     }
     std::memcpy(h_ms, md, sizeof(md));
-    // descriptors and active flags go through pinned staging; the stream
-    // is synchronised (poll) before the staging area is written again
-    char *pin = (char *)h_pinned;
-    std::memcpy(pin + PIN_MS, md, sizeof(md));
-    PSC_HIP(hipMemcpyAsync(S, pin + PIN_MS, sizeof(md), hipMemcpyHostToDevice,
-                           stream));
-    for (int s = 0; s < 2; ++s) {
-      if (!want[s]) continue;
-      prof_begin(KK_SEED);
-      k_ms_reset<<<1, 64, 0, stream>>>(C, s);
-      k_ms_seed<<<grid_for(nfront), BLOCK, 0, stream>>>(
-          S, C, s, front.as<uint32_t>(), misc.as<unsigned long long>() + 3,
-          gC.as<unsigned long long>(), A, gid_d.as<uint32_t>(),
-          gD.as<unsigned long long>());
-      prof_end();
-      kbytes[KK_SEED] += 4.0 * nfront + 32.0 * nfront;
+    // descriptors go through pinned staging, only when they changed; the
+    // stream is synchronised (poll) before the staging area is written again
+    if (!ms_dev_valid || std::memcmp(h_ms_dev, md, sizeof(md)) != 0) {
+      char *pin = (char *)h_pinned;
+      std::memcpy(pin + PIN_MS, md, sizeof(md));
+      PSC_HIP(hipMemcpyAsync(S, pin + PIN_MS, sizeof(md), hipMemcpyHostToDevice,
+                             stream));
+      std::memcpy(h_ms_dev, md, sizeof(md));
+      ms_dev_valid = true;
     }
-    uint32_t act[2] = {(uint32_t)want[0], (uint32_t)want[1]};
-    std::memcpy(pin + PIN_FLAGS, act, sizeof(act));
-    PSC_HIP(hipMemcpyAsync(&C->active[0], pin + PIN_FLAGS, sizeof(act),
-                           hipMemcpyHostToDevice, stream));
+    const uint32_t wmask = (want[0] ? 1u : 0u) | (want[1] ? 2u : 0u);
+    const uint64_t s0 = want[0] ? md[0].mask + 1 : 0, s1 = want[1] ? md[1].mask + 1 : 0;
+    prof_begin(KK_SEED);
+    k_ms_clear<<<grid_for(std::max(s0, s1), BLOCK, 8192), BLOCK, 0, stream>>>(
+        C, wmask, md[0].table, s0, md[1].table, s1);
+    k_ms_begin<<<grid_for(nfront), BLOCK, 0, stream>>>(
+        S, C, wmask, front.as<uint32_t>(), misc.as<unsigned long long>() + 3,
+        gC.as<unsigned long long>(), A, wgid_d, gD.as<unsigned long long>());
+    prof_end();
+    kbytes[KK_SEED] += 8.0 * (s0 + s1) + (4.0 * nfront + 32.0 * nfront) *
+                                             ((want[0] ? 1 : 0) + (want[1] ? 1 : 0));
     PSC_HIP(hipGetLastError());
     uint32_t L = 0;
     int K = (int)std::min<uint32_t>(std::max<uint32_t>(ms_levels_hint, 2), 32);
@@ -1628,7 +1737,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
     prof_begin(KK_MISC);
-    k_first_labels<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
+    k_first_labels<<<grid_for((n + FL_UNROLL - 1) / FL_UNROLL, BLOCK, num_cu * 8),
+                     BLOCK, 0, stream>>>(
         n, voffd, G, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
         gC.as<unsigned long long>(), big);
     prof_end();
@@ -1679,23 +1789,31 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     }
     const uint32_t A = (uint32_t)wgid.size();
     st->rounds++;
-    PSC_TRY(upload(gA, wpre.data(), A + 1, 1));
-    PSC_TRY(upload(gB, wsrc.data(), A, 2));
-    PSC_TRY(upload32(gid_d, wgid.data(), A, 3));
-    // front = not-done pivots of every window, grouped by graph; its size
-    // stays on the device (misc[3]) -- one host sync per round, in
-    // multi_search
+    PSC_TRY(upload_round(wpre, wsrc, wgid));
+    const unsigned long long *d_wpre = rnd.as<unsigned long long>();
+    const unsigned long long *d_wsrc = d_wpre + (A + 1);
+    const uint32_t *d_wgid = reinterpret_cast<const uint32_t *>(d_wsrc + A);
+    // front = not-done pivots of every window, grouped by graph, and its
+    // per-window starts (gC); its size stays on the device (misc[3]) -- one
+    // host sync per round, in multi_search
     unsigned long long *nfront_d = misc.as<unsigned long long>() + 3;
     const uint64_t V = wpre[A];
-    PSC_TRY(select(SelWindow{centers.as<uint32_t>(), gA.as<unsigned long long>(),
-                             gB.as<unsigned long long>(), A, lab},
-                   V, front.as<uint32_t>(), nullptr, nullptr, nfront_d));
+    const SelWindow selw{centers.as<uint32_t>(), d_wpre, d_wsrc, A, lab};
+    if (V <= SMALL_SELECT) {
+      prof_begin(KK_SELECT);
+      k_sel_small<SelWindow><<<1, SMALL_BLOCK, 0, stream>>>(
+          selw, V, front.as<uint32_t>(), nfront_d, d_wpre, A,
+          gC.as<unsigned long long>());
+      prof_end();
+    } else {
+      PSC_TRY(select(selw, V, front.as<uint32_t>(), nullptr, nullptr, nfront_d));
+      prof_begin(KK_MISC);
+      k_segment_starts<<<grid_for(A + 1), BLOCK, 0, stream>>>(
+          front.as<uint32_t>(), 0, nfront_d, voffd, G, d_wgid, A,
+          gC.as<unsigned long long>());
+      prof_end();
+    }
     kbytes[KK_SELECT] += 2 * 12.0 * V;
-    prof_begin(KK_MISC);
-    k_segment_starts<<<grid_for(A + 1), BLOCK, 0, stream>>>(
-        front.as<uint32_t>(), 0, nfront_d, voffd, G, gid_d.as<uint32_t>(), A,
-        gC.as<unsigned long long>());
-    prof_end();
     uint64_t slots[2];
     for (int s = 0; s < 2; ++s) {
       const uint64_t cap = 6000000ull * G;
@@ -1704,17 +1822,13 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
           (uint64_t)beta_def * ne_prev[s]);
       slots[s] = pow2_at_least(p.table_init ? p.table_init : 2 * base);
     }
-    PSC_TRY(multi_search(p, V, A, slots, st));
-    prof_begin(KK_MISC);
-    k_label_advance<<<grid_for(A), BLOCK, 0, stream>>>(
-        gC.as<unsigned long long>(), A, gid_d.as<uint32_t>(),
-        gD.as<unsigned long long>());
-    prof_end();
+    PSC_TRY(multi_search(p, V, A, d_wgid, slots, st));
     const int small = (slots[0] <= slots[1]) ? 0 : 1;
     prof_begin(KK_PARTITION);
     k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
         ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
-        misc.as<unsigned long long>() + 2);
+        misc.as<unsigned long long>() + 2, gC.as<unsigned long long>(), A, d_wgid,
+        gD.as<unsigned long long>());
     prof_end();
     PSC_HIP(hipGetLastError());
     // log reads + label read-modify-write per pair; contains() probes of
@@ -1821,7 +1935,7 @@ int Engine::prof_collect() {
 
 const char *const KK_NAMES[KK_N] = {
     "k_reach_step", "k_reach_heavy", "k_ms_expand", "k_ms_heavy",
-    "k_partition", "select (3 kernels)", "k_ms_seed", "init/trim/finalize"};
+    "k_partition", "select", "k_ms_clear+begin", "init/trim/finalize"};
 
 int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
                    uint64_t *largest) {

@@ -19,6 +19,11 @@ constexpr uint32_t HEAVY_DEG = 512;  // degree above which edges are chunked
 constexpr uint32_t CHUNK = 1024;     // edges per heavy work item
 constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests first
 constexpr uint32_t VGC_MAX_DEG = 32; // only vertices this sparse stay local
+// A wave's local search also stops taking vertices once it has scanned
+// VGC_EDGE_MULT * tau edges per seed: a lone wave walking thousands of
+// edges serially costs more than the levels it saves on low-diameter graphs,
+// while degree-<=4 lattices never reach the cap.
+constexpr uint32_t VGC_EDGE_MULT = 4;
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 
