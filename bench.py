@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--tau", type=int, default=512)
     ap.add_argument("--no-vgc", action="store_true")
+    ap.add_argument("--levels", action="store_true",
+                    help="report the first reach level by level (stderr)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
@@ -292,6 +294,8 @@ def main():
         f"device {sum(dev_ms)/len(dev_ms):.3f} ms, {gedges:.3f} Gedges/s")
     log("stats:", json.dumps(st))
 
+    if args.levels:   # per-level report of the first reach on stderr
+        s.run(**dict(prm, timing=3))
     # per-kernel profile: one extra pass with an event pair per launch
     s.run(**dict(prm, timing=2))
     prof_st = s.stats

@@ -66,7 +66,10 @@ typedef struct psc_params {
   uint64_t seed;        /* pivot permutation seed, default 1         */
   uint64_t table_init;  /* 0 = reference sizing                      */
   int32_t canonical;    /* 1: labels_out = min vertex id of the SCC  */
-  int32_t timing;       /* 1: fill the phase times of psc_stats      */
+  int32_t timing;       /* 1: fill the phase times of psc_stats;
+                           2: also per-kernel events (psc_kernel_profile);
+                           3: also one first-reach level per host poll,
+                              reported per level on stderr            */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).
@@ -149,6 +152,12 @@ int psc_device_copy(void *dst, const void *src, size_t bytes);
  * fill at full link speed. */
 int psc_host_alloc(size_t bytes, void **ptr);
 int psc_host_free(void *ptr);
+
+/* The seeded pivot order of the first round, the stand-in for
+ * parlay::random_shuffle of the non-trimmed vertices (src/scc.hpp:303):
+ * element i of a permutation of [0, n), or with inverse != 0 the position of
+ * value i (host function, i < n). */
+uint64_t psc_pivot_perm(uint64_t n, uint64_t seed, uint64_t i, int inverse);
 
 /* Message of the last error on this thread ("" if none). */
 const char *psc_last_error(void);

@@ -104,6 +104,7 @@ _sig("psc_device_alloc", C.c_int, C.c_size_t, C.POINTER(C.c_void_p))
 _sig("psc_device_free", C.c_int, C.c_void_p)
 _sig("psc_device_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
 _sig("psc_host_alloc", C.c_int, C.c_size_t, C.POINTER(C.c_void_p))
+_sig("psc_pivot_perm", C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int)
 _sig("psc_host_free", C.c_int, C.c_void_p)
 _sig("psc_host_graph_free", None, C.POINTER(CHostGraph))
 _sig("psc_read_bin", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.POINTER(CHostGraph)))

@@ -258,6 +258,28 @@ struct PivotPerm {
     while (y >= N) y = step(y);
     return y;
   }
+  // Multiplicative inverse of an odd constant mod 2^64 (Newton iteration).
+  static __host__ __device__ constexpr uint64_t inv_odd(uint64_t c) {
+    uint64_t x = c;
+    for (int i = 0; i < 6; ++i) x *= 2 - c * x;
+    return x;
+  }
+  // step^-1: the xorshift is its own inverse because 2 * sh >= k, the
+  // multiply-adds invert with the constants' inverses mod 2^k.
+  __host__ __device__ __forceinline__ uint64_t step_inv(uint64_t y) const {
+    const int sh = k / 2 + 1;
+    y ^= y >> sh;
+    y = ((y - b) * inv_odd(0x85EBCA77ull)) & mask;
+    y ^= y >> sh;
+    y = ((y - a) * inv_odd(0x9E3779B1ull)) & mask;
+    return y;
+  }
+  // Position of y in the permutation: inverse(operator()(i)) == i.
+  __host__ __device__ __forceinline__ uint64_t inverse(uint64_t y) const {
+    uint64_t x = step_inv(y);
+    while (x >= N) x = step_inv(x);
+    return x;
+  }
 };
 
 }  // namespace psc

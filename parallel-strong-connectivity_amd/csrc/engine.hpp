@@ -40,7 +40,7 @@ constexpr size_t PIN_POLL = 0, PIN_REACH = 1024, PIN_MS = 2048, PIN_FLAGS = 3072
 
 // kernel kinds timed by the per-launch profiler (timing >= 2)
 enum : int {
-  KK_STEP, KK_HEAVY, KK_MS_EXPAND, KK_MS_HEAVY, KK_PARTITION, KK_SELECT,
+  KK_STEP, KK_PULL, KK_HEAVY, KK_MS_EXPAND, KK_MS_HEAVY, KK_PARTITION, KK_SELECT,
   KK_SEED, KK_MISC, KK_N
 };
 extern const char *const KK_NAMES[KK_N];
@@ -62,10 +62,12 @@ struct Engine {
   const uint32_t *d_adj = nullptr, *d_iadj = nullptr;
   DevBuf g_off, g_adj, g_ioff, g_iadj;   // owned copies (psc_create)
   DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
-  DevBuf NZ, Z, P, centers, front, tiles, tile_off;
+  DevBuf NZ, centers, front, tiles, tile_off;
+  DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
+  DevBuf rem[2], sort_tmp;   // remaining vertices keyed by pivot position
   DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
   DevBuf table[2], log[2], mq[2][2], mhp[2], mhc[2];
-  int grid_step = 1, grid_ms = 1, grid_heavy = 1;
+  int grid_step = 1, grid_ms = 1, grid_heavy = 1, grid_pull = 1;
   int prof_level = 0;
   std::vector<hipEvent_t> pev;
   size_t pev_used = 0;
@@ -83,6 +85,7 @@ struct Engine {
   uint64_t log_n[2] = {0, 0};
   uint32_t ms_levels_hint = 2;   // sub-rounds of the previous multi-search
   uint32_t reach_levels_hint = 0; // levels of the previous first reach
+  uint64_t nnz_total = 0;         // non-trimmed vertices of the current run
   uint64_t syncs = 0;
   unsigned char h_reach[256];
   unsigned char h_ms[256];
@@ -108,8 +111,8 @@ struct Engine {
              uint64_t *largest);
 
   template <class F>
-  int select(F f, uint64_t len, uint32_t *sel, uint32_t *rej, uint64_t *count,
-             unsigned long long *count_dev = nullptr);
+  int select(F f, uint64_t len, typename F::T *sel, typename F::T *rej,
+             uint64_t *count, unsigned long long *count_dev = nullptr);
   int poll(void *dst, const void *src, size_t bytes);
   // copy the n labels to host memory on this handle's stream
   int download_labels(uint64_t *dst);

@@ -18,6 +18,9 @@
 // deterministic function of (graph, seed).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -120,8 +123,8 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scan(uint32_t *tile_cnt,
 
 template <class F>
 __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
-    F f, uint64_t len, const unsigned long long *tile_off, uint32_t *sel,
-    uint32_t *rej) {
+    F f, uint64_t len, const unsigned long long *tile_off, typename F::T *sel,
+    typename F::T *rej) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
@@ -164,7 +167,7 @@ constexpr int SMALL_BLOCK = 1024, SMALL_WAVES = SMALL_BLOCK / WAVE, SMALL_ITEMS 
 constexpr uint64_t SMALL_SELECT = (uint64_t)SMALL_BLOCK * SMALL_ITEMS * 4;
 template <class F>
 __global__ __launch_bounds__(SMALL_BLOCK) void k_sel_small(
-    F f, uint64_t len, uint32_t *sel, unsigned long long *count_dev,
+    F f, uint64_t len, typename F::T *sel, unsigned long long *count_dev,
     const unsigned long long *wpre, uint32_t A, unsigned long long *fstart) {
   __shared__ uint32_t wc[SMALL_ITEMS][SMALL_WAVES];
   const int w = threadIdx.x / WAVE;
@@ -208,18 +211,11 @@ __global__ __launch_bounds__(SMALL_BLOCK) void k_sel_small(
   }
 }
 
-// pred: vertex has non-zero in- and out-degree (src/scc.hpp:295-302)
-struct SelNonZero {
-  const uint64_t *off, *ioff;
-  __device__ bool pred(uint64_t v) const {
-    return off[v + 1] != off[v] && ioff[v + 1] != ioff[v];
-  }
-  __device__ uint32_t val(uint64_t v) const { return (uint32_t)v; }
-};
 // pred: not yet in a found SCC, over the concatenation of per-graph windows
 // of the centers list (src/scc.hpp:384-388): window a covers
 // centers[wsrc[a], wsrc[a] + wpre[a+1] - wpre[a]).
 struct SelWindow {
+  using T = uint32_t;
   const uint32_t *centers;
   const unsigned long long *wpre, *wsrc;
   uint32_t A;
@@ -230,25 +226,158 @@ struct SelWindow {
   }
   __device__ bool pred(uint64_t t) const { return !(labels[val(t)] & TOP_BIT); }
 };
-// pred: not in the first-round SCC, i.e. not reached both ways
-// (src/scc.hpp:349); reads the two visited bitmaps instead of the labels.
-struct SelNotBoth {
-  const uint32_t *src, *vf, *vb;
-  __device__ uint32_t val(uint64_t i) const { return src[i]; }
-  __device__ bool pred(uint64_t i) const {
-    const uint32_t v = src[i];
+// pred: non-trimmed vertex NZ[j] not reached both ways by the first search
+// (src/scc.hpp:349).  The value packs the vertex's position in its graph's
+// pivot order P_g = NZ_g permuted (src/scc.hpp:303), nstart[g] +
+// perm_g^-1(j - nstart[g]), above the vertex id: sorting the selected values
+// yields the remaining vertices in P order without materialising P.
+struct SelRemaining {
+  using T = unsigned long long;
+  const uint32_t *NZ, *vf, *vb;
+  const unsigned long long *nstart;
+  uint32_t G;
+  uint64_t seed;
+  __device__ bool pred(uint64_t j) const {
+    const uint32_t v = NZ[j];
     return !(bit_test(vf, v) && bit_test(vb, v));
   }
-};
-// pred: not yet in a found SCC; select from a vertex list (src/scc.hpp:349,384-388)
-struct SelNotDone {
-  const uint32_t *src;
-  const unsigned long long *labels;
-  __device__ bool pred(uint64_t i) const {
-    return !(labels[src[i]] & TOP_BIT);
+  __device__ T val(uint64_t j) const {
+    const uint32_t g = G == 1 ? 0u : seg_of(nstart, G, j);
+    const uint64_t b = nstart[g];
+    const PivotPerm perm(nstart[g + 1] - b, seed);
+    return ((unsigned long long)(b + perm.inverse(j - b)) << 32) | NZ[j];
   }
-  __device__ uint32_t val(uint64_t i) const { return src[i]; }
 };
+
+// ------------------------------------------------------------- trimming
+// Trim, pass 1 (src/scc.hpp:295-302): per SEL_TILE tile, the number of
+// vertices with non-zero in- and out-degree.  The same degree reads give
+// the dead-end bitmaps of the first reach: vis_f marks vertices without
+// out-edges, vis_b those without in-edges, trim_bm their union (one word
+// pair per bitmap, wave and 64 consecutive vertices).
+__global__ __launch_bounds__(BLOCK) void k_trim_count(
+    const uint64_t *__restrict__ off, const uint64_t *__restrict__ ioff, uint64_t n,
+    uint64_t words, uint32_t *tile_cnt, uint32_t *vis_f, uint32_t *vis_b,
+    uint32_t *trim_bm) {
+  __shared__ uint32_t wc[WPB];
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
+  const int l = lane();
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    bool zo = false, zi = false;
+    if (i < n) {
+      zo = off[i + 1] == off[i];
+      zi = ioff[i + 1] == ioff[i];
+    }
+    c += (uint32_t)__popcll(ballot(i < n && !zo && !zi));
+    const uint64_t mf = ballot(zo), mb = ballot(zi), mt = mf | mb;
+    const uint64_t wd = (i - l) / 32;   // first word of the wave's 64 vertices
+    if (l == 0 && wd < words) {
+      vis_f[wd] = (uint32_t)mf;
+      vis_b[wd] = (uint32_t)mb;
+      trim_bm[wd] = (uint32_t)mt;
+    }
+    if (l == 32 && wd + 1 < words) {
+      vis_f[wd + 1] = (uint32_t)(mf >> 32);
+      vis_b[wd + 1] = (uint32_t)(mb >> 32);
+      trim_bm[wd + 1] = (uint32_t)(mt >> 32);
+    }
+  }
+  if (l == 0) wc[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WPB; ++w) t += wc[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// Per-graph starts: nstart[g] = non-trimmed vertices before voff[g] (the
+// tile prefix plus a recount of the tile's head, one wave per graph) and
+// zstart[g] = voff[g] - nstart[g], for g in [0, G].
+__global__ void k_graph_starts(const uint64_t *__restrict__ off,
+                               const uint64_t *__restrict__ ioff, uint64_t n,
+                               const unsigned long long *voff, uint32_t G,
+                               const unsigned long long *tile_off,
+                               const unsigned long long *total,
+                               unsigned long long *nstart,
+                               unsigned long long *zstart) {
+  const int l = lane();
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; g <= G;
+       g += (uint64_t)gridDim.x * blockDim.x / WAVE) {
+    const uint64_t v0 = voff[g];
+    unsigned long long c;
+    if (v0 >= n) {
+      c = *total;
+    } else {
+      const uint64_t t = v0 / SEL_TILE;
+      c = tile_off[t];
+      for (uint64_t b = t * SEL_TILE; b < v0; b += WAVE) {
+        const uint64_t i = b + l;
+        const bool keep = i < v0 && off[i + 1] != off[i] && ioff[i + 1] != ioff[i];
+        c += (unsigned long long)__popcll(ballot(keep));
+      }
+    }
+    if (l == 0) {
+      nstart[g] = c;
+      zstart[g] = v0 - c;
+    }
+  }
+}
+
+// Trim, pass 2: non-trimmed vertices go to NZ in vertex order with label 0;
+// trimmed ones get 1 + (rank among their graph's trimmed | TOP_BIT)
+// (src/scc.hpp:304-305).  Every label is written once, coalesced.
+__global__ __launch_bounds__(BLOCK) void k_trim_scatter(
+    const uint64_t *__restrict__ off, const uint64_t *__restrict__ ioff, uint64_t n,
+    const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
+    const unsigned long long *tile_off, uint32_t *NZ, unsigned long long *labels) {
+  __shared__ uint32_t wc[SEL_ITEMS][WPB];
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
+  const int w = threadIdx.x / WAVE;
+  bool p[SEL_ITEMS];
+  uint32_t rk[SEL_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    p[j] = i < n && off[i + 1] != off[i] && ioff[i + 1] != ioff[i];
+    const uint64_t m = ballot(p[j]);
+    rk[j] = lanes_below(m);
+    if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  uint64_t r0 = tile_off[blockIdx.x];
+  const unsigned long long z0 = zstart[0];
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    uint32_t before = 0, round_tot = 0;
+#pragma unroll
+    for (int x = 0; x < WPB; ++x) {
+      before += x < w ? wc[j][x] : 0u;
+      round_tot += wc[j][x];
+    }
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    if (i < n) {
+      const uint64_t r = r0 + before + rk[j];
+      if (p[j]) {
+        NZ[r] = (uint32_t)i;
+        labels[i] = 0;
+      } else {
+        const unsigned long long zs = G == 1 ? z0 : zstart[seg_of(voff, G, i)];
+        labels[i] = 1 + ((i - r - zs) | TOP_BIT);
+      }
+    }
+    r0 += round_tot;
+  }
+}
+
+__global__ void k_low32(const unsigned long long *in, uint64_t n, uint32_t *out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)in[i];
+}
 
 // ------------------------------------------------------- per-graph helpers
 // A handle holds G graphs as one disjoint union: graph g owns the vertex id
@@ -279,34 +408,6 @@ __global__ void k_segment_starts(const uint32_t *list, uint64_t len,
   }
 }
 
-// Trimmed vertices get 1 + (rank within their graph | TOP_BIT)
-// (src/scc.hpp:304-305).
-__global__ void k_trim_labels(const uint32_t *Z, uint64_t nz,
-                              const unsigned long long *voff, uint32_t G,
-                              const unsigned long long *zstart,
-                              unsigned long long *labels) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nz;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = Z[i];
-    const uint32_t g = seg_of(voff, G, v);
-    labels[v] = 1 + ((i - zstart[g]) | TOP_BIT);
-  }
-}
-
-// P = per-graph seeded permutation of the non-trimmed vertices
-// (parlay::random_shuffle, src/scc.hpp:303).
-__global__ void k_permute(const uint32_t *NZ, uint64_t nnz,
-                          const unsigned long long *nstart, uint32_t G,
-                          uint64_t seed, uint32_t *P) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nnz;
-       j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t g = seg_of(nstart, G, j);
-    const uint64_t b = nstart[g];
-    const PivotPerm perm(nstart[g + 1] - b, seed);
-    P[j] = NZ[b + perm(j - b)];
-  }
-}
-
 // ----------------------------------------------------------- reach kernels
 // Dead ends are pre-marked visited: in a search that pushes along CSR
 // `off`, a vertex with no edges in `off` (a sink of the forward search, a
@@ -315,25 +416,10 @@ __global__ void k_permute(const uint32_t *NZ, uint64_t nnz,
 // neighbour (pull follows the opposite CSR, where it would need an edge in
 // `off`), so "visited" keeps meaning "reached" wherever a pull step reads
 // it.  First-round labels only concern the non-trimmed vertices
-// (src/scc.hpp:341-347), whose reach sets this does not change.
-// k_reach_reset starts each visited bitmap from its dead-end set and clears
-// the level control of both searches.
-__global__ void k_reach_reset(ReachSearch *S, ReachCtl *C, int ns,
-                              uint64_t words) {
-  // one lane per vertex, two bitmap words per wave (coalesced offsets)
-  const uint64_t units = (words + 1) / 2;
-  for (int s = 0; s < ns; ++s) {
-    const ReachSearch R = S[s];
-    const uint64_t n = R.n;
-    for (uint64_t t = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-         t < units; t += (uint64_t)gridDim.x * blockDim.x / WAVE) {
-      const uint64_t v = t * 64 + lane();
-      const bool dead = v < n && R.off[v + 1] == R.off[v];
-      const uint64_t m = ballot(dead);
-      if (lane() == 0) R.vis[t * 2] = (uint32_t)m;
-      if (lane() == 32 && t * 2 + 1 < words) R.vis[t * 2 + 1] = (uint32_t)(m >> 32);
-    }
-  }
+// (src/scc.hpp:341-347), whose reach sets this does not change.  The
+// bitmaps start as these dead-end sets (written by k_trim_count);
+// k_reach_reset clears the level control of both searches.
+__global__ void k_reach_reset(ReachCtl *C, int ns) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (int s = 0; s < ns; ++s) {
       ReachSlot z{};
@@ -349,14 +435,17 @@ __global__ void k_reach_reset(ReachSearch *S, ReachCtl *C, int ns,
 // (src/scc.hpp:317); the graphs are disjoint, so one multi-source reach
 // gives each graph's single reach from its own pivot.
 __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
-                                const uint32_t *P,
-                                const unsigned long long *nstart, uint32_t G) {
+                                const uint32_t *NZ,
+                                const unsigned long long *nstart, uint32_t G,
+                                uint64_t seed) {
   for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G;
        g += (uint64_t)gridDim.x * blockDim.x) {
-    if (nstart[g + 1] == nstart[g]) continue;
-    const uint32_t src = P[nstart[g]];
+    const uint64_t b = nstart[g];
+    if (nstart[g + 1] == b) continue;
+    const uint32_t src = NZ[b + PivotPerm(nstart[g + 1] - b, seed)(0)];   // P_g[0]
     for (int s = 0; s < ns; ++s) {
       atomicOr(&S[s].vis[src >> 5], 1u << (src & 31));
+      atomicAdd(&C->slot[s][0].cum_vis, 1ull);
       const uint32_t idx = atomicAdd(&C->slot[s][0].qsize, 1u);
       S[s].q[0][idx] = src;
       atomicAdd(&C->slot[s][0].degsum,
@@ -365,17 +454,48 @@ __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
   }
 }
 
+// Work split of the fused forward/backward levels by XCD.  Workgroups are
+// dispatched round-robin over the 8 XCDs (blockIdx % 8); with two searches,
+// XCDs 0-3 run the forward search and XCDs 4-7 the backward one, so each
+// XCD's L2 holds one visited bitmap instead of both.  gw / nw / total_waves
+// become the wave's index and count within its half.
+__device__ __forceinline__ void xcd_split(int ns, int w, int *s_lo, int *s_hi,
+                                          uint64_t *gw, uint64_t *nw,
+                                          uint32_t *total_waves) {
+  if (ns == 2 && (gridDim.x & 7) == 0) {
+    const uint32_t xcd = blockIdx.x & 7;
+    *s_lo = (int)(xcd >> 2);
+    *s_hi = *s_lo + 1;
+    const uint64_t hb = (uint64_t)(blockIdx.x >> 3) * 4 + (xcd & 3);
+    *gw = hb * WPB + w;
+    *nw = (uint64_t)(gridDim.x / 2) * WPB;
+    *total_waves = (uint32_t)*nw;
+  } else {
+    *s_lo = 0;
+    *s_hi = ns;
+    *gw = blockIdx.x * (uint64_t)WPB + w;
+    *nw = (uint64_t)gridDim.x * WPB;
+  }
+}
+
 // Direction choice of judge() (src/reach.hpp:86-112): sparse iff
 // n_front + out_edges < m / dense_div.  out_edges is exact after a push
 // level (summed at append time) and estimated as n_front * m / n after a
-// pull level, where the reference samples (dense_sample, :192-206).
+// pull level, where the reference samples (dense_sample, :192-206).  A
+// pull level here only visits the unvisited non-trimmed vertices (the
+// bitmap walk skips the rest), so a level also pulls when those are few
+// enough to cost less than the push would; the direction never changes
+// the reach sets.  Every input is final before the level starts.
 __device__ __forceinline__ int reach_mode(const ReachSlot *c,
                                           const ReachSearch &R) {
   const uint32_t nf = c->qsize;
   if (nf == 0) return MODE_NONE;
   unsigned long long ds = c->degsum;
   if (c->from_pull) ds = (unsigned long long)((double)nf * R.avg_deg);
-  return ((uint64_t)nf + ds < R.thresh) ? MODE_PUSH : MODE_PULL;
+  const uint64_t push = (uint64_t)nf + ds;
+  if (push >= R.thresh) return MODE_PULL;
+  const uint64_t unv = R.nnz > c->cum_vis ? R.nnz - c->cum_vis : 0;
+  return (unv * PULL_VERTEX_COST + R.n / 2048 < push) ? MODE_PULL : MODE_PUSH;
 }
 
 __device__ __forceinline__ void add_cnt(unsigned long long *c, unsigned long long v) {
@@ -427,15 +547,15 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
   __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
-  __shared__ uint32_t fl_s[WPB][WAVE];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *ring = ring_s[w];
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   const int ci = L % 3, ni = (L + 1) % 3, zi = (L + 2) % 3;
   const int p = L & 1;
-  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  int s_lo, s_hi;
+  uint64_t gw, nw;
+  xcd_split(ns, w, &s_lo, &s_hi, &gw, &nw, &total_waves);
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)ns) {
     const int s = threadIdx.x;
     ReachSlot z{};
@@ -447,12 +567,14 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
   }
   WaveOut<uint32_t, OUT_CAP> out;
   out.init(out_s[w]);
-  for (int s = 0; s < ns; ++s) {
+  for (int s = s_lo; s < s_hi; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
     const ReachSearch R = S[s];
     const int mode = reach_mode(cur, R);
     const uint32_t nf = cur->qsize;
+    // carry the visited count into the next level (once per search)
+    if (gw == 0 && l == 0 && cur->cum_vis) atomicAdd(&nxt->cum_vis, cur->cum_vis);
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
     if (mode == MODE_PUSH) {
       // contiguous share of the frontier, S seeds at a time
@@ -536,18 +658,75 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       add_cnt(&C->cnt[s][R_PUSH_EDGES], n_edges);
       add_cnt(&C->cnt[s][R_PUSH_EXP], n_exp);
       add_cnt(&C->cnt[s][R_PUSH_VIS], n_vis);
+      add_cnt(&nxt->cum_vis, n_vis);
       add_cnt(&C->cnt[s][R_PUSH_APP], n_app);
-    } else if (mode == MODE_PULL) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) nxt->from_pull = 1;
-      const uint64_t units = (R.n + 63) / 64;
-      uint32_t *fl = fl_s[w];   // per-owner "resolved" flags
-      unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_units = 0;
-      for (uint64_t t = gw; t < units; t += nw) {
-        const uint64_t v = t * 64 + l;
-        ++n_units;
+    }
+  }
+}
+
+// Pull level of both searches (src/reach.hpp:173-190): every unvisited,
+// non-trimmed vertex scans its in-edges until it meets a visited vertex;
+// testing "visited" instead of "in the previous frontier" keeps the
+// reachable set identical.  The vertex range is walked by bitmap: a wave
+// loads 64 words of (visited | trimmed) with one coalesced load (2048
+// vertices) and only works on the words with a clear bit, two words (64
+// vertices) at a time.  A vertex's first PULL_PROBE in-edges are probed
+// with independent loads; longer remainders run as load-balanced 64-edge
+// windows over the wave's vertices, or, beyond PULL_HEAVY edges, as chunks
+// on the heavy list (k_reach_heavy).  Trimmed vertices are skipped: they
+// never join a first-round reach set (src/scc.hpp:341-347) and no pull
+// probe reads them as visited.
+__global__ __launch_bounds__(BLOCK) void k_reach_pull(
+    const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t total_waves) {
+  __shared__ uint32_t out_s[WPB][OUT_CAP];
+  __shared__ uint32_t sc_s[WPB][WAVE];
+  __shared__ uint64_t st_s[WPB][WAVE];
+  __shared__ uint32_t fl_s[WPB][WAVE];
+  const int w = threadIdx.x / WAVE, l = lane();
+  uint32_t *sc = sc_s[w], *fl = fl_s[w];
+  uint64_t *st = st_s[w];
+  const int ci = L % 3, ni = (L + 1) % 3;
+  const int p = L & 1;
+  int s_lo, s_hi;
+  uint64_t gw, nw;
+  xcd_split(ns, w, &s_lo, &s_hi, &gw, &nw, &total_waves);
+  WaveOut<uint32_t, OUT_CAP> out;
+  out.init(out_s[w]);
+  for (int s = s_lo; s < s_hi; ++s) {
+    ReachSlot *cur = &C->slot[s][ci];
+    ReachSlot *nxt = &C->slot[s][ni];
+    const ReachSearch R = S[s];
+    if (reach_mode(cur, R) != MODE_PULL) continue;
+    if (gw == 0 && l == 0) nxt->from_pull = 1;   // first wave of this search
+    uint32_t *qn = (p ? R.q[0] : R.q[1]);
+    const uint64_t words = ((uint64_t)R.n + 31) / 32;
+    const uint64_t chunks = (words + WAVE - 1) / WAVE;
+    const uint32_t tail_bits = R.n & 31;
+    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0;
+    for (uint64_t c = gw; c < chunks; c += nw) {
+      const uint64_t wi = c * WAVE + l;
+      uint32_t sk = ~0u;
+      if (wi < words) {
+        sk = R.vis[wi] | R.trim[wi];
+        if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
+      }
+      ++n_chunks;
+      uint64_t act = ballot(sk != ~0u);
+      while (act) {
+        // lanes 0-31 take the first active word, lanes 32-63 the next
+        const int wa = __ffsll((unsigned long long)act) - 1;
+        act &= act - 1;
+        int wb = -1;
+        if (act) {
+          wb = __ffsll((unsigned long long)act) - 1;
+          act &= act - 1;
+        }
+        const int myw = l < 32 ? wa : wb;
+        const uint32_t mysk = __shfl(sk, myw < 0 ? 0 : myw);
+        const uint64_t v = (c * WAVE + (uint64_t)(myw < 0 ? 0 : myw)) * 32 + (l & 31);
         bool found = false;
         uint64_t b = 0, deg = 0;
-        if (v < R.n && !bit_test(R.vis, (uint32_t)v)) {
+        if (myw >= 0 && !((mysk >> (l & 31)) & 1u)) {
           ++n_scan;
           b = R.roff[v];
           deg = R.roff[v + 1] - b;
@@ -604,17 +783,18 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
         }
         const uint64_t fm = ballot(found);
         if (!fm) continue;
-        if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[t * 2], (uint32_t)fm);
-        if (l == 32 && (fm >> 32)) atomicOr(&R.vis[t * 2 + 1], (uint32_t)(fm >> 32));
+        if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[c * WAVE + wa], (uint32_t)fm);
+        if (l == 32 && (fm >> 32)) atomicOr(&R.vis[c * WAVE + wb], (uint32_t)(fm >> 32));
         n_vis += found;
         out.put(found, (uint32_t)v, &nxt->qsize, qn, R.n);
       }
-      out.flush(&nxt->qsize, qn, R.n);
-      add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
-      add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
-      add_cnt(&C->cnt[s][R_PULL_VIS], n_vis);
-      add_cnt(&C->cnt[s][R_PULL_UNITS], l == 0 ? n_units : 0);
     }
+    out.flush(&nxt->qsize, qn, R.n);
+    add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
+    add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
+    add_cnt(&C->cnt[s][R_PULL_VIS], n_vis);
+    add_cnt(&nxt->cum_vis, n_vis);
+    add_cnt(&C->cnt[s][R_PULL_UNITS], l == 0 ? n_chunks : 0);
   }
 }
 
@@ -632,10 +812,27 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
   const int l = lane();
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
-  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  // XCDs are split between the searches in proportion to their chunk counts
+  int s_lo = 0, s_hi = ns;
+  uint64_t gw = blockIdx.x * (uint64_t)WPB + threadIdx.x / WAVE;
+  if (ns == 2 && (gridDim.x & 7) == 0) {
+    const uint64_t c0 = C->slot[0][ci].heavy & H_CHUNK_MASK;
+    const uint64_t c1 = C->slot[1][ci].heavy & H_CHUNK_MASK;
+    if (c0 && c1) {
+      const uint32_t k = (uint32_t)min<uint64_t>(7, max<uint64_t>(1, (8 * c0 + (c0 + c1) / 2) / (c0 + c1)));
+      const uint32_t xcd = blockIdx.x & 7;
+      const bool second = xcd >= k;
+      const uint32_t kx = second ? 8 - k : k;
+      s_lo = second ? 1 : 0;
+      s_hi = s_lo + 1;
+      gw = ((uint64_t)(blockIdx.x >> 3) * kx + (second ? xcd - k : xcd)) * WPB +
+           threadIdx.x / WAVE;
+      total_waves = (gridDim.x >> 3) * kx * WPB;
+    }
+  }
   WaveOut<uint32_t, OUT_CAP> out;
   out.init(out_s[threadIdx.x / WAVE]);
-  for (int s = 0; s < ns; ++s) {
+  for (int s = s_lo; s < s_hi; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
     const unsigned long long hw = cur->heavy;
@@ -711,6 +908,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     add_cnt(&nxt->degsum, dsum);
     add_cnt(&C->cnt[s][R_HEAVY_EDGES], n_edges);
     add_cnt(&C->cnt[s][R_HEAVY_VIS], n_vis);
+    add_cnt(&nxt->cum_vis, n_vis);
     add_cnt(&C->cnt[s][R_HEAVY_APP], n_app);
     add_cnt(&C->cnt[s][R_HEAVY_ITEMS], n_items);
   }
@@ -1258,8 +1456,9 @@ Engine::~Engine() {
 
 std::vector<DevBuf *> Engine::all_bufs() {
   return {&labels, &vis[0], &vis[1], &rq[0][0], &rq[0][1], &rq[1][0],
-          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &Z, &P, &centers,
-          &front, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
+          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &rem[0], &rem[1],
+          &sort_tmp, &centers,
+          &front, &trim_bm, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
@@ -1296,6 +1495,8 @@ int Engine::setup() {
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_heavy,
                                                        BLOCK, 0));
   grid_heavy = std::max(1, occ) * num_cu;
+  PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_pull, BLOCK, 0));
+  grid_pull = std::max(1, occ) * num_cu;
   return PSC_OK;
 }
 
@@ -1332,12 +1533,11 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     PSC_TRY(hc[s].ensure(hcap * 8));
   }
   PSC_TRY(NZ.ensure(nn * 4));
-  PSC_TRY(Z.ensure(nn * 4));
-  PSC_TRY(P.ensure(nn * 4));
   PSC_TRY(centers.ensure(nn * 4));
   PSC_TRY(front.ensure(nn * 4));
   const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
   PSC_TRY(tiles.ensure(ntiles * 4));
+  PSC_TRY(trim_bm.ensure(words * 4));
   PSC_TRY(tile_off.ensure(ntiles * 8));
   PSC_TRY(reach_desc.ensure(sizeof(ReachSearch) * 2));
   PSC_TRY(reach_ctl.ensure(sizeof(ReachCtl)));
@@ -1345,7 +1545,8 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   PSC_TRY(ms_ctl.ensure(sizeof(MSCtl)));
   PSC_TRY(misc.ensure(256));
   PSC_TRY(scratch32.ensure((nn + G) * 4));
-  for (DevBuf *b : {&voff_d, &gA, &gB, &gC, &gD}) PSC_TRY(b->ensure((G + 1) * 8));
+  for (DevBuf *b : {&voff_d, &gB, &gC, &gD}) PSC_TRY(b->ensure((G + 1) * 8));
+  PSC_TRY(gA.ensure(2 * (G + 1) * 8));   // nstart, then zstart
   PSC_TRY(gid_d.ensure((G + 1) * 4));
   PSC_TRY(gpin(G + 2));
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
@@ -1357,6 +1558,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     rs[s].roff = s == 0 ? d_ioff : d_off;
     rs[s].radj = s == 0 ? d_iadj : d_adj;
     rs[s].vis = vis[s].as<uint32_t>();
+    rs[s].trim = trim_bm.as<uint32_t>();
     rs[s].q[0] = rq[s][0].as<uint32_t>();
     rs[s].q[1] = rq[s][1].as<uint32_t>();
     rs[s].hv = hv[s].as<uint32_t>();
@@ -1372,7 +1574,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
 
 // Stable select over [0, len) with functor f; count -> host.
 template <class F>
-int Engine::select(F f, uint64_t len, uint32_t *sel, uint32_t *rej,
+int Engine::select(F f, uint64_t len, typename F::T *sel, typename F::T *rej,
                    uint64_t *count, unsigned long long *count_dev) {
   if (len == 0) {
     if (count) *count = 0;
@@ -1474,6 +1676,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   for (int s = 0; s < 2; ++s) {
     rs[s].thresh = thresh;
     rs[s].avg_deg = n ? (double)m / (double)n : 0.0;
+    rs[s].nnz = nnz_total;
   }
   char *pin = (char *)h_pinned;
   std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
@@ -1481,23 +1684,30 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
                          hipMemcpyHostToDevice, stream));
   ReachSearch *S = reach_desc.as<ReachSearch>();
   ReachCtl *C = reach_ctl.as<ReachCtl>();
-  const uint64_t words = (n + 31) / 32;
   prof_begin(KK_MISC);
-  k_reach_reset<<<grid_for(words, BLOCK, 8192), BLOCK, 0, stream>>>(S, C, 2, words);
+  k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
   k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
-      S, C, 2, P.as<uint32_t>(), gA.as<unsigned long long>(), G);
+      S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed);
   prof_end();
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   uint32_t L = 0;
   // the previous run's level count (same graph: one poll, no idle levels)
   int K = reach_levels_hint ? (int)std::min<uint32_t>(reach_levels_hint + 1, 64) : 4;
-  ReachCtl hc_;
+  // timing >= 3: one level per poll and a per-level report on stderr
+  const bool trace = p.timing >= 3;
+  if (trace) K = 1;
+  ReachCtl hc_, prev_{};
+  auto t_lvl = std::chrono::steady_clock::now();
   while (true) {
     for (int k = 0; k < K; ++k, ++L) {
       prof_begin(KK_STEP);
       k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, L, tau,
                                                       p.local_reach, total_waves);
+      prof_end();
+      prof_begin(KK_PULL);
+      k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L,
+                                                    (uint32_t)grid_pull * WPB);
       prof_end();
       prof_begin(KK_HEAVY);
       k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L,
@@ -1506,8 +1716,35 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
     }
     PSC_HIP(hipGetLastError());
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
+    if (trace) {
+      const auto now = std::chrono::steady_clock::now();
+      const double us = std::chrono::duration<double, std::micro>(now - t_lvl).count();
+      t_lvl = now;
+      const uint32_t lv = L - 1;
+      for (int s = 0; s < 2; ++s) {
+        const ReachSlot &c = hc_.slot[s][lv % 3];
+        const unsigned long long *a = hc_.cnt[s], *b = prev_.cnt[s];
+        const double ds = c.from_pull ? (double)c.qsize * rs[s].avg_deg : (double)c.degsum;
+        const double push = (double)c.qsize + ds;
+        const double unv = nnz_total > c.cum_vis ? (double)(nnz_total - c.cum_vis) : 0.0;
+        const bool pull = push >= (double)thresh ||
+                          unv * PULL_VERTEX_COST + (double)(n / 2048) < push;
+        const char *mode = c.qsize == 0 ? "none" : (pull ? "pull" : "push");
+        std::fprintf(stderr,
+                     "level %u %s %s nf=%u degsum=%llu heavy=%llu/%llu next=%u "
+                     "push_exp=%llu push_edges=%llu pull_scans=%llu pull_edges=%llu "
+                     "heavy_edges=%llu wall=%.1fus\n",
+                     lv, s ? "bwd" : "fwd", mode, c.qsize, c.degsum,
+                     c.heavy >> H_CNT_SHIFT, c.heavy & H_CHUNK_MASK,
+                     hc_.slot[s][(lv + 1) % 3].qsize, a[R_PUSH_EXP] - b[R_PUSH_EXP],
+                     a[R_PUSH_EDGES] - b[R_PUSH_EDGES], a[R_PULL_SCANS] - b[R_PULL_SCANS],
+                     a[R_PULL_EDGES] - b[R_PULL_EDGES],
+                     a[R_HEAVY_EDGES] - b[R_HEAVY_EDGES], us);
+      }
+      prev_ = hc_;
+    }
     if (hc_.done[0] && hc_.done[1]) break;
-    K = std::min(K * 2, 64);
+    if (!trace) K = std::min(K * 2, 64);
   }
   st->levels_fwd = hc_.levels[0];
   st->levels_bwd = hc_.levels[1];
@@ -1521,17 +1758,19 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
                         c[R_PUSH_VIS] + 4.0 * c[R_PUSH_EXP] + 20.0 * c[R_PUSH_APP];
     const double heavy = 5.0 * c[R_HEAVY_EDGES] + c[R_HEAVY_VIS] +
                          20.0 * c[R_HEAVY_APP] + 32.0 * c[R_HEAVY_ITEMS];
-    // pull: bitmap words of every 64-vertex unit, in-offsets of unvisited
-    // vertices, in-edge ids + visited test per scanned edge, new vertices
-    const double pull = 8.0 * c[R_PULL_UNITS] + 16.0 * c[R_PULL_SCANS] +
+    // pull: (visited | trimmed) words of every 64-word chunk, in-offsets of
+    // scanned vertices, in-edge ids + visited test per scanned edge, new
+    // vertices
+    const double pull = 512.0 * c[R_PULL_UNITS] + 16.0 * c[R_PULL_SCANS] +
                         5.0 * c[R_PULL_EDGES] + 5.0 * c[R_PULL_VIS];
-    kbytes[KK_STEP] += push + pull;
+    kbytes[KK_STEP] += push;
+    kbytes[KK_PULL] += pull;
     kbytes[KK_HEAVY] += heavy;
     st->bytes_csr += 16 * (c[R_PUSH_EXP] + c[R_PULL_SCANS]) +
                      4 * (c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES]);
     st->bytes_label += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
                        c[R_PUSH_VIS] + c[R_HEAVY_VIS] + c[R_PULL_VIS] +
-                       8 * c[R_PULL_UNITS];
+                       512 * c[R_PULL_UNITS];
     st->bytes_front += 4 * (c[R_PUSH_APP] + c[R_HEAVY_APP] + c[R_PULL_VIS] +
                             c[R_PUSH_EXP]) +
                        16 * (c[R_PUSH_APP] + c[R_HEAVY_APP]);
@@ -1691,39 +1930,41 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   unsigned long long *voffd = voff_d.as<unsigned long long>();
   auto t_host0 = std::chrono::steady_clock::now();
   if (p.timing) PSC_HIP(hipEventRecord(ev[0], stream));
-  // ---- init + trim, per graph (src/scc.hpp:284-306)
-  prof_begin(KK_MISC);
-  k_fill_u64<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n, 0ull);
-  prof_end();
-  kbytes[KK_MISC] += 8.0 * n;
-  uint64_t nnz = 0, nz = 0;
-  PSC_TRY(select(SelNonZero{d_off, d_ioff}, n, NZ.as<uint32_t>(),
-                 Z.as<uint32_t>(), &nnz));
-  kbytes[KK_SELECT] += 2 * 32.0 * n + 4.0 * n;
-  nz = n - nnz;
-  st->n_trimmed = nz;
-  // per-graph segment starts of NZ (gA) and Z (gB)
+  // ---- init + trim, per graph (src/scc.hpp:284-306): one pass counts the
+  // non-trimmed vertices per tile (and writes the reach dead-end bitmaps),
+  // one writes NZ and every label
   unsigned long long *nstart = gA.as<unsigned long long>();
-  unsigned long long *zstart = gB.as<unsigned long long>();
-  prof_begin(KK_MISC);
-  k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
-      NZ.as<uint32_t>(), nnz, nullptr, voffd, G, nullptr, G, nstart);
-  k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
-      Z.as<uint32_t>(), nz, nullptr, voffd, G, nullptr, G, zstart);
-  if (nz)
-    k_trim_labels<<<grid_for(nz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        Z.as<uint32_t>(), nz, voffd, G, zstart, lab);
-  if (nnz)
-    k_permute<<<grid_for(nnz, BLOCK, 16384), BLOCK, 0, stream>>>(
-        NZ.as<uint32_t>(), nnz, nstart, G, p.seed, P.as<uint32_t>());
-  prof_end();
-  kbytes[KK_MISC] += 12.0 * nz + 8.0 * nnz;
-  PSC_HIP(hipGetLastError());
-  std::vector<uint64_t> h_nstart(G + 1), h_zstart(G + 1);
-  PSC_TRY(poll(h_nstart.data(), nstart, 8 * (G + 1)));
-  PSC_TRY(poll(h_zstart.data(), zstart, 8 * (G + 1)));
+  unsigned long long *zstart = nstart + (G + 1);
+  std::vector<uint64_t> h_starts(2 * (G + 1), 0);
+  if (n) {
+    const uint64_t words = (n + 31) / 32;
+    const uint64_t nt = (n + SEL_TILE - 1) / SEL_TILE;
+    unsigned long long *tot = misc.as<unsigned long long>();
+    prof_begin(KK_MISC);
+    k_trim_count<<<(unsigned)nt, BLOCK, 0, stream>>>(d_off, d_ioff, n, words,
+                                                    tiles.as<uint32_t>(),
+                                                    vis[0].as<uint32_t>(),
+                                                    vis[1].as<uint32_t>(),
+                                                    trim_bm.as<uint32_t>());
+    k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
+                                         tile_off.as<unsigned long long>(), tot);
+    k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
+        d_off, d_ioff, n, voffd, G, tile_off.as<unsigned long long>(), tot, nstart,
+        zstart);
+    k_trim_scatter<<<(unsigned)nt, BLOCK, 0, stream>>>(
+        d_off, d_ioff, n, voffd, G, zstart, tile_off.as<unsigned long long>(),
+        NZ.as<uint32_t>(), lab);
+    prof_end();
+    kbytes[KK_MISC] += 2 * 32.0 * n + n / 4.0 + 8.0 * n + 4.0 * n;
+    PSC_HIP(hipGetLastError());
+    PSC_TRY(poll(h_starts.data(), nstart, 16 * (G + 1)));
+  }
+  const uint64_t *h_nstart = h_starts.data(), *h_zstart = h_starts.data() + G + 1;
+  const uint64_t nnz = h_nstart[G], nz = n - nnz;
+  nnz_total = nnz;
+  st->n_trimmed = nz;
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
-  st->bytes_label += 8 * n + 8 * nz;
+  st->bytes_label += 8 * n;
   st->bytes_csr += 32 * n;
   // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1
   std::vector<uint64_t> label_off(G);
@@ -1744,10 +1985,31 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     prof_end();
     kbytes[KK_MISC] += 8.0 * n + 8.0 * nnz + nnz / 4.0;
     PSC_HIP(hipGetLastError());
-    PSC_TRY(select(SelNotBoth{P.as<uint32_t>(), vis[0].as<uint32_t>(),
-                              vis[1].as<uint32_t>()},
-                   nnz, centers.as<uint32_t>(), nullptr, &n_rem));
-    kbytes[KK_SELECT] += 2 * 12.0 * nnz + 4.0 * n_rem;
+    // remaining vertices (src/scc.hpp:349) in pivot order: selected in NZ
+    // order with their P position as key, then sorted by it
+    PSC_TRY(rem[0].ensure(nnz * 8));
+    PSC_TRY(select(SelRemaining{NZ.as<uint32_t>(), vis[0].as<uint32_t>(),
+                                vis[1].as<uint32_t>(), nstart, G, p.seed},
+                   nnz, rem[0].as<unsigned long long>(), nullptr, &n_rem));
+    kbytes[KK_SELECT] += 2 * 4.0 * nnz + nnz / 4.0 + 8.0 * n_rem;
+    if (n_rem) {
+      unsigned kb = 1;
+      while (kb < 32 && (1ull << kb) < nnz) ++kb;
+      size_t tb = 0;
+      PSC_HIP(rocprim::radix_sort_keys(nullptr, tb, rem[0].as<unsigned long long>(),
+                                       rem[1].as<unsigned long long>(), n_rem, 32,
+                                       32 + kb, stream));
+      PSC_TRY(rem[1].ensure(n_rem * 8));
+      PSC_TRY(sort_tmp.ensure(std::max<size_t>(tb, 8)));
+      prof_begin(KK_SELECT);
+      PSC_HIP(rocprim::radix_sort_keys(sort_tmp.ptr, tb, rem[0].as<unsigned long long>(),
+                                       rem[1].as<unsigned long long>(), n_rem, 32,
+                                       32 + kb, stream));
+      k_low32<<<grid_for(n_rem, BLOCK, 4096), BLOCK, 0, stream>>>(
+          rem[1].as<unsigned long long>(), n_rem, centers.as<uint32_t>());
+      prof_end();
+      kbytes[KK_SELECT] += 4.0 * 16.0 * n_rem + 12.0 * n_rem;
+    }
     prof_begin(KK_MISC);
     k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
         centers.as<uint32_t>(), n_rem, nullptr, voffd, G, nullptr, G,
@@ -1934,7 +2196,7 @@ int Engine::prof_collect() {
 }
 
 const char *const KK_NAMES[KK_N] = {
-    "k_reach_step", "k_reach_heavy", "k_ms_expand", "k_ms_heavy",
+    "k_reach_step", "k_reach_pull", "k_reach_heavy", "k_ms_expand", "k_ms_heavy",
     "k_partition", "select", "k_ms_clear+begin", "init/trim/finalize"};
 
 int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
@@ -2150,6 +2412,12 @@ int psc_device_free(void *ptr) {
     return PSC_ERR_HIP;
   }
   return PSC_OK;
+}
+
+uint64_t psc_pivot_perm(uint64_t n, uint64_t seed, uint64_t i, int inverse) {
+  if (n == 0 || i >= n) return 0;
+  const psc::PivotPerm perm(n, seed);
+  return inverse ? perm.inverse(i) : perm(i);
 }
 
 int psc_host_alloc(size_t bytes, void **ptr) {

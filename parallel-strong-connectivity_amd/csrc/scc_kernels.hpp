@@ -18,12 +18,14 @@ constexpr uint32_t PULL_HEAVY = 64;  // pull remainder above which it is chunked
 constexpr uint32_t HEAVY_DEG = 512;  // degree above which edges are chunked
 constexpr uint32_t CHUNK = 1024;     // edges per heavy work item
 constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests first
-constexpr uint32_t VGC_MAX_DEG = 32; // only vertices this sparse stay local
+constexpr uint32_t VGC_MAX_DEG = 8;  // only vertices this sparse stay local
 // A wave's local search also stops taking vertices once it has scanned
 // VGC_EDGE_MULT * tau edges per seed: a lone wave walking thousands of
 // edges serially costs more than the levels it saves on low-diameter graphs,
 // while degree-<=4 lattices never reach the cap.
 constexpr uint32_t VGC_EDGE_MULT = 4;
+// Pull cost estimate per unvisited non-trimmed vertex, in edge units.
+constexpr uint64_t PULL_VERTEX_COST = 8;
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 
@@ -36,6 +38,7 @@ struct ReachSearch {
   const uint64_t *roff;  // pull direction CSR (transpose)
   const uint32_t *radj;
   uint32_t *vis;         // visited bitmap, ceil(n/32) words
+  const uint32_t *trim;  // trimmed-vertex bitmap (skipped by pull levels)
   uint32_t *q[2];        // frontier ping-pong, n entries each
   uint32_t *hv;          // heavy vertices of the current level
   unsigned long long *hc;  // exclusive chunk start of each heavy vertex
@@ -43,6 +46,7 @@ struct ReachSearch {
   double avg_deg;        // m / n, out-edge estimate after a pull level
   uint32_t n;
   uint32_t pad;
+  uint64_t nnz;          // non-trimmed vertices (all graphs)
 };
 
 struct ReachSlot {
@@ -53,6 +57,7 @@ struct ReachSlot {
   unsigned long long hgrab;  // heavy chunks handed out
   uint32_t from_pull;        // frontier produced by a pull level
   uint32_t pad;
+  unsigned long long cum_vis;  // vertices visited before this level
 };
 
 // per-search counters (algorithmic traffic accounting)

@@ -237,3 +237,15 @@ def test_batch_spec_distribution(psc):
     assert np.all((rm & (rm - 1)) == 0)      # RMAT sizes are powers of two
     side = np.sqrt(n[kind == 1].astype(np.float64))
     assert np.all(side == np.round(side))    # grids are squares
+
+
+def test_pivot_perm_matches_oracle_and_inverts(psc, oracle):
+    """The device path's pivot order (element i of P_g) is the oracle's, and
+    its inverse (used to key the remaining vertices by P position without
+    materialising P) undoes it."""
+    for N in (1, 2, 3, 17, 1000, 4097, 65536):
+        for seed in (1, 3, 99):
+            fwd = [psc.lib.psc_pivot_perm(N, seed, i, 0) for i in range(N)]
+            assert fwd == [oracle.pivot_perm(N, seed, i) for i in range(N)]
+            assert sorted(fwd) == list(range(N))
+            assert [psc.lib.psc_pivot_perm(N, seed, x, 1) for x in fwd] == list(range(N))
