@@ -223,6 +223,7 @@ def main():
     ap.add_argument("--samples-log2", type=int, default=29)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--tau", type=int, default=512)
+    ap.add_argument("--dense-div", type=int, default=20)
     ap.add_argument("--no-vgc", action="store_true")
     ap.add_argument("--levels", action="store_true",
                     help="report the first reach level by level (stderr)")
@@ -272,7 +273,7 @@ def main():
     log(f"generated {workload}: n={dg.n} m={dg.m} in {gen_s:.2f}s")
     s = psc.SCC(dg)
     prm = dict(beta=1.5, local_reach=not args.no_vgc, local_scc=not args.no_vgc,
-               tau=args.tau, timing=1)
+               tau=args.tau, dense_div=args.dense_div, timing=1)
     for _ in range(args.warmup):
         s.run(**prm)
     times = []

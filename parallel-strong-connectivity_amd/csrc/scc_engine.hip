@@ -667,21 +667,94 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
 // Pull level of both searches (src/reach.hpp:173-190): every unvisited,
 // non-trimmed vertex scans its in-edges until it meets a visited vertex;
 // testing "visited" instead of "in the previous frontier" keeps the
-// reachable set identical.  The vertex range is walked by bitmap: a wave
-// loads 64 words of (visited | trimmed) with one coalesced load (2048
-// vertices) and only works on the words with a clear bit, two words (64
-// vertices) at a time.  A vertex's first PULL_PROBE in-edges are probed
-// with independent loads; longer remainders run as load-balanced 64-edge
-// windows over the wave's vertices, or, beyond PULL_HEAVY edges, as chunks
-// on the heavy list (k_reach_heavy).  Trimmed vertices are skipped: they
-// never join a first-round reach set (src/scc.hpp:341-347) and no pull
-// probe reads them as visited.
+// reachable set identical.  Trimmed vertices are skipped: they never join
+// a first-round reach set (src/scc.hpp:341-347) and no pull probe reads
+// them as visited.
+// The vertex range is walked by bitmap: a wave loads 64 words of
+// (visited | trimmed) with one coalesced load (2048 vertices) and works on
+// the words with a clear bit, PULL_PAIRS x 2 words (64 vertices each) at a
+// time with all their loads in flight together: in-offsets, then the first
+// PULL_PROBE in-edges, then their visited words.  Vertices left undecided
+// queue their remaining in-edges in LDS; a full batch of 64 queued
+// vertices is scanned as load-balanced 64-edge windows.  Remainders longer
+// than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
+constexpr int PULL_PAIRS = 2;
+constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
+
+struct PullQueue {   // per-wave LDS queue of undecided vertices
+  uint32_t *v;
+  uint64_t *b;
+  uint32_t *deg;
+  uint32_t n;
+};
+
+// Scan the first min(64, q.n) queued vertices' remaining in-edges as
+// 64-edge windows; a vertex meeting a visited in-neighbour is marked and
+// appended to the next frontier.
+template <class Out>
+__device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
+                                           uint32_t *sc, uint64_t *st, uint32_t *fl,
+                                           Out &out, uint32_t *qsize, uint32_t *qn,
+                                           unsigned long long &n_edges,
+                                           unsigned long long &n_vis) {
+  const int l = lane();
+  const uint32_t k = min(q.n, (uint32_t)WAVE);
+  const bool own = (uint32_t)l < k;
+  const uint32_t d = own ? q.deg[l] : 0u;
+  const uint64_t b = own ? q.b[l] : 0ull;
+  const uint32_t v = own ? q.v[l] : 0u;
+  wave_sync();
+  const uint32_t inc = wave_incl_scan(d);
+  const uint32_t total = __shfl(inc, WAVE - 1);
+  sc[l] = inc;
+  st[l] = b;
+  fl[l] = 0;
+  wave_sync();
+  for (uint32_t base = 0; base < total; base += WAVE) {
+    const uint32_t e = base + l;
+    if (e < total) {
+      int lo = 0, hi = WAVE - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sc[mid] > e) hi = mid; else lo = mid + 1;
+      }
+      const uint32_t ex = lo ? sc[lo - 1] : 0u;
+      if (!fl[lo]) {
+        ++n_edges;
+        if (bit_test(R.vis, R.radj[st[lo] + (e - ex)])) fl[lo] = 1;
+      }
+    }
+    wave_sync();
+    if (!ballot(d > 0 && !fl[l])) break;
+  }
+  bool found = false;
+  if (own && d > 0 && fl[l]) {
+    const uint32_t bit = 1u << (v & 31);
+    found = !(atomicOr(&R.vis[v >> 5], bit) & bit);
+  }
+  n_vis += found;
+  // move the unprocessed tail (q.n > 64) to the front
+  const uint32_t rest = q.n - k;
+  wave_sync();
+  for (uint32_t i = l; i < rest; i += WAVE) {
+    q.v[i] = q.v[k + i];
+    q.b[i] = q.b[k + i];
+    q.deg[i] = q.deg[k + i];
+  }
+  q.n = rest;
+  wave_sync();
+  out.put(found, v, qsize, qn, R.n);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t total_waves) {
   __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
   __shared__ uint32_t fl_s[WPB][WAVE];
+  __shared__ uint32_t qv_s[WPB][PULL_QCAP];
+  __shared__ uint64_t qb_s[WPB][PULL_QCAP];
+  __shared__ uint32_t qd_s[WPB][PULL_QCAP];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *sc = sc_s[w], *fl = fl_s[w];
   uint64_t *st = st_s[w];
@@ -692,6 +765,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   xcd_split(ns, w, &s_lo, &s_hi, &gw, &nw, &total_waves);
   WaveOut<uint32_t, OUT_CAP> out;
   out.init(out_s[w]);
+  PullQueue q{qv_s[w], qb_s[w], qd_s[w], 0};
   for (int s = s_lo; s < s_hi; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
@@ -713,82 +787,80 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       ++n_chunks;
       uint64_t act = ballot(sk != ~0u);
       while (act) {
-        // lanes 0-31 take the first active word, lanes 32-63 the next
-        const int wa = __ffsll((unsigned long long)act) - 1;
-        act &= act - 1;
-        int wb = -1;
-        if (act) {
-          wb = __ffsll((unsigned long long)act) - 1;
-          act &= act - 1;
-        }
-        const int myw = l < 32 ? wa : wb;
-        const uint32_t mysk = __shfl(sk, myw < 0 ? 0 : myw);
-        const uint64_t v = (c * WAVE + (uint64_t)(myw < 0 ? 0 : myw)) * 32 + (l & 31);
-        bool found = false;
-        uint64_t b = 0, deg = 0;
-        if (myw >= 0 && !((mysk >> (l & 31)) & 1u)) {
-          ++n_scan;
-          b = R.roff[v];
-          deg = R.roff[v + 1] - b;
-          // most vertices of a dense level meet a visited in-neighbour
-          // among their first few in-edges; their loads are issued together
-          const uint64_t k = min(deg, (uint64_t)PULL_PROBE);
-          uint32_t nb[PULL_PROBE];
+        // PULL_PAIRS word pairs: lanes 0-31 take the first word of a pair,
+        // lanes 32-63 the second
+        int wsel[2 * PULL_PAIRS];
 #pragma unroll
-          for (uint32_t j = 0; j < PULL_PROBE; ++j)
-            nb[j] = j < k ? R.radj[b + j] : 0u;
-          uint32_t wd[PULL_PROBE];
-#pragma unroll
-          for (uint32_t j = 0; j < PULL_PROBE; ++j)
-            wd[j] = j < k ? R.vis[nb[j] >> 5] : 0u;
-#pragma unroll
-          for (uint32_t j = 0; j < PULL_PROBE; ++j)
-            found |= j < k && ((wd[j] >> (nb[j] & 31)) & 1u);
-          n_edges += k;
-          b += k;
-          deg = found ? 0 : deg - k;
-        }
-        // long remainders go to the heavy list (k_reach_heavy, chunked)
-        const bool heavy = deg > PULL_HEAVY;
-        heavy_append(R, cur, heavy, (uint32_t)v, deg);
-        if (heavy) deg = 0;
-        // the rest: load-balanced 64-edge windows over the remainders of
-        // the wave's 64 vertices, skipping owners already resolved
-        const uint32_t inc = wave_incl_scan((uint32_t)deg);
-        const uint32_t total = __shfl(inc, WAVE - 1);
-        if (total) {
-          sc[l] = inc;
-          st[l] = b;
-          fl[l] = 0;
-          wave_sync();
-          for (uint32_t base = 0; base < total; base += WAVE) {
-            const uint32_t e = base + l;
-            if (e < total) {
-              int lo = 0, hi = WAVE - 1;
-              while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (sc[mid] > e) hi = mid; else lo = mid + 1;
-              }
-              const uint32_t ex = lo ? sc[lo - 1] : 0u;
-              if (!fl[lo]) {
-                ++n_edges;
-                if (bit_test(R.vis, R.radj[st[lo] + (e - ex)])) fl[lo] = 1;
-              }
-            }
-            wave_sync();
-            if (!ballot(deg > 0 && !fl[l])) break;
+        for (int j = 0; j < 2 * PULL_PAIRS; ++j) {
+          wsel[j] = -1;
+          if (act) {
+            wsel[j] = __ffsll((unsigned long long)act) - 1;
+            act &= act - 1;
           }
-          if (deg > 0 && fl[l]) found = true;
-          wave_sync();
         }
-        const uint64_t fm = ballot(found);
-        if (!fm) continue;
-        if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[c * WAVE + wa], (uint32_t)fm);
-        if (l == 32 && (fm >> 32)) atomicOr(&R.vis[c * WAVE + wb], (uint32_t)(fm >> 32));
-        n_vis += found;
-        out.put(found, (uint32_t)v, &nxt->qsize, qn, R.n);
+        uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
+        bool todo[PULL_PAIRS];
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) {
+          const int myw = l < 32 ? wsel[2 * j] : wsel[2 * j + 1];
+          const uint32_t mysk = __shfl(sk, myw < 0 ? 0 : myw);
+          v[j] = (c * WAVE + (uint64_t)(myw < 0 ? 0 : myw)) * 32 + (l & 31);
+          todo[j] = myw >= 0 && !((mysk >> (l & 31)) & 1u);
+          b[j] = todo[j] ? R.roff[v[j]] : 0ull;
+          deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
+        }
+        uint32_t nb[PULL_PAIRS][PULL_PROBE];
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) {
+          deg[j] -= b[j];
+          n_scan += todo[j];
+#pragma unroll
+          for (uint32_t t = 0; t < PULL_PROBE; ++t)
+            nb[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
+        }
+        uint32_t wd[PULL_PAIRS][PULL_PROBE];
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j)
+#pragma unroll
+          for (uint32_t t = 0; t < PULL_PROBE; ++t)
+            wd[j][t] = t < deg[j] ? R.vis[nb[j][t] >> 5] : 0u;
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) {
+          bool found = false;
+#pragma unroll
+          for (uint32_t t = 0; t < PULL_PROBE; ++t)
+            found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
+          const uint64_t k = min(deg[j], (uint64_t)PULL_PROBE);
+          n_edges += k;
+          const uint64_t rem = found ? 0 : deg[j] - k;
+          // long remainders go to the heavy list (k_reach_heavy, chunked),
+          // the rest to the wave's queue
+          const bool heavy = rem > PULL_HEAVY;
+          heavy_append(R, cur, heavy, (uint32_t)v[j], rem);
+          const bool qd = rem > 0 && !heavy;
+          const uint64_t qm = ballot(qd);
+          if (qd) {
+            const uint32_t at = q.n + lanes_below(qm);
+            q.v[at] = (uint32_t)v[j];
+            q.b[at] = b[j] + k;
+            q.deg[at] = (uint32_t)rem;
+          }
+          q.n += (uint32_t)__popcll(qm);
+          const uint64_t fm = ballot(found);
+          if (fm) {
+            const int wa = wsel[2 * j], wb = wsel[2 * j + 1];
+            if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[c * WAVE + wa], (uint32_t)fm);
+            if (l == 32 && (fm >> 32)) atomicOr(&R.vis[c * WAVE + wb], (uint32_t)(fm >> 32));
+            n_vis += found;
+            out.put(found, (uint32_t)v[j], &nxt->qsize, qn, R.n);
+          }
+        }
+        wave_sync();
+        while (q.n >= WAVE)
+          pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
       }
     }
+    while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, R.n);
     add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
     add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
