@@ -51,7 +51,8 @@ class CParams(C.Structure):
                 ("local_scc", C.c_int32), ("tau", C.c_uint64),
                 ("min_bag", C.c_uint64), ("dense_div", C.c_uint64),
                 ("seed", C.c_uint64), ("table_init", C.c_uint64),
-                ("canonical", C.c_int32), ("timing", C.c_int32)]
+                ("canonical", C.c_int32), ("timing", C.c_int32),
+                ("tail_max", C.c_uint64)]
 
 
 class CStats(C.Structure):

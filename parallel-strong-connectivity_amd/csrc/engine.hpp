@@ -19,6 +19,24 @@ struct MSSearch;
 void set_error(const std::string &msg);
 extern thread_local std::string g_last_error;
 
+// Error propagation of the host code: a failed HIP call records its message
+// and returns PSC_ERR_HIP; PSC_TRY forwards a non-zero status.
+#define PSC_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+      return PSC_ERR_HIP;                                                 \
+    }                                                                     \
+  } while (0)
+
+#define PSC_TRY(expr)            \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != PSC_OK) return r_; \
+  } while (0)
+
+
 // Device memory comes from the device's stream-ordered pool, allocated
 // and freed in the order of the owning handle's stream: growing a buffer
 // never waits for other streams (a plain hipFree waits for the device).
@@ -41,7 +59,7 @@ constexpr size_t PIN_POLL = 0, PIN_REACH = 1024, PIN_MS = 2048, PIN_FLAGS = 3072
 // kernel kinds timed by the per-launch profiler (timing >= 2)
 enum : int {
   KK_STEP, KK_PULL, KK_HEAVY, KK_MS_EXPAND, KK_MS_HEAVY, KK_PARTITION, KK_SELECT,
-  KK_SEED, KK_MISC, KK_N
+  KK_SEED, KK_TAIL, KK_MISC, KK_N
 };
 extern const char *const KK_NAMES[KK_N];
 
@@ -65,6 +83,8 @@ struct Engine {
   DevBuf NZ, centers, front, tiles, tile_off;
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
   DevBuf rem[2], sort_tmp;   // remaining vertices keyed by pivot position
+  // single-launch rounds of small remainders (tail.hip)
+  DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta;
   DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
   DevBuf table[2], log[2], mq[2][2], mhp[2], mhc[2];
   int grid_step = 1, grid_ms = 1, grid_heavy = 1, grid_pull = 1;
@@ -122,6 +142,9 @@ struct Engine {
   int prof_collect();
   int first_reach(const psc_params &p, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
+  int ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_rem,
+              std::vector<uint64_t> &cur, std::vector<uint64_t> &stepg, psc_stats *st,
+              bool *used);
   int multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
                    const uint32_t *wgid_d, uint64_t slots[2], psc_stats *st);
 };

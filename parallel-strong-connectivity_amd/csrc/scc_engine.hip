@@ -1433,20 +1433,6 @@ __global__ void k_status_reduce(const uint32_t *cnt, uint64_t len,
 // ======================================================================
 //                              host side
 // ======================================================================
-#define PSC_HIP(call)                                                     \
-  do {                                                                    \
-    hipError_t e_ = (call);                                               \
-    if (e_ != hipSuccess) {                                               \
-      set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
-      return PSC_ERR_HIP;                                                 \
-    }                                                                     \
-  } while (0)
-
-#define PSC_TRY(expr)            \
-  do {                           \
-    int r_ = (expr);             \
-    if (r_ != PSC_OK) return r_; \
-  } while (0)
 
 static int grid_for(uint64_t n, int per_block = BLOCK, int cap = 65536) {
   uint64_t g = (n + per_block - 1) / per_block;
@@ -1534,7 +1520,8 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd};
+          &gid_d, &rnd, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &tail_lists, &tail_meta};
 }
 
 int Engine::download_labels(uint64_t *dst) {
@@ -2105,6 +2092,15 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   const double beta_def = p.beta > 0 ? p.beta : 1.5;
   uint64_t live = 0;
   for (uint32_t g = 0; g < G; ++g) live += (h_cstart[g + 1] > h_cstart[g]);
+  if (live && p.tail_max) {
+    // small remainders: all their rounds in one launch (tail.hip)
+    bool used = false;
+    PSC_TRY(ms_tail(p, h_cstart.data(), n_rem, cur, stepg, st, &used));
+    if (used) {
+      live = 0;
+      for (uint32_t g = 0; g < G; ++g) live += cur[g] < h_cstart[g + 1] - h_cstart[g];
+    }
+  }
   while (live) {
     wpre.assign(1, 0);
     wsrc.clear();
@@ -2269,7 +2265,8 @@ int Engine::prof_collect() {
 
 const char *const KK_NAMES[KK_N] = {
     "k_reach_step", "k_reach_pull", "k_reach_heavy", "k_ms_expand", "k_ms_heavy",
-    "k_partition", "select", "k_ms_clear+begin", "init/trim/finalize"};
+    "k_partition", "select", "k_ms_clear+begin", "tail (compact + k_ms_tail)",
+    "init/trim/finalize"};
 
 int Engine::status(const unsigned long long *d_labels, uint64_t *n_scc,
                    uint64_t *largest) {
@@ -2315,6 +2312,7 @@ void psc_params_default(psc_params *p) {
   p->table_init = 0;
   p->canonical = 0;
   p->timing = 1;
+  p->tail_max = 65536;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }

@@ -45,6 +45,10 @@ def test_host_batch_equals_standalone(psc, oracle, fixture_graphs):
     labels2 = b.split(b.run(tau=1, local_reach=False, local_scc=False, table_init=16))
     for g, lab in zip(graphs, labels2):
         assert np.array_equal(lab, psc.scc(g))
+    # host-driven rounds for every graph == single-launch remainders
+    labels3 = b.split(b.run(tail_max=0))
+    for g, lab in zip(graphs, labels3):
+        assert np.array_equal(lab, psc.scc(g))
     b.close()
 
 

@@ -28,6 +28,11 @@ PARAM_SETS = [
     {"dense_div": 1},
     {"dense_div": 1 << 40},
     {"seed": 7},
+    # host-driven rounds only (no single-launch remainder)
+    {"tail_max": 0},
+    {"tail_max": 0, "beta": 2.0, "seed": 3},
+    # remainders of up to 100 vertices in one launch, larger ones on the host
+    {"tail_max": 100},
 ]
 
 
@@ -88,13 +93,19 @@ def test_repeat_runs_identical(psc, fixtures):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-def test_overflow_resize_path(psc, oracle, fixtures):
+@pytest.mark.parametrize("tail_max", [0, 65536])
+def test_overflow_resize_path(psc, oracle, fixtures, tail_max):
+    """Tables forced tiny: the host-driven rounds double them (tail_max=0);
+    the single-launch remainder stops at the overflowing round and hands the
+    graph to the host-driven rounds (default).  Labels stay exact."""
     name, g, canon = [f for f in fixtures if f[0] == "random_2000_2600"][0]
     s = psc.SCC(g)
-    labels = s.scc(table_init=16)
+    labels = s.scc(table_init=16, tail_max=tail_max)
     assert s.stats["resizes"] > 0
     s.close()
     assert np.array_equal(oracle.canonical(labels), canon)
+    ref, _ = oracle.scc_ref(g, table_init=16)
+    assert np.array_equal(labels, ref)
 
 
 def test_c1_rmat16(psc, oracle):
