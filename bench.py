@@ -104,6 +104,9 @@ def bench_batch(args, psc):
         f"in {gen_s:.1f}s")
     bss = [psc.BatchSCC(device_batch=db) for db in dbs]
     prms = [p for _, p, _ in groups]
+    if args.tail_max is not None:
+        for prm in prms:
+            prm["tail_max"] = args.tail_max
     at.run_groups(bss, prms)                       # warm-up (allocations)
     walls = [at.run_groups(bss, prms)[0] for _ in range(args.steps)]
     wall = float(np.median(walls))
@@ -129,6 +132,8 @@ def bench_batch(args, psc):
             s = psc.SCC(dg)
             kw = dict(beta=float(prm["betas"][j]), tau=prm["tau"],
                       dense_div=prm["dense_div"], timing=0)
+            if args.tail_max is not None:
+                kw["tail_max"] = args.tail_max
             s.run(**kw)                            # warm-up (allocations)
             t = time.perf_counter()
             s.run(**kw)
@@ -224,6 +229,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--tau", type=int, default=512)
     ap.add_argument("--dense-div", type=int, default=20)
+    ap.add_argument("--tail-max", type=int, default=None,
+                    help="largest remainder run in one launch (0: host rounds)")
     ap.add_argument("--no-vgc", action="store_true")
     ap.add_argument("--levels", action="store_true",
                     help="report the first reach level by level (stderr)")
@@ -274,6 +281,8 @@ def main():
     s = psc.SCC(dg)
     prm = dict(beta=1.5, local_reach=not args.no_vgc, local_scc=not args.no_vgc,
                tau=args.tau, dense_div=args.dense_div, timing=1)
+    if args.tail_max is not None:
+        prm["tail_max"] = args.tail_max
     for _ in range(args.warmup):
         s.run(**prm)
     times = []

@@ -610,6 +610,8 @@ class BatchSCC:
     def split(self, labels):
         return [labels[self.vbase[g]:self.vbase[g + 1]] for g in range(self.num_graphs)]
 
+    kernel_profile = SCC.kernel_profile
+
     def close(self):
         if self._h:
             lib.psc_destroy(self._h)

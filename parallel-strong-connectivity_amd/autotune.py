@@ -136,9 +136,11 @@ def run_groups(batches, params, labels=False, timing=0):
     def work(i):
         try:
             p = params[i]
+            extra = {"tail_max": p["tail_max"]} if "tail_max" in p else {}
             results[i] = batches[i].run(labels=labels, beta=p["beta"],
                                         beta_per_graph=p.get("betas"), tau=p["tau"],
-                                        dense_div=p["dense_div"], timing=timing)
+                                        dense_div=p["dense_div"], timing=timing,
+                                        **extra)
         except Exception as e:   # surfaced to the caller below
             errors.append(e)
 

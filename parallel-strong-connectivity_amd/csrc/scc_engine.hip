@@ -480,7 +480,7 @@ __device__ __forceinline__ void xcd_split(int ns, int w, int *s_lo, int *s_hi,
 
 // Direction choice of judge() (src/reach.hpp:86-112): sparse iff
 // n_front + out_edges < m / dense_div.  out_edges is exact after a push
-// level (summed at append time) and estimated as n_front * m / n after a
+// level (summed at append time) and estimated as n_front * m / nnz after a
 // pull level, where the reference samples (dense_sample, :192-206).  A
 // pull level here only visits the unvisited non-trimmed vertices (the
 // bitmap walk skips the rest), so a level also pulls when those are few
@@ -583,12 +583,14 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       uint64_t g = gw * per;
       const uint64_t gend = min((uint64_t)nf, g + per);
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
+      uint64_t wbud = vgc ? (uint64_t)tau * VGC_WAVE_MULT : 0;   // per launch
       while (g < gend) {
         const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
         if ((uint32_t)l < cnt) ring[l] = (p ? R.q[1] : R.q[0])[g + l];
         g += cnt;
         uint32_t head = 0, tail = cnt;
-        uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+        uint64_t budget = min((uint64_t)tau * cnt, wbud);
+        const uint64_t budget0 = budget;
         uint64_t ebudget = budget * VGC_EDGE_MULT;
         wave_sync();
         while (head < tail) {
@@ -652,6 +654,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
           }
           wave_sync();
         }
+        wbud -= budget0 - budget;
       }
       out.flush(&nxt->qsize, qn, R.n);
       add_cnt(&nxt->degsum, dsum);
@@ -1159,13 +1162,15 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     const uint64_t gend = min((uint64_t)nf, g + per);
     unsigned long long n_edges = 0, n_exp = 0, n_ins = 0, n_app = 0;
     uint32_t probes = 0;
+    uint64_t wbud = vgc ? (uint64_t)tau * VGC_WAVE_MULT : 0;   // per launch
     while (g < gend) {
       if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
       const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
       if ((uint32_t)l < cnt) ring[l] = (pq ? M.q[1] : M.q[0])[g + l];
       g += cnt;
       uint32_t head = 0, tail = cnt;
-      uint64_t budget = vgc ? (uint64_t)tau * cnt : 0;
+      uint64_t budget = min((uint64_t)tau * cnt, wbud);
+      const uint64_t budget0 = budget;
       uint64_t ebudget = budget * VGC_EDGE_MULT;
       wave_sync();
       while (head < tail) {
@@ -1255,6 +1260,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
         }
         wave_sync();
       }
+      wbud -= budget0 - budget;
     }
     out.flush(M, C, s, qn, nxt);
     add_cnt(&C->cnt[s][M_EDGES], n_edges);
@@ -1734,7 +1740,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   const uint64_t thresh = m / (p.dense_div ? p.dense_div : 20);
   for (int s = 0; s < 2; ++s) {
     rs[s].thresh = thresh;
-    rs[s].avg_deg = n ? (double)m / (double)n : 0.0;
+    // frontier vertices are non-trimmed: their mean degree is m / nnz
+    rs[s].avg_deg = nnz_total ? (double)m / (double)nnz_total : 0.0;
     rs[s].nnz = nnz_total;
   }
   char *pin = (char *)h_pinned;

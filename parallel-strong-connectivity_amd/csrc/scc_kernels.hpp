@@ -24,8 +24,14 @@ constexpr uint32_t VGC_MAX_DEG = 8;  // only vertices this sparse stay local
 // edges serially costs more than the levels it saves on low-diameter graphs,
 // while degree-<=4 lattices never reach the cap.
 constexpr uint32_t VGC_EDGE_MULT = 4;
-// Pull cost estimate per unvisited non-trimmed vertex, in edge units.
-constexpr uint64_t PULL_VERTEX_COST = 8;
+// A wave keeps at most VGC_WAVE_MULT * tau vertices local per launch, over
+// all its seed batches: with a large frontier every wave has work anyway,
+// and long serial local searches would only leave the launch waiting on a
+// few waves.
+constexpr uint32_t VGC_WAVE_MULT = 4;
+// Pull cost estimate per unvisited non-trimmed vertex, in pushed-edge
+// units (measured: a pulled vertex costs about as much as a pushed edge).
+constexpr uint64_t PULL_VERTEX_COST = 1;
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 

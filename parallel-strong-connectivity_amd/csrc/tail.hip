@@ -15,7 +15,9 @@
 //
 // A graph whose pair tables, logs or frontiers overflow stops at the start
 // of that round (its labels and schedule are written back as of the last
-// completed round) and the host-driven rounds finish it.
+// completed round) and is relaunched with larger tables.  A graph whose
+// rounds grow heavy (TAIL_ROUND_EDGES) stops after a completed round and
+// the host-driven rounds, which use the whole device, finish it.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -23,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -35,6 +38,12 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
+constexpr uint64_t TAIL_MAX_SLOTS = 1ull << 23;   // per table of one graph
+constexpr uint32_t TAIL_RING = 512;   // local-search ring per wave (LDS)
+// A graph leaves the single-workgroup rounds (the host-driven rounds, which
+// use the whole device, take over) once a round scanned so many edges that
+// the next one, beta times larger, would exceed this.
+constexpr double TAIL_ROUND_EDGES = 262144.0;
 constexpr uint32_t TAIL_PROBE_LIMIT = 4096;
 
 // Global scratch of the tail, one region per graph (offsets in slots).
@@ -44,6 +53,7 @@ struct TailGraph {
   uint64_t tbase;       // first table slot (two tables of cap slots)
   uint64_t lbase;       // first log / frontier entry (cap / 2 per list)
   uint64_t cap;         // table slots, power of two
+  uint64_t cap_max;     // largest size a relaunch may use
   uint64_t cur, step;   // pivot window schedule
   double beta;
   uint32_t g;
@@ -51,7 +61,7 @@ struct TailGraph {
 };
 
 struct TailState {    // written back per graph
-  unsigned long long cur, step, rounds, full;
+  unsigned long long cur, step, rounds, full, ins, edges, ticks;
 };
 
 // lid[centers[j]] = j, compact labels, and the compact degrees: edges to
@@ -161,6 +171,8 @@ struct TailArgs {
   uint32_t *lists;             // 4 lists of cap / 2 entries per graph
   unsigned long long *lab_off; // per graph label counter
   unsigned long long *stats;   // [0] inserts, [1] edges
+  uint32_t tau;                // local-search budget per frontier pair
+  int vgc;                     // local search on (-local_scc)
 };
 
 // One workgroup per graph: all rounds of its remainder.
@@ -168,6 +180,13 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   __shared__ uint32_t wsum[TAIL_BLOCK / WAVE];
   __shared__ unsigned int cnt_q, cnt_log, flag_full;
   __shared__ unsigned int front_n;
+  __shared__ unsigned long long round_edges;
+  // per-wave local-search ring and window scratch
+  __shared__ uint32_t ring_s[TAIL_BLOCK / WAVE][TAIL_RING];
+  __shared__ uint32_t sc_s[TAIL_BLOCK / WAVE][WAVE];
+  __shared__ uint32_t st_s[TAIL_BLOCK / WAVE][WAVE];
+  __shared__ uint32_t el_s[TAIL_BLOCK / WAVE][WAVE];
+  __shared__ unsigned long long lw_s[TAIL_BLOCK / WAVE][WAVE];
   const TailGraph tg = A.graphs[blockIdx.x];
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
   const uint64_t base = tg.base, nr = tg.nr, half = tg.cap / 2, mask = tg.cap - 1;
@@ -176,19 +195,24 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   uint32_t *const L1 = L0 + half;
   uint32_t *const qa = L0 + 2 * half, *const qb = L0 + 3 * half;   // frontiers
   unsigned long long *lab = A.tlab;
+  const uint32_t tau = A.tau;
+  const int vgc = A.vgc;
   uint64_t cur = tg.cur, step = tg.step;
   unsigned long long loff = A.lab_off[tg.g];
+  const unsigned long long t0 = wall_clock64();
   unsigned long long rounds = 0, ins = 0, edges = 0;
   uint32_t nlog0 = 0, nlog1 = 0;
-  bool full = false;
-  while (cur < nr && !full) {
+  bool full = false, handover = false;
+  while (cur < nr && !full && !handover) {
     const uint64_t end = min(cur + step, nr);
+    const unsigned long long edges0 = edges;
     // ---- front: unfinished vertices of the window, in order; seeds
     // (pivot_i, loff + i) in both tables (src/scc.hpp:147-150)
     if (tid == 0) {
       front_n = 0;
       cnt_log = 0;
       flag_full = 0;
+      round_edges = 0;
     }
     __syncthreads();
     for (uint64_t b0 = cur; b0 < end; b0 += TAIL_BLOCK) {
@@ -237,33 +261,103 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
           cnt_log = nl;
         }
         __syncthreads();
-        for (uint32_t i = tid; i < nq; i += TAIL_BLOCK) {
-          const unsigned long long pr = Td[qc[i]];
-          const uint32_t v = pair_vertex(pr), lb = pair_label(pr);
-          const unsigned long long lv = lab[v];
-          const uint32_t e1 = eo[v + 1];
-          for (uint32_t e = eo[v]; e < e1; ++e) {
-            const uint32_t x = ea[e];
-            ++edges;
-            const unsigned long long lx = lab[x];
-            if ((lx & TOP_BIT) || lx != lv) continue;
-            uint64_t slot = 0;
-            const int r = tail_insert(Td, mask, x, lb, &slot);
-            if (r < 0) {
-              flag_full = 1;
-              break;
+        // each wave takes 64 frontier pairs at a time and expands them with
+        // load-balanced 64-edge windows; new pairs of compact degree <=
+        // VGC_MAX_DEG stay in the wave's LDS ring while the budget lasts
+        // (the VGC of src/scc.hpp:177-219), the rest go to the next frontier
+        uint32_t *ring = ring_s[w];
+        uint32_t *sc = sc_s[w];
+        uint32_t *st = st_s[w];
+        uint32_t *el = el_s[w];
+        unsigned long long *lw = lw_s[w];
+        const int l = (int)(tid & (WAVE - 1));
+        for (uint32_t c0 = w * WAVE; c0 < nq && !flag_full; c0 += TAIL_BLOCK) {
+          const uint32_t cnt = min((uint32_t)WAVE, nq - c0);
+          if ((uint32_t)l < cnt) ring[l] = qc[c0 + l];
+          uint32_t head = 0, tail = cnt;
+          uint32_t budget = vgc ? tau : 0u;
+          uint64_t ebudget = (uint64_t)budget * VGC_EDGE_MULT;
+          wave_sync();
+          while (head < tail) {
+            const uint32_t k = min((uint32_t)WAVE, tail - head);
+            const bool valid = (uint32_t)l < k;
+            uint32_t deg = 0, b = 0, lb = 0;
+            unsigned long long lv = 0;
+            if (valid) {
+              const unsigned long long pr = Td[ring[(head + l) & (TAIL_RING - 1)]];
+              const uint32_t v = pair_vertex(pr);
+              lb = pair_label(pr);
+              b = eo[v];
+              deg = eo[v + 1] - b;
+              lv = lab[v];
             }
-            if (r > 0) {
-              ++ins;
-              const uint32_t at = atomicAdd(&cnt_log, 1u);
-              const uint32_t aq = atomicAdd(&cnt_q, 1u);
-              if (at >= half || aq >= half) {
-                flag_full = 1;
-                break;
+            head += k;
+            const uint32_t inc = wave_incl_scan(deg);
+            const uint32_t total = __shfl(inc, WAVE - 1);
+            wave_sync();
+            sc[l] = inc;
+            st[l] = b;
+            el[l] = lb;
+            lw[l] = lv;
+            ebudget = ebudget > total ? ebudget - total : 0;
+            if (!ebudget) budget = 0;
+            wave_sync();
+            for (uint32_t base = 0; base < total; base += WAVE) {
+              const uint32_t e = base + l;
+              bool fresh = false;
+              uint32_t x = 0, slot32 = 0, lbx = 0;
+              if (e < total) {
+                int lo = 0, hi = WAVE - 1;
+                while (lo < hi) {
+                  const int mid = (lo + hi) >> 1;
+                  if (sc[mid] > e) hi = mid; else lo = mid + 1;
+                }
+                const uint32_t ex = lo ? sc[lo - 1] : 0u;
+                x = ea[st[lo] + (e - ex)];
+                ++edges;
+                const unsigned long long lx = lab[x];
+                if (!(lx & TOP_BIT) && lx == lw[lo]) {
+                  lbx = el[lo];
+                  uint64_t slot = 0;
+                  const int r = tail_insert(Td, mask, x, lbx, &slot);
+                  if (r < 0) flag_full = 1;
+                  fresh = r > 0;
+                  slot32 = (uint32_t)slot;
+                }
               }
-              Ld[at] = (uint32_t)slot;
-              qn[aq] = (uint32_t)slot;
+              const uint64_t fm = ballot(fresh);
+              if (!fm) continue;
+              ins += fresh;
+              // log: one LDS atomic per window
+              uint32_t lbase = 0;
+              if (l == 0) lbase = atomicAdd(&cnt_log, (uint32_t)__popcll(fm));
+              lbase = __shfl(lbase, 0);
+              if (fresh) {
+                const uint32_t at = lbase + lanes_below(fm);
+                if (at < half) Ld[at] = slot32; else flag_full = 1;
+              }
+              bool want = false;
+              if (fresh && budget) want = eo[x + 1] - eo[x] <= VGC_MAX_DEG;
+              const uint64_t wm = ballot(want);
+              const uint32_t room = TAIL_RING - (tail - head);
+              const uint32_t take = min(min((uint32_t)__popcll(wm), room), budget);
+              const bool loc = want && lanes_below(wm) < take;
+              if (loc) ring[(tail + lanes_below(wm)) & (TAIL_RING - 1)] = slot32;
+              tail += take;
+              budget -= take;
+              const bool glob = fresh && !loc;
+              const uint64_t gm = ballot(glob);
+              if (gm) {
+                uint32_t qb0 = 0;
+                if (l == 0) qb0 = atomicAdd(&cnt_q, (uint32_t)__popcll(gm));
+                qb0 = __shfl(qb0, 0);
+                if (glob) {
+                  const uint32_t aq = qb0 + lanes_below(gm);
+                  if (aq < half) qn[aq] = slot32; else flag_full = 1;
+                }
+              }
             }
+            wave_sync();
           }
         }
         __syncthreads();
@@ -276,6 +370,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       }
       if (d) nlog1 = nl; else nlog0 = nl;
     }
+    if (edges != edges0) atomicAdd(&round_edges, edges - edges0);
     __syncthreads();
     full = flag_full;
     if (!full) {
@@ -300,6 +395,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       cur = end;
       step = (uint64_t)ceil((double)step * tg.beta);
       ++rounds;
+      handover = (double)round_edges * tg.beta > TAIL_ROUND_EDGES;
     }
     __syncthreads();
     // ---- empty the slots this round used
@@ -313,15 +409,17 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     // slots of a stopped round are not all logged: clear the whole region
     for (uint64_t i = tid; i < 2 * tg.cap; i += TAIL_BLOCK) T0[i] = EMPTY_SLOT;
   }
-  if (tid == 0) {
-    A.state[blockIdx.x] = TailState{cur, step, rounds, full ? 1ull : 0ull};
-    A.lab_off[tg.g] = loff;
-  }
   ins = wave_sum(ins);
   edges = wave_sum(edges);
   if ((tid & (WAVE - 1)) == 0) {
     if (ins) atomicAdd(&A.stats[0], ins);
     if (edges) atomicAdd(&A.stats[1], edges);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    A.state[blockIdx.x] = TailState{cur, step, rounds, full ? 1ull : 0ull, 0, 0,
+                                    wall_clock64() - t0};
+    A.lab_off[tg.g] = loff;
   }
 }
 
@@ -435,8 +533,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   k_tail_fill<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
       cen, n_rem, lab, lid, d_off, d_adj, d_ioff, d_iadj, toff, tioff, tadj, tiadj);
   prof_end();
-  uint64_t slots = 0;
-  for (size_t i = 0; i < tgs.size(); ++i) {
+  for (size_t i = 0; i < nt; ++i) {
     TailGraph &t = tgs[i];
     uint64_t cap = 1024;
     while (cap < 4 * (t.nr + h_e[i])) cap *= 2;
@@ -445,47 +542,102 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       while (cap < p.table_init) cap *= 2;
     }
     t.cap = cap;
-    t.tbase = slots;   // two tables of cap slots
-    t.lbase = slots;   // four lists of cap / 2 u32
-    slots += 2 * cap;
+    t.cap_max = std::min<uint64_t>(TAIL_MAX_SLOTS, cap * 64);
   }
-  PSC_TRY(tail_table.ensure(slots * 8));
-  PSC_TRY(tail_lists.ensure(slots * 4));
-  PSC_HIP(hipMemcpyAsync(d_tg, tgs.data(), nt * sizeof(TailGraph), hipMemcpyHostToDevice,
-                         stream));
-  PSC_HIP(hipMemsetAsync(d_stats, 0, 16, stream));
+  // Launch the graphs still pending; a graph whose tables overflowed stops
+  // at the start of that round and is relaunched with 4x the slots, up to
+  // 64x its first size and TAIL_MAX_SLOTS (then the host-driven rounds
+  // finish it).
+  std::vector<size_t> pending(nt);
+  for (size_t i = 0; i < nt; ++i) pending[i] = i;
+  std::vector<uint64_t> rounds(nt, 0);
+  unsigned long long hst[2] = {0, 0};
+  std::vector<TailGraph> launch;
+  std::vector<TailState> hs;
+  while (!pending.empty()) {
+    launch.clear();
+    uint64_t slots = 0;
+    for (size_t i : pending) {
+      TailGraph t = tgs[i];
+      t.tbase = slots;   // two tables of cap slots
+      t.lbase = slots;   // four lists of cap / 2 u32
+      slots += 2 * t.cap;
+      launch.push_back(t);
+    }
+    const size_t nl = launch.size();
+    PSC_TRY(tail_table.ensure(slots * 8));
+    PSC_TRY(tail_lists.ensure(slots * 4));
+    PSC_HIP(hipMemcpyAsync(d_tg, launch.data(), nl * sizeof(TailGraph),
+                           hipMemcpyHostToDevice, stream));
+    PSC_HIP(hipMemsetAsync(d_stats, 0, 16, stream));
+    prof_begin(KK_TAIL);
+    k_tail_fill_u64<<<blocks(slots), 256, 0, stream>>>(tail_table.as<unsigned long long>(),
+                                                       slots);
+    TailArgs a;
+    a.graphs = d_tg;
+    a.state = d_ts;
+    a.toff = toff;
+    a.tadj = tadj;
+    a.tioff = tioff;
+    a.tiadj = tiadj;
+    a.tlab = tail_lab.as<unsigned long long>();
+    a.table = tail_table.as<unsigned long long>();
+    a.lists = tail_lists.as<uint32_t>();
+    a.lab_off = gD.as<unsigned long long>();
+    a.stats = d_stats;
+    a.tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
+    a.vgc = p.local_scc;
+    k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
+    prof_end();
+    PSC_HIP(hipGetLastError());
+    hs.resize(nl);
+    PSC_TRY(poll(hs.data(), d_ts, nl * sizeof(TailState)));
+    unsigned long long part[2] = {0, 0};
+    PSC_TRY(poll(part, d_stats, 16));
+    hst[0] += part[0];
+    hst[1] += part[1];
+    std::vector<size_t> again;
+    for (size_t k = 0; k < nl; ++k) {
+      const size_t i = pending[k];
+      TailGraph &t = tgs[i];
+      t.cur = hs[k].cur;
+      t.step = hs[k].step;
+      cur[t.g] = hs[k].cur;
+      stepg[t.g] = hs[k].step;
+      rounds[i] += hs[k].rounds;
+      if (hs[k].full) {
+        ++st->resizes;
+        if (t.cap * 4 <= t.cap_max) {
+          t.cap *= 4;
+          again.push_back(i);
+        }
+      }
+    }
+    if (p.timing >= 3) {   // per-graph report, slowest first (wall clock 100 MHz)
+      std::vector<size_t> ord(nl);
+      for (size_t k = 0; k < nl; ++k) ord[k] = k;
+      std::sort(ord.begin(), ord.end(),
+                [&](size_t x, size_t y) { return hs[x].ticks > hs[y].ticks; });
+      for (size_t r = 0; r < std::min<size_t>(nl, 8); ++r) {
+        const size_t k = ord[r];
+        const TailGraph &t = launch[k];
+        std::fprintf(stderr,
+                     "tail graph %u: remainder %llu compact edges %llu cap %llu rounds %llu "
+                     "%s %.1f us\n",
+                     t.g, (unsigned long long)t.nr, (unsigned long long)h_e[pending[k]],
+                     (unsigned long long)t.cap, hs[k].rounds,
+                     hs[k].full ? "overflow" : (hs[k].cur < t.nr ? "handed over" : "done"),
+                     hs[k].ticks / 100.0);
+      }
+    }
+    pending.swap(again);
+  }
   prof_begin(KK_TAIL);
-  k_tail_fill_u64<<<blocks(slots), 256, 0, stream>>>(tail_table.as<unsigned long long>(),
-                                                     slots);
-  TailArgs a;
-  a.graphs = d_tg;
-  a.state = d_ts;
-  a.toff = toff;
-  a.tadj = tadj;
-  a.tioff = tioff;
-  a.tiadj = tiadj;
-  a.tlab = tail_lab.as<unsigned long long>();
-  a.table = tail_table.as<unsigned long long>();
-  a.lists = tail_lists.as<uint32_t>();
-  a.lab_off = gD.as<unsigned long long>();
-  a.stats = d_stats;
-  k_ms_tail<<<(unsigned)tgs.size(), TAIL_BLOCK, 0, stream>>>(a);
   k_tail_writeback<<<blocks(n_rem), 256, 0, stream>>>(
       cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
   prof_end();
   PSC_HIP(hipGetLastError());
-  std::vector<TailState> hs(tgs.size());
-  PSC_TRY(poll(hs.data(), d_ts, tgs.size() * sizeof(TailState)));
-  unsigned long long hst[2] = {0, 0};
-  PSC_TRY(poll(hst, d_stats, 16));
-  uint64_t max_rounds = 0;
-  for (size_t i = 0; i < tgs.size(); ++i) {
-    cur[tgs[i].g] = hs[i].cur;
-    stepg[tgs[i].g] = hs[i].step;
-    max_rounds = std::max<uint64_t>(max_rounds, hs[i].rounds);
-    if (hs[i].full) ++st->resizes;
-  }
-  st->rounds += max_rounds;
+  st->rounds += *std::max_element(rounds.begin(), rounds.end());
   st->table_inserts += hst[0];
   st->edges_scanned += hst[1];
   st->bytes_csr += 8 * (e_out + e_in) + 8 * hst[1];
