@@ -181,7 +181,8 @@ def bench_serve(args, psc):
             reqs = sv.poisson_requests(psc, args.requests, rate, 10, args.hi_log2,
                                        seed=11, table=table)
             srv = sv.Server(psc, budget_bytes=int(args.budget_gb * 2**30),
-                            lanes=args.lanes, mode=mode)
+                            lanes=args.lanes, mode=mode,
+                            max_batch_vertices=1 << args.max_batch_log2)
             summ = srv.serve(reqs)
             summ["rate"] = rate
             rows.append(summ)
@@ -251,6 +252,8 @@ def main():
     ap.add_argument("--requests", type=int, default=1000)
     ap.add_argument("--rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0])
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-batch-log2", type=int, default=27,
+                    help="serving: vertex cap of one lane's batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "

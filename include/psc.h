@@ -71,9 +71,10 @@ typedef struct psc_params {
                            3: also one first-reach level per host poll,
                               reported per level on stderr            */
   uint64_t tail_max;    /* a graph whose remainder after the first SCC
-                           has at most this many vertices runs all its
-                           multi-pivot rounds in one launch; 0 = host-
-                           driven rounds only; default 65536          */
+                           has at most this many vertices (capped at
+                           65536) runs all its multi-pivot rounds in
+                           one launch; 0 = host-driven rounds only;
+                           default 65536                              */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).

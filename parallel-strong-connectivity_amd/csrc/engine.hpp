@@ -82,7 +82,7 @@ struct Engine {
   DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
   DevBuf NZ, centers, front, tiles, tile_off;
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
-  DevBuf rem[2], sort_tmp;   // remaining vertices keyed by pivot position
+  DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
   DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta;
   DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
@@ -134,6 +134,7 @@ struct Engine {
   int select(F f, uint64_t len, typename F::T *sel, typename F::T *rej,
              uint64_t *count, unsigned long long *count_dev = nullptr);
   int poll(void *dst, const void *src, size_t bytes);
+  int scan_tiles(uint64_t nt, unsigned long long *total);
   // copy the n labels to host memory on this handle's stream
   int download_labels(uint64_t *dst);
   int prof_reset(int level);

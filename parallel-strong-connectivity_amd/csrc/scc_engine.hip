@@ -19,7 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -53,25 +53,6 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
-// Block-wide exclusive scan of one u32 per thread (BLOCK threads).
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x,
-                                                    uint32_t *total) {
-  __shared__ uint32_t wsum[WPB];
-  const int w = threadIdx.x / WAVE, l = lane();
-  uint32_t inc = wave_incl_scan(x);
-  if (l == WAVE - 1) wsum[w] = inc;
-  __syncthreads();
-  uint32_t off = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < WPB; ++i) {
-    if (i < w) off += wsum[i];
-    tot += wsum[i];
-  }
-  __syncthreads();
-  *total = tot;
-  return off + inc - x;
-}
-
 // ------------------------------------------------ stable select (3 kernels)
 constexpr int SEL_ITEMS = 8;
 constexpr int SEL_TILE = BLOCK * SEL_ITEMS;
@@ -98,27 +79,6 @@ __global__ __launch_bounds__(BLOCK) void k_sel_count(F f, uint64_t len,
     for (int w = 0; w < WPB; ++w) t += wc[w];
     tile_cnt[blockIdx.x] = t;
   }
-}
-
-// Exclusive scan of the tile counts in one block; total -> *out_total.
-__global__ __launch_bounds__(BLOCK) void k_sel_scan(uint32_t *tile_cnt,
-                                                    uint64_t ntiles,
-                                                    unsigned long long *tile_off,
-                                                    unsigned long long *out_total) {
-  __shared__ unsigned long long carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint64_t b = 0; b < ntiles; b += BLOCK) {
-    uint64_t i = b + threadIdx.x;
-    uint32_t x = i < ntiles ? tile_cnt[i] : 0;
-    uint32_t tot;
-    uint32_t ex = block_excl_scan(x, &tot);
-    if (i < ntiles) tile_off[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out_total = carry;
 }
 
 template <class F>
@@ -226,29 +186,6 @@ struct SelWindow {
   }
   __device__ bool pred(uint64_t t) const { return !(labels[val(t)] & TOP_BIT); }
 };
-// pred: non-trimmed vertex NZ[j] not reached both ways by the first search
-// (src/scc.hpp:349).  The value packs the vertex's position in its graph's
-// pivot order P_g = NZ_g permuted (src/scc.hpp:303), nstart[g] +
-// perm_g^-1(j - nstart[g]), above the vertex id: sorting the selected values
-// yields the remaining vertices in P order without materialising P.
-struct SelRemaining {
-  using T = unsigned long long;
-  const uint32_t *NZ, *vf, *vb;
-  const unsigned long long *nstart;
-  uint32_t G;
-  uint64_t seed;
-  __device__ bool pred(uint64_t j) const {
-    const uint32_t v = NZ[j];
-    return !(bit_test(vf, v) && bit_test(vb, v));
-  }
-  __device__ T val(uint64_t j) const {
-    const uint32_t g = G == 1 ? 0u : seg_of(nstart, G, j);
-    const uint64_t b = nstart[g];
-    const PivotPerm perm(nstart[g + 1] - b, seed);
-    return ((unsigned long long)(b + perm.inverse(j - b)) << 32) | NZ[j];
-  }
-};
-
 // ------------------------------------------------------------- trimming
 // Trim, pass 1 (src/scc.hpp:295-302): per SEL_TILE tile, the number of
 // vertices with non-zero in- and out-degree.  The same degree reads give
@@ -373,11 +310,6 @@ __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
   }
 }
 
-__global__ void k_low32(const unsigned long long *in, uint64_t n, uint32_t *out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = (uint32_t)in[i];
-}
 
 // ------------------------------------------------------- per-graph helpers
 // A handle holds G graphs as one disjoint union: graph g owns the vertex id
@@ -496,6 +428,30 @@ __device__ __forceinline__ int reach_mode(const ReachSlot *c,
   if (push >= R.thresh) return MODE_PULL;
   const uint64_t unv = R.nnz > c->cum_vis ? R.nnz - c->cum_vis : 0;
   return (unv * PULL_VERTEX_COST + R.n / 2048 < push) ? MODE_PULL : MODE_PUSH;
+}
+
+// K counters summed over the block, one atomic per counter and block.
+// Every thread of the block must call it (it synchronises the block).
+template <int K>
+__device__ __forceinline__ void block_add(unsigned long long *const (&ptr)[K],
+                                          const unsigned long long (&val)[K]) {
+  __shared__ unsigned long long red[WPB][K];
+  const int w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const unsigned long long x = wave_sum(val[k]);
+    if (lane() == 0) red[w][k] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (threadIdx.x == (unsigned)k) {
+      unsigned long long t = 0;
+      for (int x = 0; x < WPB; ++x) t += red[x][k];
+      if (t) atomicAdd(ptr[k], t);
+    }
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void add_cnt(unsigned long long *c, unsigned long long v) {
@@ -657,12 +613,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
         wbud -= budget0 - budget;
       }
       out.flush(&nxt->qsize, qn, R.n);
-      add_cnt(&nxt->degsum, dsum);
-      add_cnt(&C->cnt[s][R_PUSH_EDGES], n_edges);
-      add_cnt(&C->cnt[s][R_PUSH_EXP], n_exp);
-      add_cnt(&C->cnt[s][R_PUSH_VIS], n_vis);
-      add_cnt(&nxt->cum_vis, n_vis);
-      add_cnt(&C->cnt[s][R_PUSH_APP], n_app);
+      unsigned long long *const cp[6] = {&nxt->degsum, &C->cnt[s][R_PUSH_EDGES],
+                                         &C->cnt[s][R_PUSH_EXP], &C->cnt[s][R_PUSH_VIS],
+                                         &nxt->cum_vis, &C->cnt[s][R_PUSH_APP]};
+      const unsigned long long cv[6] = {dsum, n_edges, n_exp, n_vis, n_vis, n_app};
+      block_add<6>(cp, cv);
     }
   }
 }
@@ -865,11 +820,12 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     }
     while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, R.n);
-    add_cnt(&C->cnt[s][R_PULL_EDGES], n_edges);
-    add_cnt(&C->cnt[s][R_PULL_SCANS], n_scan);
-    add_cnt(&C->cnt[s][R_PULL_VIS], n_vis);
-    add_cnt(&nxt->cum_vis, n_vis);
-    add_cnt(&C->cnt[s][R_PULL_UNITS], l == 0 ? n_chunks : 0);
+    unsigned long long *const cp[5] = {&C->cnt[s][R_PULL_EDGES], &C->cnt[s][R_PULL_SCANS],
+                                       &C->cnt[s][R_PULL_VIS], &nxt->cum_vis,
+                                       &C->cnt[s][R_PULL_UNITS]};
+    const unsigned long long cv[5] = {n_edges, n_scan, n_vis, n_vis,
+                                      l == 0 ? n_chunks : 0ull};
+    block_add<5>(cp, cv);
   }
 }
 
@@ -980,61 +936,65 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
       }
     }
     out.flush(&nxt->qsize, qn, R.n);
-    add_cnt(&nxt->degsum, dsum);
-    add_cnt(&C->cnt[s][R_HEAVY_EDGES], n_edges);
-    add_cnt(&C->cnt[s][R_HEAVY_VIS], n_vis);
-    add_cnt(&nxt->cum_vis, n_vis);
-    add_cnt(&C->cnt[s][R_HEAVY_APP], n_app);
-    add_cnt(&C->cnt[s][R_HEAVY_ITEMS], n_items);
+    unsigned long long *const cp[6] = {&nxt->degsum, &C->cnt[s][R_HEAVY_EDGES],
+                                       &C->cnt[s][R_HEAVY_VIS], &nxt->cum_vis,
+                                       &C->cnt[s][R_HEAVY_APP], &C->cnt[s][R_HEAVY_ITEMS]};
+    const unsigned long long cv[6] = {dsum, n_edges, n_vis, n_vis, n_app, n_items};
+    block_add<6>(cp, cv);
   }
 }
 
-// First-SCC labelling (src/scc.hpp:340-347): both reaches -> found SCC,
-// one reach -> new subproblem label; lab1[g] = |trimmed of g| + 1.  Runs in
-// vertex order (coalesced label writes); trimmed vertices already hold a
-// non-zero label and are skipped.  Each thread issues the loads of
-// FL_UNROLL vertices before using any of them, and the found-SCC count is
-// reduced per block (one atomic per block).
-constexpr int FL_UNROLL = 4;
-__global__ __launch_bounds__(BLOCK) void k_first_labels(
-    uint64_t n, const unsigned long long *voff, uint32_t G, const uint32_t *vf,
-    const uint32_t *vb, unsigned long long *labels,
-    const unsigned long long *lab1, unsigned long long *big) {
-  __shared__ unsigned long long red[WPB];
-  unsigned long long c = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+// First-SCC labelling over the non-trimmed list (src/scc.hpp:340-349): for
+// v = NZ[j], both reaches -> found SCC (label | TOP_BIT), one reach -> new
+// subproblem label; trimmed vertices keep their labels.  A vertex not in
+// the first SCC remains: it is placed at its position in its graph's pivot
+// order P_g = NZ_g permuted (src/scc.hpp:303), pos = nstart[g] +
+// perm_g^-1(j - nstart[g]): posv[pos] = v and bit pos of pbits set, so
+// compacting pbits in order yields the remaining vertices in P order
+// without materialising or sorting P.
+__global__ __launch_bounds__(BLOCK) void k_first_mark(
+    const uint32_t *__restrict__ NZ, uint64_t nnz, const unsigned long long *voff,
+    uint32_t G, const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
+    const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
+    uint32_t *posv, uint32_t *pbits, unsigned long long *big) {
+  __shared__ unsigned long long wb[WPB];
   const unsigned long long lab_one = lab1[0];
-  for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; base < n;
-       base += stride * FL_UNROLL) {
-    unsigned long long lv[FL_UNROLL];
-    uint32_t wf[FL_UNROLL], wb[FL_UNROLL];
-#pragma unroll
-    for (int u = 0; u < FL_UNROLL; ++u) {
-      const uint64_t v = base + u * stride;
-      const bool in = v < n;
-      lv[u] = in ? labels[v] : 1ull;
-      wf[u] = in ? vf[v >> 5] : 0u;
-      wb[u] = in ? vb[v >> 5] : 0u;
-    }
-#pragma unroll
-    for (int u = 0; u < FL_UNROLL; ++u) {
-      const uint64_t v = base + u * stride;
-      const bool a = (wf[u] >> (v & 31)) & 1u, b = (wb[u] >> (v & 31)) & 1u;
-      if (lv[u] || !(a || b)) continue;
+  unsigned long long both = 0;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nnz;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = NZ[j];
+    const bool a = bit_test(vf, v), b = bit_test(vb, v);
+    if (a || b) {
       const unsigned long long label = G == 1 ? lab_one : lab1[seg_of(voff, G, v)];
       labels[v] = (a && b) ? (label | TOP_BIT) : label;
-      c += a && b;
+    }
+    if (a && b) {
+      ++both;
+    } else {
+      const uint32_t g = G == 1 ? 0u : seg_of(nstart, G, j);
+      const uint64_t b0 = nstart[g];
+      const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(j - b0);
+      posv[pos] = v;
+      atomicOr(&pbits[pos >> 5], 1u << (pos & 31));
     }
   }
-  c = wave_sum(c);
-  if (lane() == 0) red[threadIdx.x / WAVE] = c;
+  both = wave_sum(both);
+  if (lane() == 0) wb[threadIdx.x / WAVE] = both;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < WPB; ++w) t += red[w];
-    if (t) atomicAdd(big, t);
+    unsigned long long tb = 0;
+    for (int w = 0; w < WPB; ++w) tb += wb[w];
+    if (tb) atomicAdd(big, tb);
   }
 }
+
+// Remaining vertices in pivot order: the set bits of pbits, in order.
+struct SelBits {
+  using T = uint32_t;
+  const uint32_t *pbits, *posv;
+  __device__ bool pred(uint64_t pos) const { return (pbits[pos >> 5] >> (pos & 31)) & 1u; }
+  __device__ uint32_t val(uint64_t pos) const { return posv[pos]; }
+};
 
 // ----------------------------------------------------- multi-search kernels
 // Start of a multi-search attempt for the searches in `want` (bit s):
@@ -1520,7 +1480,7 @@ Engine::~Engine() {
 
 std::vector<DevBuf *> Engine::all_bufs() {
   return {&labels, &vis[0], &vis[1], &rq[0][0], &rq[0][1], &rq[1][0],
-          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &rem[0], &rem[1],
+          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &pbits,
           &sort_tmp, &centers,
           &front, &trim_bm, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
@@ -1651,14 +1611,34 @@ int Engine::select(F f, uint64_t len, typename F::T *sel, typename F::T *rej,
   unsigned long long *tot = count_dev ? count_dev : misc.as<unsigned long long>();
   prof_begin(KK_SELECT);
   k_sel_count<F><<<(unsigned)nt, BLOCK, 0, stream>>>(f, len, tiles.as<uint32_t>());
-  k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
-                                       tile_off.as<unsigned long long>(), tot);
+  PSC_TRY(scan_tiles(nt, tot));
   k_sel_scatter<F><<<(unsigned)nt, BLOCK, 0, stream>>>(
       f, len, tile_off.as<unsigned long long>(), sel, rej);
   prof_end();
   PSC_HIP(hipGetLastError());
   if (count_dev) return PSC_OK;   // count stays on the device, no sync
   return poll(count, tot, 8);
+}
+
+__global__ void k_tile_total(const uint32_t *cnt, const unsigned long long *off,
+                             uint64_t nt, unsigned long long *total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = off[nt - 1] + cnt[nt - 1];
+}
+
+// Exclusive scan of the per-tile counts (tiles -> tile_off) and their
+// total (-> *total), on the device.
+int Engine::scan_tiles(uint64_t nt, unsigned long long *total) {
+  size_t tb = 0;
+  const uint32_t *in = tiles.as<uint32_t>();
+  unsigned long long *out = tile_off.as<unsigned long long>();
+  PSC_HIP(rocprim::exclusive_scan(nullptr, tb, in, out, 0ull, nt,
+                                  rocprim::plus<unsigned long long>(), stream));
+  PSC_TRY(sort_tmp.ensure(std::max<size_t>(tb, 8)));
+  tb = sort_tmp.cap;
+  PSC_HIP(rocprim::exclusive_scan(sort_tmp.ptr, tb, in, out, 0ull, nt,
+                                  rocprim::plus<unsigned long long>(), stream));
+  k_tile_total<<<1, 64, 0, stream>>>(in, out, nt, total);
+  return PSC_OK;
 }
 
 // Pinned staging for per-graph arrays: NREG regions of (G + 2) u64.
@@ -2012,8 +1992,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
                                                     vis[0].as<uint32_t>(),
                                                     vis[1].as<uint32_t>(),
                                                     trim_bm.as<uint32_t>());
-    k_sel_scan<<<1, BLOCK, 0, stream>>>(tiles.as<uint32_t>(), nt,
-                                         tile_off.as<unsigned long long>(), tot);
+    PSC_TRY(scan_tiles(nt, tot));
     k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
         d_off, d_ioff, n, voffd, G, tile_off.as<unsigned long long>(), tot, nstart,
         zstart);
@@ -2043,39 +2022,25 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(upload(gC, label_off.data(), G, 5));
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
+    // first-SCC labels and the remaining vertices (src/scc.hpp:340-349) in
+    // pivot order: placed by P position in one pass over NZ, then the
+    // position bitmap is compacted in order
+    const uint64_t pwords = (nnz + 31) / 32;
+    PSC_TRY(pbits.ensure(pwords * 4));
+    uint32_t *posv = front.as<uint32_t>();   // front is free until the rounds
     prof_begin(KK_MISC);
-    k_first_labels<<<grid_for((n + FL_UNROLL - 1) / FL_UNROLL, BLOCK, num_cu * 8),
-                     BLOCK, 0, stream>>>(
-        n, voffd, G, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
-        gC.as<unsigned long long>(), big);
+    k_fill_u32<<<grid_for(pwords, BLOCK, 4096), BLOCK, 0, stream>>>(pbits.as<uint32_t>(),
+                                                                   pwords, 0u);
+    k_first_mark<<<grid_for(nnz, BLOCK, num_cu * 8), BLOCK, 0, stream>>>(
+        NZ.as<uint32_t>(), nnz, voffd, G, nstart, p.seed, vis[0].as<uint32_t>(),
+        vis[1].as<uint32_t>(), lab, gC.as<unsigned long long>(), posv,
+        pbits.as<uint32_t>(), big);
     prof_end();
-    kbytes[KK_MISC] += 8.0 * n + 8.0 * nnz + nnz / 4.0;
     PSC_HIP(hipGetLastError());
-    // remaining vertices (src/scc.hpp:349) in pivot order: selected in NZ
-    // order with their P position as key, then sorted by it
-    PSC_TRY(rem[0].ensure(nnz * 8));
-    PSC_TRY(select(SelRemaining{NZ.as<uint32_t>(), vis[0].as<uint32_t>(),
-                                vis[1].as<uint32_t>(), nstart, G, p.seed},
-                   nnz, rem[0].as<unsigned long long>(), nullptr, &n_rem));
-    kbytes[KK_SELECT] += 2 * 4.0 * nnz + nnz / 4.0 + 8.0 * n_rem;
-    if (n_rem) {
-      unsigned kb = 1;
-      while (kb < 32 && (1ull << kb) < nnz) ++kb;
-      size_t tb = 0;
-      PSC_HIP(rocprim::radix_sort_keys(nullptr, tb, rem[0].as<unsigned long long>(),
-                                       rem[1].as<unsigned long long>(), n_rem, 32,
-                                       32 + kb, stream));
-      PSC_TRY(rem[1].ensure(n_rem * 8));
-      PSC_TRY(sort_tmp.ensure(std::max<size_t>(tb, 8)));
-      prof_begin(KK_SELECT);
-      PSC_HIP(rocprim::radix_sort_keys(sort_tmp.ptr, tb, rem[0].as<unsigned long long>(),
-                                       rem[1].as<unsigned long long>(), n_rem, 32,
-                                       32 + kb, stream));
-      k_low32<<<grid_for(n_rem, BLOCK, 4096), BLOCK, 0, stream>>>(
-          rem[1].as<unsigned long long>(), n_rem, centers.as<uint32_t>());
-      prof_end();
-      kbytes[KK_SELECT] += 4.0 * 16.0 * n_rem + 12.0 * n_rem;
-    }
+    kbytes[KK_MISC] += 4.0 * nnz + nnz / 4.0 + 8.0 * nnz + nnz / 8.0;
+    PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
+                   nullptr, &n_rem));
+    kbytes[KK_SELECT] += 2 * nnz / 8.0 + 8.0 * n_rem;
     prof_begin(KK_MISC);
     k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
         centers.as<uint32_t>(), n_rem, nullptr, voffd, G, nullptr, G,

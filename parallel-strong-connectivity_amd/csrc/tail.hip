@@ -38,6 +38,8 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
+// a thread's share of a window is at most 64 vertices (one bit mask)
+constexpr uint64_t TAIL_MAX_V = (uint64_t)TAIL_BLOCK * 64;
 constexpr uint64_t TAIL_MAX_SLOTS = 1ull << 23;   // per table of one graph
 constexpr uint32_t TAIL_RING = 512;   // local-search ring per wave (LDS)
 // A graph leaves the single-workgroup rounds (the host-driven rounds, which
@@ -215,32 +217,41 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       round_edges = 0;
     }
     __syncthreads();
-    for (uint64_t b0 = cur; b0 < end; b0 += TAIL_BLOCK) {
-      const uint64_t j = b0 + tid;
-      const bool take = j < end && !(lab[base + j] & TOP_BIT);
-      const uint64_t m = ballot(take);
-      if ((tid & (WAVE - 1)) == 0) wsum[w] = (uint32_t)__popcll(m);
+    {
+      // thread t owns a contiguous segment of the window (<= 64 vertices,
+      // since remainders are at most 65536): count, one block scan, place
+      const uint64_t W = end - cur, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;
+      const uint64_t s0 = cur + min((uint64_t)tid * per, W);
+      const uint64_t s1 = cur + min((uint64_t)(tid + 1) * per, W);
+      uint64_t mine = 0;
+      for (uint64_t j = s0; j < s1; ++j)
+        if (!(lab[base + j] & TOP_BIT)) mine |= 1ull << (j - s0);
+      const uint32_t c = (uint32_t)__popcll(mine);
+      const uint32_t inc = wave_incl_scan(c);
+      if ((tid & (WAVE - 1)) == WAVE - 1) wsum[w] = inc;
       __syncthreads();
       uint32_t before = 0, tot = 0;
       for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
         before += x < w ? wsum[x] : 0u;
         tot += wsum[x];
       }
-      const uint32_t i = front_n + before + lanes_below(m);
-      if (take) {
-        const uint32_t v = (uint32_t)(base + j), lb = (uint32_t)(loff + i);
-        uint64_t s0 = 0, s1 = 0;
-        const int r0 = tail_insert(T0, mask, v, lb, &s0);
-        const int r1 = tail_insert(T1, mask, v, lb, &s1);
+      uint32_t i = before + inc - c;
+      while (mine) {
+        const int b = __ffsll((unsigned long long)mine) - 1;
+        mine &= mine - 1;
+        const uint32_t v = (uint32_t)(base + s0 + b), lb = (uint32_t)(loff + i);
+        uint64_t q0 = 0, q1 = 0;
+        const int r0 = tail_insert(T0, mask, v, lb, &q0);
+        const int r1 = tail_insert(T1, mask, v, lb, &q1);
         if (r0 < 0 || r1 < 0 || i >= half) {
           flag_full = 1;
         } else {
-          L0[i] = (uint32_t)s0;
-          L1[i] = (uint32_t)s1;
+          L0[i] = (uint32_t)q0;
+          L1[i] = (uint32_t)q1;
         }
+        ++i;
       }
-      __syncthreads();
-      if (tid == 0) front_n += tot;
+      if (tid == 0) front_n = tot;
       __syncthreads();
     }
     const uint32_t nfront = front_n;
@@ -473,7 +484,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   std::vector<TailGraph> tgs;
   for (uint32_t g = 0; g < G; ++g) {
     const uint64_t r = h_cstart[g + 1] - h_cstart[g];
-    if (r == 0 || cur[g] == r || r > p.tail_max) continue;
+    if (r == 0 || cur[g] == r || r > std::min<uint64_t>(p.tail_max, TAIL_MAX_V)) continue;
     TailGraph t{};
     t.base = h_cstart[g];
     t.nr = r;

@@ -110,6 +110,7 @@ class Server:
         self.errors = []
         # seconds spent per phase (summed over threads)
         self.phase = {"generate": 0.0, "gather": 0.0, "scc": 0.0, "labels": 0.0}
+        self.calls = {"generate": 0, "batches": 0}
 
     # ---------------------------------------------------------- admission
     def _admit_loop(self, reqs, t0):
@@ -166,6 +167,7 @@ class Server:
             shared = _Pool(db, len(group))
             adm = time.perf_counter() - t0
             self.phase["generate"] += adm + t0 - tg
+            self.calls["generate"] += 1
             with self.cv:
                 for j, (r, need) in enumerate(group):
                     r.admitted = adm
@@ -237,6 +239,7 @@ class Server:
                 done = time.perf_counter() - t0
                 scc_s = handle.stats["ms_total"] / 1e3
                 self.phase["gather"] += t1 - t0 - start
+                self.calls["batches"] += 1
                 self.phase["scc"] += scc_s
                 self.phase["labels"] += done + t0 - t1 - scc_s
                 parts = handle.split(labels)
@@ -286,4 +289,5 @@ class Server:
             "peak_pool_GB": round(self.peak_in_use / 2**30, 2),
             "budget_GB": round(self.budget / 2**30, 2),
             "phase_s": {k: round(v, 3) for k, v in self.phase.items()},
+            "calls": dict(self.calls),
         }
