@@ -85,6 +85,9 @@ struct Engine {
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
   DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta;
+  void *tail_pin = nullptr;   // pinned staging of the tail descriptors
+  size_t tail_pin_cap = 0;
+  int tail_pin_ensure(size_t bytes);
   DevBuf reach_desc, reach_ctl, ms_desc, ms_ctl, misc, scratch32;
   DevBuf table[2], log[2], mq[2][2], mhp[2], mhc[2];
   int grid_step = 1, grid_ms = 1, grid_heavy = 1, grid_pull = 1;

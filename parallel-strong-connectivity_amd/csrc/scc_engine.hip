@@ -1473,6 +1473,7 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (h_gpin) (void)hipHostFree(h_gpin);
+  if (tail_pin) (void)hipHostFree(tail_pin);
   for (auto &e : ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : pev) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);

@@ -434,20 +434,21 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   }
 }
 
-// Compact edge count (both directions) of each tail graph; out[count] and
-// out[count + 1] get the total out- and in-edge counts.
-__global__ void k_tail_sizes(const TailGraph *tg, uint32_t count, const uint32_t *toff,
-                             const uint32_t *tioff, uint64_t nrem,
+// Compact edge count (both directions) of every graph, from the per-graph
+// starts of the remaining list; out[G] and out[G + 1] get the total out-
+// and in-edge counts.
+__global__ void k_tail_sizes(const unsigned long long *cstart, uint32_t G,
+                             const uint32_t *toff, const uint32_t *tioff, uint64_t nrem,
                              unsigned long long *out) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= count;
-       i += gridDim.x * blockDim.x) {
-    if (i == count) {
-      out[count] = toff[nrem];
-      out[count + 1] = tioff[nrem];
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G;
+       g += gridDim.x * blockDim.x) {
+    if (g == G) {
+      out[G] = toff[nrem];
+      out[G + 1] = tioff[nrem];
       continue;
     }
-    const uint64_t b = tg[i].base, e = b + tg[i].nr;
-    out[i] = (unsigned long long)(toff[e] - toff[b]) + (tioff[e] - tioff[b]);
+    const uint64_t b = cstart[g], e = cstart[g + 1];
+    out[g] = (unsigned long long)(toff[e] - toff[b]) + (tioff[e] - tioff[b]);
   }
 }
 
@@ -471,6 +472,17 @@ int blocks(uint64_t n, int per = 256, int cap = 16384) {
 }
 
 }  // namespace
+
+int Engine::tail_pin_ensure(size_t bytes) {
+  if (bytes <= tail_pin_cap) return PSC_OK;
+  PSC_HIP(hipStreamSynchronize(stream));
+  if (tail_pin) (void)hipHostFree(tail_pin);
+  tail_pin = nullptr;
+  tail_pin_cap = 0;
+  PSC_HIP(hipHostMalloc(&tail_pin, bytes));
+  tail_pin_cap = bytes;
+  return PSC_OK;
+}
 
 // Runs the rounds of every graph with 0 < remainder <= p.tail_max on the
 // device; cur / step of those graphs are advanced (fully, unless a graph's
@@ -524,20 +536,18 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   PSC_HIP(rocprim::exclusive_scan(sort_tmp.ptr, tb, cnt_in, tioff, 0u, n_rem + 1,
                                   rocprim::plus<uint32_t>(), stream));
   prof_end();
-  // per-graph compact edge counts size the pair tables
+  // per-graph compact edge counts size the pair tables (one poll)
   const size_t nt = tgs.size();
-  PSC_TRY(tail_meta.ensure(nt * (sizeof(TailGraph) + sizeof(TailState) + 8) + 64));
+  PSC_TRY(tail_meta.ensure(nt * (sizeof(TailGraph) + sizeof(TailState)) + (G + 4) * 8 + 64));
   TailGraph *d_tg = tail_meta.as<TailGraph>();
   TailState *d_ts = reinterpret_cast<TailState *>(d_tg + nt);
-  unsigned long long *d_e = reinterpret_cast<unsigned long long *>(d_ts + nt);
-  unsigned long long *d_stats = d_e + nt + 2;
-  PSC_HIP(hipMemcpyAsync(d_tg, tgs.data(), nt * sizeof(TailGraph), hipMemcpyHostToDevice,
-                         stream));
-  k_tail_sizes<<<blocks(nt + 1), 256, 0, stream>>>(d_tg, (uint32_t)nt, toff, tioff, n_rem,
-                                                   d_e);
-  std::vector<unsigned long long> h_e(nt + 2);
-  PSC_TRY(poll(h_e.data(), d_e, (nt + 2) * 8));
-  const uint64_t e_out = h_e[nt], e_in = h_e[nt + 1];
+  unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(d_ts + nt);
+  unsigned long long *d_e = d_stats + 2;
+  k_tail_sizes<<<blocks(G + 1), 256, 0, stream>>>(gB.as<unsigned long long>(), G, toff,
+                                                  tioff, n_rem, d_e);
+  std::vector<unsigned long long> h_e(G + 2);
+  PSC_TRY(poll(h_e.data(), d_e, (G + 2) * 8));
+  const uint64_t e_out = h_e[G], e_in = h_e[G + 1];
   PSC_TRY(tail_adj.ensure((e_out + e_in + 2) * 4));
   uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj + e_out + 1;
   prof_begin(KK_TAIL);
@@ -547,7 +557,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   for (size_t i = 0; i < nt; ++i) {
     TailGraph &t = tgs[i];
     uint64_t cap = 1024;
-    while (cap < 4 * (t.nr + h_e[i])) cap *= 2;
+    while (cap < 4 * (t.nr + h_e[t.g])) cap *= 2;
     if (p.table_init) {   // forced small tables: exercises the hand-over
       cap = 16;
       while (cap < p.table_init) cap *= 2;
@@ -578,8 +588,12 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     const size_t nl = launch.size();
     PSC_TRY(tail_table.ensure(slots * 8));
     PSC_TRY(tail_lists.ensure(slots * 4));
-    PSC_HIP(hipMemcpyAsync(d_tg, launch.data(), nl * sizeof(TailGraph),
-                           hipMemcpyHostToDevice, stream));
+    // descriptors go through pinned staging (the stream is synchronised
+    // by the poll below before the staging is written again)
+    PSC_TRY(tail_pin_ensure(nl * sizeof(TailGraph)));
+    std::memcpy(tail_pin, launch.data(), nl * sizeof(TailGraph));
+    PSC_HIP(hipMemcpyAsync(d_tg, tail_pin, nl * sizeof(TailGraph), hipMemcpyHostToDevice,
+                           stream));
     PSC_HIP(hipMemsetAsync(d_stats, 0, 16, stream));
     prof_begin(KK_TAIL);
     k_tail_fill_u64<<<blocks(slots), 256, 0, stream>>>(tail_table.as<unsigned long long>(),
@@ -601,10 +615,14 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
     prof_end();
     PSC_HIP(hipGetLastError());
+    // the nt state slots and the two statistics counters are contiguous:
+    // one poll (slots past this launch's nl are not read)
+    std::vector<unsigned char> back(nt * sizeof(TailState) + 16);
+    PSC_TRY(poll(back.data(), d_ts, back.size()));
     hs.resize(nl);
-    PSC_TRY(poll(hs.data(), d_ts, nl * sizeof(TailState)));
+    std::memcpy(hs.data(), back.data(), nl * sizeof(TailState));
     unsigned long long part[2] = {0, 0};
-    PSC_TRY(poll(part, d_stats, 16));
+    std::memcpy(part, back.data() + nt * sizeof(TailState), 16);
     hst[0] += part[0];
     hst[1] += part[1];
     std::vector<size_t> again;
@@ -635,7 +653,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
         std::fprintf(stderr,
                      "tail graph %u: remainder %llu compact edges %llu cap %llu rounds %llu "
                      "%s %.1f us\n",
-                     t.g, (unsigned long long)t.nr, (unsigned long long)h_e[pending[k]],
+                     t.g, (unsigned long long)t.nr, (unsigned long long)h_e[t.g],
                      (unsigned long long)t.cap, hs[k].rounds,
                      hs[k].full ? "overflow" : (hs[k].cur < t.nr ? "handed over" : "done"),
                      hs[k].ticks / 100.0);
