@@ -179,9 +179,9 @@ struct TailArgs {
 
 // One workgroup per graph: all rounds of its remainder.
 __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
-  __shared__ uint32_t wsum[TAIL_BLOCK / WAVE];
+  __shared__ uint32_t wsum[TAIL_BLOCK / WAVE], wsum2[TAIL_BLOCK / WAVE];
   __shared__ unsigned int cnt_q, cnt_log, flag_full;
-  __shared__ unsigned int front_n;
+  __shared__ unsigned int front_n, seed_n;
   __shared__ unsigned long long round_edges;
   // per-wave local-search ring and window scratch
   __shared__ uint32_t ring_s[TAIL_BLOCK / WAVE][TAIL_RING];
@@ -217,54 +217,82 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       round_edges = 0;
     }
     __syncthreads();
+    uint64_t iso_mine = 0, iso_all = 0, iso_s0 = 0;
+    uint32_t iso_i0 = 0;
     {
       // thread t owns a contiguous segment of the window (<= 64 vertices,
-      // since remainders are at most 65536): count, one block scan, place
+      // since remainders are at most 65536): count, one block scan, place.
+      // A pivot without compact edges in either direction is its own SCC
+      // this round (nothing reaches it, it reaches nothing): it gets
+      // (loff + i) | TOP_BIT at once, exactly what its pair would give in
+      // the partition; only the others are seeded into the tables.
       const uint64_t W = end - cur, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;
       const uint64_t s0 = cur + min((uint64_t)tid * per, W);
       const uint64_t s1 = cur + min((uint64_t)(tid + 1) * per, W);
-      uint64_t mine = 0;
-      for (uint64_t j = s0; j < s1; ++j)
-        if (!(lab[base + j] & TOP_BIT)) mine |= 1ull << (j - s0);
-      const uint32_t c = (uint32_t)__popcll(mine);
-      const uint32_t inc = wave_incl_scan(c);
-      if ((tid & (WAVE - 1)) == WAVE - 1) wsum[w] = inc;
+      uint64_t mine = 0, linked = 0;
+      for (uint64_t j = s0; j < s1; ++j) {
+        const uint64_t v = base + j;
+        if (!(lab[v] & TOP_BIT)) {
+          mine |= 1ull << (j - s0);
+          if (A.toff[v + 1] != A.toff[v] || A.tioff[v + 1] != A.tioff[v])
+            linked |= 1ull << (j - s0);
+        }
+      }
+      const uint32_t c = (uint32_t)__popcll(mine), cl = (uint32_t)__popcll(linked);
+      const uint32_t inc = wave_incl_scan(c), incl = wave_incl_scan(cl);
+      if ((tid & (WAVE - 1)) == WAVE - 1) {
+        wsum[w] = inc;
+        wsum2[w] = incl;
+      }
       __syncthreads();
-      uint32_t before = 0, tot = 0;
+      uint32_t before = 0, tot = 0, beforel = 0, totl = 0;
       for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
         before += x < w ? wsum[x] : 0u;
         tot += wsum[x];
+        beforel += x < w ? wsum2[x] : 0u;
+        totl += wsum2[x];
       }
-      uint32_t i = before + inc - c;
-      while (mine) {
-        const int b = __ffsll((unsigned long long)mine) - 1;
-        mine &= mine - 1;
-        const uint32_t v = (uint32_t)(base + s0 + b), lb = (uint32_t)(loff + i);
-        uint64_t q0 = 0, q1 = 0;
-        const int r0 = tail_insert(T0, mask, v, lb, &q0);
-        const int r1 = tail_insert(T1, mask, v, lb, &q1);
-        if (r0 < 0 || r1 < 0 || i >= half) {
-          flag_full = 1;
-        } else {
-          L0[i] = (uint32_t)q0;
-          L1[i] = (uint32_t)q1;
+      const uint32_t i0 = before + inc - c;
+      uint32_t i = i0, k = beforel + incl - cl;
+      for (uint64_t mm = mine; mm; mm &= mm - 1) {
+        const int b = __ffsll((unsigned long long)mm) - 1;
+        if ((linked >> b) & 1ull) {
+          const uint32_t v = (uint32_t)(base + s0 + b), lb = (uint32_t)(loff + i);
+          uint64_t q0 = 0, q1 = 0;
+          const int r0 = tail_insert(T0, mask, v, lb, &q0);
+          const int r1 = tail_insert(T1, mask, v, lb, &q1);
+          if (r0 < 0 || r1 < 0 || k >= half) {
+            flag_full = 1;
+          } else {
+            L0[k] = (uint32_t)q0;
+            L1[k] = (uint32_t)q1;
+          }
+          ++k;
         }
         ++i;
       }
-      if (tid == 0) front_n = tot;
+      if (tid == 0) {
+        front_n = tot;
+        seed_n = totl;
+      }
       __syncthreads();
+      iso_mine = mine & ~linked;   // labelled in the partition step
+      iso_i0 = i0;
+      iso_s0 = s0;
+      iso_all = mine;
     }
-    const uint32_t nfront = front_n;
+    const uint32_t nfront = front_n, nseed = seed_n;
+    nlog0 = nlog1 = 0;
     // ---- forward and backward multi-search over pairs (src/scc.hpp:160-252)
-    for (int d = 0; d < 2 && !flag_full; ++d) {
+    for (int d = 0; d < 2 && nseed && !flag_full; ++d) {
       unsigned long long *const Td = d ? T1 : T0;
       uint32_t *const Ld = d ? L1 : L0;
       const uint32_t *const eo = d ? A.tioff : A.toff;
       const uint32_t *const ea = d ? A.tiadj : A.tadj;
-      uint32_t nl = nfront;
-      for (uint32_t i = tid; i < nfront; i += TAIL_BLOCK) qa[i] = Ld[i];
+      uint32_t nl = nseed;
+      for (uint32_t i = tid; i < nseed; i += TAIL_BLOCK) qa[i] = Ld[i];
       uint32_t *qc = qa, *qn = qb;
-      uint32_t nq = nfront;
+      uint32_t nq = nseed;
       __syncthreads();
       while (nq && !flag_full) {
         if (tid == 0) {
@@ -385,6 +413,17 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     __syncthreads();
     full = flag_full;
     if (!full) {
+      // isolated pivots: their own SCC, (loff + i) | TOP_BIT (written only
+      // now: a stopped round leaves every label as it was)
+      {
+        uint32_t i = iso_i0;
+        for (uint64_t mm = iso_all; mm; mm &= mm - 1) {
+          const int b = __ffsll((unsigned long long)mm) - 1;
+          if ((iso_mine >> b) & 1ull)
+            lab[base + iso_s0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
+          ++i;
+        }
+      }
       // ---- partition (src/scc.hpp:437-464): a pair in both tables is in
       // the pivot's SCC; labels take the maximum
       for (uint32_t i = tid; i < nlog0; i += TAIL_BLOCK) {
