@@ -629,10 +629,12 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
 // a first-round reach set (src/scc.hpp:341-347) and no pull probe reads
 // them as visited.
 // The vertex range is walked by bitmap: a wave loads 64 words of
-// (visited | trimmed) with one coalesced load (2048 vertices) and works on
-// the words with a clear bit, PULL_PAIRS x 2 words (64 vertices each) at a
-// time with all their loads in flight together: in-offsets, then the first
-// PULL_PROBE in-edges, then their visited words.  Vertices left undecided
+// (visited | trimmed) with one coalesced load (2048 vertices), lists the
+// chunk's candidates (clear bits) in LDS, and takes them PULL_PAIRS x 64 at
+// a time with all their loads in flight together: in-offsets, then the
+// first PULL_PROBE in-edges, then their visited words.  Found vertices set
+// their bits in an LDS copy of the chunk's words, flushed with one atomic
+// per word.  Vertices left undecided
 // queue their remaining in-edges in LDS; a full batch of 64 queued
 // vertices is scanned as load-balanced 64-edge windows.  Remainders longer
 // than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
@@ -713,9 +715,13 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   __shared__ uint32_t qv_s[WPB][PULL_QCAP];
   __shared__ uint64_t qb_s[WPB][PULL_QCAP];
   __shared__ uint32_t qd_s[WPB][PULL_QCAP];
+  __shared__ uint16_t cand_s[WPB][WAVE * 32];   // candidate offsets of a chunk
+  __shared__ uint32_t fw_s[WPB][WAVE];          // found bits of a chunk's words
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *sc = sc_s[w], *fl = fl_s[w];
   uint64_t *st = st_s[w];
+  uint16_t *cand = cand_s[w];
+  uint32_t *fw = fw_s[w];
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
   int s_lo, s_hi;
@@ -743,27 +749,30 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
       }
       ++n_chunks;
-      uint64_t act = ballot(sk != ~0u);
-      while (act) {
-        // PULL_PAIRS word pairs: lanes 0-31 take the first word of a pair,
-        // lanes 32-63 the second
-        int wsel[2 * PULL_PAIRS];
-#pragma unroll
-        for (int j = 0; j < 2 * PULL_PAIRS; ++j) {
-          wsel[j] = -1;
-          if (act) {
-            wsel[j] = __ffsll((unsigned long long)act) - 1;
-            act &= act - 1;
-          }
-        }
+      // the chunk's candidates (clear bits) as offsets in LDS, in order
+      const uint32_t todo_w = ~sk;
+      const uint32_t cnt = (uint32_t)__popc(todo_w);
+      const uint32_t inc = wave_incl_scan(cnt);
+      const uint32_t total = __shfl(inc, WAVE - 1);
+      if (total == 0) continue;
+      wave_sync();
+      {
+        uint32_t at = inc - cnt;
+        for (uint32_t tt = todo_w; tt; tt &= tt - 1)
+          cand[at++] = (uint16_t)(l * 32 + (__ffs(tt) - 1));
+      }
+      fw[l] = 0;
+      wave_sync();
+      for (uint32_t cb = 0; cb < total; cb += WAVE * PULL_PAIRS) {
         uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
+        uint32_t co[PULL_PAIRS];
         bool todo[PULL_PAIRS];
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
-          const int myw = l < 32 ? wsel[2 * j] : wsel[2 * j + 1];
-          const uint32_t mysk = __shfl(sk, myw < 0 ? 0 : myw);
-          v[j] = (c * WAVE + (uint64_t)(myw < 0 ? 0 : myw)) * 32 + (l & 31);
-          todo[j] = myw >= 0 && !((mysk >> (l & 31)) & 1u);
+          const uint32_t idx = cb + j * WAVE + l;
+          todo[j] = idx < total;
+          co[j] = todo[j] ? cand[idx] : 0u;
+          v[j] = c * (uint64_t)(WAVE * 32) + co[j];
           b[j] = todo[j] ? R.roff[v[j]] : 0ull;
           deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
         }
@@ -804,19 +813,18 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             q.deg[at] = (uint32_t)rem;
           }
           q.n += (uint32_t)__popcll(qm);
-          const uint64_t fm = ballot(found);
-          if (fm) {
-            const int wa = wsel[2 * j], wb = wsel[2 * j + 1];
-            if (l == 0 && (uint32_t)fm) atomicOr(&R.vis[c * WAVE + wa], (uint32_t)fm);
-            if (l == 32 && (fm >> 32)) atomicOr(&R.vis[c * WAVE + wb], (uint32_t)(fm >> 32));
-            n_vis += found;
-            out.put(found, (uint32_t)v[j], &nxt->qsize, qn, R.n);
-          }
+          // found bits gather in the chunk's LDS words, one global atomic
+          // per word at the end of the chunk
+          if (found) atomicOr(&fw[co[j] >> 5], 1u << (co[j] & 31));
+          n_vis += found;
+          out.put(found, (uint32_t)v[j], &nxt->qsize, qn, R.n);
         }
         wave_sync();
         while (q.n >= WAVE)
           pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
       }
+      wave_sync();
+      if (fw[l]) atomicOr(&R.vis[wi], fw[l]);
     }
     while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, R.n);
