@@ -70,11 +70,16 @@ typedef struct psc_params {
                            2: also per-kernel events (psc_kernel_profile);
                            3: also one first-reach level per host poll,
                               reported per level on stderr            */
-  uint64_t tail_max;    /* a graph whose remainder after the first SCC
-                           has at most this many vertices (capped at
-                           65536) runs all its multi-pivot rounds in
-                           one launch; 0 = host-driven rounds only;
-                           default 65536                              */
+  uint64_t tail_max;    /* a graph with at most this many unfinished
+                           vertices after the first SCC, or after a
+                           light host-driven round (and at most 2^20
+                           remainder vertices), runs its remaining
+                           multi-pivot rounds in one launch; 0 =
+                           host-driven rounds only; default 65536     */
+  int32_t pivot_reach;  /* 1 (default): a host-driven round whose
+                           windows hold one pivot position each runs
+                           as bitmap reaches (same labels as the
+                           multi-search); 0: always multi-search      */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).

@@ -52,7 +52,8 @@ class CParams(C.Structure):
                 ("min_bag", C.c_uint64), ("dense_div", C.c_uint64),
                 ("seed", C.c_uint64), ("table_init", C.c_uint64),
                 ("canonical", C.c_int32), ("timing", C.c_int32),
-                ("tail_max", C.c_uint64)]
+                ("tail_max", C.c_uint64),
+                ("pivot_reach", C.c_int32)]
 
 
 class CStats(C.Structure):

@@ -52,6 +52,10 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// A remainder stays in (or returns to) the single-workgroup rounds of the
+// tail while a round scans at most this many edges times beta.
+constexpr double TAIL_ROUND_EDGES = 262144.0;
+
 // Pinned staging layout (bytes): poll buffer, reach descriptors,
 // multi-search descriptors, flags.
 constexpr size_t PIN_POLL = 0, PIN_REACH = 1024, PIN_MS = 2048, PIN_FLAGS = 3072;
@@ -103,11 +107,13 @@ struct Engine {
   std::vector<double> beta_g;     // per-graph pivot-batch growth (batches)
   DevBuf voff_d, gA, gB, gC, gD, gid_d;
   DevBuf rnd;   // per-round windows: prefix (A+1), sources (A), graphs (A)
+  DevBuf rr_plab;   // per-graph pivot label of a single-pivot round
   unsigned long long *h_gpin = nullptr;   // pinned per-graph staging
   size_t h_gpin_cap = 0;
   uint64_t log_n[2] = {0, 0};
   uint32_t ms_levels_hint = 2;   // sub-rounds of the previous multi-search
   uint32_t reach_levels_hint = 0; // levels of the previous first reach
+  uint32_t round_levels_hint = 0; // levels of the previous single-pivot round
   uint64_t nnz_total = 0;         // non-trimmed vertices of the current run
   uint64_t syncs = 0;
   unsigned char h_reach[256];
@@ -144,7 +150,18 @@ struct Engine {
   void prof_begin(int kind);
   void prof_end();
   int prof_collect();
-  int first_reach(const psc_params &p, psc_stats *st);
+  // rr != nullptr: the reaches of a single-pivot round (round_reach)
+  struct ReachRound {
+    const uint32_t *front;             // the round's pivots (device)
+    const unsigned long long *nfront;  // their count (device)
+    uint32_t A;                        // windows, their graphs (device)
+    const uint32_t *wgid;
+    uint64_t n_rem;                    // remainder positions
+    uint64_t reached[2];               // out: vertices reached per direction
+  };
+  int first_reach(const psc_params &p, psc_stats *st, ReachRound *rr = nullptr);
+  int round_reach(const psc_params &p, uint32_t A, const uint32_t *wgid_d,
+                  uint64_t n_rem, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
   int ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_rem,
               std::vector<uint64_t> &cur, std::vector<uint64_t> &stepg, psc_stats *st,

@@ -386,6 +386,110 @@ __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
   }
 }
 
+// ------------------------------------------------- single-pivot rounds
+// A round whose windows hold one position each has at most one pivot per
+// graph.  Its multi-searches then insert exactly the pairs (v, label_g) of
+// the vertices reachable from the pivot inside its subproblem (edges to
+// unfinished vertices of the same label, src/scc.hpp:177-219), so the
+// round runs as plain fused reaches over visited bitmaps: a vertex is
+// included when it carries its graph's pivot label, excluded vertices are
+// never claimed (push) nor scanned (pull), and the partition becomes a
+// bitmap test per included vertex.  Labels are the same as the multi-
+// search's: label_off_g | TOP_BIT in both reaches, label_off_g in one.
+
+// Pivot label of every graph with a pivot this round (plab: EMPTY_SLOT
+// elsewhere); resets the included counts.
+__global__ void k_rr_plab(const uint32_t *front, const unsigned long long *nfront_ptr,
+                          const unsigned long long *fstart, uint32_t A,
+                          const uint32_t *wgid, const unsigned long long *labels,
+                          unsigned long long *plab, ReachSearch *S) {
+  const uint64_t nf = *nfront_ptr;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nf;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    plab[wgid[seg_of(fstart, A, i)]] = labels[front[i]];
+  if (blockIdx.x == 0 && threadIdx.x < 2) S[threadIdx.x].nnz = 0;
+}
+
+// Remainder positions [0, n_rem) grouped by graph (cstart): vertices with
+// their graph's pivot label leave the exclusion bitmap; their count is the
+// nnz of both searches.
+__global__ __launch_bounds__(BLOCK) void k_rr_include(
+    const uint32_t *centers, uint64_t n_rem, const unsigned long long *cstart, uint32_t G,
+    const unsigned long long *plab, const unsigned long long *labels, uint32_t *excl,
+    ReachSearch *S) {
+  __shared__ unsigned long long red[WPB];
+  unsigned long long c = 0;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n_rem;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long pl = plab[G == 1 ? 0u : seg_of(cstart, G, j)];
+    if (pl == EMPTY_SLOT) continue;
+    const uint32_t v = centers[j];
+    if (labels[v] == pl) {
+      atomicAnd(&excl[v >> 5], ~(1u << (v & 31)));
+      ++c;
+    }
+  }
+  c = wave_sum(c);
+  if (lane() == 0) red[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < WPB; ++w) t += red[w];
+    if (t) {
+      atomicAdd((unsigned long long *)&S[0].nnz, t);
+      atomicAdd((unsigned long long *)&S[1].nnz, t);
+    }
+  }
+}
+
+// Level 0 of a single-pivot round: the round's pivots.
+__global__ void k_reach_sources_front(ReachSearch *S, ReachCtl *C, const uint32_t *front,
+                                      const unsigned long long *nfront_ptr) {
+  const uint64_t nf = *nfront_ptr;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nf;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t src = front[i];
+    for (int s = 0; s < 2; ++s) {
+      atomicOr(&S[s].vis[src >> 5], 1u << (src & 31));
+      S[s].q[0][i] = src;
+      atomicAdd(&C->slot[s][0].degsum,
+                (unsigned long long)(S[s].off[src + 1] - S[s].off[src]));
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) {
+    C->slot[threadIdx.x][0].qsize = (uint32_t)nf;
+    C->slot[threadIdx.x][0].cum_vis = nf;
+  }
+}
+
+// Partition of a single-pivot round (src/scc.hpp:437-464 for one label per
+// graph): included vertices reached both ways take label_off_g | TOP_BIT,
+// one way label_off_g (write_max); the rest keep their labels.
+__global__ __launch_bounds__(BLOCK) void k_rr_mark(
+    const uint32_t *centers, uint64_t n_rem, const unsigned long long *cstart, uint32_t G,
+    const unsigned long long *plab, const uint32_t *excl, const uint32_t *vf,
+    const uint32_t *vb, const unsigned long long *lab_off, unsigned long long *labels) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n_rem;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = G == 1 ? 0u : seg_of(cstart, G, j);
+    if (plab[g] == EMPTY_SLOT) continue;
+    const uint32_t v = centers[j];
+    if (bit_test(excl, v)) continue;
+    const bool a = bit_test(vf, v), b = bit_test(vb, v);
+    if (!(a || b)) continue;
+    const unsigned long long ell = (uint32_t)lab_off[g];
+    const unsigned long long val = (a && b) ? (ell | TOP_BIT) : ell;
+    if (labels[v] < val) labels[v] = val;
+  }
+}
+
+// label_offset_g += pivots of g in this round (src/scc.hpp:425).
+__global__ void k_rr_advance(const unsigned long long *fstart, uint32_t A,
+                             const uint32_t *wgid, unsigned long long *lab_off) {
+  for (uint32_t a = threadIdx.x; a < A; a += blockDim.x)
+    lab_off[wgid[a]] += fstart[a + 1] - fstart[a];
+}
+
 // Work split of the fused forward/backward levels by XCD.  Workgroups are
 // dispatched round-robin over the 8 XCDs (blockIdx % 8); with two searches,
 // XCDs 0-3 run the forward search and XCDs 4-7 the backward one, so each
@@ -457,6 +561,13 @@ __device__ __forceinline__ void block_add(unsigned long long *const (&ptr)[K],
 __device__ __forceinline__ void add_cnt(unsigned long long *c, unsigned long long v) {
   v = wave_sum(v);
   if (lane() == 0 && v) atomicAdd(c, v);
+}
+
+// Push-side claim of v: outside the searched subproblems it is never
+// taken; otherwise the visited bit is set once.
+__device__ __forceinline__ bool reach_claim(const ReachSearch &R, uint32_t v) {
+  if (R.excl && bit_test(R.excl, v)) return false;
+  return bit_set(R.vis, v);
 }
 
 // Heavy-list append shared by push and pull levels: vertex u with `deg`
@@ -582,7 +693,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
               }
               const uint32_t ex = lo ? sc[lo - 1] : 0u;
               v = R.adj[st[lo] + (e - ex)];
-              found = bit_set(R.vis, v);
+              found = reach_claim(R, v);
               ++n_edges;
             }
             const uint64_t fm = ballot(found);
@@ -910,7 +1021,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
           uint32_t v = 0;
           if (j < ee) {
             v = adj[j];
-            found = bit_set(R.vis, v);
+            found = reach_claim(R, v);
             ++n_edges;
           }
           if (found) {
@@ -1495,7 +1606,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta};
 }
 
@@ -1579,7 +1690,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   PSC_TRY(ms_ctl.ensure(sizeof(MSCtl)));
   PSC_TRY(misc.ensure(256));
   PSC_TRY(scratch32.ensure((nn + G) * 4));
-  for (DevBuf *b : {&voff_d, &gB, &gC, &gD}) PSC_TRY(b->ensure((G + 1) * 8));
+  for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) PSC_TRY(b->ensure((G + 1) * 8));
   PSC_TRY(gA.ensure(2 * (G + 1) * 8));   // nstart, then zstart
   PSC_TRY(gid_d.ensure((G + 1) * 4));
   PSC_TRY(gpin(G + 2));
@@ -1723,15 +1834,17 @@ int Engine::poll(void *dst, const void *src, size_t bytes) {
 // Levels are enqueued in batches of K without host round trips; kernels of
 // levels past the end exit at once.  The host reads the done flags after
 // each batch.
-int Engine::first_reach(const psc_params &p, psc_stats *st) {
+int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   ReachSearch rs[2];
+  static_assert(2 * sizeof(ReachSearch) <= sizeof(h_reach), "reach descriptors");
   std::memcpy(rs, h_reach, sizeof(rs));
   const uint64_t thresh = m / (p.dense_div ? p.dense_div : 20);
   for (int s = 0; s < 2; ++s) {
     rs[s].thresh = thresh;
     // frontier vertices are non-trimmed: their mean degree is m / nnz
     rs[s].avg_deg = nnz_total ? (double)m / (double)nnz_total : 0.0;
-    rs[s].nnz = nnz_total;
+    rs[s].nnz = nnz_total;   // a single-pivot round counts it on the device
+    rs[s].excl = rr ? trim_bm.as<uint32_t>() : nullptr;
   }
   char *pin = (char *)h_pinned;
   std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
@@ -1741,16 +1854,42 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
   ReachCtl *C = reach_ctl.as<ReachCtl>();
   prof_begin(KK_MISC);
   k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
-  k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
-      S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed);
+  if (!rr) {
+    k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
+        S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed);
+  } else {
+    // visited bitmaps empty, every vertex excluded but those carrying
+    // their graph's pivot label (the exclusion bitmap is also the pull
+    // walk's skip set)
+    const uint64_t words = (std::max<uint64_t>(n, 1) + 31) / 32;
+    k_fill_u32<<<grid_for(words, BLOCK, 16384), BLOCK, 0, stream>>>(
+        vis[0].as<uint32_t>(), words, 0u);
+    k_fill_u32<<<grid_for(words, BLOCK, 16384), BLOCK, 0, stream>>>(
+        vis[1].as<uint32_t>(), words, 0u);
+    k_fill_u32<<<grid_for(words, BLOCK, 16384), BLOCK, 0, stream>>>(
+        trim_bm.as<uint32_t>(), words, ~0u);
+    k_fill_u64<<<grid_for(G), BLOCK, 0, stream>>>(rr_plab.as<unsigned long long>(), G,
+                                                  EMPTY_SLOT);
+    k_rr_plab<<<grid_for(rr->A), BLOCK, 0, stream>>>(
+        rr->front, rr->nfront, gC.as<unsigned long long>(), rr->A, rr->wgid,
+        labels.as<unsigned long long>(), rr_plab.as<unsigned long long>(), S);
+    k_rr_include<<<grid_for(rr->n_rem, BLOCK, 16384), BLOCK, 0, stream>>>(
+        centers.as<uint32_t>(), rr->n_rem, gB.as<unsigned long long>(), G,
+        rr_plab.as<unsigned long long>(), labels.as<unsigned long long>(),
+        trim_bm.as<uint32_t>(), S);
+    k_reach_sources_front<<<grid_for(rr->A), BLOCK, 0, stream>>>(S, C, rr->front,
+                                                                 rr->nfront);
+    kbytes[KK_MISC] += 12.0 * words + 24.0 * rr->n_rem;
+  }
   prof_end();
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   uint32_t L = 0;
   // the previous run's level count (same graph: one poll, no idle levels)
-  int K = reach_levels_hint ? (int)std::min<uint32_t>(reach_levels_hint + 1, 64) : 4;
+  uint32_t &hint = rr ? round_levels_hint : reach_levels_hint;
+  int K = hint ? (int)std::min<uint32_t>(hint + 1, 64) : 4;
   // timing >= 3: one level per poll and a per-level report on stderr
-  const bool trace = p.timing >= 3;
+  const bool trace = p.timing >= 3 && !rr;
   if (trace) K = 1;
   ReachCtl hc_, prev_{};
   auto t_lvl = std::chrono::steady_clock::now();
@@ -1801,9 +1940,16 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
     if (hc_.done[0] && hc_.done[1]) break;
     if (!trace) K = std::min(K * 2, 64);
   }
-  st->levels_fwd = hc_.levels[0];
-  st->levels_bwd = hc_.levels[1];
-  reach_levels_hint = std::max(hc_.levels[0], hc_.levels[1]);
+  if (!rr) {
+    st->levels_fwd = hc_.levels[0];
+    st->levels_bwd = hc_.levels[1];
+  } else {
+    st->sub_rounds += std::max(hc_.levels[0], hc_.levels[1]);
+    for (int s = 0; s < 2; ++s)
+      rr->reached[s] = hc_.cnt[s][R_PUSH_VIS] + hc_.cnt[s][R_HEAVY_VIS] +
+                       hc_.cnt[s][R_PULL_VIS];
+  }
+  hint = std::max(hc_.levels[0], hc_.levels[1]);
   for (int s = 0; s < 2; ++s) {
     const unsigned long long *c = hc_.cnt[s];
     st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES];
@@ -1830,6 +1976,29 @@ int Engine::first_reach(const psc_params &p, psc_stats *st) {
                             c[R_PUSH_EXP]) +
                        16 * (c[R_PUSH_APP] + c[R_HEAVY_APP]);
   }
+  return PSC_OK;
+}
+
+// A single-pivot round (every window one position): fused reaches from the
+// round's pivots inside their subproblems, then the bitmap partition.
+int Engine::round_reach(const psc_params &p, uint32_t A, const uint32_t *wgid_d,
+                        uint64_t n_rem, psc_stats *st) {
+  ReachRound rr{front.as<uint32_t>(), misc.as<unsigned long long>() + 3, A, wgid_d,
+                n_rem, {0, 0}};
+  PSC_TRY(first_reach(p, st, &rr));
+  prof_begin(KK_PARTITION);
+  k_rr_mark<<<grid_for(n_rem, BLOCK, 16384), BLOCK, 0, stream>>>(
+      centers.as<uint32_t>(), n_rem, gB.as<unsigned long long>(), G,
+      rr_plab.as<unsigned long long>(), trim_bm.as<uint32_t>(), vis[0].as<uint32_t>(),
+      vis[1].as<uint32_t>(), gD.as<unsigned long long>(), labels.as<unsigned long long>());
+  k_rr_advance<<<1, BLOCK, 0, stream>>>(gC.as<unsigned long long>(), A, wgid_d,
+                                        gD.as<unsigned long long>());
+  prof_end();
+  PSC_HIP(hipGetLastError());
+  kbytes[KK_PARTITION] += 30.0 * n_rem;
+  st->bytes_label += 16 * n_rem;
+  log_n[0] = rr.reached[0];
+  log_n[1] = rr.reached[1];
   return PSC_OK;
 }
 
@@ -2071,6 +2240,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   std::vector<uint32_t> wgid;
   uint64_t ne_prev[2] = {1, 1};
   const double beta_def = p.beta > 0 ? p.beta : 1.5;
+  double beta_max = beta_def;
+  for (double b : beta_g) beta_max = std::max(beta_max, b);
   uint64_t live = 0;
   for (uint32_t g = 0; g < G; ++g) live += (h_cstart[g + 1] > h_cstart[g]);
   if (live && p.tail_max) {
@@ -2082,6 +2253,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
       for (uint32_t g = 0; g < G; ++g) live += cur[g] < h_cstart[g + 1] - h_cstart[g];
     }
   }
+  uint32_t tail_wait = 0, tail_backoff = 1;
   while (live) {
     wpre.assign(1, 0);
     wsrc.clear();
@@ -2100,6 +2272,10 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     }
     const uint32_t A = (uint32_t)wgid.size();
     st->rounds++;
+    // timing >= 3: one line per host-driven round on stderr
+    const auto t_round = std::chrono::steady_clock::now();
+    const uint64_t sub0 = st->sub_rounds, ins0 = st->table_inserts,
+                   edg0 = st->edges_scanned;
     PSC_TRY(upload_round(wpre, wsrc, wgid));
     const unsigned long long *d_wpre = rnd.as<unsigned long long>();
     const unsigned long long *d_wsrc = d_wpre + (A + 1);
@@ -2125,30 +2301,67 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
       prof_end();
     }
     kbytes[KK_SELECT] += 2 * 12.0 * V;
-    uint64_t slots[2];
-    for (int s = 0; s < 2; ++s) {
-      const uint64_t cap = 6000000ull * G;
-      const uint64_t base = std::max<uint64_t>(
-          std::min<uint64_t>((uint64_t)std::ceil(0.3 * (double)n_rem), cap),
-          (uint64_t)beta_def * ne_prev[s]);
-      slots[s] = pow2_at_least(p.table_init ? p.table_init : 2 * base);
+    if (p.pivot_reach && V == A) {
+      // one position per window: at most one pivot per graph
+      PSC_TRY(round_reach(p, A, d_wgid, n_rem, st));
+    } else {
+      uint64_t slots[2];
+      for (int s = 0; s < 2; ++s) {
+        const uint64_t cap = 6000000ull * G;
+        const uint64_t base = std::max<uint64_t>(
+            std::min<uint64_t>((uint64_t)std::ceil(0.3 * (double)n_rem), cap),
+            (uint64_t)beta_def * ne_prev[s]);
+        slots[s] = pow2_at_least(p.table_init ? p.table_init : 2 * base);
+      }
+      PSC_TRY(multi_search(p, V, A, d_wgid, slots, st));
+      const int small = (slots[0] <= slots[1]) ? 0 : 1;
+      prof_begin(KK_PARTITION);
+      k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
+          ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
+          misc.as<unsigned long long>() + 2, gC.as<unsigned long long>(), A, d_wgid,
+          gD.as<unsigned long long>());
+      prof_end();
+      PSC_HIP(hipGetLastError());
+      // log reads + label read-modify-write per pair; contains() probes of
+      // the small log's pairs are added after the run from misc[2]
+      kbytes[KK_PARTITION] += 24.0 * (log_n[0] + log_n[1]);
+      st->bytes_label += 16 * (log_n[0] + log_n[1]);
+      st->bytes_front += 8 * (log_n[0] + log_n[1]);
     }
-    PSC_TRY(multi_search(p, V, A, d_wgid, slots, st));
-    const int small = (slots[0] <= slots[1]) ? 0 : 1;
-    prof_begin(KK_PARTITION);
-    k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
-        ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
-        misc.as<unsigned long long>() + 2, gC.as<unsigned long long>(), A, d_wgid,
-        gD.as<unsigned long long>());
-    prof_end();
-    PSC_HIP(hipGetLastError());
-    // log reads + label read-modify-write per pair; contains() probes of
-    // the small log's pairs are added after the run from misc[2]
-    kbytes[KK_PARTITION] += 24.0 * (log_n[0] + log_n[1]);
-    st->bytes_label += 16 * (log_n[0] + log_n[1]);
-    st->bytes_front += 8 * (log_n[0] + log_n[1]);
     ne_prev[0] = log_n[0];
     ne_prev[1] = log_n[1];
+    const uint64_t round_edges = st->edges_scanned - edg0;
+    if (p.timing >= 3) {
+      PSC_HIP(hipStreamSynchronize(stream));
+      std::fprintf(stderr,
+                   "round %llu: windows %u pivots %llu levels %llu pairs %llu/%llu "
+                   "inserts %llu edges %llu %.1f us\n",
+                   (unsigned long long)st->rounds, A, (unsigned long long)V,
+                   (unsigned long long)(st->sub_rounds - sub0), (unsigned long long)log_n[0],
+                   (unsigned long long)log_n[1],
+                   (unsigned long long)(st->table_inserts - ins0),
+                   (unsigned long long)(st->edges_scanned - edg0),
+                   std::chrono::duration<double, std::micro>(
+                       std::chrono::steady_clock::now() - t_round).count());
+    }
+    // Remainders whose rounds turned light (typically right after the round
+    // that found their largest SCC) go back to the single-launch rounds.
+    // A return that advances no graph waits twice as many rounds before
+    // the next try.
+    if (live && p.tail_max && (double)round_edges * beta_max <= TAIL_ROUND_EDGES &&
+        ++tail_wait >= tail_backoff) {
+      uint64_t before = 0, after = 0;
+      for (uint32_t g = 0; g < G; ++g) before += cur[g];
+      bool used = false;
+      PSC_TRY(ms_tail(p, h_cstart.data(), n_rem, cur, stepg, st, &used));
+      tail_wait = 0;
+      for (uint32_t g = 0; g < G; ++g) after += cur[g];
+      if (after == before) tail_backoff *= 2;
+      if (used) {
+        live = 0;
+        for (uint32_t g = 0; g < G; ++g) live += cur[g] < h_cstart[g + 1] - h_cstart[g];
+      }
+    }
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
   // ---- finalize (src/scc.hpp:475-476)
@@ -2294,6 +2507,7 @@ void psc_params_default(psc_params *p) {
   p->canonical = 0;
   p->timing = 1;
   p->tail_max = 65536;
+  p->pivot_reach = 1;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }

@@ -45,6 +45,8 @@ struct ReachSearch {
   const uint32_t *radj;
   uint32_t *vis;         // visited bitmap, ceil(n/32) words
   const uint32_t *trim;  // trimmed-vertex bitmap (skipped by pull levels)
+  const uint32_t *excl;  // vertices outside the searched subproblems (push
+                         // skips them), or nullptr
   uint32_t *q[2];        // frontier ping-pong, n entries each
   uint32_t *hv;          // heavy vertices of the current level
   unsigned long long *hc;  // exclusive chunk start of each heavy vertex

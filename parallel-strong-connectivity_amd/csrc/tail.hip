@@ -38,15 +38,20 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
-// a thread's share of a window is at most 64 vertices (one bit mask)
-constexpr uint64_t TAIL_MAX_V = (uint64_t)TAIL_BLOCK * 64;
+// Windows are scanned in parts of TAIL_PART positions, so that a thread's
+// share of a part is at most 64 positions (one bit mask).
+constexpr uint64_t TAIL_PART = (uint64_t)TAIL_BLOCK * 64;
+// Remainder positions (finished vertices included) a graph may have here:
+// every round scans its window's positions.
+constexpr uint64_t TAIL_MAX_POS = 1ull << 20;
 constexpr uint64_t TAIL_MAX_SLOTS = 1ull << 23;   // per table of one graph
 constexpr uint32_t TAIL_RING = 512;   // local-search ring per wave (LDS)
 // A graph leaves the single-workgroup rounds (the host-driven rounds, which
 // use the whole device, take over) once a round scanned so many edges that
-// the next one, beta times larger, would exceed this.
-constexpr double TAIL_ROUND_EDGES = 262144.0;
+// the next one, beta times larger, would exceed TAIL_ROUND_EDGES.
 constexpr uint32_t TAIL_PROBE_LIMIT = 4096;
+// Compact edges (both directions) above which a remainder is not taken.
+constexpr uint64_t TAIL_MAX_EDGES = 1ull << 20;
 
 // Global scratch of the tail, one region per graph (offsets in slots).
 struct TailGraph {
@@ -59,7 +64,7 @@ struct TailGraph {
   uint64_t cur, step;   // pivot window schedule
   double beta;
   uint32_t g;
-  uint32_t pad;
+  uint32_t nu;          // unfinished vertices (host-side table sizing)
 };
 
 struct TailState {    // written back per graph
@@ -164,6 +169,58 @@ __device__ __forceinline__ int tail_insert(unsigned long long *T, uint64_t mask,
   return -1;
 }
 
+// One part [c0, c1) of a round's window (c1 - c0 <= TAIL_PART): thread t
+// owns a contiguous segment of at most 64 positions starting at s0; `mine`
+// marks its unfinished vertices, `linked` those with compact edges in
+// either direction.  i0 / k0 are the block-wide exclusive prefixes of the
+// two counts, tot / totl their totals.  Every thread of the block calls
+// it (it synchronises the block).
+struct WinPart {
+  uint64_t s0, mine, linked;
+  uint32_t i0, k0, tot, totl;
+};
+
+__device__ __forceinline__ WinPart tail_part(const unsigned long long *lab,
+                                             const uint32_t *toff, const uint32_t *tioff,
+                                             uint64_t base, uint64_t c0, uint64_t c1,
+                                             uint32_t *wsum, uint32_t *wsum2) {
+  const uint32_t tid = threadIdx.x, w = tid / WAVE;
+  const uint64_t W = c1 - c0, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;
+  WinPart r;
+  r.s0 = c0 + min((uint64_t)tid * per, W);
+  const uint64_t s1 = c0 + min((uint64_t)(tid + 1) * per, W);
+  r.mine = 0;
+  r.linked = 0;
+  for (uint64_t j = r.s0; j < s1; ++j) {
+    const uint64_t v = base + j;
+    if (!(lab[v] & TOP_BIT)) {
+      r.mine |= 1ull << (j - r.s0);
+      if (toff[v + 1] != toff[v] || tioff[v + 1] != tioff[v])
+        r.linked |= 1ull << (j - r.s0);
+    }
+  }
+  const uint32_t c = (uint32_t)__popcll(r.mine), cl = (uint32_t)__popcll(r.linked);
+  const uint32_t inc = wave_incl_scan(c), incl = wave_incl_scan(cl);
+  if ((tid & (WAVE - 1)) == WAVE - 1) {
+    wsum[w] = inc;
+    wsum2[w] = incl;
+  }
+  __syncthreads();
+  uint32_t before = 0, beforel = 0;
+  r.tot = 0;
+  r.totl = 0;
+  for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
+    before += x < w ? wsum[x] : 0u;
+    r.tot += wsum[x];
+    beforel += x < w ? wsum2[x] : 0u;
+    r.totl += wsum2[x];
+  }
+  r.i0 = before + inc - c;
+  r.k0 = beforel + incl - cl;
+  __syncthreads();   // wsum is reused by the next part
+  return r;
+}
+
 struct TailArgs {
   const TailGraph *graphs;
   TailState *state;
@@ -217,47 +274,23 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       round_edges = 0;
     }
     __syncthreads();
-    uint64_t iso_mine = 0, iso_all = 0, iso_s0 = 0;
-    uint32_t iso_i0 = 0;
     {
-      // thread t owns a contiguous segment of the window (<= 64 vertices,
-      // since remainders are at most 65536): count, one block scan, place.
-      // A pivot without compact edges in either direction is its own SCC
-      // this round (nothing reaches it, it reaches nothing): it gets
-      // (loff + i) | TOP_BIT at once, exactly what its pair would give in
-      // the partition; only the others are seeded into the tables.
-      const uint64_t W = end - cur, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;
-      const uint64_t s0 = cur + min((uint64_t)tid * per, W);
-      const uint64_t s1 = cur + min((uint64_t)(tid + 1) * per, W);
-      uint64_t mine = 0, linked = 0;
-      for (uint64_t j = s0; j < s1; ++j) {
-        const uint64_t v = base + j;
-        if (!(lab[v] & TOP_BIT)) {
-          mine |= 1ull << (j - s0);
-          if (A.toff[v + 1] != A.toff[v] || A.tioff[v + 1] != A.tioff[v])
-            linked |= 1ull << (j - s0);
-        }
-      }
-      const uint32_t c = (uint32_t)__popcll(mine), cl = (uint32_t)__popcll(linked);
-      const uint32_t inc = wave_incl_scan(c), incl = wave_incl_scan(cl);
-      if ((tid & (WAVE - 1)) == WAVE - 1) {
-        wsum[w] = inc;
-        wsum2[w] = incl;
-      }
-      __syncthreads();
-      uint32_t before = 0, tot = 0, beforel = 0, totl = 0;
-      for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
-        before += x < w ? wsum[x] : 0u;
-        tot += wsum[x];
-        beforel += x < w ? wsum2[x] : 0u;
-        totl += wsum2[x];
-      }
-      const uint32_t i0 = before + inc - c;
-      uint32_t i = i0, k = beforel + incl - cl;
-      for (uint64_t mm = mine; mm; mm &= mm - 1) {
-        const int b = __ffsll((unsigned long long)mm) - 1;
-        if ((linked >> b) & 1ull) {
-          const uint32_t v = (uint32_t)(base + s0 + b), lb = (uint32_t)(loff + i);
+      // Unfinished vertices of the window, part by part: pivot i of the
+      // round gets label loff + i.  A pivot without compact edges in either
+      // direction is its own SCC this round (nothing reaches it, it reaches
+      // nothing): it is labelled (loff + i) | TOP_BIT at the round's commit,
+      // exactly what its pair would give in the partition; only the others
+      // are seeded into the tables.
+      uint32_t fb = 0, kb = 0;
+      for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
+        const WinPart wp = tail_part(lab, A.toff, A.tioff, base, c0,
+                                     min(c0 + TAIL_PART, end), wsum, wsum2);
+        uint32_t k = kb + wp.k0;
+        for (uint64_t mm = wp.linked; mm; mm &= mm - 1) {
+          const int b = __ffsll((unsigned long long)mm) - 1;
+          const uint32_t v = (uint32_t)(base + wp.s0 + b);
+          const uint32_t i = fb + wp.i0 + (uint32_t)__popcll(wp.mine & ((1ull << b) - 1));
+          const uint32_t lb = (uint32_t)(loff + i);
           uint64_t q0 = 0, q1 = 0;
           const int r0 = tail_insert(T0, mask, v, lb, &q0);
           const int r1 = tail_insert(T1, mask, v, lb, &q1);
@@ -269,17 +302,14 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
           }
           ++k;
         }
-        ++i;
+        fb += wp.tot;
+        kb += wp.totl;
       }
       if (tid == 0) {
-        front_n = tot;
-        seed_n = totl;
+        front_n = fb;
+        seed_n = kb;
       }
       __syncthreads();
-      iso_mine = mine & ~linked;   // labelled in the partition step
-      iso_i0 = i0;
-      iso_s0 = s0;
-      iso_all = mine;
     }
     const uint32_t nfront = front_n, nseed = seed_n;
     nlog0 = nlog1 = 0;
@@ -414,14 +444,22 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     full = flag_full;
     if (!full) {
       // isolated pivots: their own SCC, (loff + i) | TOP_BIT (written only
-      // now: a stopped round leaves every label as it was)
+      // now: a stopped round leaves every label as it was).  The window is
+      // scanned again; no label has changed since the first scan, and a
+      // part's labels are written after the whole part was read.
       {
-        uint32_t i = iso_i0;
-        for (uint64_t mm = iso_all; mm; mm &= mm - 1) {
-          const int b = __ffsll((unsigned long long)mm) - 1;
-          if ((iso_mine >> b) & 1ull)
-            lab[base + iso_s0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
-          ++i;
+        uint32_t fb = 0;
+        for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
+          const WinPart wp = tail_part(lab, A.toff, A.tioff, base, c0,
+                                       min(c0 + TAIL_PART, end), wsum, wsum2);
+          uint32_t i = fb + wp.i0;
+          for (uint64_t mm = wp.mine; mm; mm &= mm - 1) {
+            const int b = __ffsll((unsigned long long)mm) - 1;
+            if (!((wp.linked >> b) & 1ull))
+              lab[base + wp.s0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
+            ++i;
+          }
+          fb += wp.tot;
         }
       }
       // ---- partition (src/scc.hpp:437-464): a pair in both tables is in
@@ -473,10 +511,30 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   }
 }
 
+// Unfinished vertices of each described graph at positions [cur, nr) of
+// its remainder (every earlier position was a pivot and is finished).
+__global__ void k_tail_unfinished(const TailGraph *tg, const uint32_t *centers,
+                                  const unsigned long long *labels,
+                                  unsigned long long *out) {
+  __shared__ unsigned long long red[TAIL_BLOCK / WAVE];
+  const TailGraph t = tg[blockIdx.x];
+  unsigned long long c = 0;
+  for (uint64_t j = t.cur + threadIdx.x; j < t.nr; j += blockDim.x)
+    c += !(labels[centers[t.base + j]] & TOP_BIT);
+  c = wave_sum(c);
+  if (lane() == 0) red[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (uint32_t x = 0; x < blockDim.x / WAVE; ++x) tot += red[x];
+    out[blockIdx.x] = tot;
+  }
+}
+
 // Compact edge count (both directions) of every graph, from the per-graph
 // starts of the remaining list; out[G] and out[G + 1] get the total out-
 // and in-edge counts.
-__global__ void k_tail_sizes(const unsigned long long *cstart, uint32_t G,
+__global__ void k_tail_sizes(const unsigned long long *cstart, uint32_t G, uint64_t lo,
                              const uint32_t *toff, const uint32_t *tioff, uint64_t nrem,
                              unsigned long long *out) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G;
@@ -486,7 +544,9 @@ __global__ void k_tail_sizes(const unsigned long long *cstart, uint32_t G,
       out[G + 1] = tioff[nrem];
       continue;
     }
-    const uint64_t b = cstart[g], e = cstart[g + 1];
+    // graphs outside the compacted range [lo, lo + nrem) count 0
+    const uint64_t b = min(max(cstart[g], lo), lo + nrem) - lo;
+    const uint64_t e = min(max(cstart[g + 1], lo), lo + nrem) - lo;
     out[g] = (unsigned long long)(toff[e] - toff[b]) + (tioff[e] - tioff[b]);
   }
 }
@@ -523,19 +583,27 @@ int Engine::tail_pin_ensure(size_t bytes) {
   return PSC_OK;
 }
 
-// Runs the rounds of every graph with 0 < remainder <= p.tail_max on the
-// device; cur / step of those graphs are advanced (fully, unless a graph's
-// tables overflowed).  Returns PSC_OK with *used = false when no graph
+// Runs the remaining rounds of every unfinished graph with at most
+// p.tail_max unfinished vertices (and at most TAIL_MAX_POS remainder
+// positions, TAIL_MAX_EDGES compact edges) on the device; cur / step of
+// those graphs are advanced (fully, unless a graph's tables overflowed or
+// its rounds grew heavy).  Returns PSC_OK with *used = false when no graph
 // qualifies.
 int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_rem,
                     std::vector<uint64_t> &cur, std::vector<uint64_t> &stepg,
                     psc_stats *st, bool *used) {
   *used = false;
   const double beta_def = p.beta > 0 ? p.beta : 1.5;
+  // Candidates: unfinished graphs of at most TAIL_MAX_POS remainder
+  // positions.  A graph qualifies with at most tail_max unfinished
+  // vertices; there are at most nr - cur of them (every position before
+  // cur was a pivot and is finished), and only graphs above that bound
+  // are counted (one small kernel and one poll).
   std::vector<TailGraph> tgs;
+  bool count = false;
   for (uint32_t g = 0; g < G; ++g) {
     const uint64_t r = h_cstart[g + 1] - h_cstart[g];
-    if (r == 0 || cur[g] == r || r > std::min<uint64_t>(p.tail_max, TAIL_MAX_V)) continue;
+    if (r == 0 || cur[g] == r || r > TAIL_MAX_POS) continue;
     TailGraph t{};
     t.base = h_cstart[g];
     t.nr = r;
@@ -543,11 +611,45 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     t.step = stepg[g];
     t.beta = beta_g.empty() ? beta_def : beta_g[g];
     t.g = g;
+    t.nu = (uint32_t)(r - cur[g]);
+    count |= t.nu > p.tail_max;
     tgs.push_back(t);
   }
   if (tgs.empty() || n_rem == 0) return PSC_OK;
+  const size_t ns = tgs.size();   // descriptor / state slots
+  PSC_TRY(tail_meta.ensure(ns * (sizeof(TailGraph) + sizeof(TailState)) + (2 * G + 4) * 8 +
+                           64));
+  TailGraph *d_tg = tail_meta.as<TailGraph>();
+  TailState *d_ts = reinterpret_cast<TailState *>(d_tg + ns);
+  unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(d_ts + ns);
+  unsigned long long *d_e = d_stats + 2;
+  if (count) {
+    unsigned long long *d_u = d_e + (G + 2);
+    PSC_TRY(tail_pin_ensure(ns * sizeof(TailGraph)));
+    std::memcpy(tail_pin, tgs.data(), ns * sizeof(TailGraph));
+    PSC_HIP(hipMemcpyAsync(d_tg, tail_pin, ns * sizeof(TailGraph), hipMemcpyHostToDevice,
+                           stream));
+    prof_begin(KK_TAIL);
+    k_tail_unfinished<<<(unsigned)ns, TAIL_BLOCK, 0, stream>>>(
+        d_tg, centers.as<uint32_t>(), labels.as<unsigned long long>(), d_u);
+    prof_end();
+    std::vector<unsigned long long> h_u(ns);
+    PSC_TRY(poll(h_u.data(), d_u, ns * 8));
+    std::vector<TailGraph> keep;
+    for (size_t i = 0; i < ns; ++i) {
+      if (h_u[i] > p.tail_max) continue;
+      tgs[i].nu = (uint32_t)h_u[i];
+      keep.push_back(tgs[i]);
+    }
+    tgs.swap(keep);
+    if (tgs.empty()) return PSC_OK;
+  }
+  // compact only the span of the selected graphs' remainders
+  const uint64_t lo = tgs.front().base, span_end = h_cstart[tgs.back().g + 1];
+  n_rem = span_end - lo;
+  for (TailGraph &t : tgs) t.base -= lo;
   const unsigned long long *lab = labels.as<unsigned long long>();
-  const uint32_t *cen = centers.as<uint32_t>();
+  const uint32_t *cen = centers.as<uint32_t>() + lo;
   // ---- compact remainder: ids by pivot position, same-subproblem edges
   PSC_TRY(tail_cnt.ensure(2 * (n_rem + 1) * 4));
   PSC_TRY(tail_lab.ensure(n_rem * 8));
@@ -576,16 +678,21 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
                                   rocprim::plus<uint32_t>(), stream));
   prof_end();
   // per-graph compact edge counts size the pair tables (one poll)
-  const size_t nt = tgs.size();
-  PSC_TRY(tail_meta.ensure(nt * (sizeof(TailGraph) + sizeof(TailState)) + (G + 4) * 8 + 64));
-  TailGraph *d_tg = tail_meta.as<TailGraph>();
-  TailState *d_ts = reinterpret_cast<TailState *>(d_tg + nt);
-  unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(d_ts + nt);
-  unsigned long long *d_e = d_stats + 2;
-  k_tail_sizes<<<blocks(G + 1), 256, 0, stream>>>(gB.as<unsigned long long>(), G, toff,
-                                                  tioff, n_rem, d_e);
+  k_tail_sizes<<<blocks(G + 1), 256, 0, stream>>>(gB.as<unsigned long long>(), G, lo,
+                                                  toff, tioff, n_rem, d_e);
   std::vector<unsigned long long> h_e(G + 2);
   PSC_TRY(poll(h_e.data(), d_e, (G + 2) * 8));
+  // A remainder that keeps many edges runs its next rounds on the whole
+  // device: one workgroup scans them far slower, and the host-driven rounds
+  // return the graph here once its rounds are light.
+  {
+    std::vector<TailGraph> keep;
+    for (const TailGraph &t : tgs)
+      if (h_e[t.g] <= TAIL_MAX_EDGES) keep.push_back(t);
+    tgs.swap(keep);
+  }
+  if (tgs.empty()) return PSC_OK;
+  const size_t nt = tgs.size();
   const uint64_t e_out = h_e[G], e_in = h_e[G + 1];
   PSC_TRY(tail_adj.ensure((e_out + e_in + 2) * 4));
   uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj + e_out + 1;
@@ -596,7 +703,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   for (size_t i = 0; i < nt; ++i) {
     TailGraph &t = tgs[i];
     uint64_t cap = 1024;
-    while (cap < 4 * (t.nr + h_e[t.g])) cap *= 2;
+    while (cap < 4 * ((uint64_t)t.nu + h_e[t.g])) cap *= 2;
     if (p.table_init) {   // forced small tables: exercises the hand-over
       cap = 16;
       while (cap < p.table_init) cap *= 2;
@@ -654,14 +761,14 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
     prof_end();
     PSC_HIP(hipGetLastError());
-    // the nt state slots and the two statistics counters are contiguous:
+    // the ns state slots and the two statistics counters are contiguous:
     // one poll (slots past this launch's nl are not read)
-    std::vector<unsigned char> back(nt * sizeof(TailState) + 16);
+    std::vector<unsigned char> back(ns * sizeof(TailState) + 16);
     PSC_TRY(poll(back.data(), d_ts, back.size()));
     hs.resize(nl);
     std::memcpy(hs.data(), back.data(), nl * sizeof(TailState));
     unsigned long long part[2] = {0, 0};
-    std::memcpy(part, back.data() + nt * sizeof(TailState), 16);
+    std::memcpy(part, back.data() + ns * sizeof(TailState), 16);
     hst[0] += part[0];
     hst[1] += part[1];
     std::vector<size_t> again;

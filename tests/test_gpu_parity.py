@@ -31,6 +31,10 @@ PARAM_SETS = [
     # host-driven rounds only (no single-launch remainder)
     {"tail_max": 0},
     {"tail_max": 0, "beta": 2.0, "seed": 3},
+    # single-pivot host rounds as multi-searches instead of bitmap reaches
+    {"tail_max": 0, "pivot_reach": 0},
+    # every host round single-pivot (windows never grow): all bitmap reaches
+    {"tail_max": 0, "beta": 1.0},
     # remainders of up to 100 vertices in one launch, larger ones on the host
     {"tail_max": 100},
 ]
