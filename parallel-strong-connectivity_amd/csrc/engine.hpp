@@ -88,7 +88,8 @@ struct Engine {
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
-  DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta;
+  DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta,
+      tail_spec;
   void *tail_pin = nullptr;   // pinned staging of the tail descriptors
   size_t tail_pin_cap = 0;
   int tail_pin_ensure(size_t bytes);

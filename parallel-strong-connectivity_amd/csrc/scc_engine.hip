@@ -266,9 +266,11 @@ __global__ void k_graph_starts(const uint64_t *__restrict__ off,
 
 // Trim, pass 2: non-trimmed vertices go to NZ in vertex order with label 0;
 // trimmed ones get 1 + (rank among their graph's trimmed | TOP_BIT)
-// (src/scc.hpp:304-305).  Every label is written once, coalesced.
+// (src/scc.hpp:304-305).  Every label is written once, coalesced.  The
+// trimmed set is read from pass 1's bitmap (n/8 bytes) instead of the
+// degree arrays again (16n bytes).
 __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
-    const uint64_t *__restrict__ off, const uint64_t *__restrict__ ioff, uint64_t n,
+    const uint32_t *__restrict__ trim_bm, uint64_t n,
     const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
     const unsigned long long *tile_off, uint32_t *NZ, unsigned long long *labels) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
     const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    p[j] = i < n && off[i + 1] != off[i] && ioff[i + 1] != ioff[i];
+    p[j] = i < n && !bit_test(trim_bm, (uint32_t)i);
     const uint64_t m = ballot(p[j]);
     rk[j] = lanes_below(m);
     if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
@@ -1607,7 +1609,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
           &gid_d, &rnd, &rr_plab, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
-          &tail_lists, &tail_meta};
+          &tail_lists, &tail_meta, &tail_spec};
 }
 
 int Engine::download_labels(uint64_t *dst) {
@@ -2175,10 +2177,10 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
         d_off, d_ioff, n, voffd, G, tile_off.as<unsigned long long>(), tot, nstart,
         zstart);
     k_trim_scatter<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        d_off, d_ioff, n, voffd, G, zstart, tile_off.as<unsigned long long>(),
+        trim_bm.as<uint32_t>(), n, voffd, G, zstart, tile_off.as<unsigned long long>(),
         NZ.as<uint32_t>(), lab);
     prof_end();
-    kbytes[KK_MISC] += 2 * 32.0 * n + n / 4.0 + 8.0 * n + 4.0 * n;
+    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 8.0 * n + 4.0 * n;
     PSC_HIP(hipGetLastError());
     PSC_TRY(poll(h_starts.data(), nstart, 16 * (G + 1)));
   }

@@ -38,9 +38,9 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
-// Windows are scanned in parts of TAIL_PART positions, so that a thread's
-// share of a part is at most 64 positions (one bit mask).
-constexpr uint64_t TAIL_PART = (uint64_t)TAIL_BLOCK * 64;
+// Windows are scanned in parts of TAIL_PART positions: a part covers at
+// most TAIL_BLOCK words of the special-position bitmap, one per thread.
+constexpr uint64_t TAIL_PART = 32ull * (TAIL_BLOCK - 1);
 // Remainder positions (finished vertices included) a graph may have here:
 // every round scans its window's positions.
 constexpr uint64_t TAIL_MAX_POS = 1ull << 20;
@@ -169,56 +169,82 @@ __device__ __forceinline__ int tail_insert(unsigned long long *T, uint64_t mask,
   return -1;
 }
 
+// Special positions of a remainder (bit j of `spec`, span positions):
+// those with compact edges or already finished when the graph entered the
+// tail.  Every other position is isolated: nothing reaches it and it
+// reaches nothing, so it stays unfinished until its own round makes it a
+// singleton SCC.  A round therefore reads the done flags of its window's
+// special positions only, and labels the isolated ones by arithmetic.
+//
 // One part [c0, c1) of a round's window (c1 - c0 <= TAIL_PART): thread t
-// owns a contiguous segment of at most 64 positions starting at s0; `mine`
-// marks its unfinished vertices, `linked` those with compact edges in
-// either direction.  i0 / k0 are the block-wide exclusive prefixes of the
-// two counts, tot / totl their totals.  Every thread of the block calls
-// it (it synchronises the block).
-struct WinPart {
-  uint64_t s0, mine, linked;
-  uint32_t i0, k0, tot, totl;
+// owns span word (base + c0) / 32 + t, masked to the part.  done marks its
+// special positions that are finished now; u0 / ns0 are the block-wide
+// exclusive prefixes of unfinished positions / unfinished special
+// positions (seeds), tot / nstot their totals.  Every thread of the block
+// calls it (it synchronises the block).
+struct SpecWord {
+  uint64_t p0;                 // span position of bit 0
+  uint32_t inwin, spec, done;  // bit masks
+  uint32_t u0, tot, ns0, nstot;
 };
 
-__device__ __forceinline__ WinPart tail_part(const unsigned long long *lab,
-                                             const uint32_t *toff, const uint32_t *tioff,
-                                             uint64_t base, uint64_t c0, uint64_t c1,
-                                             uint32_t *wsum, uint32_t *wsum2) {
+__device__ __forceinline__ SpecWord tail_spec_part(const uint32_t *specbm,
+                                                   const unsigned long long *lab,
+                                                   uint64_t base, uint64_t c0, uint64_t c1,
+                                                   uint32_t *wsum, uint32_t *wsum2) {
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
-  const uint64_t W = c1 - c0, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;
-  WinPart r;
-  r.s0 = c0 + min((uint64_t)tid * per, W);
-  const uint64_t s1 = c0 + min((uint64_t)(tid + 1) * per, W);
-  r.mine = 0;
-  r.linked = 0;
-  for (uint64_t j = r.s0; j < s1; ++j) {
-    const uint64_t v = base + j;
-    if (!(lab[v] & TOP_BIT)) {
-      r.mine |= 1ull << (j - r.s0);
-      if (toff[v + 1] != toff[v] || tioff[v + 1] != tioff[v])
-        r.linked |= 1ull << (j - r.s0);
-    }
+  const uint64_t a0 = base + c0, a1 = base + c1;
+  SpecWord r;
+  r.p0 = ((a0 >> 5) + tid) << 5;
+  r.inwin = 0;
+  if (r.p0 < a1) {
+    const uint32_t lo = a0 > r.p0 ? (uint32_t)(a0 - r.p0) : 0u;   // < 32
+    const uint64_t hi = min<uint64_t>(32, a1 - r.p0);
+    r.inwin = (hi >= 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
   }
-  const uint32_t c = (uint32_t)__popcll(r.mine), cl = (uint32_t)__popcll(r.linked);
-  const uint32_t inc = wave_incl_scan(c), incl = wave_incl_scan(cl);
+  r.spec = r.inwin ? specbm[r.p0 >> 5] & r.inwin : 0u;
+  r.done = 0;
+  for (uint32_t m = r.spec; m; m &= m - 1) {
+    const int b = __ffs(m) - 1;
+    if (lab[r.p0 + b] & TOP_BIT) r.done |= 1u << b;
+  }
+  const uint32_t unf = (uint32_t)__popc(r.inwin & ~r.done);
+  const uint32_t sd = (uint32_t)__popc(r.spec & ~r.done);
+  const uint32_t inc = wave_incl_scan(unf), incs = wave_incl_scan(sd);
   if ((tid & (WAVE - 1)) == WAVE - 1) {
     wsum[w] = inc;
-    wsum2[w] = incl;
+    wsum2[w] = incs;
   }
   __syncthreads();
-  uint32_t before = 0, beforel = 0;
+  uint32_t before = 0, befores = 0;
   r.tot = 0;
-  r.totl = 0;
+  r.nstot = 0;
   for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
     before += x < w ? wsum[x] : 0u;
     r.tot += wsum[x];
-    beforel += x < w ? wsum2[x] : 0u;
-    r.totl += wsum2[x];
+    befores += x < w ? wsum2[x] : 0u;
+    r.nstot += wsum2[x];
   }
-  r.i0 = before + inc - c;
-  r.k0 = beforel + incl - cl;
+  r.u0 = before + inc - unf;
+  r.ns0 = befores + incs - sd;
   __syncthreads();   // wsum is reused by the next part
   return r;
+}
+
+// spec bitmap of the span positions [0, nrem) (bits past nrem clear).
+__global__ void k_tail_special(const unsigned long long *tlab, const uint32_t *toff,
+                               const uint32_t *tioff, uint64_t nrem, uint32_t *spec) {
+  const int l = lane();
+  for (uint64_t i0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~(uint64_t)(WAVE - 1);
+       i0 < nrem; i0 += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = i0 + l;
+    bool sp = false;
+    if (i < nrem)
+      sp = (tlab[i] & TOP_BIT) || toff[i + 1] != toff[i] || tioff[i + 1] != tioff[i];
+    const uint64_t m = ballot(sp);
+    if (l == 0) spec[i0 >> 5] = (uint32_t)m;
+    if (l == 32 && i0 + 32 < nrem) spec[(i0 >> 5) + 1] = (uint32_t)(m >> 32);
+  }
 }
 
 struct TailArgs {
@@ -226,6 +252,7 @@ struct TailArgs {
   TailState *state;
   const uint32_t *toff, *tadj, *tioff, *tiadj;
   unsigned long long *tlab;
+  const uint32_t *spec;        // special positions (k_tail_special)
   unsigned long long *table;   // 2 cap slots per graph, all EMPTY on entry
   uint32_t *lists;             // 4 lists of cap / 2 entries per graph
   unsigned long long *lab_off; // per graph label counter
@@ -283,13 +310,14 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       // are seeded into the tables.
       uint32_t fb = 0, kb = 0;
       for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
-        const WinPart wp = tail_part(lab, A.toff, A.tioff, base, c0,
-                                     min(c0 + TAIL_PART, end), wsum, wsum2);
-        uint32_t k = kb + wp.k0;
-        for (uint64_t mm = wp.linked; mm; mm &= mm - 1) {
-          const int b = __ffsll((unsigned long long)mm) - 1;
-          const uint32_t v = (uint32_t)(base + wp.s0 + b);
-          const uint32_t i = fb + wp.i0 + (uint32_t)__popcll(wp.mine & ((1ull << b) - 1));
+        const SpecWord sw = tail_spec_part(A.spec, lab, base, c0, min(c0 + TAIL_PART, end),
+                                           wsum, wsum2);
+        const uint32_t seeds = sw.spec & ~sw.done, unf = sw.inwin & ~sw.done;
+        uint32_t k = kb + sw.ns0;
+        for (uint32_t mm = seeds; mm; mm &= mm - 1) {
+          const int b = __ffs(mm) - 1;
+          const uint32_t v = (uint32_t)(sw.p0 + b);
+          const uint32_t i = fb + sw.u0 + (uint32_t)__popc(unf & ((1u << b) - 1u));
           const uint32_t lb = (uint32_t)(loff + i);
           uint64_t q0 = 0, q1 = 0;
           const int r0 = tail_insert(T0, mask, v, lb, &q0);
@@ -302,8 +330,8 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
           }
           ++k;
         }
-        fb += wp.tot;
-        kb += wp.totl;
+        fb += sw.tot;
+        kb += sw.nstot;
       }
       if (tid == 0) {
         front_n = fb;
@@ -444,22 +472,21 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     full = flag_full;
     if (!full) {
       // isolated pivots: their own SCC, (loff + i) | TOP_BIT (written only
-      // now: a stopped round leaves every label as it was).  The window is
-      // scanned again; no label has changed since the first scan, and a
-      // part's labels are written after the whole part was read.
+      // now: a stopped round leaves every label as it was).  The window's
+      // parts are scanned again; only isolated positions are written, and
+      // only special positions are read.
       {
         uint32_t fb = 0;
         for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
-          const WinPart wp = tail_part(lab, A.toff, A.tioff, base, c0,
-                                       min(c0 + TAIL_PART, end), wsum, wsum2);
-          uint32_t i = fb + wp.i0;
-          for (uint64_t mm = wp.mine; mm; mm &= mm - 1) {
-            const int b = __ffsll((unsigned long long)mm) - 1;
-            if (!((wp.linked >> b) & 1ull))
-              lab[base + wp.s0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
-            ++i;
+          const SpecWord sw = tail_spec_part(A.spec, lab, base, c0,
+                                             min(c0 + TAIL_PART, end), wsum, wsum2);
+          const uint32_t unf = sw.inwin & ~sw.done;
+          for (uint32_t mm = sw.inwin & ~sw.spec; mm; mm &= mm - 1) {
+            const int b = __ffs(mm) - 1;
+            const uint32_t i = fb + sw.u0 + (uint32_t)__popc(unf & ((1u << b) - 1u));
+            lab[sw.p0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
           }
-          fb += wp.tot;
+          fb += sw.tot;
         }
       }
       // ---- partition (src/scc.hpp:437-464): a pair in both tables is in
@@ -696,9 +723,12 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   const uint64_t e_out = h_e[G], e_in = h_e[G + 1];
   PSC_TRY(tail_adj.ensure((e_out + e_in + 2) * 4));
   uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj + e_out + 1;
+  PSC_TRY(tail_spec.ensure(((n_rem + 31) / 32 + 1) * 4));
   prof_begin(KK_TAIL);
   k_tail_fill<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
       cen, n_rem, lab, lid, d_off, d_adj, d_ioff, d_iadj, toff, tioff, tadj, tiadj);
+  k_tail_special<<<blocks(n_rem), 256, 0, stream>>>(tail_lab.as<unsigned long long>(), toff,
+                                                    tioff, n_rem, tail_spec.as<uint32_t>());
   prof_end();
   for (size_t i = 0; i < nt; ++i) {
     TailGraph &t = tgs[i];
@@ -752,6 +782,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     a.tioff = tioff;
     a.tiadj = tiadj;
     a.tlab = tail_lab.as<unsigned long long>();
+    a.spec = tail_spec.as<uint32_t>();
     a.table = tail_table.as<unsigned long long>();
     a.lists = tail_lists.as<uint32_t>();
     a.lab_off = gD.as<unsigned long long>();
