@@ -291,6 +291,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   bool full = false, handover = false;
   while (cur < nr && !full && !handover) {
     const uint64_t end = min(cur + step, nr);
+    SpecWord sw_keep{};
     const unsigned long long edges0 = edges;
     // ---- front: unfinished vertices of the window, in order; seeds
     // (pivot_i, loff + i) in both tables (src/scc.hpp:147-150)
@@ -312,6 +313,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
         const SpecWord sw = tail_spec_part(A.spec, lab, base, c0, min(c0 + TAIL_PART, end),
                                            wsum, wsum2);
+        sw_keep = sw;   // the commit reuses a one-part window's scan
         const uint32_t seeds = sw.spec & ~sw.done, unf = sw.inwin & ~sw.done;
         uint32_t k = kb + sw.ns0;
         for (uint32_t mm = seeds; mm; mm &= mm - 1) {
@@ -478,8 +480,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       {
         uint32_t fb = 0;
         for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
-          const SpecWord sw = tail_spec_part(A.spec, lab, base, c0,
-                                             min(c0 + TAIL_PART, end), wsum, wsum2);
+          const SpecWord sw = end - cur <= TAIL_PART
+                                  ? sw_keep
+                                  : tail_spec_part(A.spec, lab, base, c0,
+                                                   min(c0 + TAIL_PART, end), wsum, wsum2);
           const uint32_t unf = sw.inwin & ~sw.done;
           for (uint32_t mm = sw.inwin & ~sw.spec; mm; mm &= mm - 1) {
             const int b = __ffs(mm) - 1;
