@@ -80,6 +80,9 @@ typedef struct psc_params {
                            windows hold one pivot position each runs
                            as bitmap reaches (same labels as the
                            multi-search); 0: always multi-search      */
+  int32_t capture;      /* 1 (default): the level batches enqueued
+                           between host polls are captured once into
+                           hipGraphs and replayed; 0: plain launches  */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).

@@ -169,6 +169,27 @@ struct Engine {
               bool *used);
   int multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
                    const uint32_t *wgid_d, uint64_t slots[2], psc_stats *st);
+
+  // Captured level batches: the K levels enqueued between two host polls
+  // (K x the level's launches) are captured once into a hipGraph per
+  // distinct launch sequence and replayed afterwards.  The key holds every
+  // kernel argument of the batch; kernels read everything else from
+  // device descriptors.
+  struct BatchKey {
+    int kind;                 // 0 = reach levels, 1 = multi-search levels
+    uint32_t L0, K, tau, waves;
+    int vgc, g0, g1;
+    const void *S, *C, *lab;
+    bool operator==(const BatchKey &o) const {
+      return kind == o.kind && L0 == o.L0 && K == o.K && tau == o.tau &&
+             waves == o.waves && vgc == o.vgc && g0 == o.g0 && g1 == o.g1 && S == o.S &&
+             C == o.C && lab == o.lab;
+    }
+  };
+  std::vector<std::pair<BatchKey, hipGraphExec_t>> gcache;
+  uint64_t graph_replays = 0, graph_captures = 0;
+  template <class F>
+  int replay(const BatchKey &k, F enqueue);
 };
 
 }  // namespace psc

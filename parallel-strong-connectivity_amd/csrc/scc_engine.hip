@@ -1588,6 +1588,9 @@ Engine::Engine() {
 }
 
 Engine::~Engine() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &g : gcache) (void)hipGraphExecDestroy(g.second);
+  gcache.clear();
   for (DevBuf *b : all_bufs()) b->release();
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
@@ -1819,6 +1822,48 @@ int Engine::upload_round(const std::vector<uint64_t> &wpre,
   return PSC_OK;
 }
 
+// Replays the captured level batch `k`, capturing it first from
+// `enqueue` (which launches the batch's kernels on the stream).  At most
+// 32 batches are kept; the oldest is dropped once the stream is idle.
+template <class F>
+int Engine::replay(const BatchKey &k, F enqueue) {
+  hipGraphExec_t ex = nullptr;
+  for (auto &g : gcache)
+    if (g.first == k) {
+      ex = g.second;
+      break;
+    }
+  if (!ex) {
+    hipGraph_t graph = nullptr;
+    PSC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    enqueue();
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(stream, &graph);
+    if (le != hipSuccess || ee != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      set_error(std::string("level batch capture: ") +
+                hipGetErrorString(le != hipSuccess ? le : ee));
+      return PSC_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      return PSC_ERR_HIP;
+    }
+    if (gcache.size() >= 32) {
+      PSC_HIP(hipStreamSynchronize(stream));
+      (void)hipGraphExecDestroy(gcache.front().second);
+      gcache.erase(gcache.begin());
+    }
+    gcache.push_back({k, ex});
+    ++graph_captures;
+  }
+  PSC_HIP(hipGraphLaunch(ex, stream));
+  ++graph_replays;
+  return PSC_OK;
+}
+
 int Engine::poll(void *dst, const void *src, size_t bytes) {
   char *buf = (char *)h_pinned + PIN_POLL;
   if (bytes > PIN_REACH - PIN_POLL) {
@@ -1896,20 +1941,31 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   ReachCtl hc_, prev_{};
   auto t_lvl = std::chrono::steady_clock::now();
   while (true) {
-    for (int k = 0; k < K; ++k, ++L) {
-      prof_begin(KK_STEP);
-      k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, L, tau,
-                                                      p.local_reach, total_waves);
-      prof_end();
-      prof_begin(KK_PULL);
-      k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, L,
-                                                    (uint32_t)grid_pull * WPB);
-      prof_end();
-      prof_begin(KK_HEAVY);
-      k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, L,
-                                                       (uint32_t)grid_heavy * WPB);
-      prof_end();
+    auto levels = [&](uint32_t L0) {
+      for (int k = 0; k < K; ++k) {
+        const uint32_t Lk = L0 + (uint32_t)k;
+        prof_begin(KK_STEP);
+        k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, Lk, tau,
+                                                        p.local_reach, total_waves);
+        prof_end();
+        prof_begin(KK_PULL);
+        k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, Lk,
+                                                      (uint32_t)grid_pull * WPB);
+        prof_end();
+        prof_begin(KK_HEAVY);
+        k_reach_heavy<<<grid_heavy, BLOCK, 0, stream>>>(S, C, 2, Lk,
+                                                         (uint32_t)grid_heavy * WPB);
+        prof_end();
+      }
+    };
+    if (p.capture && !trace && prof_level < 2) {
+      const BatchKey key{0, L, (uint32_t)K, tau, total_waves, p.local_reach, grid_pull,
+                         grid_heavy, S, C, nullptr};
+      PSC_TRY(replay(key, [&] { levels(key.L0); }));
+    } else {
+      levels(L);
     }
+    L += (uint32_t)K;
     PSC_HIP(hipGetLastError());
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
     if (trace) {
@@ -2087,18 +2143,29 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
     int K = (int)std::min<uint32_t>(std::max<uint32_t>(ms_levels_hint, 2), 32);
     MSCtl hc_;
     while (true) {
-      for (int k = 0; k < K; ++k, ++L) {
-        prof_begin(KK_MS_EXPAND);
-        k_ms_expand<<<grid_ms, BLOCK, 0, stream>>>(
-            S, C, 2, L, labels.as<unsigned long long>(), tau, p.local_scc,
-            total_waves);
-        prof_end();
-        prof_begin(KK_MS_HEAVY);
-        k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
-            S, C, 2, L, labels.as<unsigned long long>(),
-            (uint32_t)grid_heavy * WPB);
-        prof_end();
+      auto levels = [&](uint32_t L0) {
+        for (int k = 0; k < K; ++k) {
+          const uint32_t Lk = L0 + (uint32_t)k;
+          prof_begin(KK_MS_EXPAND);
+          k_ms_expand<<<grid_ms, BLOCK, 0, stream>>>(
+              S, C, 2, Lk, labels.as<unsigned long long>(), tau, p.local_scc,
+              total_waves);
+          prof_end();
+          prof_begin(KK_MS_HEAVY);
+          k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
+              S, C, 2, Lk, labels.as<unsigned long long>(),
+              (uint32_t)grid_heavy * WPB);
+          prof_end();
+        }
+      };
+      if (p.capture && prof_level < 2) {
+        const BatchKey key{1, L, (uint32_t)K, tau, total_waves, p.local_scc, grid_heavy,
+                           0, S, C, labels.ptr};
+        PSC_TRY(replay(key, [&] { levels(key.L0); }));
+      } else {
+        levels(L);
       }
+      L += (uint32_t)K;
       PSC_HIP(hipGetLastError());
       PSC_TRY(poll(&hc_, C, sizeof(MSCtl)));
       bool live = false;
@@ -2510,6 +2577,7 @@ void psc_params_default(psc_params *p) {
   p->timing = 1;
   p->tail_max = 65536;
   p->pivot_reach = 1;
+  p->capture = 1;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }

@@ -38,9 +38,9 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
-// Windows are scanned in parts of TAIL_PART positions: a part covers at
-// most TAIL_BLOCK words of the special-position bitmap, one per thread.
-constexpr uint64_t TAIL_PART = 32ull * (TAIL_BLOCK - 1);
+// Windows are scanned in parts of TAIL_PART positions, so that a thread's
+// share of a part is at most 32 positions (one bit mask).
+constexpr uint64_t TAIL_PART = 32ull * TAIL_BLOCK;
 // Remainder positions (finished vertices included) a graph may have here:
 // every round scans its window's positions.
 constexpr uint64_t TAIL_MAX_POS = 1ull << 20;
@@ -177,11 +177,12 @@ __device__ __forceinline__ int tail_insert(unsigned long long *T, uint64_t mask,
 // special positions only, and labels the isolated ones by arithmetic.
 //
 // One part [c0, c1) of a round's window (c1 - c0 <= TAIL_PART): thread t
-// owns span word (base + c0) / 32 + t, masked to the part.  done marks its
-// special positions that are finished now; u0 / ns0 are the block-wide
-// exclusive prefixes of unfinished positions / unfinished special
-// positions (seeds), tot / nstot their totals.  Every thread of the block
-// calls it (it synchronises the block).
+// owns a contiguous segment of at most 32 positions starting at span
+// position p0 (its bits come from one or two bitmap words).  done marks
+// its special positions that are finished now; u0 / ns0 are the
+// block-wide exclusive prefixes of unfinished positions / unfinished
+// special positions (seeds), tot / nstot their totals.  Every thread of
+// the block calls it (it synchronises the block).
 struct SpecWord {
   uint64_t p0;                 // span position of bit 0
   uint32_t inwin, spec, done;  // bit masks
@@ -193,16 +194,20 @@ __device__ __forceinline__ SpecWord tail_spec_part(const uint32_t *specbm,
                                                    uint64_t base, uint64_t c0, uint64_t c1,
                                                    uint32_t *wsum, uint32_t *wsum2) {
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
-  const uint64_t a0 = base + c0, a1 = base + c1;
+  const uint64_t W = c1 - c0, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;   // <= 32
+  const uint64_t s0 = min((uint64_t)tid * per, W), s1 = min(s0 + per, W);
   SpecWord r;
-  r.p0 = ((a0 >> 5) + tid) << 5;
-  r.inwin = 0;
-  if (r.p0 < a1) {
-    const uint32_t lo = a0 > r.p0 ? (uint32_t)(a0 - r.p0) : 0u;   // < 32
-    const uint64_t hi = min<uint64_t>(32, a1 - r.p0);
-    r.inwin = (hi >= 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+  r.p0 = base + c0 + s0;
+  const uint32_t len = (uint32_t)(s1 - s0);
+  r.inwin = len >= 32 ? ~0u : ((1u << len) - 1u);
+  r.spec = 0;
+  if (len) {
+    const uint64_t wd = r.p0 >> 5;
+    const uint32_t sh = (uint32_t)(r.p0 & 31);
+    uint64_t bits = specbm[wd];
+    if (sh + len > 32) bits |= (uint64_t)specbm[wd + 1] << 32;
+    r.spec = (uint32_t)(bits >> sh) & r.inwin;
   }
-  r.spec = r.inwin ? specbm[r.p0 >> 5] & r.inwin : 0u;
   r.done = 0;
   for (uint32_t m = r.spec; m; m &= m - 1) {
     const int b = __ffs(m) - 1;

@@ -35,6 +35,9 @@ PARAM_SETS = [
     {"tail_max": 0, "pivot_reach": 0},
     # every host round single-pivot (windows never grow): all bitmap reaches
     {"tail_max": 0, "beta": 1.0},
+    # level batches launched one by one instead of replayed from hipGraphs
+    {"capture": 0},
+    {"tail_max": 0, "capture": 0},
     # remainders of up to 100 vertices in one launch, larger ones on the host
     {"tail_max": 100},
 ]
@@ -95,6 +98,24 @@ def test_repeat_runs_identical(psc, fixtures):
     c = s.scc(tau=3).copy()
     s.close()
     assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_captured_batches_replay(psc, oracle, fixtures):
+    """Captured level batches replayed many times on one handle, with plain
+    launches, other handles' work and synchronisations in between: every
+    replay gives the labels of the plain launches."""
+    for name in ("rmat_12", "grid_40x40_p03", "random_2000_2600", "knn_3000"):
+        _, g, canon = [f for f in fixtures if f[0] == name][0]
+        s = psc.SCC(g)
+        for tm in (0, 65536):
+            eager = s.scc(capture=0, tail_max=tm).copy()
+            assert np.array_equal(oracle.canonical(eager), canon), name
+            for r in range(4):
+                got = s.scc(capture=1, tail_max=tm).copy()
+                assert np.array_equal(got, eager), (name, tm, r)
+                other = psc.scc(g, capture=0, tail_max=tm)   # other handle
+                assert np.array_equal(other, eager), (name, tm, r)
+        s.close()
 
 
 @pytest.mark.parametrize("tail_max", [0, 65536])
