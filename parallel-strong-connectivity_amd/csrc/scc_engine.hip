@@ -653,6 +653,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
       const uint64_t gend = min((uint64_t)nf, g + per);
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
       uint64_t wbud = vgc ? (uint64_t)tau * VGC_WAVE_MULT : 0;   // per launch
+      uint32_t nbatch = 0;
       while (g < gend) {
         const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
         if ((uint32_t)l < cnt) ring[l] = (p ? R.q[1] : R.q[0])[g + l];
@@ -663,6 +664,9 @@ __global__ __launch_bounds__(BLOCK) void k_reach_step(
         uint64_t ebudget = budget * VGC_EDGE_MULT;
         wave_sync();
         while (head < tail) {
+          // a wave's serial batches per launch are capped: past the cap its
+          // new vertices go to the next frontier, where every wave shares them
+          if (++nbatch > VGC_MAX_BATCHES) budget = 0;
           const uint32_t k = min(64u, tail - head);
           const bool valid = (uint32_t)l < k;
           const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
@@ -1244,6 +1248,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
     unsigned long long n_edges = 0, n_exp = 0, n_ins = 0, n_app = 0;
     uint32_t probes = 0;
     uint64_t wbud = vgc ? (uint64_t)tau * VGC_WAVE_MULT : 0;   // per launch
+    uint32_t nbatch = 0;
     while (g < gend) {
       if (__atomic_load_n(&C->full[s], __ATOMIC_RELAXED)) break;
       const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
@@ -1255,6 +1260,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
       uint64_t ebudget = budget * VGC_EDGE_MULT;
       wave_sync();
       while (head < tail) {
+        if (++nbatch > VGC_MAX_BATCHES) budget = 0;   // as in k_reach_step
         const uint32_t k = min(64u, tail - head);
         const bool valid = (uint32_t)l < k;
         const unsigned long long pr = valid ? ring[(head + l) & (MS_RING - 1)] : 0ull;

@@ -29,6 +29,10 @@ constexpr uint32_t VGC_EDGE_MULT = 4;
 // and long serial local searches would only leave the launch waiting on a
 // few waves.
 constexpr uint32_t VGC_WAVE_MULT = 4;
+// ... and at most VGC_MAX_BATCHES serial 64-vertex batches per launch: the
+// cost of a local search is its number of batches (each a chain of
+// dependent loads), and a launch waits for its slowest wave.
+constexpr uint32_t VGC_MAX_BATCHES = 64;
 // Pull cost estimate per unvisited non-trimmed vertex, in pushed-edge
 // units (measured: a pulled vertex costs about as much as a pushed edge).
 constexpr uint64_t PULL_VERTEX_COST = 1;
