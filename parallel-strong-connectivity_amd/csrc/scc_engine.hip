@@ -371,7 +371,14 @@ __global__ void k_reach_reset(ReachCtl *C, int ns) {
 __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
                                 const uint32_t *NZ,
                                 const unsigned long long *nstart, uint32_t G,
-                                uint64_t seed) {
+                                uint64_t seed, uint64_t m) {
+  // the non-trimmed count (and the mean degree of a frontier vertex) of
+  // the direction rule come from the trim on the device
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)ns) {
+    const unsigned long long nz = nstart[G];
+    S[threadIdx.x].nnz = nz;
+    S[threadIdx.x].avg_deg = nz ? (double)m / (double)nz : 0.0;
+  }
   for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G;
        g += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t b = nstart[g];
@@ -1909,7 +1916,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
   if (!rr) {
     k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
-        S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed);
+        S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed, m);
   } else {
     // visited bitmaps empty, every vertex excluded but those carrying
     // their graph's pivot label (the exclusion bitmap is also the pull
@@ -2255,13 +2262,29 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     prof_end();
     kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 8.0 * n + 4.0 * n;
     PSC_HIP(hipGetLastError());
-    PSC_TRY(poll(h_starts.data(), nstart, 16 * (G + 1)));
+    // the per-graph starts travel to pinned memory while the first reach
+    // runs (its kernels take the counts from the device); the reach's
+    // host poll has them in place
+    PSC_TRY(gpin(2 * (G + 1)));
+    PSC_HIP(hipMemcpyAsync(region(2), nstart, 16 * (G + 1), hipMemcpyDeviceToHost,
+                           stream));
+  }
+  if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
+  const bool trace_first = p.timing >= 3;   // the level report wants nnz first
+  if (n && trace_first) {
+    PSC_HIP(hipStreamSynchronize(stream));
+    std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
+    nnz_total = h_starts[G];
+  }
+  if (n) {
+    // ---- first SCC of every graph (src/scc.hpp:316-355)
+    PSC_TRY(first_reach(p, st));
+    if (!trace_first) std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
   }
   const uint64_t *h_nstart = h_starts.data(), *h_zstart = h_starts.data() + G + 1;
   const uint64_t nnz = h_nstart[G], nz = n - nnz;
   nnz_total = nnz;
   st->n_trimmed = nz;
-  if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
   st->bytes_label += 8 * n;
   st->bytes_csr += 32 * n;
   // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1
@@ -2270,8 +2293,6 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   uint64_t n_rem = 0;
   std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
-    // ---- first SCC of every graph (src/scc.hpp:316-355)
-    PSC_TRY(first_reach(p, st));
     PSC_TRY(upload(gC, label_off.data(), G, 5));
     unsigned long long *big = misc.as<unsigned long long>() + 1;
     PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
@@ -2291,18 +2312,20 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     prof_end();
     PSC_HIP(hipGetLastError());
     kbytes[KK_MISC] += 4.0 * nnz + nnz / 4.0 + 8.0 * nnz + nnz / 8.0;
+    // the remaining count stays on the device: the per-graph starts
+    // (cstart[G] = the count) come back in one poll; the first SCC's size
+    // is read with the run's last poll
+    unsigned long long *rem_d = misc.as<unsigned long long>();
     PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
-                   nullptr, &n_rem));
-    kbytes[KK_SELECT] += 2 * nnz / 8.0 + 8.0 * n_rem;
+                   nullptr, nullptr, rem_d));
     prof_begin(KK_MISC);
     k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
-        centers.as<uint32_t>(), n_rem, nullptr, voffd, G, nullptr, G,
+        centers.as<uint32_t>(), 0, rem_d, voffd, G, nullptr, G,
         gB.as<unsigned long long>());
     prof_end();
     PSC_TRY(poll(h_cstart.data(), gB.ptr, 8 * (G + 1)));
-    uint64_t bigh = 0;
-    PSC_TRY(poll(&bigh, big, 8));
-    st->first_scc_size = bigh;
+    n_rem = h_cstart[G];
+    kbytes[KK_SELECT] += 2 * nnz / 8.0 + 8.0 * n_rem;
     st->bytes_label += 8 * nnz + nnz / 4;
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
@@ -2460,10 +2483,11 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   PSC_HIP(hipStreamSynchronize(stream));
   ++syncs;
   {
-    unsigned long long pprobe = 0;
-    PSC_TRY(poll(&pprobe, misc.as<unsigned long long>() + 2, 8));
-    kbytes[KK_PARTITION] += 8.0 * pprobe;
-    st->bytes_table += 8 * pprobe;
+    unsigned long long tail2[2] = {0, 0};   // first SCC size, contains() probes
+    PSC_TRY(poll(tail2, misc.as<unsigned long long>() + 1, 16));
+    st->first_scc_size = nnz ? tail2[0] : 0;
+    kbytes[KK_PARTITION] += 8.0 * tail2[1];
+    st->bytes_table += 8 * tail2[1];
   }
   if (p.timing) {
     float a = 0;
