@@ -352,9 +352,11 @@ __global__ void k_segment_starts(const uint32_t *list, uint64_t len,
 // it.  First-round labels only concern the non-trimmed vertices
 // (src/scc.hpp:341-347), whose reach sets this does not change.  The
 // bitmaps start as these dead-end sets (written by k_trim_count);
-// k_reach_reset clears the level control of both searches.
-__global__ void k_reach_reset(ReachCtl *C, int ns) {
+// k_reach_reset clears the level control of both searches (and *zero,
+// when given).
+__global__ void k_reach_reset(ReachCtl *C, int ns, unsigned long long *zero) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (zero) *zero = 0;
     for (int s = 0; s < ns; ++s) {
       ReachSlot z{};
       for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
@@ -1913,7 +1915,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   ReachSearch *S = reach_desc.as<ReachSearch>();
   ReachCtl *C = reach_ctl.as<ReachCtl>();
   prof_begin(KK_MISC);
-  k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
+  // the first round also clears the first-SCC size counter of k_first_mark
+  k_reach_reset<<<1, 64, 0, stream>>>(C, 2, rr ? nullptr : misc.as<unsigned long long>() + 1);
   if (!rr) {
     k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
         S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed, m);
@@ -2294,8 +2297,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
     PSC_TRY(upload(gC, label_off.data(), G, 5));
-    unsigned long long *big = misc.as<unsigned long long>() + 1;
-    PSC_HIP(hipMemsetAsync(big, 0, 8, stream));
+    unsigned long long *big = misc.as<unsigned long long>() + 1;   // cleared by k_reach_reset
     // first-SCC labels and the remaining vertices (src/scc.hpp:340-349) in
     // pivot order: placed by P position in one pass over NZ, then the
     // position bitmap is compacted in order

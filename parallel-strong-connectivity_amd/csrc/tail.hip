@@ -80,6 +80,10 @@ __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
                                const uint64_t *ioff, const uint32_t *iadj,
                                uint32_t *cnt_out, uint32_t *cnt_in) {
   const int l = lane();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {   // scan sentinels
+    cnt_out[nrem] = 0;
+    cnt_in[nrem] = 0;
+  }
   for (uint64_t j = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; j < nrem;
        j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
     const uint32_t v = centers[j];
@@ -595,7 +599,9 @@ __global__ void k_tail_writeback(const uint32_t *centers, uint64_t nrem,
     labels[centers[j]] = tlab[j];
 }
 
-__global__ void k_tail_fill_u64(unsigned long long *p, uint64_t n) {
+__global__ void k_tail_fill_u64(unsigned long long *p, uint64_t n,
+                                unsigned long long *zero2) {
+  if (blockIdx.x == 0 && threadIdx.x < 2) zero2[threadIdx.x] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = EMPTY_SLOT;
@@ -696,8 +702,6 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
                                                 tail_lab.as<unsigned long long>());
   k_tail_degrees<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
       cen, n_rem, lab, d_off, d_adj, d_ioff, d_iadj, cnt_out, cnt_in);
-  PSC_HIP(hipMemsetAsync(cnt_out + n_rem, 0, 4, stream));
-  PSC_HIP(hipMemsetAsync(cnt_in + n_rem, 0, 4, stream));
   PSC_TRY(tail_off.ensure(2 * (n_rem + 1) * 4));
   uint32_t *toff = tail_off.as<uint32_t>(), *tioff = toff + (n_rem + 1);
   size_t tb = 0, tb2 = 0;
@@ -779,10 +783,9 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     std::memcpy(tail_pin, launch.data(), nl * sizeof(TailGraph));
     PSC_HIP(hipMemcpyAsync(d_tg, tail_pin, nl * sizeof(TailGraph), hipMemcpyHostToDevice,
                            stream));
-    PSC_HIP(hipMemsetAsync(d_stats, 0, 16, stream));
     prof_begin(KK_TAIL);
     k_tail_fill_u64<<<blocks(slots), 256, 0, stream>>>(tail_table.as<unsigned long long>(),
-                                                       slots);
+                                                       slots, d_stats);
     TailArgs a;
     a.graphs = d_tg;
     a.state = d_ts;
