@@ -618,7 +618,7 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // "in the previous frontier" keeps the reachable set identical.  Long
 // remainders (> PULL_HEAVY edges after the first probes) are chunked onto
 // the heavy list.
-__global__ __launch_bounds__(BLOCK) void k_reach_step(
+__global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
     const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
@@ -1661,7 +1661,7 @@ int Engine::setup() {
                                                        BLOCK, 0));
   grid_heavy = std::max(1, occ) * num_cu;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_pull, BLOCK, 0));
-  grid_pull = std::max(1, occ) * num_cu;
+  grid_pull = std::max(1, std::min(occ, 2)) * num_cu;
   return PSC_OK;
 }
 
