@@ -80,9 +80,13 @@ typedef struct psc_params {
                            windows hold one pivot position each runs
                            as bitmap reaches (same labels as the
                            multi-search); 0: always multi-search      */
-  int32_t capture;      /* 1 (default): the level batches enqueued
-                           between host polls are captured once into
-                           hipGraphs and replayed; 0: plain launches  */
+  int32_t capture;      /* 1: the level batches enqueued between host
+                           polls are captured once into hipGraphs and
+                           replayed; 0 (default): plain launches.  Only
+                           for a process whose other threads make no
+                           HIP calls while a handle runs (no serving
+                           lanes): capture is a stream mode that other
+                           threads' allocating calls can disturb      */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).

@@ -2609,7 +2609,7 @@ void psc_params_default(psc_params *p) {
   p->timing = 1;
   p->tail_max = 65536;
   p->pivot_reach = 1;
-  p->capture = 1;
+  p->capture = 0;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }
