@@ -2482,15 +2482,15 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_HIP(hipMemcpyAsync(d_out, lab, n * 8, hipMemcpyDeviceToDevice, stream));
   PSC_HIP(hipGetLastError());
   if (p.timing) PSC_HIP(hipEventRecord(ev[4], stream));
+  // first SCC size and contains() probes ride on the run's final sync
+  unsigned long long *tail2 = (unsigned long long *)((char *)h_pinned + PIN_POLL);
+  PSC_HIP(hipMemcpyAsync(tail2, misc.as<unsigned long long>() + 1, 16,
+                         hipMemcpyDeviceToHost, stream));
   PSC_HIP(hipStreamSynchronize(stream));
   ++syncs;
-  {
-    unsigned long long tail2[2] = {0, 0};   // first SCC size, contains() probes
-    PSC_TRY(poll(tail2, misc.as<unsigned long long>() + 1, 16));
-    st->first_scc_size = nnz ? tail2[0] : 0;
-    kbytes[KK_PARTITION] += 8.0 * tail2[1];
-    st->bytes_table += 8 * tail2[1];
-  }
+  st->first_scc_size = nnz ? tail2[0] : 0;
+  kbytes[KK_PARTITION] += 8.0 * tail2[1];
+  st->bytes_table += 8 * tail2[1];
   if (p.timing) {
     float a = 0;
     PSC_HIP(hipEventElapsedTime(&a, ev[0], ev[4]));

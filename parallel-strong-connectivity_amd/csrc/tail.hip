@@ -38,6 +38,7 @@ namespace psc {
 namespace {
 
 constexpr int TAIL_BLOCK = 1024;
+constexpr uint32_t HALF_WAVES = TAIL_BLOCK / WAVE / 2;   // waves per search direction
 // Windows are scanned in parts of TAIL_PART positions, so that a thread's
 // share of a part is at most 32 positions (one bit mask).
 constexpr uint64_t TAIL_PART = 32ull * TAIL_BLOCK;
@@ -58,7 +59,7 @@ struct TailGraph {
   uint64_t base;        // first center index of the graph
   uint64_t nr;          // remaining vertices
   uint64_t tbase;       // first table slot (two tables of cap slots)
-  uint64_t lbase;       // first log / frontier entry (cap / 2 per list)
+  uint64_t lbase;       // first log / frontier entry (6 lists of cap / 2)
   uint64_t cap;         // table slots, power of two
   uint64_t cap_max;     // largest size a relaunch may use
   uint64_t cur, step;   // pivot window schedule
@@ -263,7 +264,7 @@ struct TailArgs {
   unsigned long long *tlab;
   const uint32_t *spec;        // special positions (k_tail_special)
   unsigned long long *table;   // 2 cap slots per graph, all EMPTY on entry
-  uint32_t *lists;             // 4 lists of cap / 2 entries per graph
+  uint32_t *lists;             // 6 lists of cap / 2 entries per graph
   unsigned long long *lab_off; // per graph label counter
   unsigned long long *stats;   // [0] inserts, [1] edges
   uint32_t tau;                // local-search budget per frontier pair
@@ -273,7 +274,7 @@ struct TailArgs {
 // One workgroup per graph: all rounds of its remainder.
 __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   __shared__ uint32_t wsum[TAIL_BLOCK / WAVE], wsum2[TAIL_BLOCK / WAVE];
-  __shared__ unsigned int cnt_q, cnt_log, flag_full;
+  __shared__ unsigned int cnt_q[2], cnt_log[2], flag_full;
   __shared__ unsigned int front_n, seed_n;
   __shared__ unsigned long long round_edges;
   // per-wave local-search ring and window scratch
@@ -288,7 +289,9 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   unsigned long long *const T0 = A.table + tg.tbase, *const T1 = T0 + tg.cap;
   uint32_t *const L0 = A.lists + tg.lbase;   // slot indices of inserted pairs
   uint32_t *const L1 = L0 + half;
-  uint32_t *const qa = L0 + 2 * half, *const qb = L0 + 3 * half;   // frontiers
+  // frontier ping-pong of each direction
+  uint32_t *const qa0 = L0 + 2 * half, *const qb0 = L0 + 3 * half;
+  uint32_t *const qa1 = L0 + 4 * half, *const qb1 = L0 + 5 * half;
   unsigned long long *lab = A.tlab;
   const uint32_t tau = A.tau;
   const int vgc = A.vgc;
@@ -306,7 +309,6 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     // (pivot_i, loff + i) in both tables (src/scc.hpp:147-150)
     if (tid == 0) {
       front_n = 0;
-      cnt_log = 0;
       flag_full = 0;
       round_edges = 0;
     }
@@ -351,24 +353,34 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       __syncthreads();
     }
     const uint32_t nfront = front_n, nseed = seed_n;
+    // ---- forward and backward multi-search over pairs (src/scc.hpp:160-252),
+    // both at once: waves [0, HALF_WAVES) expand the forward search, the
+    // others the backward one, so a round waits for the longer of the two
+    // searches instead of their sum
     nlog0 = nlog1 = 0;
-    // ---- forward and backward multi-search over pairs (src/scc.hpp:160-252)
-    for (int d = 0; d < 2 && nseed && !flag_full; ++d) {
-      unsigned long long *const Td = d ? T1 : T0;
-      uint32_t *const Ld = d ? L1 : L0;
-      const uint32_t *const eo = d ? A.tioff : A.toff;
-      const uint32_t *const ea = d ? A.tiadj : A.tadj;
-      uint32_t nl = nseed;
-      for (uint32_t i = tid; i < nseed; i += TAIL_BLOCK) qa[i] = Ld[i];
-      uint32_t *qc = qa, *qn = qb;
-      uint32_t nq = nseed;
+    if (nseed && !flag_full) {
+      for (uint32_t i = tid; i < nseed; i += TAIL_BLOCK) {
+        qa0[i] = L0[i];
+        qa1[i] = L1[i];
+      }
+      uint32_t nq0 = nseed, nq1 = nseed, nl0 = nseed, nl1 = nseed;
+      uint32_t *qc0 = qa0, *qn0 = qb0, *qc1 = qa1, *qn1 = qb1;
+      const int dw = w >= HALF_WAVES ? 1 : 0;
+      const uint32_t wd = w - (uint32_t)dw * HALF_WAVES;
+      unsigned long long *const Td = dw ? T1 : T0;
+      uint32_t *const Ld = dw ? L1 : L0;
+      const uint32_t *const eo = dw ? A.tioff : A.toff;
+      const uint32_t *const ea = dw ? A.tiadj : A.tadj;
       __syncthreads();
-      while (nq && !flag_full) {
-        if (tid == 0) {
-          cnt_q = 0;
-          cnt_log = nl;
+      while ((nq0 || nq1) && !flag_full) {
+        if (tid < 2) {
+          cnt_q[tid] = 0;
+          cnt_log[tid] = tid ? nl1 : nl0;
         }
         __syncthreads();
+        const uint32_t nq = dw ? nq1 : nq0;
+        const uint32_t *const qc = dw ? qc1 : qc0;
+        uint32_t *const qn = dw ? qn1 : qn0;
         // each wave takes 64 frontier pairs at a time and expands them with
         // load-balanced 64-edge windows; new pairs of compact degree <=
         // VGC_MAX_DEG stay in the wave's LDS ring while the budget lasts
@@ -379,7 +391,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
         uint32_t *el = el_s[w];
         unsigned long long *lw = lw_s[w];
         const int l = (int)(tid & (WAVE - 1));
-        for (uint32_t c0 = w * WAVE; c0 < nq && !flag_full; c0 += TAIL_BLOCK) {
+        for (uint32_t c0 = wd * WAVE; c0 < nq && !flag_full; c0 += HALF_WAVES * WAVE) {
           const uint32_t cnt = min((uint32_t)WAVE, nq - c0);
           if ((uint32_t)l < cnt) ring[l] = qc[c0 + l];
           uint32_t head = 0, tail = cnt;
@@ -438,7 +450,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
               ins += fresh;
               // log: one LDS atomic per window
               uint32_t lbase = 0;
-              if (l == 0) lbase = atomicAdd(&cnt_log, (uint32_t)__popcll(fm));
+              if (l == 0) lbase = atomicAdd(&cnt_log[dw], (uint32_t)__popcll(fm));
               lbase = __shfl(lbase, 0);
               if (fresh) {
                 const uint32_t at = lbase + lanes_below(fm);
@@ -457,7 +469,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
               const uint64_t gm = ballot(glob);
               if (gm) {
                 uint32_t qb0 = 0;
-                if (l == 0) qb0 = atomicAdd(&cnt_q, (uint32_t)__popcll(gm));
+                if (l == 0) qb0 = atomicAdd(&cnt_q[dw], (uint32_t)__popcll(gm));
                 qb0 = __shfl(qb0, 0);
                 if (glob) {
                   const uint32_t aq = qb0 + lanes_below(gm);
@@ -469,14 +481,24 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
           }
         }
         __syncthreads();
-        nq = min(cnt_q, (unsigned int)half);
-        nl = min(cnt_log, (unsigned int)half);
-        uint32_t *t = qc;
-        qc = qn;
-        qn = t;
+        if (nq0) {
+          nq0 = min(cnt_q[0], (unsigned int)half);
+          nl0 = min(cnt_log[0], (unsigned int)half);
+          uint32_t *t = qc0;
+          qc0 = qn0;
+          qn0 = t;
+        }
+        if (nq1) {
+          nq1 = min(cnt_q[1], (unsigned int)half);
+          nl1 = min(cnt_log[1], (unsigned int)half);
+          uint32_t *t = qc1;
+          qc1 = qn1;
+          qn1 = t;
+        }
         __syncthreads();
       }
-      if (d) nlog1 = nl; else nlog0 = nl;
+      nlog0 = nl0;
+      nlog1 = nl1;
     }
     if (edges != edges0) atomicAdd(&round_edges, edges - edges0);
     __syncthreads();
@@ -766,17 +788,18 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   std::vector<TailState> hs;
   while (!pending.empty()) {
     launch.clear();
-    uint64_t slots = 0;
+    uint64_t slots = 0, lslots = 0;
     for (size_t i : pending) {
       TailGraph t = tgs[i];
       t.tbase = slots;   // two tables of cap slots
-      t.lbase = slots;   // four lists of cap / 2 u32
+      t.lbase = lslots;   // six lists of cap / 2 u32
       slots += 2 * t.cap;
+      lslots += 3 * t.cap;
       launch.push_back(t);
     }
     const size_t nl = launch.size();
     PSC_TRY(tail_table.ensure(slots * 8));
-    PSC_TRY(tail_lists.ensure(slots * 4));
+    PSC_TRY(tail_lists.ensure(lslots * 4));
     // descriptors go through pinned staging (the stream is synchronised
     // by the poll below before the staging is written again)
     PSC_TRY(tail_pin_ensure(nl * sizeof(TailGraph)));
