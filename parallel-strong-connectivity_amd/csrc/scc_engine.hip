@@ -200,14 +200,24 @@ __global__ __launch_bounds__(BLOCK) void k_trim_count(
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int l = lane();
   uint32_t c = 0;
+  // every offset load of the tile first, then the tests
+  uint64_t o0[SEL_ITEMS], o1[SEL_ITEMS], i0[SEL_ITEMS], i1[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
     const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    bool zo = false, zi = false;
+    o0[j] = o1[j] = i0[j] = i1[j] = 0;
     if (i < n) {
-      zo = off[i + 1] == off[i];
-      zi = ioff[i + 1] == ioff[i];
+      o0[j] = off[i];
+      o1[j] = off[i + 1];
+      i0[j] = ioff[i];
+      i1[j] = ioff[i + 1];
     }
+  }
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    const bool zo = i < n && o1[j] == o0[j];
+    const bool zi = i < n && i1[j] == i0[j];
     c += (uint32_t)__popcll(ballot(i < n && !zo && !zi));
     const uint64_t mf = ballot(zo), mb = ballot(zi), mt = mf | mb;
     const uint64_t wd = (i - l) / 32;   // first word of the wave's 64 vertices
