@@ -16,7 +16,7 @@ constexpr int OUT_CAP = 512;         // per-wave staged appends (vertices)
 constexpr int OUT_CAP64 = 256;       // per-wave staged appends (pairs)
 constexpr uint32_t PULL_HEAVY = 64;  // pull remainder above which it is chunked
 constexpr uint32_t HEAVY_DEG = 512;  // degree above which edges are chunked
-constexpr uint32_t CHUNK = 1024;     // edges per heavy work item
+constexpr uint32_t CHUNK = 256;     // edges per heavy work item
 constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests first
 constexpr uint32_t VGC_MAX_DEG = 8;  // only vertices this sparse stay local
 // A wave's local search also stops taking vertices once it has scanned
