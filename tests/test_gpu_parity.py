@@ -195,3 +195,22 @@ def test_device_graph_run_and_status(psc, oracle):
     expect = oracle.tarjan_canonical(h)
     assert np.array_equal(labels.astype(np.uint32), expect)
     assert (nscc, big) == oracle.status(expect)
+
+
+def test_first_pivot_misses_giant_scc(psc, oracle):
+    """RMAT 2^16 (graph 1408 of the seed-4 batch mix) whose first pivot is
+    not in the giant SCC: the giant SCC is found by a later single-pivot
+    round (a bitmap reach on the whole device), the remainder then returns
+    to the single-launch rounds.  Every schedule gives the oracle's raw
+    labels."""
+    g = psc.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, seed=psc.graph_seed(4, 1408))
+    ref, _ = oracle.scc_ref(g, tau=4096)
+    canon = oracle.tarjan_canonical(g)
+    assert np.array_equal(oracle.canonical(ref), canon)
+    s = psc.SCC(g)
+    for kw in ({}, {"pivot_reach": 0}, {"tail_max": 0}):
+        labels = s.scc(tau=4096, **kw).copy()
+        print(kw, s.stats["first_scc_size"], s.stats["rounds"], flush=True)
+        assert s.stats["first_scc_size"] < g.n // 4, "first pivot hit the giant SCC"
+        assert np.array_equal(labels, ref), kw
+    s.close()
