@@ -362,11 +362,9 @@ __global__ void k_segment_starts(const uint32_t *list, uint64_t len,
 // it.  First-round labels only concern the non-trimmed vertices
 // (src/scc.hpp:341-347), whose reach sets this does not change.  The
 // bitmaps start as these dead-end sets (written by k_trim_count);
-// k_reach_reset clears the level control of both searches (and *zero,
-// when given).
-__global__ void k_reach_reset(ReachCtl *C, int ns, unsigned long long *zero) {
+// k_reach_reset clears the level control of both searches.
+__global__ void k_reach_reset(ReachCtl *C, int ns) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (zero) *zero = 0;
     for (int s = 0; s < ns; ++s) {
       ReachSlot z{};
       for (int j = 0; j < 3; ++j) C->slot[s][j] = z;
@@ -1100,10 +1098,8 @@ __global__ __launch_bounds__(BLOCK) void k_first_mark(
     const uint32_t *__restrict__ NZ, uint64_t nnz, const unsigned long long *voff,
     uint32_t G, const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
-    uint32_t *posv, uint32_t *pbits, unsigned long long *big) {
-  __shared__ unsigned long long wb[WPB];
+    uint32_t *posv, uint32_t *pbits) {
   const unsigned long long lab_one = lab1[0];
-  unsigned long long both = 0;
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nnz;
        j += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = NZ[j];
@@ -1112,23 +1108,13 @@ __global__ __launch_bounds__(BLOCK) void k_first_mark(
       const unsigned long long label = G == 1 ? lab_one : lab1[seg_of(voff, G, v)];
       labels[v] = (a && b) ? (label | TOP_BIT) : label;
     }
-    if (a && b) {
-      ++both;
-    } else {
+    if (!(a && b)) {
       const uint32_t g = G == 1 ? 0u : seg_of(nstart, G, j);
       const uint64_t b0 = nstart[g];
       const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(j - b0);
       posv[pos] = v;
       atomicOr(&pbits[pos >> 5], 1u << (pos & 31));
     }
-  }
-  both = wave_sum(both);
-  if (lane() == 0) wb[threadIdx.x / WAVE] = both;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long tb = 0;
-    for (int w = 0; w < WPB; ++w) tb += wb[w];
-    if (tb) atomicAdd(big, tb);
   }
 }
 
@@ -1925,8 +1911,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   ReachSearch *S = reach_desc.as<ReachSearch>();
   ReachCtl *C = reach_ctl.as<ReachCtl>();
   prof_begin(KK_MISC);
-  // the first round also clears the first-SCC size counter of k_first_mark
-  k_reach_reset<<<1, 64, 0, stream>>>(C, 2, rr ? nullptr : misc.as<unsigned long long>() + 1);
+  k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
   if (!rr) {
     k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
         S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed, m);
@@ -2307,7 +2292,6 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
     PSC_TRY(upload(gC, label_off.data(), G, 5));
-    unsigned long long *big = misc.as<unsigned long long>() + 1;   // cleared by k_reach_reset
     // first-SCC labels and the remaining vertices (src/scc.hpp:340-349) in
     // pivot order: placed by P position in one pass over NZ, then the
     // position bitmap is compacted in order
@@ -2320,13 +2304,12 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_first_mark<<<grid_for(nnz, BLOCK, num_cu * 8), BLOCK, 0, stream>>>(
         NZ.as<uint32_t>(), nnz, voffd, G, nstart, p.seed, vis[0].as<uint32_t>(),
         vis[1].as<uint32_t>(), lab, gC.as<unsigned long long>(), posv,
-        pbits.as<uint32_t>(), big);
+        pbits.as<uint32_t>());
     prof_end();
     PSC_HIP(hipGetLastError());
     kbytes[KK_MISC] += 4.0 * nnz + nnz / 4.0 + 8.0 * nnz + nnz / 8.0;
     // the remaining count stays on the device: the per-graph starts
-    // (cstart[G] = the count) come back in one poll; the first SCC's size
-    // is read with the run's last poll
+    // (cstart[G] = the count) come back in one poll
     unsigned long long *rem_d = misc.as<unsigned long long>();
     PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
                    nullptr, nullptr, rem_d));
@@ -2492,15 +2475,16 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_HIP(hipMemcpyAsync(d_out, lab, n * 8, hipMemcpyDeviceToDevice, stream));
   PSC_HIP(hipGetLastError());
   if (p.timing) PSC_HIP(hipEventRecord(ev[4], stream));
-  // first SCC size and contains() probes ride on the run's final sync
-  unsigned long long *tail2 = (unsigned long long *)((char *)h_pinned + PIN_POLL);
-  PSC_HIP(hipMemcpyAsync(tail2, misc.as<unsigned long long>() + 1, 16,
+  // the contains() probe count rides on the run's final sync; the first
+  // SCC holds every non-trimmed vertex that does not remain
+  unsigned long long *probes = (unsigned long long *)((char *)h_pinned + PIN_POLL);
+  PSC_HIP(hipMemcpyAsync(probes, misc.as<unsigned long long>() + 2, 8,
                          hipMemcpyDeviceToHost, stream));
   PSC_HIP(hipStreamSynchronize(stream));
   ++syncs;
-  st->first_scc_size = nnz ? tail2[0] : 0;
-  kbytes[KK_PARTITION] += 8.0 * tail2[1];
-  st->bytes_table += 8 * tail2[1];
+  st->first_scc_size = nnz - n_rem;
+  kbytes[KK_PARTITION] += 8.0 * probes[0];
+  st->bytes_table += 8 * probes[0];
   if (p.timing) {
     float a = 0;
     PSC_HIP(hipEventElapsedTime(&a, ev[0], ev[4]));

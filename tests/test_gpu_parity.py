@@ -204,7 +204,7 @@ def test_first_pivot_misses_giant_scc(psc, oracle):
     to the single-launch rounds.  Every schedule gives the oracle's raw
     labels."""
     g = psc.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, seed=psc.graph_seed(4, 1408))
-    ref, _ = oracle.scc_ref(g, tau=4096)
+    ref, ref_st = oracle.scc_ref(g, tau=4096)
     canon = oracle.tarjan_canonical(g)
     assert np.array_equal(oracle.canonical(ref), canon)
     s = psc.SCC(g)
@@ -212,5 +212,6 @@ def test_first_pivot_misses_giant_scc(psc, oracle):
         labels = s.scc(tau=4096, **kw).copy()
         print(kw, s.stats["first_scc_size"], s.stats["rounds"], flush=True)
         assert s.stats["first_scc_size"] < g.n // 4, "first pivot hit the giant SCC"
+        assert s.stats["first_scc_size"] == ref_st["first_scc_size"]
         assert np.array_equal(labels, ref), kw
     s.close()
