@@ -274,15 +274,13 @@ __global__ void k_graph_starts(const uint64_t *__restrict__ off,
   }
 }
 
-// Trim, pass 2: non-trimmed vertices go to NZ in vertex order with label 0;
-// trimmed ones get 1 + (rank among their graph's trimmed | TOP_BIT)
-// (src/scc.hpp:304-305).  Every label is written once, coalesced.  The
-// trimmed set is read from pass 1's bitmap (n/8 bytes) instead of the
-// degree arrays again (16n bytes).
+// Trim, pass 2: non-trimmed vertices go to NZ in vertex order
+// (src/scc.hpp:295-303).  The trimmed set is read from pass 1's bitmap
+// (n/8 bytes) instead of the degree arrays again (16n bytes).  Labels are
+// written once, after the first reach (k_first_label).
 __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
     const uint32_t *__restrict__ trim_bm, uint64_t n,
-    const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
-    const unsigned long long *tile_off, uint32_t *NZ, unsigned long long *labels) {
+    const unsigned long long *tile_off, uint32_t *NZ) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
@@ -298,7 +296,6 @@ __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
   }
   __syncthreads();
   uint64_t r0 = tile_off[blockIdx.x];
-  const unsigned long long z0 = zstart[0];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
     uint32_t before = 0, round_tot = 0;
@@ -308,16 +305,7 @@ __global__ __launch_bounds__(BLOCK) void k_trim_scatter(
       round_tot += wc[j][x];
     }
     const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    if (i < n) {
-      const uint64_t r = r0 + before + rk[j];
-      if (p[j]) {
-        NZ[r] = (uint32_t)i;
-        labels[i] = 0;
-      } else {
-        const unsigned long long zs = G == 1 ? z0 : zstart[seg_of(voff, G, i)];
-        labels[i] = 1 + ((i - r - zs) | TOP_BIT);
-      }
-    }
+    if (p[j]) NZ[r0 + before + rk[j]] = (uint32_t)i;
     r0 += round_tot;
   }
 }
@@ -1086,35 +1074,88 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
   }
 }
 
-// First-SCC labelling over the non-trimmed list (src/scc.hpp:340-349): for
-// v = NZ[j], both reaches -> found SCC (label | TOP_BIT), one reach -> new
-// subproblem label; trimmed vertices keep their labels.  A vertex not in
-// the first SCC remains: it is placed at its position in its graph's pivot
-// order P_g = NZ_g permuted (src/scc.hpp:303), pos = nstart[g] +
-// perm_g^-1(j - nstart[g]): posv[pos] = v and bit pos of pbits set, so
-// compacting pbits in order yields the remaining vertices in P order
-// without materialising or sorting P.
-__global__ __launch_bounds__(BLOCK) void k_first_mark(
-    const uint32_t *__restrict__ NZ, uint64_t nnz, const unsigned long long *voff,
-    uint32_t G, const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
+// Every label, once, in vertex order (whole lines), after the first reach:
+// trimmed vertices get 1 + (rank among their graph's trimmed | TOP_BIT)
+// (src/scc.hpp:304-305); a non-trimmed vertex reached both ways is in the
+// first SCC (label | TOP_BIT), one way gets the new subproblem label, none
+// keeps 0 (src/scc.hpp:340-349).  A non-trimmed vertex not in the first
+// SCC remains: it is placed at its position in its graph's pivot order
+// P_g = NZ_g permuted (src/scc.hpp:303), pos = nstart[g] +
+// perm_g^-1(j - nstart[g]) with j its NZ index: posv[pos] = v and bit pos
+// of pbits set, so compacting pbits in order yields the remaining vertices
+// in P order without materialising or sorting P.  Tiles and ranks as in
+// k_trim_scatter.
+__global__ __launch_bounds__(BLOCK) void k_first_label(
+    const uint32_t *__restrict__ trim_bm, uint64_t n, const unsigned long long *tile_off,
+    const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
+    const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
     uint32_t *posv, uint32_t *pbits) {
-  const unsigned long long lab_one = lab1[0];
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nnz;
-       j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = NZ[j];
-    const bool a = bit_test(vf, v), b = bit_test(vb, v);
-    if (a || b) {
-      const unsigned long long label = G == 1 ? lab_one : lab1[seg_of(voff, G, v)];
-      labels[v] = (a && b) ? (label | TOP_BIT) : label;
+  __shared__ uint32_t wc[SEL_ITEMS][WPB];
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
+  const int w = threadIdx.x / WAVE;
+  // every bitmap word of the tile first (trimmed, both reaches)
+  uint32_t tw[SEL_ITEMS], fw[SEL_ITEMS], bw[SEL_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    tw[j] = ~0u;
+    fw[j] = bw[j] = 0;
+    if (i < n) {
+      tw[j] = trim_bm[i >> 5];
+      fw[j] = vf[i >> 5];
+      bw[j] = vb[i >> 5];
     }
-    if (!(a && b)) {
-      const uint32_t g = G == 1 ? 0u : seg_of(nstart, G, j);
-      const uint64_t b0 = nstart[g];
-      const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(j - b0);
-      posv[pos] = v;
-      atomicOr(&pbits[pos >> 5], 1u << (pos & 31));
+  }
+  bool p[SEL_ITEMS];
+  uint32_t rk[SEL_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    p[j] = i < n && !((tw[j] >> (i & 31)) & 1u);
+    const uint64_t m = ballot(p[j]);
+    rk[j] = lanes_below(m);
+    if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  uint64_t r0 = tile_off[blockIdx.x];
+  const unsigned long long z0 = zstart[0], lab_one = lab1[0];
+  uint32_t place = 0;            // items left to place by pivot position
+  uint64_t rr[SEL_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SEL_ITEMS; ++j) {
+    uint32_t before = 0, round_tot = 0;
+#pragma unroll
+    for (int x = 0; x < WPB; ++x) {
+      before += x < w ? wc[j][x] : 0u;
+      round_tot += wc[j][x];
     }
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    const uint64_t r = r0 + before + rk[j];
+    rr[j] = r;
+    r0 += round_tot;
+    if (i >= n) continue;
+    const uint32_t g = G == 1 ? 0u : seg_of(voff, G, i);
+    if (!p[j]) {
+      const unsigned long long zs = G == 1 ? z0 : zstart[g];
+      labels[i] = 1 + ((i - r - zs) | TOP_BIT);
+      continue;
+    }
+    const bool a = (fw[j] >> (i & 31)) & 1u, b = (bw[j] >> (i & 31)) & 1u;
+    const unsigned long long label = G == 1 ? lab_one : lab1[g];
+    labels[i] = (a && b) ? (label | TOP_BIT) : ((a || b) ? label : 0ull);
+    if (!(a && b)) place |= 1u << j;
+  }
+  // placement of the few remaining vertices, out of the unrolled loop
+#pragma unroll 1
+  for (uint32_t mm = place; mm; mm &= mm - 1) {
+    const int j = __ffs(mm) - 1;
+    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    const uint32_t g = G == 1 ? 0u : seg_of(voff, G, i);
+    const uint64_t b0 = nstart[g];
+    const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(rr[j] - b0);
+    posv[pos] = (uint32_t)i;
+    atomicOr(&pbits[pos >> 5], 1u << (pos & 31));
   }
 }
 
@@ -2255,10 +2296,9 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
         d_off, d_ioff, n, voffd, G, tile_off.as<unsigned long long>(), tot, nstart,
         zstart);
     k_trim_scatter<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        trim_bm.as<uint32_t>(), n, voffd, G, zstart, tile_off.as<unsigned long long>(),
-        NZ.as<uint32_t>(), lab);
+        trim_bm.as<uint32_t>(), n, tile_off.as<unsigned long long>(), NZ.as<uint32_t>());
     prof_end();
-    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 8.0 * n + 4.0 * n;
+    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 4.0 * n;
     PSC_HIP(hipGetLastError());
     // the per-graph starts travel to pinned memory while the first reach
     // runs (its kernels take the counts from the device); the reach's
@@ -2283,33 +2323,37 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   const uint64_t nnz = h_nstart[G], nz = n - nnz;
   nnz_total = nnz;
   st->n_trimmed = nz;
-  st->bytes_label += 8 * n;
-  st->bytes_csr += 32 * n;
+  st->bytes_csr += 16 * n;
   // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1
   std::vector<uint64_t> label_off(G);
   for (uint32_t g = 0; g < G; ++g) label_off[g] = h_zstart[g + 1] - h_zstart[g] + 1;
   uint64_t n_rem = 0;
   std::vector<uint64_t> h_cstart(G + 1, 0);
-  if (nnz) {
+  if (n) {
     PSC_TRY(upload(gC, label_off.data(), G, 5));
-    // first-SCC labels and the remaining vertices (src/scc.hpp:340-349) in
-    // pivot order: placed by P position in one pass over NZ, then the
-    // position bitmap is compacted in order
-    const uint64_t pwords = (nnz + 31) / 32;
+    // every label (trimmed ranks, first-SCC labels) in one vertex-order
+    // pass, and the remaining vertices placed by P position; the position
+    // bitmap is then compacted in order
+    const uint64_t pwords = (std::max<uint64_t>(nnz, 1) + 31) / 32;
     PSC_TRY(pbits.ensure(pwords * 4));
     uint32_t *posv = front.as<uint32_t>();   // front is free until the rounds
+    const uint64_t nt = (n + SEL_TILE - 1) / SEL_TILE;
     prof_begin(KK_MISC);
     k_fill_u32<<<grid_for(pwords, BLOCK, 4096), BLOCK, 0, stream>>>(pbits.as<uint32_t>(),
                                                                    pwords, 0u);
-    k_first_mark<<<grid_for(nnz, BLOCK, num_cu * 8), BLOCK, 0, stream>>>(
-        NZ.as<uint32_t>(), nnz, voffd, G, nstart, p.seed, vis[0].as<uint32_t>(),
-        vis[1].as<uint32_t>(), lab, gC.as<unsigned long long>(), posv,
-        pbits.as<uint32_t>());
+    k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
+        trim_bm.as<uint32_t>(), n, tile_off.as<unsigned long long>(), voffd, G, zstart,
+        nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
+        gC.as<unsigned long long>(), posv, pbits.as<uint32_t>());
     prof_end();
     PSC_HIP(hipGetLastError());
-    kbytes[KK_MISC] += 4.0 * nnz + nnz / 4.0 + 8.0 * nnz + nnz / 8.0;
+    kbytes[KK_MISC] += 8.0 * n + 3 * n / 8.0 + (nnz + 31) / 8.0;
+    st->bytes_label += 8 * n;
+  }
+  if (nnz) {
     // the remaining count stays on the device: the per-graph starts
     // (cstart[G] = the count) come back in one poll
+    uint32_t *posv = front.as<uint32_t>();
     unsigned long long *rem_d = misc.as<unsigned long long>();
     PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
                    nullptr, nullptr, rem_d));
@@ -2321,7 +2365,6 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(poll(h_cstart.data(), gB.ptr, 8 * (G + 1)));
     n_rem = h_cstart[G];
     kbytes[KK_SELECT] += 2 * nnz / 8.0 + 8.0 * n_rem;
-    st->bytes_label += 8 * nnz + nnz / 4;
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
   for (uint32_t g = 0; g < G; ++g) label_off[g]++;
