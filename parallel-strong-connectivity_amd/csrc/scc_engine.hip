@@ -1279,6 +1279,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
   MSOut out;
   out.log.init(olog_s[w]);
   out.q.init(oq_s[w]);
+  uint32_t acc0[5] = {}, acc1[5] = {};   // per-thread counters of search 0 / 1
   for (int s = 0; s < ns; ++s) {
     if (!C->active[s]) continue;
     MSSlot *cur = &C->slot[s][ci];
@@ -1396,12 +1397,21 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
       wbud -= budget0 - budget;
     }
     out.flush(M, C, s, qn, nxt);
-    add_cnt(&C->cnt[s][M_EDGES], n_edges);
-    add_cnt(&C->cnt[s][M_EXP], n_exp);
-    add_cnt(&C->cnt[s][M_INS], n_ins);
-    add_cnt(&C->cnt[s][M_APP], n_app);
-    add_cnt(&C->cnt[s][M_PROBES], (unsigned long long)probes);
+    const uint32_t v5[5] = {(uint32_t)n_edges, (uint32_t)n_exp, (uint32_t)n_ins,
+                            (uint32_t)n_app, probes};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (s) acc1[k] += v5[k]; else acc0[k] += v5[k];
+    }
   }
+  // counters: one atomic per block and counter (every thread gets here)
+  unsigned long long *const cp[10] = {
+      &C->cnt[0][M_EDGES], &C->cnt[0][M_EXP], &C->cnt[0][M_INS], &C->cnt[0][M_APP],
+      &C->cnt[0][M_PROBES], &C->cnt[1][M_EDGES], &C->cnt[1][M_EXP], &C->cnt[1][M_INS],
+      &C->cnt[1][M_APP], &C->cnt[1][M_PROBES]};
+  const unsigned long long cv[10] = {acc0[0], acc0[1], acc0[2], acc0[3], acc0[4],
+                                     acc1[0], acc1[1], acc1[2], acc1[3], acc1[4]};
+  block_add<10>(cp, cv);
 }
 
 // Heavy pairs of this sub-round: CHUNK-edge items, a contiguous range per
@@ -1418,6 +1428,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
   MSOut out;
   out.log.init(olog_s[threadIdx.x / WAVE]);
   out.q.init(oq_s[threadIdx.x / WAVE]);
+  uint32_t acc0[5] = {}, acc1[5] = {};   // per-thread counters of search 0 / 1
   for (int s = 0; s < ns; ++s) {
     if (!C->active[s]) continue;
     MSSlot *cur = &C->slot[s][ci];
@@ -1472,12 +1483,22 @@ __global__ __launch_bounds__(BLOCK) void k_ms_heavy(
       }
     }
     out.flush(M, C, s, qn, nxt);
-    add_cnt(&C->cnt[s][M_HEAVY_EDGES], n_edges);
-    add_cnt(&C->cnt[s][M_HEAVY_INS], n_ins);
-    add_cnt(&C->cnt[s][M_HEAVY_APP], n_app);
-    add_cnt(&C->cnt[s][M_HEAVY_PROBES], (unsigned long long)probes);
-    add_cnt(&C->cnt[s][M_HEAVY_ITEMS], n_items);
+    const uint32_t v5[5] = {(uint32_t)n_edges, (uint32_t)n_ins, (uint32_t)n_app, probes,
+                            (uint32_t)n_items};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (s) acc1[k] += v5[k]; else acc0[k] += v5[k];
+    }
   }
+  // counters: one atomic per block and counter (every thread gets here)
+  unsigned long long *const cp[10] = {
+      &C->cnt[0][M_HEAVY_EDGES], &C->cnt[0][M_HEAVY_INS], &C->cnt[0][M_HEAVY_APP],
+      &C->cnt[0][M_HEAVY_PROBES], &C->cnt[0][M_HEAVY_ITEMS], &C->cnt[1][M_HEAVY_EDGES],
+      &C->cnt[1][M_HEAVY_INS], &C->cnt[1][M_HEAVY_APP], &C->cnt[1][M_HEAVY_PROBES],
+      &C->cnt[1][M_HEAVY_ITEMS]};
+  const unsigned long long cv[10] = {acc0[0], acc0[1], acc0[2], acc0[3], acc0[4],
+                                     acc1[0], acc1[1], acc1[2], acc1[3], acc1[4]};
+  block_add<10>(cp, cv);
 }
 
 // Partition (src/scc.hpp:437-464): pairs of the smaller table get
@@ -1512,8 +1533,9 @@ __global__ __launch_bounds__(BLOCK) void k_partition(
     unsigned long long *dst = &labels[pair_vertex(pr)];
     if (*dst < val) atomicMax(dst, val);
   }
-  unsigned long long pb = wave_sum((unsigned long long)probes);
-  if (lane() == 0 && pb) atomicAdd(probe_ctr, pb);
+  unsigned long long *const cp[1] = {probe_ctr};
+  const unsigned long long cv[1] = {probes};
+  block_add<1>(cp, cv);   // one atomic per block
 }
 
 // -------------------------------------------------------- finalize, status
@@ -1972,7 +1994,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
     k_rr_plab<<<grid_for(rr->A), BLOCK, 0, stream>>>(
         rr->front, rr->nfront, gC.as<unsigned long long>(), rr->A, rr->wgid,
         labels.as<unsigned long long>(), rr_plab.as<unsigned long long>(), S);
-    k_rr_include<<<grid_for(rr->n_rem, BLOCK, 16384), BLOCK, 0, stream>>>(
+    // few blocks: each adds its count to both searches' nnz with one atomic
+    k_rr_include<<<grid_for(rr->n_rem, BLOCK, 1024), BLOCK, 0, stream>>>(
         centers.as<uint32_t>(), rr->n_rem, gB.as<unsigned long long>(), G,
         rr_plab.as<unsigned long long>(), labels.as<unsigned long long>(),
         trim_bm.as<uint32_t>(), S);
@@ -2452,7 +2475,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
       PSC_TRY(multi_search(p, V, A, d_wgid, slots, st));
       const int small = (slots[0] <= slots[1]) ? 0 : 1;
       prof_begin(KK_PARTITION);
-      k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 16384), BLOCK, 0, stream>>>(
+      k_partition<<<grid_for(log_n[0] + log_n[1], BLOCK, 2048), BLOCK, 0, stream>>>(
           ms_desc.as<MSSearch>(), ms_ctl.as<MSCtl>(), small, lab,
           misc.as<unsigned long long>() + 2, gC.as<unsigned long long>(), A, d_wgid,
           gD.as<unsigned long long>());
