@@ -79,7 +79,8 @@ __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
                                const unsigned long long *labels,
                                const uint64_t *off, const uint32_t *adj,
                                const uint64_t *ioff, const uint32_t *iadj,
-                               uint32_t *cnt_out, uint32_t *cnt_in) {
+                               uint32_t *cnt_out, uint32_t *cnt_in, uint32_t *lid,
+                               unsigned long long *tlab) {
   const int l = lane();
   if (blockIdx.x == 0 && threadIdx.x == 0) {   // scan sentinels
     cnt_out[nrem] = 0;
@@ -89,6 +90,10 @@ __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
        j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
     const uint32_t v = centers[j];
     const unsigned long long lv = labels[v];
+    if (l == 0) {
+      lid[v] = (uint32_t)j;
+      tlab[j] = lv;
+    }
     for (int dir = 0; dir < 2; ++dir) {
       const uint64_t *o = dir ? ioff : off;
       const uint32_t *a = dir ? iadj : adj;
@@ -104,17 +109,6 @@ __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
       }
       if (l == 0) (dir ? cnt_in : cnt_out)[j] = c;
     }
-  }
-}
-
-__global__ void k_tail_lid(const uint32_t *centers, uint64_t nrem,
-                           const unsigned long long *labels, uint32_t *lid,
-                           unsigned long long *tlab) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < nrem;
-       j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = centers[j];
-    lid[v] = (uint32_t)j;
-    tlab[j] = labels[v];
   }
 }
 
@@ -720,23 +714,20 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   uint32_t *cnt_out = tail_cnt.as<uint32_t>(), *cnt_in = cnt_out + (n_rem + 1);
   uint32_t *lid = scratch32.as<uint32_t>();
   prof_begin(KK_TAIL);
-  k_tail_lid<<<blocks(n_rem), 256, 0, stream>>>(cen, n_rem, lab, lid,
-                                                tail_lab.as<unsigned long long>());
   k_tail_degrees<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
-      cen, n_rem, lab, d_off, d_adj, d_ioff, d_iadj, cnt_out, cnt_in);
+      cen, n_rem, lab, d_off, d_adj, d_ioff, d_iadj, cnt_out, cnt_in, lid,
+      tail_lab.as<unsigned long long>());
+  // one scan over both degree arrays (contiguous, each with a 0 sentinel):
+  // out-edges at [toff[j], toff[j+1]), in-edges at [tioff[j], tioff[j+1])
+  // of the same compact edge array
   PSC_TRY(tail_off.ensure(2 * (n_rem + 1) * 4));
   uint32_t *toff = tail_off.as<uint32_t>(), *tioff = toff + (n_rem + 1);
-  size_t tb = 0, tb2 = 0;
-  PSC_HIP(rocprim::exclusive_scan(nullptr, tb, cnt_out, toff, 0u, n_rem + 1,
+  size_t tb = 0;
+  PSC_HIP(rocprim::exclusive_scan(nullptr, tb, cnt_out, toff, 0u, 2 * (n_rem + 1),
                                   rocprim::plus<uint32_t>(), stream));
-  PSC_HIP(rocprim::exclusive_scan(nullptr, tb2, cnt_in, tioff, 0u, n_rem + 1,
-                                  rocprim::plus<uint32_t>(), stream));
-  PSC_TRY(sort_tmp.ensure(std::max<size_t>(std::max(tb, tb2), 8)));
+  PSC_TRY(sort_tmp.ensure(std::max<size_t>(tb, 8)));
   tb = sort_tmp.cap;
-  PSC_HIP(rocprim::exclusive_scan(sort_tmp.ptr, tb, cnt_out, toff, 0u, n_rem + 1,
-                                  rocprim::plus<uint32_t>(), stream));
-  tb = sort_tmp.cap;
-  PSC_HIP(rocprim::exclusive_scan(sort_tmp.ptr, tb, cnt_in, tioff, 0u, n_rem + 1,
+  PSC_HIP(rocprim::exclusive_scan(sort_tmp.ptr, tb, cnt_out, toff, 0u, 2 * (n_rem + 1),
                                   rocprim::plus<uint32_t>(), stream));
   prof_end();
   // per-graph compact edge counts size the pair tables (one poll)
@@ -755,9 +746,13 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   }
   if (tgs.empty()) return PSC_OK;
   const size_t nt = tgs.size();
-  const uint64_t e_out = h_e[G], e_in = h_e[G + 1];
-  PSC_TRY(tail_adj.ensure((e_out + e_in + 2) * 4));
-  uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj + e_out + 1;
+  // out- and in-edges of the compact copy: the same edges, so twice the
+  // out-edges unless the 32-bit offsets wrapped (then the host-driven
+  // rounds take the remainders)
+  const uint64_t e_all = h_e[G + 1];
+  if (h_e[G] >= (1ull << 31) || e_all != 2 * h_e[G]) return PSC_OK;
+  PSC_TRY(tail_adj.ensure((e_all + 2) * 4));
+  uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj;
   PSC_TRY(tail_spec.ensure(((n_rem + 31) / 32 + 1) * 4));
   prof_begin(KK_TAIL);
   k_tail_fill<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
@@ -825,6 +820,11 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     a.tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
     a.vgc = p.local_scc;
     k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
+    // labels as of every completed round, enqueued before the poll (a
+    // relaunch continues from the compact labels; the host rounds and the
+    // finalize read these)
+    k_tail_writeback<<<blocks(n_rem), 256, 0, stream>>>(
+        cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
     prof_end();
     PSC_HIP(hipGetLastError());
     // the ns state slots and the two statistics counters are contiguous:
@@ -873,15 +873,10 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     }
     pending.swap(again);
   }
-  prof_begin(KK_TAIL);
-  k_tail_writeback<<<blocks(n_rem), 256, 0, stream>>>(
-      cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
-  prof_end();
-  PSC_HIP(hipGetLastError());
   st->rounds += *std::max_element(rounds.begin(), rounds.end());
   st->table_inserts += hst[0];
   st->edges_scanned += hst[1];
-  st->bytes_csr += 8 * (e_out + e_in) + 8 * hst[1];
+  st->bytes_csr += 8 * e_all + 8 * hst[1];
   st->bytes_label += 16 * hst[1];
   st->bytes_table += 16 * hst[0];
   *used = true;
