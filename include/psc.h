@@ -121,11 +121,15 @@ typedef struct psc_handle psc_handle;
 void psc_params_default(psc_params *p);
 
 /* Upload both CSRs once and allocate the device working set
- * (SCC::SCC, src/scc.hpp:81-92).  Host arrays are only read. */
+ * (SCC::SCC, src/scc.hpp:81-92).  Host arrays are only read.  The set-up
+ * also gathers each vertex's first in- and out-neighbour (read by the pull
+ * levels). */
 int psc_create(const psc_graph *g, psc_handle **out);
 
 /* Same, for a graph whose four arrays already live in device memory of the
- * current HIP device; the handle borrows them (caller keeps them alive). */
+ * current HIP device; the handle borrows them (caller keeps them alive, and
+ * unchanged while the handle is bound to them: the gathered neighbours
+ * would go stale; re-bind with psc_batch_rebind or re-create instead). */
 int psc_create_device(const psc_graph *dev_g, psc_handle **out);
 
 /* SCC::scc (src/scc.hpp:277-479): labels_out[n] on the host receives

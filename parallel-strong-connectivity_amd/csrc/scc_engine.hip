@@ -744,6 +744,17 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
   }
 }
 
+// head[v] = first entry of v's list (v itself for an empty list).
+__global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ adj,
+                                                      uint64_t n, uint32_t *head) {
+  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t b = off[v];
+    head[v] = off[v + 1] > b ? adj[b] : (uint32_t)v;
+  }
+}
+
 // Pull level of both searches (src/reach.hpp:173-190): every unvisited,
 // non-trimmed vertex scans its in-edges until it meets a visited vertex;
 // testing "visited" instead of "in the previous frontier" keeps the
@@ -753,8 +764,10 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
 // The vertex range is walked by bitmap: a wave loads 64 words of
 // (visited | trimmed) with one coalesced load (2048 vertices), lists the
 // chunk's candidates (clear bits) in LDS, and takes them PULL_PAIRS x 64 at
-// a time with all their loads in flight together: in-offsets, then the
-// first PULL_PROBE in-edges, then their visited words.  Found vertices set
+// a time with all their loads in flight together: the first in-neighbour
+// (from the per-vertex head array, coalesced) and the in-offsets, then the
+// head's visited word; only vertices whose head is unvisited go on to load
+// the next PULL_PROBE in-edges and their visited words.  Found vertices set
 // their bits in an LDS copy of the chunk's words, flushed with one atomic
 // per word.  Vertices left undecided
 // queue their remaining in-edges in LDS; a full batch of 64 queued
@@ -887,22 +900,31 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       wave_sync();
       for (uint32_t cb = 0; cb < total; cb += WAVE * PULL_PAIRS) {
         uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
-        uint32_t co[PULL_PAIRS];
-        bool todo[PULL_PAIRS];
+        uint32_t co[PULL_PAIRS], hd[PULL_PAIRS], hw[PULL_PAIRS];
+        bool todo[PULL_PAIRS], hf[PULL_PAIRS];
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
           const uint32_t idx = cb + j * WAVE + l;
           todo[j] = idx < total;
           co[j] = todo[j] ? cand[idx] : 0u;
           v[j] = c * (uint64_t)(WAVE * 32) + co[j];
+          hd[j] = todo[j] ? R.rhead[v[j]] : 0u;
           b[j] = todo[j] ? R.roff[v[j]] : 0ull;
           deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
         }
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
         uint32_t nb[PULL_PAIRS][PULL_PROBE];
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
           deg[j] -= b[j];
           n_scan += todo[j];
+          // the head is in-edge 0; the probes go on from in-edge 1
+          const bool has = deg[j] > 0;
+          hf[j] = has && ((hw[j] >> (hd[j] & 31)) & 1u);
+          n_edges += has;
+          b[j] += has;
+          deg[j] = hf[j] ? 0ull : deg[j] - has;
 #pragma unroll
           for (uint32_t t = 0; t < PULL_PROBE; ++t)
             nb[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
@@ -915,7 +937,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             wd[j][t] = t < deg[j] ? R.vis[nb[j][t] >> 5] : 0u;
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
-          bool found = false;
+          bool found = hf[j];
 #pragma unroll
           for (uint32_t t = 0; t < PULL_PROBE; ++t)
             found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
@@ -1004,7 +1026,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const bool pull = reach_mode(cur, R) == MODE_PULL;
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
-    const uint64_t skip = pull ? PULL_PROBE : 0;
+    // a pull remainder starts after the head and the PULL_PROBE probed edges
+    const uint64_t skip = pull ? PULL_PROBE + 1 : 0;
     unsigned long long n_edges = 0, n_vis = 0, n_app = 0, n_items = 0, dsum = 0;
     const uint64_t per = (Ct + total_waves - 1) / total_waves;
     uint64_t c = gw * per;
@@ -1685,7 +1708,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &rr_plab, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &rhead[0], &rhead[1], &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta, &tail_spec};
 }
 
@@ -1774,6 +1797,17 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   PSC_TRY(gid_d.ensure((G + 1) * 4));
   PSC_TRY(gpin(G + 2));
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
+  // per-graph state read by every pull level: the first neighbour of each
+  // vertex in the pull direction, gathered once here so that a pull probe of
+  // it is one coalesced 4-byte load instead of an offset pair and a line of
+  // the adjacency array
+  for (int s = 0; s < 2; ++s) {
+    PSC_TRY(rhead[s].ensure(nn * 4));
+    const uint64_t blocks = std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 65536);
+    k_pull_heads<<<(unsigned)blocks, BLOCK, 0, stream>>>(
+        s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, n, rhead[s].as<uint32_t>());
+    PSC_HIP(hipGetLastError());
+  }
 
   ReachSearch rs[2];
   for (int s = 0; s < 2; ++s) {
@@ -1781,6 +1815,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     rs[s].adj = s == 0 ? d_adj : d_iadj;
     rs[s].roff = s == 0 ? d_ioff : d_off;
     rs[s].radj = s == 0 ? d_iadj : d_adj;
+    rs[s].rhead = rhead[s].as<uint32_t>();
     rs[s].vis = vis[s].as<uint32_t>();
     rs[s].trim = trim_bm.as<uint32_t>();
     rs[s].q[0] = rq[s][0].as<uint32_t>();

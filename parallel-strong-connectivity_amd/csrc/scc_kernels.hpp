@@ -47,6 +47,7 @@ struct ReachSearch {
   const uint32_t *adj;
   const uint64_t *roff;  // pull direction CSR (transpose)
   const uint32_t *radj;
+  const uint32_t *rhead; // radj[roff[v]] per vertex (v itself if no in-edge)
   uint32_t *vis;         // visited bitmap, ceil(n/32) words
   const uint32_t *trim;  // trimmed-vertex bitmap (skipped by pull levels)
   const uint32_t *excl;  // vertices outside the searched subproblems (push
