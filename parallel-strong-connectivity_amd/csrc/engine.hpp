@@ -3,6 +3,7 @@
 // pivot order, label tables, logs and pair frontiers.  Buffers grow on
 // demand and are reused across runs.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -86,6 +87,7 @@ struct Engine {
   DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
   DevBuf rhead[2];   // first pull-direction neighbour of each vertex, per search
   DevBuf NZ, centers, front, tiles, tile_off;
+  DevBuf trim_off;   // exclusive non-trimmed count per tile (kept past selects)
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
@@ -145,7 +147,8 @@ struct Engine {
   int select(F f, uint64_t len, typename F::T *sel, typename F::T *rej,
              uint64_t *count, unsigned long long *count_dev = nullptr);
   int poll(void *dst, const void *src, size_t bytes);
-  int scan_tiles(uint64_t nt, unsigned long long *total);
+  int scan_tiles(uint64_t nt, unsigned long long *total,
+                 unsigned long long *out = nullptr);
   // copy the n labels to host memory on this handle's stream
   int download_labels(uint64_t *dst);
   int prof_reset(int level);
@@ -161,10 +164,20 @@ struct Engine {
     uint64_t n_rem;                    // remainder positions
     uint64_t reached[2];               // out: vertices reached per direction
   };
-  int first_reach(const psc_params &p, psc_stats *st, ReachRound *rr = nullptr);
+  // `after_levels` (may be empty) enqueues work behind a batch of levels,
+  // before the poll that tells whether the reach has ended, once the batch
+  // reaches the previous run's level count; *after_ran tells whether it ran
+  // behind the last batch
+  int first_reach(const psc_params &p, psc_stats *st, ReachRound *rr = nullptr,
+                  const std::function<int()> &after_levels = {},
+                  bool *after_ran = nullptr);
   int round_reach(const psc_params &p, uint32_t A, const uint32_t *wgid_d,
                   uint64_t n_rem, psc_stats *st);
   int ms_prepare(int s, uint64_t slots, MSSearch *d);
+  // labels -> (label & VAL_MASK) - 1, skipped on the device while `go`
+  // (may be null) reads zero
+  int finalize_labels(const unsigned long long *go);
+  bool finalized = false;   // this run's finalize is already enqueued
   int ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_rem,
               std::vector<uint64_t> &cur, std::vector<uint64_t> &stepg, psc_stats *st,
               bool *used);

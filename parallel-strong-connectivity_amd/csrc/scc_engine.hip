@@ -1562,7 +1562,11 @@ __global__ __launch_bounds__(BLOCK) void k_partition(
 }
 
 // -------------------------------------------------------- finalize, status
-__global__ void k_finalize(unsigned long long *labels, uint64_t n) {
+// `go` (may be null): skip unless *go is nonzero (a finalize enqueued
+// before the host knows that every graph is finished).
+__global__ void k_finalize(unsigned long long *labels, uint64_t n,
+                           const unsigned long long *go) {
+  if (go && *go == 0) return;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     labels[i] = (labels[i] & VAL_MASK) - 1;   // src/scc.hpp:475-476
@@ -1704,7 +1708,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
   return {&labels, &vis[0], &vis[1], &rq[0][0], &rq[0][1], &rq[1][0],
           &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &pbits,
           &sort_tmp, &centers,
-          &front, &trim_bm, &tiles, &tile_off, &reach_desc, &reach_ctl, &ms_desc,
+          &front, &trim_bm, &tiles, &tile_off, &trim_off, &reach_desc, &reach_ctl, &ms_desc,
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
@@ -1786,6 +1790,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   PSC_TRY(tiles.ensure(ntiles * 4));
   PSC_TRY(trim_bm.ensure(words * 4));
   PSC_TRY(tile_off.ensure(ntiles * 8));
+  PSC_TRY(trim_off.ensure(ntiles * 8));
   PSC_TRY(reach_desc.ensure(sizeof(ReachSearch) * 2));
   PSC_TRY(reach_ctl.ensure(sizeof(ReachCtl)));
   PSC_TRY(ms_desc.ensure(sizeof(MSSearch) * 2));
@@ -1861,10 +1866,10 @@ __global__ void k_tile_total(const uint32_t *cnt, const unsigned long long *off,
 
 // Exclusive scan of the per-tile counts (tiles -> tile_off) and their
 // total (-> *total), on the device.
-int Engine::scan_tiles(uint64_t nt, unsigned long long *total) {
+int Engine::scan_tiles(uint64_t nt, unsigned long long *total, unsigned long long *out) {
   size_t tb = 0;
   const uint32_t *in = tiles.as<uint32_t>();
-  unsigned long long *out = tile_off.as<unsigned long long>();
+  if (!out) out = tile_off.as<unsigned long long>();
   PSC_HIP(rocprim::exclusive_scan(nullptr, tb, in, out, 0ull, nt,
                                   rocprim::plus<unsigned long long>(), stream));
   PSC_TRY(sort_tmp.ensure(std::max<size_t>(tb, 8)));
@@ -1990,7 +1995,9 @@ int Engine::poll(void *dst, const void *src, size_t bytes) {
 // Levels are enqueued in batches of K without host round trips; kernels of
 // levels past the end exit at once.  The host reads the done flags after
 // each batch.
-int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
+int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
+                        const std::function<int()> &after_levels, bool *after_ran) {
+  if (after_ran) *after_ran = false;
   ReachSearch rs[2];
   static_assert(2 * sizeof(ReachSearch) <= sizeof(h_reach), "reach descriptors");
   std::memcpy(rs, h_reach, sizeof(rs));
@@ -2044,6 +2051,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
   uint32_t L = 0;
   // the previous run's level count (same graph: one poll, no idle levels)
   uint32_t &hint = rr ? round_levels_hint : reach_levels_hint;
+  const uint32_t hint0 = hint;
   int K = hint ? (int)std::min<uint32_t>(hint + 1, 64) : 4;
   // timing >= 3: one level per poll and a per-level report on stderr
   const bool trace = p.timing >= 3 && !rr;
@@ -2077,6 +2085,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
     }
     L += (uint32_t)K;
     PSC_HIP(hipGetLastError());
+    const bool ran = after_levels && !trace && hint0 && L >= hint0 + 1;
+    if (ran) PSC_TRY(after_levels());
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
     if (trace) {
       const auto now = std::chrono::steady_clock::now();
@@ -2105,7 +2115,10 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr) {
       }
       prev_ = hc_;
     }
-    if (hc_.done[0] && hc_.done[1]) break;
+    if (hc_.done[0] && hc_.done[1]) {
+      if (after_ran) *after_ran = ran;
+      break;
+    }
     if (!trace) K = std::min(K * 2, 64);
   }
   if (!rr) {
@@ -2325,9 +2338,16 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
   return PSC_OK;
 }
 
+int Engine::finalize_labels(const unsigned long long *go) {
+  k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
+      labels.as<unsigned long long>(), n, go);
+  return PSC_OK;
+}
+
 int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   *st = psc_stats{};
   syncs = 0;
+  finalized = false;
   PSC_TRY(prof_reset(p.timing));
   unsigned long long *lab = labels.as<unsigned long long>();
   unsigned long long *voffd = voff_d.as<unsigned long long>();
@@ -2349,49 +2369,60 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
                                                     vis[0].as<uint32_t>(),
                                                     vis[1].as<uint32_t>(),
                                                     trim_bm.as<uint32_t>());
-    PSC_TRY(scan_tiles(nt, tot));
+    PSC_TRY(scan_tiles(nt, tot, trim_off.as<unsigned long long>()));
     k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
-        d_off, d_ioff, n, voffd, G, tile_off.as<unsigned long long>(), tot, nstart,
+        d_off, d_ioff, n, voffd, G, trim_off.as<unsigned long long>(), tot, nstart,
         zstart);
     k_trim_scatter<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        trim_bm.as<uint32_t>(), n, tile_off.as<unsigned long long>(), NZ.as<uint32_t>());
+        trim_bm.as<uint32_t>(), n, trim_off.as<unsigned long long>(), NZ.as<uint32_t>());
     prof_end();
     kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 4.0 * n;
     PSC_HIP(hipGetLastError());
     // the per-graph starts travel to pinned memory while the first reach
-    // runs (its kernels take the counts from the device); the reach's
-    // host poll has them in place
+    // runs (its kernels take the counts from the device); the host waits
+    // for this copy only, not for the reach
     PSC_TRY(gpin(2 * (G + 1)));
     PSC_HIP(hipMemcpyAsync(region(2), nstart, 16 * (G + 1), hipMemcpyDeviceToHost,
                            stream));
+    PSC_HIP(hipEventRecord(ev[7], stream));
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
   const bool trace_first = p.timing >= 3;   // the level report wants nnz first
-  if (n && trace_first) {
-    PSC_HIP(hipStreamSynchronize(stream));
+  bool have_starts = false, counted = false;
+  auto read_starts = [&]() -> int {
+    if (have_starts) return PSC_OK;
+    PSC_HIP(hipEventSynchronize(ev[7]));
     std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
     nnz_total = h_starts[G];
-  }
-  if (n) {
-    // ---- first SCC of every graph (src/scc.hpp:316-355)
-    PSC_TRY(first_reach(p, st));
-    if (!trace_first) std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
-  }
+    have_starts = true;
+    return PSC_OK;
+  };
+  if (n && trace_first) PSC_TRY(read_starts());
   const uint64_t *h_nstart = h_starts.data(), *h_zstart = h_starts.data() + G + 1;
-  const uint64_t nnz = h_nstart[G], nz = n - nnz;
-  nnz_total = nnz;
-  st->n_trimmed = nz;
-  st->bytes_csr += 16 * n;
-  // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1
-  std::vector<uint64_t> label_off(G);
-  for (uint32_t g = 0; g < G; ++g) label_off[g] = h_zstart[g + 1] - h_zstart[g] + 1;
-  uint64_t n_rem = 0;
-  std::vector<uint64_t> h_cstart(G + 1, 0);
-  if (n) {
+  // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1,
+  // the rounds' labels start at nz_g + 2
+  std::vector<uint64_t> label_off(G), label_next(G);
+  auto set_label_offs = [&]() {
+    for (uint32_t g = 0; g < G; ++g) {
+      label_off[g] = h_zstart[g + 1] - h_zstart[g] + 1;
+      label_next[g] = label_off[g] + 1;
+    }
+  };
+  // The labelling after the first reach: every label (trimmed ranks,
+  // first-SCC and subproblem labels) in one vertex-order pass, the
+  // remaining vertices placed by pivot position, the position bitmap
+  // compacted in order, and the per-graph starts of the remaining vertices
+  // copied back.  It needs only the trim counts from the host, so it is
+  // enqueued behind the batch of reach levels that is expected to end the
+  // reach, before that batch's poll: the poll returns with it done.  If the
+  // reach went on, it is enqueued again after the last batch (it rewrites
+  // everything it writes; the trim's tile offsets live in their own buffer).
+  auto label_pass = [&]() -> int {
+    PSC_TRY(read_starts());
+    const uint64_t nnz = h_nstart[G];
+    set_label_offs();
     PSC_TRY(upload(gC, label_off.data(), G, 5));
-    // every label (trimmed ranks, first-SCC labels) in one vertex-order
-    // pass, and the remaining vertices placed by P position; the position
-    // bitmap is then compacted in order
+    PSC_TRY(upload(gD, label_next.data(), G, 4));
     const uint64_t pwords = (std::max<uint64_t>(nnz, 1) + 31) / 32;
     PSC_TRY(pbits.ensure(pwords * 4));
     uint32_t *posv = front.as<uint32_t>();   // front is free until the rounds
@@ -2400,33 +2431,59 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_fill_u32<<<grid_for(pwords, BLOCK, 4096), BLOCK, 0, stream>>>(pbits.as<uint32_t>(),
                                                                    pwords, 0u);
     k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        trim_bm.as<uint32_t>(), n, tile_off.as<unsigned long long>(), voffd, G, zstart,
+        trim_bm.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
         gC.as<unsigned long long>(), posv, pbits.as<uint32_t>());
     prof_end();
     PSC_HIP(hipGetLastError());
-    kbytes[KK_MISC] += 8.0 * n + 3 * n / 8.0 + (nnz + 31) / 8.0;
-    st->bytes_label += 8 * n;
+    if (nnz) {
+      // the remaining count stays on the device: the per-graph starts
+      // (cstart[G] = the count) come back with the poll
+      unsigned long long *rem_d = misc.as<unsigned long long>();
+      PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
+                     nullptr, nullptr, rem_d));
+      prof_begin(KK_MISC);
+      k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
+          centers.as<uint32_t>(), 0, rem_d, voffd, G, nullptr, G,
+          gB.as<unsigned long long>());
+      prof_end();
+      PSC_HIP(hipMemcpyAsync(region(3), gB.ptr, 8 * (G + 1), hipMemcpyDeviceToHost,
+                             stream));
+    }
+    if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
+    if (!counted) {
+      kbytes[KK_MISC] += 8.0 * n + 3 * n / 8.0 + (nnz + 31) / 8.0;
+      kbytes[KK_SELECT] += 2 * nnz / 8.0;
+      st->bytes_label += 8 * n;
+      counted = true;
+    }
+    return PSC_OK;
+  };
+  if (n) {
+    // ---- first SCC of every graph (src/scc.hpp:316-355)
+    bool labelled = false;
+    PSC_TRY(first_reach(p, st, nullptr, label_pass, &labelled));
+    if (!labelled) {
+      PSC_TRY(label_pass());
+      PSC_HIP(hipStreamSynchronize(stream));
+      ++syncs;
+    }
+  } else {
+    set_label_offs();
+    PSC_TRY(upload(gD, label_next.data(), G, 4));
+    if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
   }
+  const uint64_t nnz = h_nstart[G], nz = n - nnz;
+  nnz_total = nnz;
+  st->n_trimmed = nz;
+  st->bytes_csr += 16 * n;
+  uint64_t n_rem = 0;
+  std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
-    // the remaining count stays on the device: the per-graph starts
-    // (cstart[G] = the count) come back in one poll
-    uint32_t *posv = front.as<uint32_t>();
-    unsigned long long *rem_d = misc.as<unsigned long long>();
-    PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
-                   nullptr, nullptr, rem_d));
-    prof_begin(KK_MISC);
-    k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
-        centers.as<uint32_t>(), 0, rem_d, voffd, G, nullptr, G,
-        gB.as<unsigned long long>());
-    prof_end();
-    PSC_TRY(poll(h_cstart.data(), gB.ptr, 8 * (G + 1)));
+    std::memcpy(h_cstart.data(), region(3), 8 * (G + 1));
     n_rem = h_cstart[G];
-    kbytes[KK_SELECT] += 2 * nnz / 8.0 + 8.0 * n_rem;
+    kbytes[KK_SELECT] += 8.0 * n_rem;
   }
-  if (p.timing) PSC_HIP(hipEventRecord(ev[2], stream));
-  for (uint32_t g = 0; g < G; ++g) label_off[g]++;
-  PSC_TRY(upload(gD, label_off.data(), G, 4));
   // ---- multi-pivot rounds (src/scc.hpp:369-473), every graph on its own
   // pivot-batch schedule; the rounds of all graphs advance together
   std::vector<uint64_t> cur(G, 0), stepg(G, 1);
@@ -2557,10 +2614,11 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
       }
     }
   }
-  if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
-  // ---- finalize (src/scc.hpp:475-476)
+  if (p.timing && !finalized) PSC_HIP(hipEventRecord(ev[3], stream));
+  // ---- finalize (src/scc.hpp:475-476), unless the remainder rounds'
+  // launch already enqueued it
   prof_begin(KK_MISC);
-  k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(lab, n);
+  if (!finalized) PSC_TRY(finalize_labels(nullptr));
   kbytes[KK_MISC] += 16.0 * n;
   if (p.canonical) {
     uint32_t *minv = scratch32.as<uint32_t>();

@@ -72,6 +72,16 @@ struct TailState {    // written back per graph
   unsigned long long cur, step, rounds, full, ins, edges, ticks;
 };
 
+// *go = 1 if every graph of the launch finished all its rounds (no table
+// overflow), else 0.  One wave.
+__global__ void k_tail_go(const TailState *ts, const TailGraph *tg, uint32_t nl,
+                          unsigned long long *go) {
+  bool ok = true;
+  for (uint32_t k = lane(); k < nl; k += WAVE) ok = ok && ts[k].cur == tg[k].nr && !ts[k].full;
+  const bool all = ballot(!ok) == 0;
+  if (threadIdx.x == 0) *go = all ? 1ull : 0ull;
+}
+
 // lid[centers[j]] = j, compact labels, and the compact degrees: edges to
 // vertices that are unfinished and carry the same label (one wave per
 // remaining vertex, 64-edge windows).
@@ -681,6 +691,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   TailState *d_ts = reinterpret_cast<TailState *>(d_tg + ns);
   unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(d_ts + ns);
   unsigned long long *d_e = d_stats + 2;
+  unsigned long long *d_go = d_stats + (2 * G + 4);   // in the 64 spare bytes
   if (count) {
     unsigned long long *d_u = d_e + (G + 2);
     PSC_TRY(tail_pin_ensure(ns * sizeof(TailGraph)));
@@ -827,6 +838,26 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
         cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
     prof_end();
     PSC_HIP(hipGetLastError());
+    // When every graph with rounds left is in this launch, the finalize
+    // goes behind it before the poll, skipped on the device unless all of
+    // them finished: the poll then returns with the labels final.
+    bool spec_fin = false;
+    {
+      std::vector<char> in_launch(G, 0);
+      for (const TailGraph &t : launch) in_launch[t.g] = 1;
+      bool all_in = true;
+      for (uint32_t g = 0; g < G && all_in; ++g)
+        if (!in_launch[g] && cur[g] < h_cstart[g + 1] - h_cstart[g]) all_in = false;
+      if (all_in) {
+        if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
+        prof_begin(KK_MISC);
+        k_tail_go<<<1, WAVE, 0, stream>>>(d_ts, d_tg, (uint32_t)nl, d_go);
+        PSC_TRY(finalize_labels(d_go));
+        prof_end();
+        PSC_HIP(hipGetLastError());
+        spec_fin = true;
+      }
+    }
     // the ns state slots and the two statistics counters are contiguous:
     // one poll (slots past this launch's nl are not read)
     std::vector<unsigned char> back(ns * sizeof(TailState) + 16);
@@ -853,6 +884,12 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
           again.push_back(i);
         }
       }
+    }
+    if (spec_fin) {   // the same test as k_tail_go's, on the polled states
+      bool all_done = true;
+      for (uint32_t g = 0; g < G && all_done; ++g)
+        if (cur[g] < h_cstart[g + 1] - h_cstart[g]) all_done = false;
+      finalized = all_done;
     }
     if (p.timing >= 3) {   // per-graph report, slowest first (wall clock 100 MHz)
       std::vector<size_t> ord(nl);
