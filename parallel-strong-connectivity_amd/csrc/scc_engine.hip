@@ -763,17 +763,18 @@ __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict
 // them as visited.
 // The vertex range is walked by bitmap: a wave loads 64 words of
 // (visited | trimmed) with one coalesced load (2048 vertices), lists the
-// chunk's candidates (clear bits) in LDS, and takes them PULL_PAIRS x 64 at
-// a time with all their loads in flight together: the first in-neighbour
-// (from the per-vertex head array, coalesced) and the in-offsets, then the
-// head's visited word; only vertices whose head is unvisited go on to load
-// the next PULL_PROBE in-edges and their visited words.  Found vertices set
+// chunk's candidates (clear bits) in LDS.  Every candidate first tests its
+// first in-neighbour (from the per-vertex head array, coalesced), PULL_HEADS
+// x 64 at a time; the misses are compacted in the list and then take
+// PULL_PAIRS x 64 at a time with their loads in flight together: in-offsets,
+// the next PULL_PROBE in-edges, their visited words.  Found vertices set
 // their bits in an LDS copy of the chunk's words, flushed with one atomic
 // per word.  Vertices left undecided
 // queue their remaining in-edges in LDS; a full batch of 64 queued
 // vertices is scanned as load-balanced 64-edge windows.  Remainders longer
 // than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 constexpr int PULL_PAIRS = 2;
+constexpr int PULL_HEADS = 8;   // head tests per lane in flight
 constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
 
 struct PullQueue {   // per-wave LDS queue of undecided vertices
@@ -898,22 +899,57 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       }
       fw[l] = 0;
       wave_sync();
-      for (uint32_t cb = 0; cb < total; cb += WAVE * PULL_PAIRS) {
-        uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
-        uint32_t co[PULL_PAIRS], hd[PULL_PAIRS], hw[PULL_PAIRS];
-        bool todo[PULL_PAIRS], hf[PULL_PAIRS];
+      const uint64_t vbase = c * (uint64_t)(WAVE * 32);
+      // phase 1: every candidate tests its head, PULL_HEADS per lane at a
+      // time (two dependent loads); the misses are compacted to the front of
+      // the candidate list (the entries a batch writes lie at or below the
+      // batch's own, which are already in registers)
+      uint32_t nmiss = 0;
+      for (uint32_t cb = 0; cb < total; cb += WAVE * PULL_HEADS) {
+        uint32_t co[PULL_HEADS], hd[PULL_HEADS], hw[PULL_HEADS];
+        bool todo[PULL_HEADS];
 #pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j) {
+        for (int j = 0; j < PULL_HEADS; ++j) {
           const uint32_t idx = cb + j * WAVE + l;
           todo[j] = idx < total;
           co[j] = todo[j] ? cand[idx] : 0u;
-          v[j] = c * (uint64_t)(WAVE * 32) + co[j];
-          hd[j] = todo[j] ? R.rhead[v[j]] : 0u;
+          hd[j] = todo[j] ? R.rhead[vbase + co[j]] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < PULL_HEADS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < PULL_HEADS; ++j) {
+          // v's own bit is clear, so an empty list's head (v) misses
+          const bool hit = todo[j] && ((hw[j] >> (hd[j] & 31)) & 1u);
+          const bool miss = todo[j] && !hit;
+          n_edges += todo[j];
+          const uint64_t mm = ballot(miss);
+          if (miss) cand[nmiss + lanes_below(mm)] = (uint16_t)co[j];
+          nmiss += (uint32_t)__popcll(mm);
+          // found bits gather in the chunk's LDS words, one global atomic
+          // per word at the end of the chunk
+          if (hit) atomicOr(&fw[co[j] >> 5], 1u << (co[j] & 31));
+          n_vis += hit;
+          out.put(hit, (uint32_t)(vbase + co[j]), &nxt->qsize, qn, R.n);
+        }
+      }
+      wave_sync();
+      // phase 2: the misses load their offsets and probe in-edges
+      // 1..PULL_PROBE, PULL_PAIRS per lane
+      for (uint32_t cb = 0; cb < nmiss; cb += WAVE * PULL_PAIRS) {
+        uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
+        uint32_t co[PULL_PAIRS];
+        bool todo[PULL_PAIRS];
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) {
+          const uint32_t idx = cb + j * WAVE + l;
+          todo[j] = idx < nmiss;
+          co[j] = todo[j] ? cand[idx] : 0u;
+          v[j] = vbase + co[j];
           b[j] = todo[j] ? R.roff[v[j]] : 0ull;
           deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
         }
-#pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
         uint32_t nb[PULL_PAIRS][PULL_PROBE];
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
@@ -921,10 +957,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           n_scan += todo[j];
           // the head is in-edge 0; the probes go on from in-edge 1
           const bool has = deg[j] > 0;
-          hf[j] = has && ((hw[j] >> (hd[j] & 31)) & 1u);
-          n_edges += has;
           b[j] += has;
-          deg[j] = hf[j] ? 0ull : deg[j] - has;
+          deg[j] -= has;
 #pragma unroll
           for (uint32_t t = 0; t < PULL_PROBE; ++t)
             nb[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
@@ -937,7 +971,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             wd[j][t] = t < deg[j] ? R.vis[nb[j][t] >> 5] : 0u;
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
-          bool found = hf[j];
+          bool found = false;
 #pragma unroll
           for (uint32_t t = 0; t < PULL_PROBE; ++t)
             found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
