@@ -538,7 +538,7 @@ __device__ __forceinline__ int reach_mode(const ReachSlot *c,
   const uint64_t push = (uint64_t)nf + ds;
   if (push >= R.thresh) return MODE_PULL;
   const uint64_t unv = R.nnz > c->cum_vis ? R.nnz - c->cum_vis : 0;
-  return (unv * PULL_VERTEX_COST + R.n / 2048 < push) ? MODE_PULL : MODE_PUSH;
+  return (unv * PULL_VERTEX_COST + R.n / PULL_WALK_DIV < push) ? MODE_PULL : MODE_PUSH;
 }
 
 // K counters summed over the block, one atomic per counter and block.
@@ -774,7 +774,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict
 // vertices is scanned as load-balanced 64-edge windows.  Remainders longer
 // than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 constexpr int PULL_PAIRS = 2;
-constexpr int PULL_HEADS = 8;   // head tests per lane in flight
+constexpr int PULL_HEADS = 4;   // head tests per lane in flight
 constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
 
 struct PullQueue {   // per-wave LDS queue of undecided vertices
@@ -2134,7 +2134,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
         const double push = (double)c.qsize + ds;
         const double unv = nnz_total > c.cum_vis ? (double)(nnz_total - c.cum_vis) : 0.0;
         const bool pull = push >= (double)thresh ||
-                          unv * PULL_VERTEX_COST + (double)(n / 2048) < push;
+                          unv * PULL_VERTEX_COST + (double)(n / PULL_WALK_DIV) < push;
         const char *mode = c.qsize == 0 ? "none" : (pull ? "pull" : "push");
         std::fprintf(stderr,
                      "level %u %s %s nf=%u degsum=%llu heavy=%llu/%llu next=%u "

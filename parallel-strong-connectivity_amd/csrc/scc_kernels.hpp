@@ -17,7 +17,7 @@ constexpr int OUT_CAP64 = 256;       // per-wave staged appends (pairs)
 constexpr uint32_t PULL_HEAVY = 64;  // pull remainder above which it is chunked
 constexpr uint32_t HEAVY_DEG = 512;  // degree above which edges are chunked
 constexpr uint32_t CHUNK = 256;     // edges per heavy work item
-constexpr uint32_t PULL_PROBE = 4;   // in-edges a pull lane tests first
+constexpr uint32_t PULL_PROBE = 8;   // in-edges a pull lane tests first
 constexpr uint32_t VGC_MAX_DEG = 8;  // only vertices this sparse stay local
 // A wave's local search also stops taking vertices once it has scanned
 // VGC_EDGE_MULT * tau edges per seed: a lone wave walking thousands of
@@ -36,6 +36,8 @@ constexpr uint32_t VGC_MAX_BATCHES = 64;
 // Pull cost estimate per unvisited non-trimmed vertex, in pushed-edge
 // units (measured: a pulled vertex costs about as much as a pushed edge).
 constexpr uint64_t PULL_VERTEX_COST = 1;
+// Fixed cost of a pull level's bitmap walk, in pushed-edge units: n / this.
+constexpr uint64_t PULL_WALK_DIV = 64;
 constexpr unsigned long long H_CNT_SHIFT = 40;
 constexpr unsigned long long H_CHUNK_MASK = (1ull << 40) - 1;
 
