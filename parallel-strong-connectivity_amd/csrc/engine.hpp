@@ -86,7 +86,7 @@ struct Engine {
   DevBuf g_off, g_adj, g_ioff, g_iadj;   // owned copies (psc_create)
   DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
   DevBuf rhead[2];   // first pull-direction neighbour of each vertex, per search
-  DevBuf NZ, centers, front, tiles, tile_off;
+  DevBuf centers, front, tiles, tile_off;
   DevBuf trim_off;   // exclusive non-trimmed count per tile (kept past selects)
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch

@@ -274,43 +274,6 @@ __global__ void k_graph_starts(const uint64_t *__restrict__ off,
   }
 }
 
-// Trim, pass 2: non-trimmed vertices go to NZ in vertex order
-// (src/scc.hpp:295-303).  The trimmed set is read from pass 1's bitmap
-// (n/8 bytes) instead of the degree arrays again (16n bytes).  Labels are
-// written once, after the first reach (k_first_label).
-__global__ __launch_bounds__(BLOCK) void k_trim_scatter(
-    const uint32_t *__restrict__ trim_bm, uint64_t n,
-    const unsigned long long *tile_off, uint32_t *NZ) {
-  __shared__ uint32_t wc[SEL_ITEMS][WPB];
-  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
-  const int w = threadIdx.x / WAVE;
-  bool p[SEL_ITEMS];
-  uint32_t rk[SEL_ITEMS];
-#pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    p[j] = i < n && !bit_test(trim_bm, (uint32_t)i);
-    const uint64_t m = ballot(p[j]);
-    rk[j] = lanes_below(m);
-    if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
-  }
-  __syncthreads();
-  uint64_t r0 = tile_off[blockIdx.x];
-#pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j) {
-    uint32_t before = 0, round_tot = 0;
-#pragma unroll
-    for (int x = 0; x < WPB; ++x) {
-      before += x < w ? wc[j][x] : 0u;
-      round_tot += wc[j][x];
-    }
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    if (p[j]) NZ[r0 + before + rk[j]] = (uint32_t)i;
-    r0 += round_tot;
-  }
-}
-
-
 // ------------------------------------------------------- per-graph helpers
 // A handle holds G graphs as one disjoint union: graph g owns the vertex id
 // range [voff[g], voff[g+1]).  Per-graph lists (trimmed vertices, pivot
@@ -366,10 +329,18 @@ __global__ void k_reach_reset(ReachCtl *C, int ns) {
 // Level 0: the first pivot P_g[0] of every graph with non-trimmed vertices
 // (src/scc.hpp:317); the graphs are disjoint, so one multi-source reach
 // gives each graph's single reach from its own pivot.
+// The first pivot of every graph, P_g[0] (src/scc.hpp:303,316): the
+// vertex of rank nstart[g] + perm_g(0) among the non-trimmed vertices in
+// vertex order, found by one wave per graph without listing them: a 64-way
+// search over the trim's tile prefix (tile_off, SEL_TILE vertices a tile)
+// picks the tile, a scan of its 64 bitmap words the word, then the bit.
+static_assert(SEL_TILE == 32 * WAVE, "one bitmap word per lane and tile");
 __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
-                                const uint32_t *NZ,
-                                const unsigned long long *nstart, uint32_t G,
-                                uint64_t seed, uint64_t m) {
+                                const uint32_t *__restrict__ trim_bm, uint64_t words,
+                                const unsigned long long *__restrict__ tile_off,
+                                uint64_t ntiles, const unsigned long long *nstart,
+                                uint32_t G, uint64_t seed, uint64_t m) {
+  const int l = lane();
   // the non-trimmed count (and the mean degree of a frontier vertex) of
   // the direction rule come from the trim on the device
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)ns) {
@@ -377,18 +348,44 @@ __global__ void k_reach_sources(ReachSearch *S, ReachCtl *C, int ns,
     S[threadIdx.x].nnz = nz;
     S[threadIdx.x].avg_deg = nz ? (double)m / (double)nz : 0.0;
   }
-  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G;
-       g += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint64_t g = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; g < G;
+       g += (uint64_t)gridDim.x * blockDim.x / WAVE) {
     const uint64_t b = nstart[g];
     if (nstart[g + 1] == b) continue;
-    const uint32_t src = NZ[b + PivotPerm(nstart[g + 1] - b, seed)(0)];   // P_g[0]
-    for (int s = 0; s < ns; ++s) {
-      atomicOr(&S[s].vis[src >> 5], 1u << (src & 31));
-      atomicAdd(&C->slot[s][0].cum_vis, 1ull);
-      const uint32_t idx = atomicAdd(&C->slot[s][0].qsize, 1u);
-      S[s].q[0][idx] = src;
-      atomicAdd(&C->slot[s][0].degsum,
-                (unsigned long long)(S[s].off[src + 1] - S[s].off[src]));
+    const uint64_t k = b + PivotPerm(nstart[g + 1] - b, seed)(0);   // its rank
+    // last tile t with tile_off[t] <= k (tile_off[0] = 0)
+    uint64_t lo = 0, hi = ntiles;
+    while (hi - lo > 1) {
+      const uint64_t step = (hi - lo + WAVE - 1) / WAVE;
+      const uint64_t p = lo + (uint64_t)l * step;
+      const uint64_t ok = ballot(p < hi && tile_off[p] <= k);   // lanes 0..x
+      const uint64_t nlo = lo + (uint64_t)(63 - __clzll((unsigned long long)ok)) * step;
+      hi = min(hi, nlo + step);
+      lo = nlo;
+    }
+    const uint64_t w = lo * WAVE + l;
+    const uint32_t bits = w < words ? ~trim_bm[w] : 0u;
+    const uint32_t cnt = (uint32_t)__popc(bits);
+    const uint64_t incl = wave_incl_scan((uint64_t)cnt);
+    const uint64_t r = k - tile_off[lo];   // rank inside the tile
+    // bits past n read as non-trimmed but rank after every vertex < n
+    const bool mine = incl - cnt <= r && r < incl;
+    uint32_t src = 0;
+    if (mine) {
+      uint32_t x = bits;
+      for (uint64_t t = incl - cnt; t < r; ++t) x &= x - 1;
+      src = (uint32_t)(w * 32 + (uint64_t)(__ffs(x) - 1));
+    }
+    src = __shfl(src, __ffsll((unsigned long long)ballot(mine)) - 1);
+    if (l == 0) {
+      for (int s = 0; s < ns; ++s) {
+        atomicOr(&S[s].vis[src >> 5], 1u << (src & 31));
+        atomicAdd(&C->slot[s][0].cum_vis, 1ull);
+        const uint32_t idx = atomicAdd(&C->slot[s][0].qsize, 1u);
+        S[s].q[0][idx] = src;
+        atomicAdd(&C->slot[s][0].degsum,
+                  (unsigned long long)(S[s].off[src + 1] - S[s].off[src]));
+      }
     }
   }
 }
@@ -1138,10 +1135,10 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
 // keeps 0 (src/scc.hpp:340-349).  A non-trimmed vertex not in the first
 // SCC remains: it is placed at its position in its graph's pivot order
 // P_g = NZ_g permuted (src/scc.hpp:303), pos = nstart[g] +
-// perm_g^-1(j - nstart[g]) with j its NZ index: posv[pos] = v and bit pos
-// of pbits set, so compacting pbits in order yields the remaining vertices
-// in P order without materialising or sorting P.  Tiles and ranks as in
-// k_trim_scatter.
+// perm_g^-1(j - nstart[g]) with j its rank among the non-trimmed vertices
+// (NZ index): posv[pos] = v and bit pos of pbits set, so compacting pbits
+// in order yields the remaining vertices in P order without materialising
+// or sorting P.  Ranks come from the trim's tile prefix and ballots.
 __global__ __launch_bounds__(BLOCK) void k_first_label(
     const uint32_t *__restrict__ trim_bm, uint64_t n, const unsigned long long *tile_off,
     const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
@@ -1740,7 +1737,7 @@ Engine::~Engine() {
 
 std::vector<DevBuf *> Engine::all_bufs() {
   return {&labels, &vis[0], &vis[1], &rq[0][0], &rq[0][1], &rq[1][0],
-          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &NZ, &pbits,
+          &rq[1][1], &hv[0], &hv[1], &hc[0], &hc[1], &pbits,
           &sort_tmp, &centers,
           &front, &trim_bm, &tiles, &tile_off, &trim_off, &reach_desc, &reach_ctl, &ms_desc,
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
@@ -1817,7 +1814,6 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     PSC_TRY(hv[s].ensure(hcap * 4));
     PSC_TRY(hc[s].ensure(hcap * 8));
   }
-  PSC_TRY(NZ.ensure(nn * 4));
   PSC_TRY(centers.ensure(nn * 4));
   PSC_TRY(front.ensure(nn * 4));
   const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
@@ -2052,8 +2048,11 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
   prof_begin(KK_MISC);
   k_reach_reset<<<1, 64, 0, stream>>>(C, 2);
   if (!rr) {
-    k_reach_sources<<<grid_for(G), BLOCK, 0, stream>>>(
-        S, C, 2, NZ.as<uint32_t>(), gA.as<unsigned long long>(), G, p.seed, m);
+    const uint64_t words = (std::max<uint64_t>(n, 1) + 31) / 32;
+    k_reach_sources<<<grid_for((uint64_t)G * WAVE), BLOCK, 0, stream>>>(
+        S, C, 2, trim_bm.as<uint32_t>(), words, trim_off.as<unsigned long long>(),
+        (std::max<uint64_t>(n, 1) + SEL_TILE - 1) / SEL_TILE, gA.as<unsigned long long>(),
+        G, p.seed, m);
   } else {
     // visited bitmaps empty, every vertex excluded but those carrying
     // their graph's pivot label (the exclusion bitmap is also the pull
@@ -2388,8 +2387,9 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   auto t_host0 = std::chrono::steady_clock::now();
   if (p.timing) PSC_HIP(hipEventRecord(ev[0], stream));
   // ---- init + trim, per graph (src/scc.hpp:284-306): one pass counts the
-  // non-trimmed vertices per tile (and writes the reach dead-end bitmaps),
-  // one writes NZ and every label
+  // non-trimmed vertices per tile and writes the trimmed and dead-end
+  // bitmaps; NZ (the non-trimmed vertices in order) is never listed: the
+  // first pivot is found by rank, every label is written after the reach
   unsigned long long *nstart = gA.as<unsigned long long>();
   unsigned long long *zstart = nstart + (G + 1);
   std::vector<uint64_t> h_starts(2 * (G + 1), 0);
@@ -2407,10 +2407,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
         d_off, d_ioff, n, voffd, G, trim_off.as<unsigned long long>(), tot, nstart,
         zstart);
-    k_trim_scatter<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        trim_bm.as<uint32_t>(), n, trim_off.as<unsigned long long>(), NZ.as<uint32_t>());
     prof_end();
-    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0 + n / 8.0 + 4.0 * n;
+    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0;
     PSC_HIP(hipGetLastError());
     // the per-graph starts travel to pinned memory while the first reach
     // runs (its kernels take the counts from the device); the host waits
