@@ -178,6 +178,11 @@ struct Engine {
   // (may be null) reads zero
   int finalize_labels(const unsigned long long *go);
   bool finalized = false;   // this run's finalize is already enqueued
+  // the first labelling wrote final values for trimmed and first-SCC
+  // vertices (all but one per graph): finalize covers the rest only
+  bool direct_final = false;
+  uint64_t final_rem = 0;   // remaining vertices (centers) of this run
+  DevBuf first_trim;        // per graph: its trimmed vertex of rank 0
   int ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_rem,
               std::vector<uint64_t> &cur, std::vector<uint64_t> &stepg, psc_stats *st,
               bool *used);

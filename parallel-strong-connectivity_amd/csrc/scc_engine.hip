@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
     const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
-    uint32_t *posv, uint32_t *pbits) {
+    uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
@@ -1192,12 +1192,19 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const uint32_t g = G == 1 ? 0u : seg_of(voff, G, i);
     if (!p[j]) {
       const unsigned long long zs = G == 1 ? z0 : zstart[g];
-      labels[i] = 1 + ((i - r - zs) | TOP_BIT);
+      const unsigned long long rank = i - r - zs;
+      if (direct && rank) {
+        labels[i] = rank;   // already final: (1 + rank) - 1
+      } else {
+        labels[i] = 1 + (rank | TOP_BIT);
+        if (direct) first_trim[g] = (uint32_t)i;   // the one final value 0
+      }
       continue;
     }
     const bool a = (fw[j] >> (i & 31)) & 1u, b = (bw[j] >> (i & 31)) & 1u;
     const unsigned long long label = G == 1 ? lab_one : lab1[g];
-    labels[i] = (a && b) ? (label | TOP_BIT) : ((a || b) ? label : 0ull);
+    const unsigned long long done = direct ? label - 1 : (label | TOP_BIT);
+    labels[i] = (a && b) ? done : ((a || b) ? label : 0ull);
     if (!(a && b)) place |= 1u << j;
   }
   // placement of the few remaining vertices, out of the unrolled loop
@@ -1602,6 +1609,27 @@ __global__ void k_finalize(unsigned long long *labels, uint64_t n,
        i += (uint64_t)gridDim.x * blockDim.x)
     labels[i] = (labels[i] & VAL_MASK) - 1;   // src/scc.hpp:475-476
 }
+// Finalize when the first labelling wrote final values directly: only the
+// remaining vertices (centers) and each graph's trimmed vertex of rank 0
+// still hold internal labels.
+__global__ void k_finalize_rest(unsigned long long *labels, const uint32_t *centers,
+                                uint64_t n_rem, const uint32_t *first_trim,
+                                const unsigned long long *zstart, uint32_t G,
+                                const unsigned long long *go) {
+  if (go && *go == 0) return;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_rem + G;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t v;
+    if (i < n_rem) {
+      v = centers[i];
+    } else {
+      const uint64_t g = i - n_rem;
+      if (zstart[g + 1] == zstart[g]) continue;   // no trimmed vertex
+      v = first_trim[g];
+    }
+    labels[v] = (labels[v] & VAL_MASK) - 1;
+  }
+}
 // Canonical labels: minimum local vertex id per (graph, label); labels
 // are in [0, n_g], so (voff[g] + g + label) indexes a table of n + G slots.
 __global__ void k_canon_min(const unsigned long long *labels, uint64_t n,
@@ -1743,7 +1771,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &rr_plab, &rhead[0], &rhead[1], &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta, &tail_spec};
 }
 
@@ -1830,6 +1858,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) PSC_TRY(b->ensure((G + 1) * 8));
   PSC_TRY(gA.ensure(2 * (G + 1) * 8));   // nstart, then zstart
   PSC_TRY(gid_d.ensure((G + 1) * 4));
+  PSC_TRY(first_trim.ensure((G + 1) * 4));
   PSC_TRY(gpin(G + 2));
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
   // per-graph state read by every pull level: the first neighbour of each
@@ -2372,8 +2401,14 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
 }
 
 int Engine::finalize_labels(const unsigned long long *go) {
-  k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
-      labels.as<unsigned long long>(), n, go);
+  if (direct_final) {
+    k_finalize_rest<<<grid_for(final_rem + G, BLOCK, 16384), BLOCK, 0, stream>>>(
+        labels.as<unsigned long long>(), centers.as<uint32_t>(), final_rem,
+        first_trim.as<uint32_t>(), gA.as<unsigned long long>() + (G + 1), G, go);
+  } else {
+    k_finalize<<<grid_for(n, BLOCK, 16384), BLOCK, 0, stream>>>(
+        labels.as<unsigned long long>(), n, go);
+  }
   return PSC_OK;
 }
 
@@ -2381,6 +2416,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   *st = psc_stats{};
   syncs = 0;
   finalized = false;
+  direct_final = false;
+  final_rem = 0;
   PSC_TRY(prof_reset(p.timing));
   unsigned long long *lab = labels.as<unsigned long long>();
   unsigned long long *voffd = voff_d.as<unsigned long long>();
@@ -2453,6 +2490,13 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(read_starts());
     const uint64_t nnz = h_nstart[G];
     set_label_offs();
+    // final values go straight into the labels unless a graph with
+    // non-trimmed vertices has no trimmed one (its first SCC's final value
+    // would be 0, which an unreached vertex's label also is)
+    direct_final = true;
+    for (uint32_t g = 0; g < G; ++g)
+      if (h_nstart[g + 1] > h_nstart[g] && h_zstart[g + 1] == h_zstart[g])
+        direct_final = false;
     PSC_TRY(upload(gC, label_off.data(), G, 5));
     PSC_TRY(upload(gD, label_next.data(), G, 4));
     const uint64_t pwords = (std::max<uint64_t>(nnz, 1) + 31) / 32;
@@ -2465,7 +2509,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
         trim_bm.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
-        gC.as<unsigned long long>(), posv, pbits.as<uint32_t>());
+        gC.as<unsigned long long>(), posv, pbits.as<uint32_t>(), direct_final ? 1 : 0,
+        first_trim.as<uint32_t>());
     prof_end();
     PSC_HIP(hipGetLastError());
     if (nnz) {
@@ -2514,6 +2559,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   if (nnz) {
     std::memcpy(h_cstart.data(), region(3), 8 * (G + 1));
     n_rem = h_cstart[G];
+    final_rem = n_rem;
     kbytes[KK_SELECT] += 8.0 * n_rem;
   }
   // ---- multi-pivot rounds (src/scc.hpp:369-473), every graph on its own
@@ -2651,7 +2697,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   // launch already enqueued it
   prof_begin(KK_MISC);
   if (!finalized) PSC_TRY(finalize_labels(nullptr));
-  kbytes[KK_MISC] += 16.0 * n;
+  kbytes[KK_MISC] += 16.0 * (direct_final ? final_rem + G : n);
   if (p.canonical) {
     uint32_t *minv = scratch32.as<uint32_t>();
     k_fill_u32<<<grid_for(n + G, BLOCK, 16384), BLOCK, 0, stream>>>(
