@@ -771,6 +771,7 @@ __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict
 // vertices is scanned as load-balanced 64-edge windows.  Remainders longer
 // than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 constexpr int PULL_PAIRS = 2;
+constexpr int PULL_CHUNKS = 8;  // chunks whose candidates one pass may hold
 constexpr int PULL_HEADS = 4;   // head tests per lane in flight
 constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
 
@@ -848,13 +849,16 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   __shared__ uint32_t qv_s[WPB][PULL_QCAP];
   __shared__ uint64_t qb_s[WPB][PULL_QCAP];
   __shared__ uint32_t qd_s[WPB][PULL_QCAP];
-  __shared__ uint16_t cand_s[WPB][WAVE * 32];   // candidate offsets of a chunk
-  __shared__ uint32_t fw_s[WPB][WAVE];          // found bits of a chunk's words
+  // listed candidates: chunk slot << 11 | offset in the chunk
+  __shared__ uint16_t cand_s[WPB][WAVE * 32];
+  __shared__ uint32_t fw_s[WPB][PULL_CHUNKS * WAVE];   // found bits per listed chunk
+  __shared__ uint64_t ck_s[WPB][PULL_CHUNKS];          // the listed chunks
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *sc = sc_s[w], *fl = fl_s[w];
   uint64_t *st = st_s[w];
   uint16_t *cand = cand_s[w];
   uint32_t *fw = fw_s[w];
+  uint64_t *ck = ck_s[w];
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
   int s_lo, s_hi;
@@ -874,43 +878,26 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
     unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0;
-    for (uint64_t c = gw; c < chunks; c += nw) {
-      const uint64_t wi = c * WAVE + l;
-      uint32_t sk = ~0u;
-      if (wi < words) {
-        sk = R.vis[wi] | R.trim[wi];
-        if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
-      }
-      ++n_chunks;
-      // the chunk's candidates (clear bits) as offsets in LDS, in order
-      const uint32_t todo_w = ~sk;
-      const uint32_t cnt = (uint32_t)__popc(todo_w);
-      const uint32_t inc = wave_incl_scan(cnt);
-      const uint32_t total = __shfl(inc, WAVE - 1);
-      if (total == 0) continue;
-      wave_sync();
-      {
-        uint32_t at = inc - cnt;
-        for (uint32_t tt = todo_w; tt; tt &= tt - 1)
-          cand[at++] = (uint16_t)(l * 32 + (__ffs(tt) - 1));
-      }
-      fw[l] = 0;
-      wave_sync();
-      const uint64_t vbase = c * (uint64_t)(WAVE * 32);
+    uint32_t ncand = 0, nck = 0;
+    // Both phases over the listed candidates (of up to PULL_CHUNKS chunks),
+    // then the found bits of the chunks' words go out, one atomic per word.
+    auto process = [&]() {
       // phase 1: every candidate tests its head, PULL_HEADS per lane at a
-      // time (two dependent loads); the misses are compacted to the front of
-      // the candidate list (the entries a batch writes lie at or below the
+      // time (two dependent loads); the misses are compacted to the front
+      // of the list (the entries a batch writes lie at or below the
       // batch's own, which are already in registers)
       uint32_t nmiss = 0;
-      for (uint32_t cb = 0; cb < total; cb += WAVE * PULL_HEADS) {
-        uint32_t co[PULL_HEADS], hd[PULL_HEADS], hw[PULL_HEADS];
+      for (uint32_t cb = 0; cb < ncand; cb += WAVE * PULL_HEADS) {
+        uint32_t e[PULL_HEADS], hd[PULL_HEADS], hw[PULL_HEADS];
+        uint64_t vx[PULL_HEADS];
         bool todo[PULL_HEADS];
 #pragma unroll
         for (int j = 0; j < PULL_HEADS; ++j) {
           const uint32_t idx = cb + j * WAVE + l;
-          todo[j] = idx < total;
-          co[j] = todo[j] ? cand[idx] : 0u;
-          hd[j] = todo[j] ? R.rhead[vbase + co[j]] : 0u;
+          todo[j] = idx < ncand;
+          e[j] = todo[j] ? cand[idx] : 0u;
+          vx[j] = ck[e[j] >> 11] * (uint64_t)(WAVE * 32) + (e[j] & 2047u);
+          hd[j] = todo[j] ? R.rhead[vx[j]] : 0u;
         }
 #pragma unroll
         for (int j = 0; j < PULL_HEADS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
@@ -922,13 +909,12 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           const bool miss = todo[j] && !hit;
           n_edges += todo[j];
           const uint64_t mm = ballot(miss);
-          if (miss) cand[nmiss + lanes_below(mm)] = (uint16_t)co[j];
+          if (miss) cand[nmiss + lanes_below(mm)] = (uint16_t)e[j];
           nmiss += (uint32_t)__popcll(mm);
-          // found bits gather in the chunk's LDS words, one global atomic
-          // per word at the end of the chunk
-          if (hit) atomicOr(&fw[co[j] >> 5], 1u << (co[j] & 31));
+          // found bits gather in the chunks' LDS words
+          if (hit) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += hit;
-          out.put(hit, (uint32_t)(vbase + co[j]), &nxt->qsize, qn, R.n);
+          out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, R.n);
         }
       }
       wave_sync();
@@ -936,14 +922,14 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       // 1..PULL_PROBE, PULL_PAIRS per lane
       for (uint32_t cb = 0; cb < nmiss; cb += WAVE * PULL_PAIRS) {
         uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
-        uint32_t co[PULL_PAIRS];
+        uint32_t e[PULL_PAIRS];
         bool todo[PULL_PAIRS];
 #pragma unroll
         for (int j = 0; j < PULL_PAIRS; ++j) {
           const uint32_t idx = cb + j * WAVE + l;
           todo[j] = idx < nmiss;
-          co[j] = todo[j] ? cand[idx] : 0u;
-          v[j] = vbase + co[j];
+          e[j] = todo[j] ? cand[idx] : 0u;
+          v[j] = ck[e[j] >> 11] * (uint64_t)(WAVE * 32) + (e[j] & 2047u);
           b[j] = todo[j] ? R.roff[v[j]] : 0ull;
           deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
         }
@@ -988,9 +974,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             q.deg[at] = (uint32_t)rem;
           }
           q.n += (uint32_t)__popcll(qm);
-          // found bits gather in the chunk's LDS words, one global atomic
-          // per word at the end of the chunk
-          if (found) atomicOr(&fw[co[j] >> 5], 1u << (co[j] & 31));
+          if (found) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += found;
           out.put(found, (uint32_t)v[j], &nxt->qsize, qn, R.n);
         }
@@ -999,8 +983,53 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
       }
       wave_sync();
-      if (fw[l]) atomicOr(&R.vis[wi], fw[l]);
+      for (uint32_t k = 0; k < nck; ++k) {
+        const uint32_t f = fw[k * WAVE + l];
+        if (f) atomicOr(&R.vis[ck[k] * WAVE + l], f);
+      }
+      ncand = 0;
+      nck = 0;
+      wave_sync();
+    };
+    // (visited | trimmed) word of this lane in chunk c; the next chunk's is
+    // loaded before a chunk is listed (in a pull level a vertex's bit is
+    // set only by its own processing, so an early load reads the same word)
+    auto skip_word = [&](uint64_t c) -> uint32_t {
+      const uint64_t wi = c * WAVE + l;
+      uint32_t sk = ~0u;
+      if (c < chunks && wi < words) {
+        sk = R.vis[wi] | R.trim[wi];
+        if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
+      }
+      return sk;
+    };
+    uint32_t sk_next = skip_word(gw);
+    for (uint64_t c = gw; c < chunks; c += nw) {
+      const uint32_t sk = sk_next;
+      sk_next = skip_word(c + nw);
+      ++n_chunks;
+      // the chunk's candidates (clear bits), appended to the list in order;
+      // chunks with few candidates share one pass of both phases
+      const uint32_t todo_w = ~sk;
+      const uint32_t cnt = (uint32_t)__popc(todo_w);
+      const uint32_t inc = wave_incl_scan(cnt);
+      const uint32_t total = __shfl(inc, WAVE - 1);
+      if (total == 0) continue;
+      if (ncand + total > WAVE * 32 || nck == PULL_CHUNKS) process();
+      wave_sync();
+      {
+        uint32_t at = ncand + inc - cnt;
+        for (uint32_t tt = todo_w; tt; tt &= tt - 1)
+          cand[at++] = (uint16_t)((nck << 11) | (l * 32 + (__ffs(tt) - 1)));
+      }
+      fw[nck * WAVE + l] = 0;
+      if (l == 0) ck[nck] = c;
+      ncand += total;
+      ++nck;
+      wave_sync();
+      if (ncand >= WAVE * PULL_HEADS) process();
     }
+    if (ncand) process();
     while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, R.n);
     unsigned long long *const cp[5] = {&C->cnt[s][R_PULL_EDGES], &C->cnt[s][R_PULL_SCANS],
