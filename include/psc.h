@@ -122,8 +122,8 @@ void psc_params_default(psc_params *p);
 
 /* Upload both CSRs once and allocate the device working set
  * (SCC::SCC, src/scc.hpp:81-92).  Host arrays are only read.  The set-up
- * also gathers each vertex's first in- and out-neighbour (read by the pull
- * levels). */
+ * also picks, for each vertex, one in- and one out-neighbour of large
+ * degree (read by the pull levels). */
 int psc_create(const psc_graph *g, psc_handle **out);
 
 /* Same, for a graph whose four arrays already live in device memory of the

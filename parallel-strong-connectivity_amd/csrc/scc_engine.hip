@@ -741,14 +741,38 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
   }
 }
 
-// head[v] = first entry of v's list (v itself for an empty list).
+// deg[v] = off[v + 1] - off[v], saturated to 32 bits.
+__global__ __launch_bounds__(BLOCK) void k_degrees32(const uint64_t *__restrict__ off,
+                                                     uint64_t n, uint32_t *deg) {
+  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * BLOCK)
+    deg[v] = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
+}
+
+// head[v] = the entry of v's list with the largest degree in the other
+// direction (`deg`), among its first HUB_SCAN entries, the earliest on
+// ties (v itself for an empty list).  A vertex of large degree is reached
+// early, so most pull candidates are decided by their head, and the heads
+// concentrate on few vertices whose visited words stay cached.
+constexpr uint32_t HUB_SCAN = 32;
 __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict__ off,
                                                       const uint32_t *__restrict__ adj,
+                                                      const uint32_t *__restrict__ deg,
                                                       uint64_t n, uint32_t *head) {
   for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t b = off[v];
-    head[v] = off[v + 1] > b ? adj[b] : (uint32_t)v;
+    const uint32_t d = (uint32_t)min(off[v + 1] - b, (uint64_t)HUB_SCAN);
+    uint32_t best = (uint32_t)v, bd = 0;
+    for (uint32_t t = 0; t < d; ++t) {
+      const uint32_t u = adj[b + t];
+      const uint32_t du = deg[u];
+      if (t == 0 || du > bd) {
+        best = u;
+        bd = du;
+      }
+    }
+    head[v] = best;
   }
 }
 
@@ -938,10 +962,6 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         for (int j = 0; j < PULL_PAIRS; ++j) {
           deg[j] -= b[j];
           n_scan += todo[j];
-          // the head is in-edge 0; the probes go on from in-edge 1
-          const bool has = deg[j] > 0;
-          b[j] += has;
-          deg[j] -= has;
 #pragma unroll
           for (uint32_t t = 0; t < PULL_PROBE; ++t)
             nb[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
@@ -1086,8 +1106,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const bool pull = reach_mode(cur, R) == MODE_PULL;
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
-    // a pull remainder starts after the head and the PULL_PROBE probed edges
-    const uint64_t skip = pull ? PULL_PROBE + 1 : 0;
+    // a pull remainder starts after the PULL_PROBE probed edges
+    const uint64_t skip = pull ? PULL_PROBE : 0;
     unsigned long long n_edges = 0, n_vis = 0, n_app = 0, n_items = 0, dsum = 0;
     const uint64_t per = (Ct + total_waves - 1) / total_waves;
     uint64_t c = gw * per;
@@ -1894,12 +1914,23 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   // vertex in the pull direction, gathered once here so that a pull probe of
   // it is one coalesced 4-byte load instead of an offset pair and a line of
   // the adjacency array
-  for (int s = 0; s < 2; ++s) {
-    PSC_TRY(rhead[s].ensure(nn * 4));
+  {
+    DevBuf deg32;
+    deg32.owner = &stream;
+    PSC_TRY(deg32.ensure(nn * 4));
     const uint64_t blocks = std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 65536);
-    k_pull_heads<<<(unsigned)blocks, BLOCK, 0, stream>>>(
-        s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, n, rhead[s].as<uint32_t>());
-    PSC_HIP(hipGetLastError());
+    for (int s = 0; s < 2; ++s) {
+      PSC_TRY(rhead[s].ensure(nn * 4));
+      // a pull of search s reads the other CSR; its candidates' in-neighbours
+      // are reached along the search's own CSR, whose degree ranks them
+      k_degrees32<<<(unsigned)blocks, BLOCK, 0, stream>>>(s == 0 ? d_off : d_ioff, n,
+                                                          deg32.as<uint32_t>());
+      k_pull_heads<<<(unsigned)blocks, BLOCK, 0, stream>>>(
+          s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, deg32.as<uint32_t>(), n,
+          rhead[s].as<uint32_t>());
+      PSC_HIP(hipGetLastError());
+    }
+    deg32.release();
   }
 
   ReachSearch rs[2];
