@@ -754,23 +754,29 @@ __global__ __launch_bounds__(BLOCK) void k_degrees32(const uint64_t *__restrict_
 // ties (v itself for an empty list).  A vertex of large degree is reached
 // early, so most pull candidates are decided by their head, and the heads
 // concentrate on few vertices whose visited words stay cached.
-constexpr uint32_t HUB_SCAN = 32;
+constexpr uint32_t HUB_SCAN = 128;
 __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict__ off,
                                                       const uint32_t *__restrict__ adj,
                                                       const uint32_t *__restrict__ deg,
                                                       uint64_t n, uint32_t *head) {
+  constexpr uint32_t K = 8;   // list entries (and their degrees) in flight
   for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t b = off[v];
     const uint32_t d = (uint32_t)min(off[v + 1] - b, (uint64_t)HUB_SCAN);
     uint32_t best = (uint32_t)v, bd = 0;
-    for (uint32_t t = 0; t < d; ++t) {
-      const uint32_t u = adj[b + t];
-      const uint32_t du = deg[u];
-      if (t == 0 || du > bd) {
-        best = u;
-        bd = du;
-      }
+    for (uint32_t t0 = 0; t0 < d; t0 += K) {
+      uint32_t u[K], du[K];
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) u[k] = t0 + k < d ? adj[b + t0 + k] : 0u;
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) du[k] = t0 + k < d ? deg[u[k]] : 0u;
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k)
+        if (t0 + k < d && (t0 + k == 0 || du[k] > bd)) {
+          best = u[k];
+          bd = du[k];
+        }
     }
     head[v] = best;
   }
