@@ -2177,10 +2177,14 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
   uint32_t L = 0;
-  // the previous run's level count (same graph: one poll, no idle levels)
+  // the previous run's level count (same graph: one poll, no idle levels;
+  // the poll itself shows that the level after the batch has no frontier).
+  // Single-pivot rounds share one hint across rounds of different depths:
+  // one spare level saves them a second poll.
   uint32_t &hint = rr ? round_levels_hint : reach_levels_hint;
   const uint32_t hint0 = hint;
-  int K = hint ? (int)std::min<uint32_t>(hint + 1, 64) : 4;
+  int K = hint ? (int)std::min<uint32_t>(hint + (rr ? 1 : 0), 64) : 4;
+  uint32_t lvl[2] = {0, 0};
   // timing >= 3: one level per poll and a per-level report on stderr
   const bool trace = p.timing >= 3 && !rr;
   if (trace) K = 1;
@@ -2213,7 +2217,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     }
     L += (uint32_t)K;
     PSC_HIP(hipGetLastError());
-    const bool ran = after_levels && !trace && hint0 && L >= hint0 + 1;
+    const bool ran = after_levels && !trace && hint0 && L >= hint0;
     if (ran) PSC_TRY(after_levels());
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
     if (trace) {
@@ -2243,22 +2247,30 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
       }
       prev_ = hc_;
     }
-    if (hc_.done[0] && hc_.done[1]) {
+    // a search is over once a level starts without frontier: marked by that
+    // level's step kernel, or seen here in the slot of level L (the next)
+    bool fin = true;
+    for (int s = 0; s < 2; ++s) {
+      if (hc_.done[s]) lvl[s] = hc_.levels[s];
+      else if (hc_.slot[s][L % 3].qsize == 0) lvl[s] = L;
+      else fin = false;
+    }
+    if (fin) {
       if (after_ran) *after_ran = ran;
       break;
     }
     if (!trace) K = std::min(K * 2, 64);
   }
   if (!rr) {
-    st->levels_fwd = hc_.levels[0];
-    st->levels_bwd = hc_.levels[1];
+    st->levels_fwd = lvl[0];
+    st->levels_bwd = lvl[1];
   } else {
-    st->sub_rounds += std::max(hc_.levels[0], hc_.levels[1]);
+    st->sub_rounds += std::max(lvl[0], lvl[1]);
     for (int s = 0; s < 2; ++s)
       rr->reached[s] = hc_.cnt[s][R_PUSH_VIS] + hc_.cnt[s][R_HEAVY_VIS] +
                        hc_.cnt[s][R_PULL_VIS];
   }
-  hint = std::max(hc_.levels[0], hc_.levels[1]);
+  hint = std::max(lvl[0], lvl[1]);
   for (int s = 0; s < 2; ++s) {
     const unsigned long long *c = hc_.cnt[s];
     st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES];
