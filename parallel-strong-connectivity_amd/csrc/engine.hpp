@@ -92,7 +92,7 @@ struct Engine {
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
   DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta,
-      tail_spec;
+      tail_spec, tail_trace;
   void *tail_pin = nullptr;   // pinned staging of the tail descriptors
   size_t tail_pin_cap = 0;
   int tail_pin_ensure(size_t bytes);

@@ -1827,7 +1827,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
           &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
-          &tail_lists, &tail_meta, &tail_spec};
+          &tail_lists, &tail_meta, &tail_spec, &tail_trace};
 }
 
 int Engine::download_labels(uint64_t *dst) {

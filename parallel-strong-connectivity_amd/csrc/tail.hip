@@ -273,7 +273,11 @@ struct TailArgs {
   unsigned long long *stats;   // [0] inserts, [1] edges
   uint32_t tau;                // local-search budget per frontier pair
   int vgc;                     // local search on (-local_scc)
+  // timing >= 3: per round of the first graph, (start tick, window,
+  // seeds, search levels), TAIL_TRACE rounds at most; else null
+  unsigned long long *trace;
 };
+constexpr uint32_t TAIL_TRACE = 64;
 
 // One workgroup per graph: all rounds of its remainder.
 __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
@@ -307,6 +311,11 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   bool full = false, handover = false;
   while (cur < nr && !full && !handover) {
     const uint64_t end = min(cur + step, nr);
+    const bool tr = A.trace && blockIdx.x == 0 && tid == 0 && rounds < TAIL_TRACE;
+    if (tr) {
+      A.trace[4 * rounds] = wall_clock64() - t0;
+      A.trace[4 * rounds + 1] = end - cur;
+    }
     SpecWord sw_keep{};
     const unsigned long long edges0 = edges;
     // ---- front: unfinished vertices of the window, in order; seeds
@@ -357,6 +366,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       __syncthreads();
     }
     const uint32_t nfront = front_n, nseed = seed_n;
+    uint32_t slevels = 0;
     // ---- forward and backward multi-search over pairs (src/scc.hpp:160-252),
     // both at once: waves [0, HALF_WAVES) expand the forward search, the
     // others the backward one, so a round waits for the longer of the two
@@ -377,6 +387,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       const uint32_t *const ea = dw ? A.tiadj : A.tadj;
       __syncthreads();
       while ((nq0 || nq1) && !flag_full) {
+        ++slevels;
         if (tid < 2) {
           cnt_q[tid] = 0;
           cnt_log[tid] = tid ? nl1 : nl0;
@@ -503,6 +514,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       }
       nlog0 = nl0;
       nlog1 = nl1;
+    }
+    if (tr) {
+      A.trace[4 * rounds + 2] = nseed;
+      A.trace[4 * rounds + 3] = slevels;
     }
     if (edges != edges0) atomicAdd(&round_edges, edges - edges0);
     __syncthreads();
@@ -830,6 +845,11 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     a.stats = d_stats;
     a.tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
     a.vgc = p.local_scc;
+    a.trace = nullptr;
+    if (p.timing >= 3) {
+      PSC_TRY(tail_trace.ensure((4 * TAIL_TRACE + 1) * 8));
+      a.trace = tail_trace.as<unsigned long long>();
+    }
     k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
     // labels as of every completed round, enqueued before the poll (a
     // relaunch continues from the compact labels; the host rounds and the
@@ -890,6 +910,17 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       for (uint32_t g = 0; g < G && all_done; ++g)
         if (cur[g] < h_cstart[g + 1] - h_cstart[g]) all_done = false;
       finalized = all_done;
+    }
+    if (p.timing >= 3 && nl) {   // rounds of the launch's first graph
+      std::vector<unsigned long long> tr(4 * TAIL_TRACE);
+      PSC_HIP(hipMemcpy(tr.data(), tail_trace.ptr, tr.size() * 8, hipMemcpyDeviceToHost));
+      const uint64_t nrd = std::min<uint64_t>(hs[0].rounds, TAIL_TRACE);
+      for (uint64_t r = 0; r < nrd; ++r) {
+        const unsigned long long t1 = r + 1 < nrd ? tr[4 * (r + 1)] : hs[0].ticks;
+        std::fprintf(stderr, "tail round %llu: window %llu seeds %llu search levels %llu %.1f us\n",
+                     (unsigned long long)r, tr[4 * r + 1], tr[4 * r + 2], tr[4 * r + 3],
+                     (t1 - tr[4 * r]) / 100.0);
+      }
     }
     if (p.timing >= 3) {   // per-graph report, slowest first (wall clock 100 MHz)
       std::vector<size_t> ord(nl);
