@@ -88,6 +88,7 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
+  uint64_t r0 = tile_off[blockIdx.x];   // selected items before this tile
   bool p[SEL_ITEMS];
   uint32_t rk[SEL_ITEMS];
 #pragma unroll
@@ -99,7 +100,6 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
     if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
   }
   __syncthreads();
-  uint64_t r0 = tile_off[blockIdx.x];   // selected items before this round
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
     uint32_t before = 0, round_tot = 0;
@@ -1203,7 +1203,10 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
-  // every bitmap word of the tile first (trimmed, both reaches)
+  // every load the tile needs first: its prefix, the per-graph constants
+  // of a single graph, and the bitmap words (trimmed, both reaches)
+  uint64_t r0 = tile_off[blockIdx.x];
+  const unsigned long long z0 = zstart[0], lab_one = lab1[0];
   uint32_t tw[SEL_ITEMS], fw[SEL_ITEMS], bw[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
@@ -1227,8 +1230,6 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
   }
   __syncthreads();
-  uint64_t r0 = tile_off[blockIdx.x];
-  const unsigned long long z0 = zstart[0], lab_one = lab1[0];
   uint32_t place = 0;            // items left to place by pivot position
   uint64_t rr[SEL_ITEMS];
 #pragma unroll
