@@ -789,17 +789,18 @@ __global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict
 // a first-round reach set (src/scc.hpp:341-347) and no pull probe reads
 // them as visited.
 // The vertex range is walked by bitmap: a wave loads 64 words of
-// (visited | trimmed) with one coalesced load (2048 vertices), lists the
-// chunk's candidates (clear bits) in LDS.  Every candidate first tests its
-// first in-neighbour (from the per-vertex head array, coalesced), PULL_HEADS
-// x 64 at a time; the misses are compacted in the list and then take
-// PULL_PAIRS x 64 at a time with their loads in flight together: in-offsets,
-// the next PULL_PROBE in-edges, their visited words.  Found vertices set
-// their bits in an LDS copy of the chunk's words, flushed with one atomic
-// per word.  Vertices left undecided
-// queue their remaining in-edges in LDS; a full batch of 64 queued
-// vertices is scanned as load-balanced 64-edge windows.  Remainders longer
-// than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
+// (visited | trimmed) with one coalesced load (2048 vertices) and lists the
+// chunk's candidates (clear bits) in LDS; chunks with few candidates are
+// listed together (up to PULL_CHUNKS) before both phases run over the list.
+// Phase 1: every candidate tests its head (one in-neighbour of large degree,
+// from the per-vertex head array, coalesced), PULL_HEADS x 64 at a time.
+// Phase 2: the misses, compacted in the list, take PULL_PAIRS x 64 at a time
+// with their loads in flight together: in-offsets, the first PULL_PROBE
+// in-edges, their visited words.  Found vertices set their bits in LDS
+// copies of their chunks' words, flushed with one atomic per word.  Vertices
+// left undecided queue their remaining in-edges in LDS; a full batch of 64
+// queued vertices is scanned as load-balanced 64-edge windows.  Remainders
+// longer than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 constexpr int PULL_PAIRS = 2;
 constexpr int PULL_CHUNKS = 8;  // chunks whose candidates one pass may hold
 constexpr int PULL_HEADS = 4;   // head tests per lane in flight
