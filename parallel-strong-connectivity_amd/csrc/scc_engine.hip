@@ -528,6 +528,7 @@ __device__ __forceinline__ void xcd_split(int ns, int w, int *s_lo, int *s_hi,
 // the reach sets.  Every input is final before the level starts.
 __device__ __forceinline__ int reach_mode(const ReachSlot *c,
                                           const ReachSearch &R) {
+  if (c->deferred) return MODE_PULL;
   const uint32_t nf = c->qsize;
   if (nf == 0) return MODE_NONE;
   unsigned long long ds = c->degsum;
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
     const int s = threadIdx.x;
     ReachSlot z{};
     C->slot[s][zi] = z;
-    if (C->slot[s][ci].qsize == 0 && !C->done[s]) {
+    if (C->slot[s][ci].qsize == 0 && C->slot[s][ci].deferred == 0 && !C->done[s]) {
       C->done[s] = 1;
       C->levels[s] = L;
     }
@@ -908,8 +909,16 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
-    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0;
+    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0, n_def = 0;
     uint32_t ncand = 0, nck = 0;
+    // A dense pull level (the reference's criterion) right after push
+    // levels leaves the in-list remainders of its undecided candidates to
+    // the next level, which then pulls completely: most of them are found
+    // there by their head, and the rest would be scanned there again
+    // anyway.  Reached sets do not depend on the level a vertex is found
+    // at; the search cannot end while remainders are deferred.
+    const bool defer = !cur->from_pull && !cur->deferred &&
+                       (uint64_t)cur->qsize + cur->degsum >= R.thresh;
     // Both phases over the listed candidates (of up to PULL_CHUNKS chunks),
     // then the found bits of the chunks' words go out, one atomic per word.
     auto process = [&]() {
@@ -987,7 +996,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
           const uint64_t k = min(deg[j], (uint64_t)PULL_PROBE);
           n_edges += k;
-          const uint64_t rem = found ? 0 : deg[j] - k;
+          uint64_t rem = found ? 0 : deg[j] - k;
+          if (defer && rem) {
+            ++n_def;
+            rem = 0;
+          }
           // long remainders go to the heavy list (k_reach_heavy, chunked),
           // the rest to the wave's queue
           const bool heavy = rem > PULL_HEAVY;
@@ -1065,6 +1078,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const unsigned long long cv[5] = {n_edges, n_scan, n_vis, n_vis,
                                       l == 0 ? n_chunks : 0ull};
     block_add<5>(cp, cv);
+    if (defer && __syncthreads_or(n_def != 0) && threadIdx.x == 0) atomicOr(&nxt->deferred, 1u);
   }
 }
 
@@ -2254,7 +2268,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     bool fin = true;
     for (int s = 0; s < 2; ++s) {
       if (hc_.done[s]) lvl[s] = hc_.levels[s];
-      else if (hc_.slot[s][L % 3].qsize == 0) lvl[s] = L;
+      else if (hc_.slot[s][L % 3].qsize == 0 && hc_.slot[s][L % 3].deferred == 0) lvl[s] = L;
       else fin = false;
     }
     if (fin) {

@@ -71,7 +71,9 @@ struct ReachSlot {
   unsigned long long heavy;  // (count << 40) | chunks
   unsigned long long hgrab;  // heavy chunks handed out
   uint32_t from_pull;        // frontier produced by a pull level
-  uint32_t pad;
+  // nonzero: the previous (pull) level left in-list remainders to this one,
+  // which pulls, completely, whatever its frontier
+  uint32_t deferred;
   unsigned long long cum_vis;  // vertices visited before this level
 };
 
