@@ -195,8 +195,10 @@ struct SelWindow {
 __global__ __launch_bounds__(BLOCK) void k_trim_count(
     const uint64_t *__restrict__ off, const uint64_t *__restrict__ ioff, uint64_t n,
     uint64_t words, uint32_t *tile_cnt, uint32_t *vis_f, uint32_t *vis_b,
-    uint32_t *trim_bm) {
+    uint32_t *trim_bm, unsigned long long *run_zero) {
   __shared__ uint32_t wc[WPB];
+  // the run's contains() probe counter starts here
+  if (blockIdx.x == 0 && threadIdx.x == 0) *run_zero = 0;
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int l = lane();
   uint32_t c = 0;
@@ -2533,7 +2535,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
                                                     tiles.as<uint32_t>(),
                                                     vis[0].as<uint32_t>(),
                                                     vis[1].as<uint32_t>(),
-                                                    trim_bm.as<uint32_t>());
+                                                    trim_bm.as<uint32_t>(),
+                                                    misc.as<unsigned long long>() + 2);
     PSC_TRY(scan_tiles(nt, tot, trim_off.as<unsigned long long>()));
     k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
         d_off, d_ioff, n, voffd, G, trim_off.as<unsigned long long>(), tot, nstart,
@@ -2848,7 +2851,9 @@ int Engine::prof_reset(int level) {
     kprof[k] = KProf{};
     kbytes[k] = 0;
   }
-  (void)hipMemsetAsync(misc.as<unsigned long long>() + 2, 0, 8, stream);
+  // the contains() probe counter (misc[2]) is cleared by the run's first
+  // kernel (k_trim_count); an empty graph has no kernel
+  if (n == 0) (void)hipMemsetAsync(misc.as<unsigned long long>() + 2, 0, 8, stream);
   return PSC_OK;
 }
 
