@@ -960,6 +960,12 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         }
       }
       wave_sync();
+      // a deferring level leaves every head miss to the next level, which
+      // scans it completely
+      if (defer) {
+        n_def += nmiss;
+        nmiss = 0;
+      }
       // phase 2: the misses load their offsets and probe in-edges
       // 1..PULL_PROBE, PULL_PAIRS per lane
       for (uint32_t cb = 0; cb < nmiss; cb += WAVE * PULL_PAIRS) {
