@@ -245,6 +245,16 @@ int psc_dev_gen_grid(uint32_t rows, uint32_t cols, double p_both,
                      uint64_t seed, psc_dev_graph **out);
 void psc_dev_graph_free(psc_dev_graph *dg);
 
+/* make_directed (src/graph.hpp:222-245) on the device: copies the out-CSR
+ * given in device memory (offsets u64[n+1], edges u32[m], kept as given:
+ * duplicates and self-loops stay) and builds the in-CSR on the GPU, in-lists
+ * sorted by source as the reference's sort of (v, u) pairs leaves them.
+ * The result is identical to psc_make_directed; free it with
+ * psc_dev_graph_free.  Rejects offsets that do not run from 0 to m without
+ * falling, and endpoints >= n. */
+int psc_dev_make_directed(uint64_t n, uint64_t m, const uint64_t *d_offset,
+                          const uint32_t *d_edges, psc_dev_graph **out);
+
 /* Copy a device graph back to host arrays (for checking). */
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
 

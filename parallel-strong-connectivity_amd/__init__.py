@@ -124,6 +124,8 @@ _sig("psc_dev_gen_rmat", C.c_int, C.c_uint32, C.c_uint64, C.c_double,
 _sig("psc_dev_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double,
      C.c_uint64, C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_free", None, C.POINTER(CDevGraph))
+_sig("psc_dev_make_directed", C.c_int, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+     C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_download", C.c_int, C.POINTER(CDevGraph),
      C.POINTER(C.POINTER(CHostGraph)))
 
@@ -269,6 +271,30 @@ class DeviceGraph:
         _check("psc_dev_gen_grid", lib.psc_dev_gen_grid(rows, cols, p_both, seed,
                                                         C.byref(dgp)))
         return DeviceGraph(dgp)
+
+    @staticmethod
+    def make_directed(n, offset, edges):
+        """In-CSR built on the device (make_directed, src/graph.hpp:222-245):
+        the out-CSR goes to device memory as given, the transpose runs there."""
+        off = np.ascontiguousarray(offset, dtype=np.uint64)
+        e = np.ascontiguousarray(edges, dtype=np.uint32)
+        d_off, d_e = C.c_void_p(), C.c_void_p()
+        _check("psc_device_alloc", lib.psc_device_alloc(max(off.nbytes, 8), C.byref(d_off)))
+        try:
+            _check("psc_device_alloc", lib.psc_device_alloc(max(e.nbytes, 4), C.byref(d_e)))
+            try:
+                _check("psc_device_copy", lib.psc_device_copy(d_off, off.ctypes.data, off.nbytes))
+                if e.size:
+                    _check("psc_device_copy", lib.psc_device_copy(d_e, e.ctypes.data, e.nbytes))
+                dgp = C.POINTER(CDevGraph)()
+                _check("psc_dev_make_directed",
+                       lib.psc_dev_make_directed(int(n), int(e.size), d_off, d_e,
+                                                 C.byref(dgp)))
+                return DeviceGraph(dgp)
+            finally:
+                lib.psc_device_free(d_e)
+        finally:
+            lib.psc_device_free(d_off)
 
     def cgraph(self):
         return self._p.contents.g

@@ -10,6 +10,7 @@
 #include <string>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 
 #include "../../include/psc.h"
@@ -99,6 +100,37 @@ __global__ void k_offsets(const unsigned long long *keys, uint64_t m,
       if (keys[mid] < t) lo = mid + 1; else hi = mid;
     }
     off[v] = lo;
+  }
+}
+
+// make_directed on the device.  A vertex with out-edges marks the first
+// slot of its list with its id (every other slot holds 0); a max-scan over
+// the slots then gives each edge its source.  Offsets that fall or leave
+// [0, m] set *bad.
+__global__ void k_src_marks(const uint64_t *off, uint64_t n, uint64_t m, uint32_t *mark,
+                            unsigned long long *bad) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[v], e = off[v + 1];
+    if (e < b || e > m) {
+      *bad = 1;
+      continue;
+    }
+    if (e > b) mark[b] = (uint32_t)v;
+  }
+}
+
+// keys[i] = dst << 32 | src: sorted, they list every in-edge by target and,
+// within a target, by source, as make_directed's sort of (v, u) pairs does.
+// Endpoints >= n set *bad.
+__global__ void k_dst_src_keys(const uint32_t *E, const uint32_t *src, uint64_t m,
+                               uint64_t n, unsigned long long *keys,
+                               unsigned long long *bad) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = E[i];
+    if (d >= n) *bad = 1;
+    keys[i] = ((unsigned long long)d << 32) | src[i];
   }
 }
 
@@ -218,6 +250,90 @@ int psc_dev_gen_grid(uint32_t rows, uint32_t cols, double p_both,
   rc = build_csr(keys, 2 * npairs, (uint64_t)rows * cols, out);
 done:
   if (keys) (void)hipFree(keys);
+  return rc;
+}
+
+int psc_dev_make_directed(uint64_t n, uint64_t m, const uint64_t *d_offset,
+                          const uint32_t *d_edges, psc_dev_graph **out) {
+  if (!out || !d_offset || (m && !d_edges) || n >= 0xFFFFFFFFull) {
+    psc::set_error("psc_dev_make_directed: bad arguments");
+    return PSC_ERR_INVALID;
+  }
+  *out = nullptr;
+  int rc = PSC_OK;
+  hipStream_t st = nullptr;
+  uint64_t *off = nullptr, *ioff = nullptr;
+  uint32_t *E = nullptr, *iE = nullptr, *mark = nullptr, *src = nullptr;
+  unsigned long long *keys = nullptr, *tmpk = nullptr, *bad = nullptr;
+  void *temp = nullptr;
+  size_t tb = 0, tb2 = 0;
+  uint64_t ends[2] = {0, 0};
+  unsigned long long hbad = 0;
+  psc_dev_graph *dg = nullptr;
+  GEN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  GEN_HIP(hipMalloc(&off, (n + 1) * 8));
+  GEN_HIP(hipMalloc(&ioff, (n + 1) * 8));
+  GEN_HIP(hipMalloc(&E, std::max<uint64_t>(m, 1) * 4));
+  GEN_HIP(hipMalloc(&iE, std::max<uint64_t>(m, 1) * 4));
+  GEN_HIP(hipMalloc(&bad, 8));
+  GEN_HIP(hipMemcpyAsync(off, d_offset, (n + 1) * 8, hipMemcpyDeviceToDevice, st));
+  if (m) GEN_HIP(hipMemcpyAsync(E, d_edges, m * 4, hipMemcpyDeviceToDevice, st));
+  GEN_HIP(hipMemcpyAsync(&ends[0], off, 8, hipMemcpyDeviceToHost, st));
+  GEN_HIP(hipMemcpyAsync(&ends[1], off + n, 8, hipMemcpyDeviceToHost, st));
+  GEN_HIP(hipMemsetAsync(bad, 0, 8, st));
+  GEN_HIP(hipStreamSynchronize(st));
+  if (ends[0] != 0 || ends[1] != m) {
+    psc::set_error("offsets must start at 0 and end at m");
+    rc = PSC_ERR_INVALID;
+    goto done;
+  }
+  if (m) {
+    GEN_HIP(hipMalloc(&mark, m * 4));
+    GEN_HIP(hipMalloc(&src, m * 4));
+    GEN_HIP(hipMalloc(&keys, m * 8));
+    GEN_HIP(hipMalloc(&tmpk, m * 8));
+    GEN_HIP(rocprim::inclusive_scan(nullptr, tb, mark, src, m, rocprim::maximum<uint32_t>(), st));
+    GEN_HIP(rocprim::radix_sort_keys(nullptr, tb2, keys, tmpk, m, 0, 64, st));
+    tb = std::max(tb, tb2);
+    GEN_HIP(hipMalloc(&temp, std::max<size_t>(tb, 8)));
+    GEN_HIP(hipMemsetAsync(mark, 0, m * 4, st));
+    k_src_marks<<<grid_blocks(n), 256, 0, st>>>(off, n, m, mark, bad);
+    GEN_HIP(hipGetLastError());
+    tb2 = tb;
+    GEN_HIP(rocprim::inclusive_scan(temp, tb2, mark, src, m, rocprim::maximum<uint32_t>(), st));
+    k_dst_src_keys<<<grid_blocks(m), 256, 0, st>>>(E, src, m, n, keys, bad);
+    GEN_HIP(hipGetLastError());
+    GEN_HIP(hipMemcpyAsync(&hbad, bad, 8, hipMemcpyDeviceToHost, st));
+    GEN_HIP(hipStreamSynchronize(st));
+    if (hbad) {
+      psc::set_error("edge endpoint out of range or offsets not increasing");
+      rc = PSC_ERR_INVALID;
+      goto done;
+    }
+    tb2 = tb;
+    GEN_HIP(rocprim::radix_sort_keys(temp, tb2, keys, tmpk, m, 0, 64, st));
+    k_low_halves<<<grid_blocks(m), 256, 0, st>>>(tmpk, iE, m);
+  }
+  k_offsets<<<grid_blocks(n + 1), 256, 0, st>>>(tmpk, m, n, ioff);
+  GEN_HIP(hipGetLastError());
+  GEN_HIP(hipStreamSynchronize(st));
+  dg = new psc_dev_graph;
+  dg->g = psc_graph{n, m, off, E, ioff, iE};
+  *out = dg;
+  off = ioff = nullptr;
+  E = iE = nullptr;
+done:
+  if (temp) (void)hipFree(temp);
+  if (mark) (void)hipFree(mark);
+  if (src) (void)hipFree(src);
+  if (keys) (void)hipFree(keys);
+  if (tmpk) (void)hipFree(tmpk);
+  if (bad) (void)hipFree(bad);
+  if (off) (void)hipFree(off);
+  if (ioff) (void)hipFree(ioff);
+  if (E) (void)hipFree(E);
+  if (iE) (void)hipFree(iE);
+  if (st) (void)hipStreamDestroy(st);
   return rc;
 }
 
