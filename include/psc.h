@@ -255,6 +255,12 @@ void psc_dev_graph_free(psc_dev_graph *dg);
 int psc_dev_make_directed(uint64_t n, uint64_t m, const uint64_t *d_offset,
                           const uint32_t *d_edges, psc_dev_graph **out);
 
+/* The .bin reader of psc_read_bin with the graph left in device memory:
+ * for one CSR in the file (large=0, not symmetric) the out-CSR is uploaded
+ * and the in-CSR built on the device (psc_dev_make_directed) instead of by
+ * a host transpose.  Free with psc_dev_graph_free. */
+int psc_read_bin_device(const char *path, int large, psc_dev_graph **out);
+
 /* Copy a device graph back to host arrays (for checking). */
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
 

@@ -181,6 +181,74 @@ int psc_read_bin(const char *path, int large, psc_host_graph **out) {
   return PSC_OK;
 }
 
+int psc_read_bin_device(const char *path, int large, psc_dev_graph **out) {
+  if (!path || !out) return PSC_ERR_INVALID;
+  *out = nullptr;
+  if (large || strstr(path, "sym")) {
+    // both CSRs come from the file: read, check and upload them
+    psc_host_graph *hg = nullptr;
+    int rc = psc_read_bin(path, large, &hg);
+    if (rc) return rc;
+    const uint64_t n = hg->g.n, m = hg->g.m;
+    void *p[4] = {nullptr, nullptr, nullptr, nullptr};
+    const void *src[4] = {hg->offset, hg->edges, hg->in_offset, hg->in_edges};
+    const size_t bytes[4] = {(n + 1) * 8, m * 4, (n + 1) * 8, m * 4};
+    for (int i = 0; i < 4 && rc == PSC_OK; ++i) {
+      rc = psc_device_alloc(bytes[i] ? bytes[i] : 4, &p[i]);
+      if (rc == PSC_OK && bytes[i]) rc = psc_device_copy(p[i], src[i], bytes[i]);
+    }
+    psc_host_graph_free(hg);
+    if (rc) {
+      for (void *q : p)
+        if (q) psc_device_free(q);
+      return rc;
+    }
+    auto *dg = new psc_dev_graph;
+    dg->g = psc_graph{n, m, (const uint64_t *)p[0], (const uint32_t *)p[1],
+                      (const uint64_t *)p[2], (const uint32_t *)p[3]};
+    *out = dg;
+    return PSC_OK;
+  }
+  // one CSR in the file: read it, upload it, transpose on the device
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    psc::set_error(std::string("cannot open ") + path);
+    return PSC_ERR_IO;
+  }
+  uint64_t hdr[3];
+  if (!read_exact(f, hdr, sizeof(hdr))) {
+    fclose(f);
+    psc::set_error("truncated header");
+    return PSC_ERR_IO;
+  }
+  const uint64_t n = hdr[0], m = hdr[1];
+  if (hdr[2] != (n + 1) * 8 + m * 4 + 3 * 8 || n >= 0xFFFFFFFFull) {
+    fclose(f);
+    psc::set_error("bad .bin header (sizes field mismatch)");
+    return PSC_ERR_IO;
+  }
+  std::vector<uint64_t> off(n + 1);
+  std::vector<uint32_t> E(m);
+  bool ok = read_exact(f, off.data(), (n + 1) * 8) && read_exact(f, E.data(), m * 4);
+  if (ok) ok = fgetc(f) == EOF;   // "Bad data" check, src/graph.hpp:178-181
+  fclose(f);
+  if (!ok) {
+    psc::set_error("malformed .bin graph");
+    return PSC_ERR_IO;
+  }
+  void *d_off = nullptr, *d_e = nullptr;
+  int rc = psc_device_alloc((n + 1) * 8, &d_off);
+  if (rc == PSC_OK) rc = psc_device_alloc(m ? m * 4 : 4, &d_e);
+  if (rc == PSC_OK) rc = psc_device_copy(d_off, off.data(), (n + 1) * 8);
+  if (rc == PSC_OK && m) rc = psc_device_copy(d_e, E.data(), m * 4);
+  // offsets and endpoints are checked on the device
+  if (rc == PSC_OK)
+    rc = psc_dev_make_directed(n, m, (const uint64_t *)d_off, (const uint32_t *)d_e, out);
+  if (d_off) psc_device_free(d_off);
+  if (d_e) psc_device_free(d_e);
+  return rc;
+}
+
 int psc_write_bin(const char *path, const psc_graph *g, int large) {
   if (!path || !g) return PSC_ERR_INVALID;
   FILE *f = fopen(path, "wb");

@@ -66,9 +66,11 @@ int main(int argc, char **argv) {
   psc_params_default(&prm);
   prm.beta = P.getd("-beta", 1.5);
   printf("--beta %g--\n", prm.beta);
-  psc_host_graph *hg = nullptr;
-  int rc = psc_read_bin(fileName, P.has("-large") ? 1 : 0, &hg);
+  // the graph goes to device memory; a one-CSR file is transposed there
+  psc_dev_graph *dg = nullptr;
+  int rc = psc_read_bin_device(fileName, P.has("-large") ? 1 : 0, &dg);
   if (rc) return fail("read graph", rc);
+  const psc_graph *g = &dg->g;
   prm.local_reach = P.has("-local_reach");
   prm.local_scc = P.has("-local_scc");
   prm.min_bag = (uint64_t)P.geti("-lambda", 1 << 10);
@@ -80,14 +82,14 @@ int main(int argc, char **argv) {
   printf("tau %llu\n", (unsigned long long)prm.tau);
   printf("input graph: %s\n", fileName);
   printf("number of vertices: %llu, number of edges: %llu\n",
-         (unsigned long long)hg->g.n, (unsigned long long)hg->g.m);
+         (unsigned long long)g->n, (unsigned long long)g->m);
   printf("enable VGC1: %d, enable VGC_multi: %d\n", prm.local_reach, prm.local_scc);
   fflush(stdout);
 
   psc_handle *h = nullptr;
-  rc = psc_create(&hg->g, &h);
-  if (rc) return fail("psc_create", rc);
-  std::vector<uint64_t> label(hg->g.n);
+  rc = psc_create_device(g, &h);
+  if (rc) return fail("psc_create_device", rc);
+  std::vector<uint64_t> label(g->n);
   const int repeat = (int)P.geti("-t", 10);
   psc_stats st;
   rc = psc_run(h, &prm, label.data(), &st);   // warm-up, src/scc.cpp:58
@@ -112,8 +114,8 @@ int main(int argc, char **argv) {
   printf("average multi_search time %g\n", multi / rep);
   printf("average hash table resize time %g\n", resize / rep);
   if (P.has("-status")) {   // src/scc.cpp:13-29
-    std::vector<uint32_t> flag(hg->g.n + 1, 0);
-    for (uint64_t i = 0; i < hg->g.n; i++) flag[label[i] & 0x7FFFFFFFu]++;
+    std::vector<uint32_t> flag(g->n + 1, 0);
+    for (uint64_t i = 0; i < g->n; i++) flag[label[i] & 0x7FFFFFFFu]++;
     uint64_t nscc = 0, big = 0;
     for (uint32_t f : flag) {
       nscc += f != 0;
@@ -131,6 +133,6 @@ int main(int argc, char **argv) {
     fclose(f);
   }
   psc_destroy(h);
-  psc_host_graph_free(hg);
+  psc_dev_graph_free(dg);
   return 0;
 }
