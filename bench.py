@@ -220,6 +220,41 @@ def bench_copy(args, psc):
     b.free()
 
 
+def bench_cc(args, psc):
+    """Connectivity (src/Connectivity.h:37-119) of the undirected graph under
+    the RMAT arcs (2^scale vertices, 2^samples_log2 samples, seed): kernel
+    time from events, labels checked against the oracle's union-find."""
+    import numpy as np
+    import oracle
+    scale, samples = args.scale, 1 << args.samples_log2
+    dg = psc.DeviceGraph.rmat(scale, samples, 0.57, 0.19, 0.19, seed=args.seed, permute=True)
+    for _ in range(args.warmup):
+        psc.connectivity(dg)
+    ms = []
+    for _ in range(args.steps):
+        labels = psc.connectivity(dg)
+        ms.append(psc.connectivity.ms)
+    dev_ms = float(np.median(ms))
+    log(f"connectivity: {dev_ms:.3f} ms (kernels), {dg.m / (dev_ms * 1e-3) / 1e9:.1f} Gedges/s")
+    ok = None
+    cpu_s = None
+    if not args.no_cpu:
+        g = dg.to_host()
+        t = time.perf_counter()
+        expect = oracle.cc(g)
+        cpu_s = time.perf_counter() - t
+        ok = bool(np.array_equal(labels, expect))
+        log(f"oracle union-find (1 thread): {cpu_s:.2f} s; labels identical: {ok}")
+    print(json.dumps({"metric": "connectivity Gedges/s on RMAT",
+                      "value": round(dg.m / (dev_ms * 1e-3) / 1e9, 2), "unit": "Gedges/s",
+                      "ms": round(dev_ms, 3), "n": dg.n, "m": dg.m,
+                      "components": int(np.unique(labels).size),
+                      "labels_equal_oracle": ok,
+                      "oracle_seconds_1_thread": None if cpu_s is None else round(cpu_s, 2)}),
+          flush=True)
+    dg.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,7 +274,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="warm-up + steps only (for rocprofv3 runs)")
-    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve", "copy"),
+    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve", "copy", "cc"),
                     default="c3")
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--grid-side", type=int, default=2048)
@@ -263,6 +298,8 @@ def main():
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
     psc = ge.load_package()
+    if args.mode == "cc":
+        return bench_cc(args, psc)
     if args.mode == "copy":
         return bench_copy(args, psc)
     if args.mode == "grid":

@@ -261,6 +261,18 @@ int psc_dev_make_directed(uint64_t n, uint64_t m, const uint64_t *d_offset,
  * a host transpose.  Free with psc_dev_graph_free. */
 int psc_read_bin_device(const char *path, int large, psc_dev_graph **out);
 
+/* ------------------------------------------------------ connectivity
+ * Connected components of the undirected graph under an out-CSR (every arc
+ * joins its endpoints): the partition of Connectivity
+ * (src/Connectivity.h:37-119, LDD + union-find).  The reference's label
+ * values depend on the order of its concurrent CASes; here labels[v] is
+ * the minimum vertex id of v's component, the same on every run.  Only the
+ * out-CSR is read (in_offset / in_edges may be NULL).  The device version
+ * takes device pointers and returns the kernels' time in *ms (may be
+ * NULL). */
+int psc_connectivity(const psc_graph *g, uint32_t *labels);
+int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms);
+
 /* Copy a device graph back to host arrays (for checking). */
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
 

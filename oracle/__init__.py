@@ -11,6 +11,9 @@ ctypes bindings to liboracle.so:
   scc_ref(graph, ...)  CPU restatement of src/scc.hpp / reach.hpp /
                        hash_bag.h / resizable_table.h (scc_ref.cpp)
   reach(...)           REACH::reach, src/reach.hpp:208-243
+  cc(graph)            connected components (min vertex id per component)
+                       of the undirected graph under the out-CSR: the
+                       partition of src/Connectivity.h:37-119 (cc.c)
   Table / bag_roundtrip / pivot_perm   per-function hooks
 
 Parity of these functions is pinned in tests/test_oracle.py against
@@ -66,6 +69,7 @@ def _sig(name, res, *args):
 
 
 _sig("psc_oracle_tarjan", C.c_int64, C.c_uint64, u64p, u32p, u32p)
+_sig("psc_oracle_cc", C.c_int, C.c_uint64, u64p, u32p, u32p)
 _sig("psc_oracle_canonical", C.c_int, C.c_uint64, u64p, u32p)
 _sig("psc_oracle_status", C.c_int, C.c_uint64, u32p, u64p, u64p)
 _sig("psc_oracle_scc_create", C.c_void_p, C.c_uint64, C.c_uint64, u64p, u32p,
@@ -96,6 +100,20 @@ def _arrays(g):
     ioff = np.ascontiguousarray(g.in_offset, dtype=np.uint64)
     ie = np.ascontiguousarray(g.in_edges, dtype=np.uint32)
     return off, e, ioff, ie
+
+
+def cc_csr(n, offset, edges):
+    off = np.ascontiguousarray(offset, dtype=np.uint64)
+    e = np.ascontiguousarray(edges, dtype=np.uint32)
+    out = np.empty(n, dtype=np.uint32)
+    rc = lib.psc_oracle_cc(n, _p(off, u64p), _p(e, u32p), _p(out, u32p))
+    if rc != 0:
+        raise ValueError("psc_oracle_cc failed")
+    return out
+
+
+def cc(g):
+    return cc_csr(g.n, g.offset, g.edges)
 
 
 def tarjan_csr(n, offset, edges):

@@ -124,6 +124,9 @@ _sig("psc_dev_gen_rmat", C.c_int, C.c_uint32, C.c_uint64, C.c_double,
 _sig("psc_dev_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double,
      C.c_uint64, C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_free", None, C.POINTER(CDevGraph))
+_sig("psc_connectivity", C.c_int, C.POINTER(CGraph), u32p)
+_sig("psc_connectivity_device", C.c_int, C.POINTER(CGraph), C.c_void_p,
+     C.POINTER(C.c_double))
 _sig("psc_dev_make_directed", C.c_int, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
      C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_download", C.c_int, C.POINTER(CDevGraph),
@@ -229,6 +232,30 @@ class Graph:
         g = self.cgraph()
         _check("psc_write_bin", lib.psc_write_bin(os.fsencode(path), C.byref(g),
                                                   int(large)))
+
+
+def connectivity(g):
+    """Connected components of the undirected graph under g's out-CSR
+    (Connectivity, src/Connectivity.h:37-119): labels[v] = the minimum
+    vertex id of v's component.  g is a Graph (host arrays) or a
+    DeviceGraph; for a DeviceGraph the kernels' time is in connectivity.ms."""
+    out = np.empty(g.n, dtype=np.uint32)
+    cg = g.cgraph()
+    if isinstance(g, DeviceGraph):
+        d = C.c_void_p()
+        _check("psc_device_alloc", lib.psc_device_alloc(max(g.n * 4, 4), C.byref(d)))
+        try:
+            ms = C.c_double()
+            _check("psc_connectivity_device",
+                   lib.psc_connectivity_device(C.byref(cg), d, C.byref(ms)))
+            if g.n:
+                _check("psc_device_copy", lib.psc_device_copy(out.ctypes.data, d, g.n * 4))
+            connectivity.ms = ms.value
+        finally:
+            lib.psc_device_free(d)
+    else:
+        _check("psc_connectivity", lib.psc_connectivity(C.byref(cg), _ptr(out, u32p)))
+    return out
 
 
 def gen_rmat(scale, samples, a=0.57, b=0.19, c=0.19, seed=1, permute=True):

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wextra",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip"]
+HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip", "cc.hip"]
 CXX_SOURCES = ["graph_host.cpp", "serve.cpp"]
 
 

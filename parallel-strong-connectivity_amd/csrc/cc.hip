@@ -1,0 +1,226 @@
+// Connected components of the undirected graph under an out-CSR (every arc
+// joins its endpoints): the partition Connectivity computes
+// (src/Connectivity.h:37-119) through LDD (src/ldd.hpp) and union-find
+// (src/union_find_rules.h).
+//
+// On the GPU the union-find runs over the arcs directly, lock-free: a root
+// is hooked only under a smaller root, by one CAS on its own parent slot,
+// and finds halve their paths.  Parent pointers therefore only ever point
+// to smaller ids (no cycles, every pass ends), and a component's last root
+// is its minimum vertex id: the labels are the minimum vertex id of each
+// component whatever order the CASes land in.  The reference needs LDD to
+// keep its CPU union-find short; here each arc is one short find pair and
+// at most a few CASes, so the arcs go to the union-find at once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/psc.h"
+#include "device_common.hpp"
+#include "engine.hpp"
+
+namespace psc {
+namespace {
+
+constexpr uint32_t CC_HEAVY = 256;   // longer lists are shared by a wave
+
+__global__ void k_cc_init(uint32_t *p, uint64_t n) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    p[v] = (uint32_t)v;
+}
+
+// root of x; every visited slot is re-pointed at its grandparent (path
+// halving: the new parent is an ancestor, so parents keep decreasing)
+__device__ __forceinline__ uint32_t cc_find(uint32_t *p, uint32_t x) {
+  uint32_t px = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+  while (px != x) {
+    const uint32_t gp = __atomic_load_n(&p[px], __ATOMIC_RELAXED);
+    if (gp != px) __atomic_store_n(&p[x], gp, __ATOMIC_RELAXED);
+    x = px;
+    px = gp;
+  }
+  return x;
+}
+
+// union of the components of u and v: the larger root goes under the
+// smaller one if it is still a root, else both roots are found again
+__device__ __forceinline__ void cc_unite(uint32_t *p, uint32_t u, uint32_t v) {
+  while (true) {
+    u = cc_find(p, u);
+    v = cc_find(p, v);
+    if (u == v) return;
+    const uint32_t hi = u > v ? u : v, lo = u > v ? v : u;
+    if (atomicCAS(&p[hi], hi, lo) == hi) return;
+    u = hi;
+    v = lo;
+  }
+}
+
+// thread per vertex for lists up to CC_HEAVY arcs; longer lists are listed
+// for k_cc_heavy
+__global__ __launch_bounds__(BLOCK) void k_cc_light(const uint64_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ adj,
+                                                    uint64_t n, uint32_t *p, uint32_t *heavy,
+                                                    uint32_t *nheavy) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[u], e = off[u + 1];
+    const bool big = e - b > CC_HEAVY;
+    const uint32_t at = wave_append(nheavy, big);
+    if (big) {
+      heavy[at] = (uint32_t)u;
+      continue;
+    }
+    for (uint64_t j = b; j < e; ++j) cc_unite(p, (uint32_t)u, adj[j]);
+  }
+}
+
+// one wave per listed vertex, lanes over its arcs
+__global__ __launch_bounds__(BLOCK) void k_cc_heavy(const uint64_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ adj,
+                                                    const uint32_t *heavy,
+                                                    const uint32_t *nheavy, uint32_t *p) {
+  const uint32_t nh = *nheavy;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / WAVE);
+  for (uint64_t h = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; h < nh;
+       h += waves) {
+    const uint32_t u = heavy[h];
+    for (uint64_t j = off[u] + lane(); j < off[u + 1]; j += WAVE) cc_unite(p, u, adj[j]);
+  }
+}
+
+// *bad = 1 if an offset falls or leaves [0, m], or an endpoint is >= n
+__global__ void k_cc_check(const uint64_t *off, const uint32_t *adj, uint64_t n, uint64_t m,
+                           uint32_t *bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  for (uint64_t v = i0; v < n; v += stride)
+    if (off[v + 1] < off[v] || off[v + 1] > m) *bad = 1;
+  if (i0 == 0 && (off[0] != 0 || off[n] != m)) *bad = 1;
+  for (uint64_t j = i0; j < m; j += stride)
+    if (adj[j] >= n) *bad = 1;
+}
+
+// Final labels: every slot is written only by its own thread, with a root
+// walk that writes nothing (a halving store from another thread could
+// otherwise replace a written root by an older ancestor).  No hook happens
+// in this kernel, so every walk ends at the same root.
+__global__ void k_cc_labels(uint32_t *p, uint64_t n) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x = (uint32_t)v, px = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+    while (px != x) {
+      x = px;
+      px = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+    }
+    __atomic_store_n(&p[v], x, __ATOMIC_RELAXED);
+  }
+}
+
+unsigned grid_for_cc(uint64_t n) {
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 65536));
+}
+
+#define CC_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+      rc = PSC_ERR_HIP;                                                   \
+      goto done;                                                          \
+    }                                                                     \
+  } while (0)
+
+}  // namespace
+}  // namespace psc
+
+extern "C" {
+
+int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms) {
+  using namespace psc;
+  if (!dev_g || (dev_g->n && (!d_labels || !dev_g->offset)) ||
+      (dev_g->m && !dev_g->edges) || dev_g->n >= 0xFFFFFFFFull) {
+    set_error("psc_connectivity_device: bad arguments");
+    return PSC_ERR_INVALID;
+  }
+  const uint64_t n = dev_g->n;
+  int rc = PSC_OK;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  uint32_t *heavy = nullptr, *nheavy = nullptr;
+  uint32_t hbad = 0;
+  float t = 0;
+  if (n == 0) {
+    if (ms) *ms = 0;
+    return PSC_OK;
+  }
+  CC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  CC_HIP(hipEventCreate(&e0));
+  CC_HIP(hipEventCreate(&e1));
+  CC_HIP(hipMalloc(&heavy, n * 4));
+  CC_HIP(hipMalloc(&nheavy, 4));
+  // the input is checked before any kernel indexes with it
+  k_cc_init<<<1, 64, 0, st>>>(nheavy, 1);   // used as the flag: 0
+  k_cc_check<<<grid_for_cc(std::max<uint64_t>(n, dev_g->m)), BLOCK, 0, st>>>(
+      dev_g->offset, dev_g->edges, n, dev_g->m, nheavy);
+  CC_HIP(hipGetLastError());
+  CC_HIP(hipMemcpyAsync(&hbad, nheavy, 4, hipMemcpyDeviceToHost, st));
+  CC_HIP(hipStreamSynchronize(st));
+  if (hbad) {
+    set_error("psc_connectivity: offsets must run from 0 to m without falling, "
+              "endpoints must be < n");
+    rc = PSC_ERR_INVALID;
+    goto done;
+  }
+  CC_HIP(hipEventRecord(e0, st));
+  k_cc_init<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
+  k_cc_init<<<1, 64, 0, st>>>(nheavy, 1);   // nheavy = 0
+  k_cc_light<<<grid_for_cc(n), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges, n, d_labels,
+                                               heavy, nheavy);
+  k_cc_heavy<<<grid_for_cc(n * WAVE / 16), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges,
+                                                          heavy, nheavy, d_labels);
+  k_cc_labels<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
+  CC_HIP(hipGetLastError());
+  CC_HIP(hipEventRecord(e1, st));
+  CC_HIP(hipStreamSynchronize(st));
+  CC_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+done:
+  if (heavy) (void)hipFree(heavy);
+  if (nheavy) (void)hipFree(nheavy);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
+int psc_connectivity(const psc_graph *g, uint32_t *labels) {
+  using namespace psc;
+  if (!g || (g->n && (!labels || !g->offset)) || (g->m && !g->edges)) {
+    set_error("psc_connectivity: bad arguments");
+    return PSC_ERR_INVALID;
+  }
+  const uint64_t n = g->n, m = g->m;
+  int rc = PSC_OK;
+  uint64_t *off = nullptr;
+  uint32_t *adj = nullptr, *lab = nullptr;
+  psc_graph dg{};
+  if (n == 0) return PSC_OK;
+  CC_HIP(hipMalloc(&off, (n + 1) * 8));
+  CC_HIP(hipMalloc(&adj, std::max<uint64_t>(m, 1) * 4));
+  CC_HIP(hipMalloc(&lab, n * 4));
+  CC_HIP(hipMemcpy(off, g->offset, (n + 1) * 8, hipMemcpyHostToDevice));
+  if (m) CC_HIP(hipMemcpy(adj, g->edges, m * 4, hipMemcpyHostToDevice));
+  dg = psc_graph{n, m, off, adj, nullptr, nullptr};
+  rc = psc_connectivity_device(&dg, lab, nullptr);
+  if (rc == PSC_OK) CC_HIP(hipMemcpy(labels, lab, n * 4, hipMemcpyDeviceToHost));
+done:
+  if (off) (void)hipFree(off);
+  if (adj) (void)hipFree(adj);
+  if (lab) (void)hipFree(lab);
+  return rc;
+}
+
+}  // extern "C"

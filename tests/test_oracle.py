@@ -240,3 +240,61 @@ def test_pivot_perm_is_bijection(oracle):
     for N in (1, 2, 3, 7, 64, 1000, 4097):
         vals = [oracle.pivot_perm(N, 5, i) for i in range(N)]
         assert sorted(vals) == list(range(N))
+
+
+# ---------------------------------------------------------------- connectivity
+def _bfs_components(n, off, adj):
+    """Min vertex id per connected component of the undirected graph under
+    the arcs, by breadth-first search over an explicit neighbour list."""
+    nb = [[] for _ in range(n)]
+    for u in range(n):
+        for j in range(int(off[u]), int(off[u + 1])):
+            v = int(adj[j])
+            nb[u].append(v)
+            nb[v].append(u)
+    lab = [-1] * n
+    for s in range(n):
+        if lab[s] >= 0:
+            continue
+        lab[s] = s
+        q = [s]
+        for x in q:
+            for y in nb[x]:
+                if lab[y] < 0:
+                    lab[y] = s
+                    q.append(y)
+    return np.array(lab, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("n,m,seed", [(1, 0, 1), (2, 1, 2), (40, 30, 3), (200, 150, 4),
+                                      (300, 900, 5)])
+def test_cc_oracle_vs_bfs(psc, oracle, n, m, seed):
+    g = random_digraph(psc, n, m, seed, self_loops=True, dups=True)
+    assert np.array_equal(oracle.cc(g), _bfs_components(g.n, g.offset, g.edges))
+
+
+def test_cc_oracle_vs_scipy(psc, oracle):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    for n, m, seed in ((5000, 4000, 11), (20000, 30000, 12)):
+        g = random_digraph(psc, n, m, seed)
+        a = csr_matrix((np.ones(g.m), g.edges.astype(np.int64), g.offset.astype(np.int64)),
+                       shape=(n, n))
+        _, comp = connected_components(a, directed=True, connection="weak")
+        first = np.full(comp.max() + 1, n, dtype=np.int64)
+        np.minimum.at(first, comp, np.arange(n))
+        assert np.array_equal(oracle.cc(g), first[comp].astype(np.uint32))
+
+
+def test_cc_oracle_planted(oracle):
+    # two triangles {0, 2, 4} and {1, 3, 5} joined by nothing, 6 isolated;
+    # arcs point towards larger ids and away, so direction must not matter
+    src = [4, 2, 0, 5, 1]
+    dst = [2, 0, 4, 3, 5]
+    n = 7
+    order = np.argsort(src, kind="stable")
+    s, d = np.array(src)[order], np.array(dst)[order]
+    off = np.zeros(n + 1, dtype=np.uint64)
+    np.add.at(off, s + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    assert oracle.cc_csr(n, off, d).tolist() == [0, 1, 0, 1, 0, 1, 6]
