@@ -255,6 +255,41 @@ def bench_cc(args, psc):
     dg.free()
 
 
+def bench_le(args, psc):
+    """LE-lists (src/LeList.hpp:246-328) of the RMAT arcs (2^scale vertices,
+    2^samples_log2 samples, seed) for a seeded random source order, beta
+    1.5 (the reference's default, src/LeList.cpp:21): kernel time from
+    events, rows checked against the oracle's seq_le_list."""
+    import numpy as np
+    import oracle
+    scale, samples = args.scale, 1 << args.samples_log2
+    dg = psc.DeviceGraph.rmat(scale, samples, 0.57, 0.19, 0.19, seed=args.seed, permute=True)
+    order = psc.random_order(dg.n, args.seed)
+    for _ in range(args.warmup):
+        psc.le_lists(dg, order)
+    ms = []
+    for _ in range(args.steps):
+        rows = psc.le_lists(dg, order)
+        ms.append(psc.le_lists.ms)
+    dev_ms = float(np.median(ms))
+    log(f"le_lists: {dev_ms:.3f} ms, {rows.shape[0]} rows")
+    ok = None
+    cpu_s = None
+    if not args.no_cpu:
+        g = dg.to_host()
+        t = time.perf_counter()
+        expect = oracle.le_lists(g, order)
+        cpu_s = time.perf_counter() - t
+        ok = bool(np.array_equal(rows, expect))
+        log(f"oracle seq_le_list (1 thread): {cpu_s:.2f} s; rows identical: {ok}")
+    print(json.dumps({"metric": "LE-lists ms on RMAT", "value": round(dev_ms, 3), "unit": "ms",
+                      "higher_is_better": False, "n": dg.n, "m": dg.m,
+                      "rows": int(rows.shape[0]), "rows_equal_oracle": ok,
+                      "oracle_seconds_1_thread": None if cpu_s is None else round(cpu_s, 2)}),
+          flush=True)
+    dg.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +309,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-only", action="store_true",
                     help="warm-up + steps only (for rocprofv3 runs)")
-    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve", "copy", "cc"),
+    ap.add_argument("--mode", choices=("c3", "grid", "batch", "serve", "copy", "cc", "le"),
                     default="c3")
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--grid-side", type=int, default=2048)
@@ -298,6 +333,8 @@ def main():
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
     psc = ge.load_package()
+    if args.mode == "le":
+        return bench_le(args, psc)
     if args.mode == "cc":
         return bench_cc(args, psc)
     if args.mode == "copy":

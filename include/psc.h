@@ -273,6 +273,28 @@ int psc_read_bin_device(const char *path, int large, psc_dev_graph **out);
 int psc_connectivity(const psc_graph *g, uint32_t *labels);
 int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms);
 
+/* ------------------------------------------------------------ LE-lists
+ * Least-element lists (LE_LIST::le_list, src/LeList.hpp:127-328; the
+ * LeList app, src/LeList.cpp:5-58).  `order` is the reference's `centers`
+ * (a permutation of 0..n-1; the reference draws it at random): for each
+ * target t, in order of the sources, the rows (t, s, d(s, t)) whose
+ * distance along the out-arcs is below that of every earlier source.
+ * Rows are 3 uint32 (target, source vertex, distance) sorted by (target,
+ * position of the source in order) -- the reference's `list` after the
+ * filter.  beta (>= 1) grows the source batches as -beta does (default
+ * 1.5); the rows do not depend on it.  Only the out-CSR is read.
+ *
+ * psc_le_lists takes host arrays and returns *rows in host memory (free
+ * with psc_le_free); psc_le_lists_device takes device pointers, returns
+ * *d_rows in device memory (free with psc_le_free_device) and the kernels'
+ * time in *ms (may be NULL).  *count receives the number of rows. */
+int psc_le_lists(const psc_graph *g, const uint32_t *order, double beta, uint32_t **rows,
+                 uint64_t *count);
+int psc_le_lists_device(const psc_graph *dev_g, const uint32_t *d_order, double beta,
+                        uint32_t **d_rows, uint64_t *count, double *ms);
+void psc_le_free(uint32_t *rows);
+int psc_le_free_device(uint32_t *d_rows);
+
 /* Copy a device graph back to host arrays (for checking). */
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out);
 

@@ -11,6 +11,8 @@ ctypes bindings to liboracle.so:
   scc_ref(graph, ...)  CPU restatement of src/scc.hpp / reach.hpp /
                        hash_bag.h / resizable_table.h (scc_ref.cpp)
   reach(...)           REACH::reach, src/reach.hpp:208-243
+  le_lists(graph, order)  LE-lists, seq_le_list of src/LeList.hpp:76-115
+                       (le_list.c)
   cc(graph)            connected components (min vertex id per component)
                        of the undirected graph under the out-CSR: the
                        partition of src/Connectivity.h:37-119 (cc.c)
@@ -70,6 +72,9 @@ def _sig(name, res, *args):
 
 _sig("psc_oracle_tarjan", C.c_int64, C.c_uint64, u64p, u32p, u32p)
 _sig("psc_oracle_cc", C.c_int, C.c_uint64, u64p, u32p, u32p)
+_sig("psc_oracle_le_lists", C.c_uint64, C.c_uint64, u64p, u32p, u32p,
+     C.POINTER(u32p))
+_sig("psc_oracle_free", None, C.c_void_p)
 _sig("psc_oracle_canonical", C.c_int, C.c_uint64, u64p, u32p)
 _sig("psc_oracle_status", C.c_int, C.c_uint64, u32p, u64p, u64p)
 _sig("psc_oracle_scc_create", C.c_void_p, C.c_uint64, C.c_uint64, u64p, u32p,
@@ -114,6 +119,29 @@ def cc_csr(n, offset, edges):
 
 def cc(g):
     return cc_csr(g.n, g.offset, g.edges)
+
+
+def le_lists_csr(n, offset, edges, order):
+    """LE-lists of seq_le_list (src/LeList.hpp:76-115): (count, 3) uint32
+    rows (target, source vertex, distance) sorted by (target, position of
+    the source in `order`)."""
+    off = np.ascontiguousarray(offset, dtype=np.uint64)
+    e = np.ascontiguousarray(edges, dtype=np.uint32)
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    ptr = u32p()
+    cnt = lib.psc_oracle_le_lists(n, _p(off, u64p), _p(e, u32p), _p(o, u32p), C.byref(ptr))
+    if cnt == 2**64 - 1:
+        raise ValueError("psc_oracle_le_lists: out of memory or order not a permutation")
+    if cnt == 0:
+        return np.zeros((0, 3), dtype=np.uint32)
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(cnt, 3)).copy()
+    finally:
+        lib.psc_oracle_free(ptr)
+
+
+def le_lists(g, order):
+    return le_lists_csr(g.n, g.offset, g.edges, order)
 
 
 def tarjan_csr(n, offset, edges):

@@ -125,6 +125,12 @@ _sig("psc_dev_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double,
      C.c_uint64, C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_free", None, C.POINTER(CDevGraph))
 _sig("psc_connectivity", C.c_int, C.POINTER(CGraph), u32p)
+_sig("psc_le_lists", C.c_int, C.POINTER(CGraph), u32p, C.c_double, C.POINTER(u32p),
+     C.POINTER(C.c_uint64))
+_sig("psc_le_lists_device", C.c_int, C.POINTER(CGraph), C.c_void_p, C.c_double,
+     C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_double))
+_sig("psc_le_free", None, u32p)
+_sig("psc_le_free_device", C.c_int, C.c_void_p)
 _sig("psc_connectivity_device", C.c_int, C.POINTER(CGraph), C.c_void_p,
      C.POINTER(C.c_double))
 _sig("psc_dev_make_directed", C.c_int, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
@@ -256,6 +262,58 @@ def connectivity(g):
     else:
         _check("psc_connectivity", lib.psc_connectivity(C.byref(cg), _ptr(out, u32p)))
     return out
+
+
+def random_order(n, seed=1):
+    """A seeded random permutation of 0..n-1 (the role of the reference's
+    parlay::random_permutation for LE_LIST::centers, src/LeList.hpp:64)."""
+    return np.random.default_rng(seed).permutation(n).astype(np.uint32)
+
+
+def le_lists(g, order=None, beta=1.5, seed=1):
+    """LE-lists (LE_LIST::le_list, src/LeList.hpp:246-328) of g's out-arcs
+    for the source order `order` (default: random_order(g.n, seed)): a
+    (count, 3) uint32 array of rows (target, source vertex, distance),
+    sorted by (target, position of the source in order).  g is a Graph or
+    a DeviceGraph; for a DeviceGraph the kernels' time is in le_lists.ms."""
+    if order is None:
+        order = random_order(g.n, seed)
+    order = np.ascontiguousarray(order, dtype=np.uint32)
+    if order.shape != (g.n,):
+        raise PscError(f"le_lists: order must hold n = {g.n} entries")
+    cg = g.cgraph()
+    cnt = C.c_uint64()
+    if isinstance(g, DeviceGraph):
+        d_ord, d_rows = C.c_void_p(), C.c_void_p()
+        _check("psc_device_alloc", lib.psc_device_alloc(max(g.n * 4, 4), C.byref(d_ord)))
+        try:
+            if g.n:
+                _check("psc_device_copy",
+                       lib.psc_device_copy(d_ord, order.ctypes.data, g.n * 4))
+            ms = C.c_double()
+            _check("psc_le_lists_device",
+                   lib.psc_le_lists_device(C.byref(cg), d_ord, float(beta), C.byref(d_rows),
+                                           C.byref(cnt), C.byref(ms)))
+            out = np.empty((cnt.value, 3), dtype=np.uint32)
+            try:
+                if cnt.value:
+                    _check("psc_device_copy",
+                           lib.psc_device_copy(out.ctypes.data, d_rows, cnt.value * 12))
+            finally:
+                lib.psc_le_free_device(d_rows)
+            le_lists.ms = ms.value
+        finally:
+            lib.psc_device_free(d_ord)
+        return out
+    rows = u32p()
+    _check("psc_le_lists", lib.psc_le_lists(C.byref(cg), _ptr(order, u32p), float(beta),
+                                            C.byref(rows), C.byref(cnt)))
+    try:
+        if cnt.value == 0:
+            return np.zeros((0, 3), dtype=np.uint32)
+        return np.ctypeslib.as_array(rows, shape=(cnt.value, 3)).copy()
+    finally:
+        lib.psc_le_free(rows)
 
 
 def gen_rmat(scale, samples, a=0.57, b=0.19, c=0.19, seed=1, permute=True):

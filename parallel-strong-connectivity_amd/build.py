@@ -31,7 +31,8 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wextra",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip", "cc.hip"]
+HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip", "cc.hip",
+               "le_list.hip"]
 CXX_SOURCES = ["graph_host.cpp", "serve.cpp"]
 
 

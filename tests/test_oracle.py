@@ -298,3 +298,66 @@ def test_cc_oracle_planted(oracle):
     np.add.at(off, s + 1, 1)
     off = np.cumsum(off).astype(np.uint64)
     assert oracle.cc_csr(n, off, d).tolist() == [0, 1, 0, 1, 0, 1, 6]
+
+
+# ------------------------------------------------------------ LE-lists
+
+def _le_lists_brute(n, off, adj, order):
+    """The LE-list definition, computed without pruning: a full BFS from
+    every source (scipy's unweighted shortest paths), then for each target
+    the sources in order whose distance is below every earlier source's."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    a = csr_matrix((np.ones(len(adj)), np.asarray(adj, dtype=np.int64),
+                    np.asarray(off, dtype=np.int64)), shape=(n, n))
+    d = shortest_path(a, directed=True, unweighted=True, indices=np.asarray(order))
+    rows = []
+    for t in range(n):
+        best = np.inf
+        for i, s in enumerate(order):
+            if d[i, t] < best:
+                best = d[i, t]
+                rows.append((t, s, int(best)))
+    return np.array(rows, dtype=np.uint32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("n,m,seed", [(1, 0, 1), (6, 9, 2), (40, 80, 3), (200, 600, 4),
+                                      (300, 250, 5), (150, 3000, 6)])
+def test_le_lists_oracle_vs_brute_force(psc, oracle, n, m, seed):
+    g = random_digraph(psc, n, m, seed, self_loops=True, dups=True)
+    order = np.random.default_rng(seed).permutation(n).astype(np.uint32)
+    assert np.array_equal(oracle.le_lists(g, order), _le_lists_brute(g.n, g.offset, g.edges, order))
+
+
+def test_le_lists_oracle_closed_forms(psc, oracle):
+    n = 50
+    path = psc.Graph.from_edges(n, np.arange(0, n - 1), np.arange(1, n))   # 0 -> 1 -> ...
+    # order 0, 1, ...: every earlier source reaches t, each one step closer
+    got = oracle.le_lists(path, np.arange(n, dtype=np.uint32))
+    expect = [(t, s, t - s) for t in range(n) for s in range(t + 1)]
+    assert np.array_equal(got, np.array(expect, dtype=np.uint32))
+    # order n-1, ..., 0: the first source that reaches t is t itself
+    got = oracle.le_lists(path, np.arange(n - 1, -1, -1).astype(np.uint32))
+    assert np.array_equal(got, np.array([(t, t, 0) for t in range(n)], dtype=np.uint32))
+    # a star with arcs out of the hub 0, hub first: one entry per vertex
+    # from the hub, plus each leaf's own entry at distance 0
+    star = psc.Graph.from_edges(n, np.zeros(n - 1, dtype=np.int64), np.arange(1, n))
+    got = oracle.le_lists(star, np.arange(n, dtype=np.uint32))
+    expect = [(0, 0, 0)] + [r for t in range(1, n) for r in ((t, 0, 1), (t, t, 0))]
+    assert np.array_equal(got, np.array(expect, dtype=np.uint32))
+
+
+def test_le_lists_oracle_rejects_bad_order(psc, oracle):
+    g = psc.Graph.from_edges(3, [0, 1], [1, 2])
+    with pytest.raises(ValueError):
+        oracle.le_lists(g, np.array([0, 0, 1], dtype=np.uint32))
+
+
+def test_le_lists_brute_force_catches_planted_mistake(psc, oracle):
+    # a plausible slip (keeping entries whose distance only ties an earlier
+    # source's) must differ from the definition on a graph with ties
+    g = psc.Graph.from_edges(4, [0, 1, 0, 2], [1, 3, 2, 3])
+    order = np.array([0, 1, 2, 3], dtype=np.uint32)
+    good = _le_lists_brute(4, g.offset, g.edges, order)
+    assert [3, 2, 1] not in good.tolist()   # d(2,3) = 1 ties d(1,3) = 1
+    assert np.array_equal(oracle.le_lists(g, order), good)
