@@ -37,7 +37,8 @@
 // sum exceeds the buffers as they are, the arcs that pass both tests are
 // counted first for a tight bound.  The expansion walks, a wave at a time,
 // the concatenated arc lists of 64 frontier pairs (coalesced, balanced
-// across lanes); the count gives a lane its own list up to LE_LIGHT arcs
+// across lanes); lists over LE_HEAVY arcs are dealt over every wave of the
+// grid by a second kernel; the count gives a lane its own list up to LE_LIGHT arcs
 // and shares longer lists across the wave.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -66,6 +67,7 @@ struct LeCtl {
   unsigned long long nnext;   // pairs added by the current sub-round
   unsigned long long deg;     // out-degree sum of the current frontier
   unsigned long long dnext;   // out-degree sum of the pairs added so far
+  unsigned long long nheavy;  // frontier pairs with more than LE_HEAVY arcs
   uint32_t bad;               // input check
   uint32_t pad;
 };
@@ -115,6 +117,7 @@ __global__ void k_le_seed(const uint32_t *order, uint64_t s0, uint64_t k,
 
 // a batch of k sources begins
 __global__ void k_le_begin(LeCtl *ctl, uint64_t k) {
+  ctl->nheavy = 0;
   ctl->nf = k;
   ctl->nnext = 0;
   ctl->cnt = 0;
@@ -165,6 +168,9 @@ __global__ __launch_bounds__(BLOCK) void k_le_count(const unsigned long long *fr
 }
 
 constexpr uint32_t LE_STAGE = 256;   // new pairs staged per wave before a flush
+constexpr uint64_t LE_HEAVY = 4096;  // longer lists are spread over the grid
+constexpr uint32_t LE_HSEG = 2048;   // heavy pairs per LDS prefix segment
+static_assert(LE_HSEG % BLOCK == 0, "a block builds the segment prefix in equal shares");
 
 // the staged pairs of a wave go to the next frontier and to the list at
 // lbase + their frontier slot, with one atomic for all of them
@@ -185,19 +191,45 @@ __device__ __forceinline__ void le_flush(const unsigned long long *stage, uint32
   wave_sync();
 }
 
+// one arc (po -> w) of a wave step: a pair (p, w) is new when an
+// atomicMin lowers bmin[w] from above p (:190-215); new pairs are staged
+// in the wave's LDS slice and flushed LE_STAGE at a time.  The whole wave
+// calls it together.
+__device__ __forceinline__ void le_take(uint32_t po, uint32_t w, bool want, uint32_t r,
+                                        const uint32_t *__restrict__ dist, uint32_t *bmin,
+                                        const uint64_t *__restrict__ off,
+                                        unsigned long long *stage, uint32_t &sn,
+                                        uint64_t &degs, unsigned long long *nxt,
+                                        unsigned long long *lkey, uint32_t *ld, uint64_t lbase,
+                                        uint32_t bw, uint32_t *next_dist, LeCtl *ctl) {
+  bool ins = false;
+  if (want && le_passes(po, w, r, dist, bmin)) ins = atomicMin(&bmin[w], po) > po;
+  const uint64_t m = ballot(ins);
+  if (ins) {
+    stage[sn + lanes_below(m)] = make_pair(po, w);
+    // every writer of this launch writes r; earlier sub-rounds wrote less
+    if (next_dist[w] > r) next_dist[w] = r;
+    degs += off[w + 1] - off[w];
+  }
+  sn += (uint32_t)__popcll(m);
+  if (sn > LE_STAGE - WAVE) {
+    le_flush(stage, sn, nxt, lkey, ld, lbase, bw, r, ctl);
+    sn = 0;
+  }
+}
+
 // A wave takes 64 frontier pairs and walks the concatenation of their arc
 // lists, 64 consecutive arcs per step: lane t of a step finds the pair that
 // owns arc t by a binary search over the wave's exclusive degree prefix
 // (held one entry per lane), so adjacency reads are coalesced and no lane
-// idles while another walks a long list.  A pair (p, w) is new when an
-// atomicMin lowers bmin[w] from above p (:190-215); new pairs are staged in
-// LDS and flushed LE_STAGE at a time.  The out-degree sum of the new pairs
-// bounds the next sub-round.
+// idles while another walks a long list.  Pairs with more than LE_HEAVY
+// arcs are listed for k_le_heavy instead.  The out-degree sum of the new
+// pairs bounds the next sub-round.
 __global__ __launch_bounds__(BLOCK) void k_le_expand(
     const unsigned long long *front, unsigned long long *nxt, LeCtl *ctl,
     const uint64_t *__restrict__ off, const uint32_t *__restrict__ adj,
     const uint32_t *__restrict__ dist, uint32_t r, uint32_t *bmin, unsigned long long *lkey,
-    uint32_t *ld, uint64_t lbase, uint32_t bw, uint32_t *next_dist) {
+    uint32_t *ld, uint64_t lbase, uint32_t bw, uint32_t *next_dist, unsigned long long *heavy) {
   __shared__ unsigned long long stage_all[BLOCK / WAVE][LE_STAGE];
   unsigned long long *stage = stage_all[threadIdx.x / WAVE];
   uint32_t sn = 0;   // staged pairs (the same in every lane)
@@ -210,7 +242,11 @@ __global__ __launch_bounds__(BLOCK) void k_le_expand(
     const bool valid = i < nf;
     const unsigned long long pr = valid ? front[i] : 0;
     const uint32_t p = pair_vertex(pr), v = pair_label(pr);
-    const uint64_t b = valid ? off[v] : 0, len = valid ? off[v + 1] - b : 0;
+    const uint64_t b = valid ? off[v] : 0, full = valid ? off[v + 1] - b : 0;
+    const bool big = full > LE_HEAVY;
+    const unsigned long long hat = wave_append64(&ctl->nheavy, big);
+    if (big) heavy[hat] = pr;
+    const uint64_t len = big ? 0 : full;
     const uint64_t incl = wave_incl_scan(len), excl = incl - len;
     const uint64_t total = __shfl(incl, WAVE - 1);
     for (uint64_t t0 = 0; t0 < total; t0 += WAVE) {
@@ -224,25 +260,81 @@ __global__ __launch_bounds__(BLOCK) void k_le_expand(
       }
       const uint32_t po = __shfl(p, o);
       const uint64_t j = __shfl(b, o) + (t - __shfl(excl, o));
-      bool ins = false;
-      uint32_t w = 0;
-      if (t < total) {
-        w = adj[j];
-        if (le_passes(po, w, r, dist, bmin)) ins = atomicMin(&bmin[w], po) > po;
-      }
-      const uint64_t m = ballot(ins);
-      if (ins) {
-        stage[sn + lanes_below(m)] = make_pair(po, w);
-        // every writer of this launch writes r; earlier sub-rounds wrote less
-        if (next_dist[w] > r) next_dist[w] = r;
-        degs += off[w + 1] - off[w];
-      }
-      sn += (uint32_t)__popcll(m);
-      if (sn > LE_STAGE - WAVE) {
-        le_flush(stage, sn, nxt, lkey, ld, lbase, bw, r, ctl);
-        sn = 0;
-      }
+      le_take(po, t < total ? adj[j] : 0, t < total, r, dist, bmin, off, stage, sn, degs, nxt,
+              lkey, ld, lbase, bw, next_dist, ctl);
     }
+  }
+  if (sn) le_flush(stage, sn, nxt, lkey, ld, lbase, bw, r, ctl);
+  degs = wave_sum(degs);
+  if (lane() == 0 && degs) atomicAdd(&ctl->dnext, (unsigned long long)degs);
+}
+
+// The arcs of the listed heavy pairs, spread over every wave of the grid:
+// a segment of up to LE_HSEG pairs has its degree prefix built in LDS by
+// each block, and wave steps of 64 consecutive arcs are dealt round-robin
+// over the grid's waves; a lane finds its arc's pair by binary search in
+// the prefix.
+__global__ __launch_bounds__(BLOCK) void k_le_heavy(
+    const unsigned long long *heavy, unsigned long long *nxt, LeCtl *ctl,
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ adj,
+    const uint32_t *__restrict__ dist, uint32_t r, uint32_t *bmin, unsigned long long *lkey,
+    uint32_t *ld, uint64_t lbase, uint32_t bw, uint32_t *next_dist) {
+  __shared__ unsigned long long stage_all[BLOCK / WAVE][LE_STAGE];
+  __shared__ unsigned long long pre[LE_HSEG + 1];
+  __shared__ unsigned long long part[BLOCK / WAVE];
+  unsigned long long *stage = stage_all[threadIdx.x / WAVE];
+  uint32_t sn = 0;
+  uint64_t degs = 0;
+  const uint64_t nh = ctl->nheavy;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (BLOCK / WAVE);
+  const uint64_t gw = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
+  constexpr uint32_t PER = LE_HSEG / BLOCK;   // prefix entries per thread
+  for (uint64_t s0 = 0; s0 < nh; s0 += LE_HSEG) {
+    const uint32_t cnt = (uint32_t)(nh - s0 < LE_HSEG ? nh - s0 : LE_HSEG);
+    // pre[i] = arcs of the segment's pairs before pair i (block scan)
+    uint64_t mine[PER];
+    uint64_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+      const uint32_t i = threadIdx.x * PER + q;
+      uint64_t len = 0;
+      if (i < cnt) {
+        const uint32_t v = pair_label(heavy[s0 + i]);
+        len = off[v + 1] - off[v];
+      }
+      mine[q] = len;
+      sum += len;
+    }
+    const uint64_t winc = wave_incl_scan(sum);
+    if (lane() == WAVE - 1) part[threadIdx.x / WAVE] = winc;
+    __syncthreads();
+    uint64_t before = winc - sum;
+    for (uint32_t wv = 0; wv < threadIdx.x / WAVE; ++wv) before += part[wv];
+#pragma unroll
+    for (uint32_t q = 0; q < PER; ++q) {
+      pre[threadIdx.x * PER + q] = before;
+      before += mine[q];
+    }
+    if (threadIdx.x == BLOCK - 1) pre[LE_HSEG] = before;
+    __syncthreads();
+    const uint64_t total = pre[LE_HSEG];
+    for (uint64_t t0 = gw * WAVE; t0 < total; t0 += nwaves * WAVE) {
+      const uint64_t t = t0 + lane();
+      const bool want = t < total;
+      // largest i < cnt with pre[i] <= t
+      uint32_t lo = 0, hi = cnt - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const unsigned long long pr = heavy[s0 + lo];
+      const uint32_t v = pair_label(pr);
+      const uint64_t j = off[v] + (t - pre[lo]);
+      le_take(pair_vertex(pr), want ? adj[j] : 0, want, r, dist, bmin, off, stage, sn, degs,
+              nxt, lkey, ld, lbase, bw, next_dist, ctl);
+    }
+    __syncthreads();   // pre[] is rebuilt for the next segment
   }
   if (sn) le_flush(stage, sn, nxt, lkey, ld, lbase, bw, r, ctl);
   degs = wave_sum(degs);
@@ -252,6 +344,7 @@ __global__ __launch_bounds__(BLOCK) void k_le_expand(
 // next sub-round: the added pairs become the frontier, their degree sum
 // the bound of its arcs
 __global__ void k_le_advance(LeCtl *ctl) {
+  ctl->nheavy = 0;
   ctl->nf = ctl->nnext;
   ctl->nnext = 0;
   ctl->cnt = 0;
@@ -337,7 +430,7 @@ struct LeBuf {
 struct LeRun {
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  LeBuf ctl, dist, next_dist, seen, bmin, front[2], lkey, ld, skey, sd, keep, pos, temp;
+  LeBuf ctl, dist, next_dist, seen, bmin, heavy, front[2], lkey, ld, skey, sd, keep, pos, temp;
   LeCtl h{};
   ~LeRun() {
     if (e0) (void)hipEventDestroy(e0);
@@ -426,10 +519,16 @@ struct LeRun {
           if (bound == 0) break;
         }
         LE_HIP(front[cf ^ 1].ensure(bound * 8, 0, st));
+        LE_HIP(heavy.ensure(std::max<uint64_t>(h.nf, 1) * 8, 0, st));   // at most nf listed
         LE_HIP(lkey.ensure((nlist + bound) * 8, nlist * 8, st));
         LE_HIP(ld.ensure((nlist + bound) * 4, nlist * 4, st));
         k_le_expand<<<gwave, BLOCK, 0, st>>>(
             front[cf].as<unsigned long long>(), front[cf ^ 1].as<unsigned long long>(),
+            ctl.as<LeCtl>(), g->offset, g->edges, dist.as<uint32_t>(), r, bmin.as<uint32_t>(),
+            lkey.as<unsigned long long>(), ld.as<uint32_t>(), nlist, bw,
+            next_dist.as<uint32_t>(), heavy.as<unsigned long long>());
+        k_le_heavy<<<gwave, BLOCK, 0, st>>>(
+            heavy.as<unsigned long long>(), front[cf ^ 1].as<unsigned long long>(),
             ctl.as<LeCtl>(), g->offset, g->edges, dist.as<uint32_t>(), r, bmin.as<uint32_t>(),
             lkey.as<unsigned long long>(), ld.as<uint32_t>(), nlist, bw,
             next_dist.as<uint32_t>());
