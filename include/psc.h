@@ -266,8 +266,10 @@ int psc_read_bin_device(const char *path, int large, psc_dev_graph **out);
  * joins its endpoints): the partition of Connectivity
  * (src/Connectivity.h:37-119, LDD + union-find).  The reference's label
  * values depend on the order of its concurrent CASes; here labels[v] is
- * the minimum vertex id of v's component, the same on every run.  Only the
- * out-CSR is read (in_offset / in_edges may be NULL).  The device version
+ * the minimum vertex id of v's component, the same on every run.  The
+ * in-CSR is optional (in_offset / in_edges may be NULL); when given it must
+ * be the transpose of the out-CSR, and vertices of the largest component
+ * found by sampling then skip their remaining arcs.  The device version
  * takes device pointers and returns the kernels' time in *ms (may be
  * NULL). */
 int psc_connectivity(const psc_graph *g, uint32_t *labels);

@@ -9,8 +9,12 @@
 // to smaller ids (no cycles, every pass ends), and a component's last root
 // is its minimum vertex id: the labels are the minimum vertex id of each
 // component whatever order the CASes land in.  The reference needs LDD to
-// keep its CPU union-find short; here each arc is one short find pair and
-// at most a few CASes, so the arcs go to the union-find at once.
+// keep its CPU union-find short; here the arcs go to the union-find at once,
+// after Afforest's sampling (Sutton et al., IPDPS 2018): every vertex first
+// links CC_SAMPLE out-arcs, the most frequent root among CC_PROBES spread
+// vertices is taken as the giant component, and -- when the graph carries
+// its in-CSR -- vertices already in it skip their remaining arcs, which
+// their other ends link from the in-CSR or the out-CSR.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,36 +62,80 @@ __device__ __forceinline__ void cc_unite(uint32_t *p, uint32_t u, uint32_t v) {
   }
 }
 
-// thread per vertex for lists up to CC_HEAVY arcs; longer lists are listed
-// for k_cc_heavy
-__global__ __launch_bounds__(BLOCK) void k_cc_light(const uint64_t *__restrict__ off,
-                                                    const uint32_t *__restrict__ adj,
-                                                    uint64_t n, uint32_t *p, uint32_t *heavy,
-                                                    uint32_t *nheavy) {
+constexpr uint32_t CC_SAMPLE = 2;      // out-arcs per vertex linked first
+constexpr uint32_t CC_PROBES = 1024;   // vertices whose roots elect the giant component
+constexpr uint32_t CC_NONE = 0xFFFFFFFFu;
+
+// sampling (Afforest): every vertex links its first CC_SAMPLE out-arcs
+__global__ void k_cc_sample(const uint64_t *__restrict__ off, const uint32_t *__restrict__ adj,
+                            uint64_t n, uint32_t *p) {
   for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
        u += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t b = off[u], e = off[u + 1];
-    const bool big = e - b > CC_HEAVY;
+    for (uint64_t j = b; j < e && j < b + CC_SAMPLE; ++j) cc_unite(p, (uint32_t)u, adj[j]);
+  }
+}
+
+// roots of CC_PROBES spread vertices, for the host to elect the most
+// frequent one
+__global__ void k_cc_probe(uint32_t *p, uint64_t n, uint32_t *roots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < CC_PROBES) roots[i] = cc_find(p, (uint32_t)(hash32(i) % n));
+}
+
+// The remaining arcs, a thread per vertex for lists up to CC_HEAVY arcs
+// (longer ones are listed for k_cc_heavy): out-arcs after the sampled ones
+// and, with an in-CSR, every in-arc.  A vertex whose root is `giant` (its
+// set holds the elected vertex) is skipped: an arc with both ends in that
+// set joins nothing new, and any other arc is also an in- or out-arc of
+// its end outside the set, which links it.  Skipping only ever happens
+// for a vertex already joined to the giant root, so it is exact whatever
+// the order of hooks.
+__device__ __forceinline__ bool cc_skip(uint32_t *p, uint32_t u, uint32_t giant) {
+  return giant != CC_NONE && cc_find(p, u) == giant;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cc_light(const uint64_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ adj,
+                                                    const uint64_t *__restrict__ ioff,
+                                                    const uint32_t *__restrict__ iadj,
+                                                    uint64_t n, uint32_t skip0, uint32_t giant,
+                                                    uint32_t *p, uint32_t *heavy,
+                                                    uint32_t *nheavy) {
+  for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = off[u + 1], b = off[u] + skip0 < e ? off[u] + skip0 : e;
+    const uint64_t ib = ioff ? ioff[u] : 0, ie = ioff ? ioff[u + 1] : 0;
+    const bool any = e > b || ie > ib;
+    const bool run = any && !cc_skip(p, (uint32_t)u, giant);
+    const bool big = run && (e - b) + (ie - ib) > CC_HEAVY;
     const uint32_t at = wave_append(nheavy, big);
     if (big) {
       heavy[at] = (uint32_t)u;
       continue;
     }
+    if (!run) continue;
     for (uint64_t j = b; j < e; ++j) cc_unite(p, (uint32_t)u, adj[j]);
+    for (uint64_t j = ib; j < ie; ++j) cc_unite(p, (uint32_t)u, iadj[j]);
   }
 }
 
-// one wave per listed vertex, lanes over its arcs
+// one wave per listed vertex, lanes over its remaining arcs
 __global__ __launch_bounds__(BLOCK) void k_cc_heavy(const uint64_t *__restrict__ off,
                                                     const uint32_t *__restrict__ adj,
-                                                    const uint32_t *heavy,
+                                                    const uint64_t *__restrict__ ioff,
+                                                    const uint32_t *__restrict__ iadj,
+                                                    uint32_t skip0, const uint32_t *heavy,
                                                     const uint32_t *nheavy, uint32_t *p) {
   const uint32_t nh = *nheavy;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / WAVE);
   for (uint64_t h = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; h < nh;
        h += waves) {
     const uint32_t u = heavy[h];
-    for (uint64_t j = off[u] + lane(); j < off[u + 1]; j += WAVE) cc_unite(p, u, adj[j]);
+    const uint64_t e = off[u + 1], b = off[u] + skip0 < e ? off[u] + skip0 : e;
+    for (uint64_t j = b + lane(); j < e; j += WAVE) cc_unite(p, u, adj[j]);
+    if (ioff)
+      for (uint64_t j = ioff[u] + lane(); j < ioff[u + 1]; j += WAVE) cc_unite(p, u, iadj[j]);
   }
 }
 
@@ -149,9 +197,13 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   int rc = PSC_OK;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  uint32_t *heavy = nullptr, *nheavy = nullptr;
-  uint32_t hbad = 0;
+  uint32_t *heavy = nullptr, *nheavy = nullptr, *roots = nullptr;
+  uint32_t hbad = 0, giant = CC_NONE;
+  uint32_t hroots[CC_PROBES];
   float t = 0;
+  // with the in-CSR (the transpose of the out-CSR) the giant component's
+  // vertices can be skipped; without it every arc is linked
+  const bool has_in = dev_g->m && dev_g->in_offset && dev_g->in_edges;
   if (n == 0) {
     if (ms) *ms = 0;
     return PSC_OK;
@@ -161,10 +213,14 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   CC_HIP(hipEventCreate(&e1));
   CC_HIP(hipMalloc(&heavy, n * 4));
   CC_HIP(hipMalloc(&nheavy, 4));
+  CC_HIP(hipMalloc(&roots, CC_PROBES * 4));
   // the input is checked before any kernel indexes with it
   k_cc_init<<<1, 64, 0, st>>>(nheavy, 1);   // used as the flag: 0
   k_cc_check<<<grid_for_cc(std::max<uint64_t>(n, dev_g->m)), BLOCK, 0, st>>>(
       dev_g->offset, dev_g->edges, n, dev_g->m, nheavy);
+  if (has_in)
+    k_cc_check<<<grid_for_cc(std::max<uint64_t>(n, dev_g->m)), BLOCK, 0, st>>>(
+        dev_g->in_offset, dev_g->in_edges, n, dev_g->m, nheavy);
   CC_HIP(hipGetLastError());
   CC_HIP(hipMemcpyAsync(&hbad, nheavy, 4, hipMemcpyDeviceToHost, st));
   CC_HIP(hipStreamSynchronize(st));
@@ -177,10 +233,28 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   CC_HIP(hipEventRecord(e0, st));
   k_cc_init<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
   k_cc_init<<<1, 64, 0, st>>>(nheavy, 1);   // nheavy = 0
-  k_cc_light<<<grid_for_cc(n), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges, n, d_labels,
-                                               heavy, nheavy);
-  k_cc_heavy<<<grid_for_cc(n * WAVE / 16), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges,
-                                                          heavy, nheavy, d_labels);
+  k_cc_sample<<<grid_for_cc(n), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges, n, d_labels);
+  if (has_in) {   // elect the most frequent root of the probes
+    k_cc_probe<<<(CC_PROBES + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(d_labels, n, roots);
+    CC_HIP(hipGetLastError());
+    CC_HIP(hipMemcpyAsync(hroots, roots, sizeof hroots, hipMemcpyDeviceToHost, st));
+    CC_HIP(hipStreamSynchronize(st));
+    std::sort(hroots, hroots + CC_PROBES);
+    uint32_t best = 0;
+    for (uint32_t i = 0, j; i < CC_PROBES; i = j) {
+      for (j = i; j < CC_PROBES && hroots[j] == hroots[i]; ++j) {}
+      if (j - i > best) {
+        best = j - i;
+        giant = hroots[i];
+      }
+    }
+  }
+  k_cc_light<<<grid_for_cc(n), BLOCK, 0, st>>>(
+      dev_g->offset, dev_g->edges, has_in ? dev_g->in_offset : nullptr,
+      has_in ? dev_g->in_edges : nullptr, n, CC_SAMPLE, giant, d_labels, heavy, nheavy);
+  k_cc_heavy<<<grid_for_cc(n * WAVE / 16), BLOCK, 0, st>>>(
+      dev_g->offset, dev_g->edges, has_in ? dev_g->in_offset : nullptr,
+      has_in ? dev_g->in_edges : nullptr, CC_SAMPLE, heavy, nheavy, d_labels);
   k_cc_labels<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
   CC_HIP(hipGetLastError());
   CC_HIP(hipEventRecord(e1, st));
@@ -190,6 +264,7 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
 done:
   if (heavy) (void)hipFree(heavy);
   if (nheavy) (void)hipFree(nheavy);
+  if (roots) (void)hipFree(roots);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (st) (void)hipStreamDestroy(st);
@@ -204,21 +279,30 @@ int psc_connectivity(const psc_graph *g, uint32_t *labels) {
   }
   const uint64_t n = g->n, m = g->m;
   int rc = PSC_OK;
-  uint64_t *off = nullptr;
-  uint32_t *adj = nullptr, *lab = nullptr;
+  uint64_t *off = nullptr, *ioff = nullptr;
+  uint32_t *adj = nullptr, *iadj = nullptr, *lab = nullptr;
   psc_graph dg{};
+  const bool has_in = m && g->in_offset && g->in_edges;
   if (n == 0) return PSC_OK;
   CC_HIP(hipMalloc(&off, (n + 1) * 8));
   CC_HIP(hipMalloc(&adj, std::max<uint64_t>(m, 1) * 4));
   CC_HIP(hipMalloc(&lab, n * 4));
   CC_HIP(hipMemcpy(off, g->offset, (n + 1) * 8, hipMemcpyHostToDevice));
   if (m) CC_HIP(hipMemcpy(adj, g->edges, m * 4, hipMemcpyHostToDevice));
-  dg = psc_graph{n, m, off, adj, nullptr, nullptr};
+  if (has_in) {
+    CC_HIP(hipMalloc(&ioff, (n + 1) * 8));
+    CC_HIP(hipMalloc(&iadj, m * 4));
+    CC_HIP(hipMemcpy(ioff, g->in_offset, (n + 1) * 8, hipMemcpyHostToDevice));
+    CC_HIP(hipMemcpy(iadj, g->in_edges, m * 4, hipMemcpyHostToDevice));
+  }
+  dg = psc_graph{n, m, off, adj, ioff, iadj};
   rc = psc_connectivity_device(&dg, lab, nullptr);
   if (rc == PSC_OK) CC_HIP(hipMemcpy(labels, lab, n * 4, hipMemcpyDeviceToHost));
 done:
   if (off) (void)hipFree(off);
   if (adj) (void)hipFree(adj);
+  if (ioff) (void)hipFree(ioff);
+  if (iadj) (void)hipFree(iadj);
   if (lab) (void)hipFree(lab);
   return rc;
 }

@@ -48,3 +48,28 @@ def test_cc_rejects_bad_input(psc):
     g.edges[1] = 9   # endpoint >= n, caught before any kernel indexes with it
     with pytest.raises(psc.PscError):
         psc.connectivity(g)
+
+
+def test_cc_without_in_csr(psc, oracle):
+    # out-CSR only: no vertex may skip its arcs, every arc is linked
+    import ctypes as C
+    g = random_digraph(psc, 20000, 30000, 8, self_loops=True, dups=True)
+    cg = g.cgraph()
+    cg.in_offset = None
+    cg.in_edges = None
+    out = np.empty(g.n, dtype=np.uint32)
+    rc = psc.lib.psc_connectivity(C.byref(cg), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    assert rc == 0
+    assert np.array_equal(out, oracle.cc(g))
+
+
+def test_cc_device_transpose_random(psc, oracle):
+    # device graphs built by make_directed carry the in-CSR: the skip path
+    for n, m, seed in ((3000, 2500, 11), (5000, 40000, 12), (200, 150, 13)):
+        h = random_digraph(psc, n, m, seed, self_loops=True, dups=True)
+        d = psc.DeviceGraph.make_directed(h.n, h.offset, h.edges)
+        try:
+            got = psc.connectivity(d)
+        finally:
+            d.free()
+        assert np.array_equal(got, oracle.cc(h))
