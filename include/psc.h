@@ -275,6 +275,18 @@ int psc_read_bin_device(const char *path, int large, psc_dev_graph **out);
 int psc_connectivity(const psc_graph *g, uint32_t *labels);
 int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms);
 
+/* Connectivity with spanning_forest = true (src/Connectivity.h:37-40,
+ * 100-118): the labels as above, and in forest[2 i], forest[2 i + 1]
+ * (i < *count = n - number of components) the arcs (u, v) of a spanning
+ * forest -- the arcs whose union-find hook joined two components.  The
+ * forest holds up to n - 1 arcs (2 (n - 1) uint32).  Which arcs are
+ * chosen depends on the order of concurrent hooks, as in the reference;
+ * the partition they span does not. */
+int psc_spanning_forest(const psc_graph *g, uint32_t *labels, uint32_t *forest,
+                        uint64_t *count);
+int psc_spanning_forest_device(const psc_graph *dev_g, uint32_t *d_labels, uint32_t *d_forest,
+                               uint64_t *count, double *ms);
+
 /* ------------------------------------------------------------ LE-lists
  * Least-element lists (LE_LIST::le_list, src/LeList.hpp:127-328; the
  * LeList app, src/LeList.cpp:5-58).  `order` is the reference's `centers`

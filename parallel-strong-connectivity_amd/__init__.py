@@ -125,6 +125,9 @@ _sig("psc_dev_gen_grid", C.c_int, C.c_uint32, C.c_uint32, C.c_double,
      C.c_uint64, C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_free", None, C.POINTER(CDevGraph))
 _sig("psc_connectivity", C.c_int, C.POINTER(CGraph), u32p)
+_sig("psc_spanning_forest", C.c_int, C.POINTER(CGraph), u32p, u32p, C.POINTER(C.c_uint64))
+_sig("psc_spanning_forest_device", C.c_int, C.POINTER(CGraph), C.c_void_p, C.c_void_p,
+     C.POINTER(C.c_uint64), C.POINTER(C.c_double))
 _sig("psc_le_lists", C.c_int, C.POINTER(CGraph), u32p, C.c_double, C.POINTER(u32p),
      C.POINTER(C.c_uint64))
 _sig("psc_le_lists_device", C.c_int, C.POINTER(CGraph), C.c_void_p, C.c_double,
@@ -262,6 +265,42 @@ def connectivity(g):
     else:
         _check("psc_connectivity", lib.psc_connectivity(C.byref(cg), _ptr(out, u32p)))
     return out
+
+
+def spanning_forest(g):
+    """Connectivity(G, beta, pred, spanning_forest=True)
+    (src/Connectivity.h:37-40,100-118): (labels, forest) with labels as in
+    connectivity(g) and forest a (n - components, 2) uint32 array of arcs
+    (u, v) of g that span its components."""
+    labels = np.empty(g.n, dtype=np.uint32)
+    forest = np.empty((max(g.n - 1, 1), 2), dtype=np.uint32)
+    cnt = C.c_uint64()
+    cg = g.cgraph()
+    if isinstance(g, DeviceGraph):
+        d, df = C.c_void_p(), C.c_void_p()
+        _check("psc_device_alloc", lib.psc_device_alloc(max(g.n * 4, 4), C.byref(d)))
+        try:
+            _check("psc_device_alloc",
+                   lib.psc_device_alloc(max((g.n - 1) * 8, 8), C.byref(df)))
+            ms = C.c_double()
+            _check("psc_spanning_forest_device",
+                   lib.psc_spanning_forest_device(C.byref(cg), d, df, C.byref(cnt),
+                                                  C.byref(ms)))
+            if g.n:
+                _check("psc_device_copy", lib.psc_device_copy(labels.ctypes.data, d, g.n * 4))
+            if cnt.value:
+                _check("psc_device_copy",
+                       lib.psc_device_copy(forest.ctypes.data, df, cnt.value * 8))
+            spanning_forest.ms = ms.value
+        finally:
+            lib.psc_device_free(d)
+            if df.value:
+                lib.psc_device_free(df)
+    else:
+        _check("psc_spanning_forest",
+               lib.psc_spanning_forest(C.byref(cg), _ptr(labels, u32p), _ptr(forest, u32p),
+                                       C.byref(cnt)))
+    return labels, forest[:cnt.value].copy()
 
 
 def random_order(n, seed=1):

@@ -49,14 +49,31 @@ __device__ __forceinline__ uint32_t cc_find(uint32_t *p, uint32_t x) {
 }
 
 // union of the components of u and v: the larger root goes under the
-// smaller one if it is still a root, else both roots are found again
-__device__ __forceinline__ void cc_unite(uint32_t *p, uint32_t u, uint32_t v) {
+// smaller one if it is still a root, else both roots are found again.
+// With a forest buffer, the arc (u, v) of every hook is recorded: each
+// hook joins two sets, so the recorded arcs form a spanning forest
+// (Connectivity's spanning_forest, src/Connectivity.h:100-118).
+struct CcForest {
+  uint32_t *arcs;             // 2 per hook, at most n - 1 hooks
+  unsigned long long *count;
+};
+
+__device__ __forceinline__ void cc_unite(uint32_t *p, uint32_t u0, uint32_t v0,
+                                         CcForest f) {
+  uint32_t u = u0, v = v0;
   while (true) {
     u = cc_find(p, u);
     v = cc_find(p, v);
     if (u == v) return;
     const uint32_t hi = u > v ? u : v, lo = u > v ? v : u;
-    if (atomicCAS(&p[hi], hi, lo) == hi) return;
+    if (atomicCAS(&p[hi], hi, lo) == hi) {
+      if (f.arcs) {
+        const unsigned long long at = atomicAdd(f.count, 1ull);
+        f.arcs[2 * at] = u0;
+        f.arcs[2 * at + 1] = v0;
+      }
+      return;
+    }
     u = hi;
     v = lo;
   }
@@ -68,11 +85,11 @@ constexpr uint32_t CC_NONE = 0xFFFFFFFFu;
 
 // sampling (Afforest): every vertex links its first CC_SAMPLE out-arcs
 __global__ void k_cc_sample(const uint64_t *__restrict__ off, const uint32_t *__restrict__ adj,
-                            uint64_t n, uint32_t *p) {
+                            uint64_t n, uint32_t *p, CcForest f) {
   for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
        u += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t b = off[u], e = off[u + 1];
-    for (uint64_t j = b; j < e && j < b + CC_SAMPLE; ++j) cc_unite(p, (uint32_t)u, adj[j]);
+    for (uint64_t j = b; j < e && j < b + CC_SAMPLE; ++j) cc_unite(p, (uint32_t)u, adj[j], f);
   }
 }
 
@@ -101,7 +118,7 @@ __global__ __launch_bounds__(BLOCK) void k_cc_light(const uint64_t *__restrict__
                                                     const uint32_t *__restrict__ iadj,
                                                     uint64_t n, uint32_t skip0, uint32_t giant,
                                                     uint32_t *p, uint32_t *heavy,
-                                                    uint32_t *nheavy) {
+                                                    uint32_t *nheavy, CcForest f) {
   for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
        u += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e = off[u + 1], b = off[u] + skip0 < e ? off[u] + skip0 : e;
@@ -115,8 +132,8 @@ __global__ __launch_bounds__(BLOCK) void k_cc_light(const uint64_t *__restrict__
       continue;
     }
     if (!run) continue;
-    for (uint64_t j = b; j < e; ++j) cc_unite(p, (uint32_t)u, adj[j]);
-    for (uint64_t j = ib; j < ie; ++j) cc_unite(p, (uint32_t)u, iadj[j]);
+    for (uint64_t j = b; j < e; ++j) cc_unite(p, (uint32_t)u, adj[j], f);
+    for (uint64_t j = ib; j < ie; ++j) cc_unite(p, iadj[j], (uint32_t)u, f);   // arc iadj[j] -> u
   }
 }
 
@@ -126,16 +143,17 @@ __global__ __launch_bounds__(BLOCK) void k_cc_heavy(const uint64_t *__restrict__
                                                     const uint64_t *__restrict__ ioff,
                                                     const uint32_t *__restrict__ iadj,
                                                     uint32_t skip0, const uint32_t *heavy,
-                                                    const uint32_t *nheavy, uint32_t *p) {
+                                                    const uint32_t *nheavy, uint32_t *p,
+                                                    CcForest f) {
   const uint32_t nh = *nheavy;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / WAVE);
   for (uint64_t h = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; h < nh;
        h += waves) {
     const uint32_t u = heavy[h];
     const uint64_t e = off[u + 1], b = off[u] + skip0 < e ? off[u] + skip0 : e;
-    for (uint64_t j = b + lane(); j < e; j += WAVE) cc_unite(p, u, adj[j]);
+    for (uint64_t j = b + lane(); j < e; j += WAVE) cc_unite(p, u, adj[j], f);
     if (ioff)
-      for (uint64_t j = ioff[u] + lane(); j < ioff[u + 1]; j += WAVE) cc_unite(p, u, iadj[j]);
+      for (uint64_t j = ioff[u] + lane(); j < ioff[u + 1]; j += WAVE) cc_unite(p, iadj[j], u, f);
   }
 }
 
@@ -186,7 +204,8 @@ unsigned grid_for_cc(uint64_t n) {
 
 extern "C" {
 
-int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms) {
+static int cc_device(const psc_graph *dev_g, uint32_t *d_labels, uint32_t *d_forest,
+                     uint64_t *forest_count, double *ms) {
   using namespace psc;
   if (!dev_g || (dev_g->n && (!d_labels || !dev_g->offset)) ||
       (dev_g->m && !dev_g->edges) || dev_g->n >= 0xFFFFFFFFull) {
@@ -198,17 +217,25 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   uint32_t *heavy = nullptr, *nheavy = nullptr, *roots = nullptr;
+  unsigned long long *fcount = nullptr, hfc = 0;
+  CcForest f{nullptr, nullptr};
   uint32_t hbad = 0, giant = CC_NONE;
   uint32_t hroots[CC_PROBES];
   float t = 0;
   // with the in-CSR (the transpose of the out-CSR) the giant component's
   // vertices can be skipped; without it every arc is linked
   const bool has_in = dev_g->m && dev_g->in_offset && dev_g->in_edges;
+  if (forest_count) *forest_count = 0;
   if (n == 0) {
     if (ms) *ms = 0;
     return PSC_OK;
   }
   CC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  if (d_forest) {
+    CC_HIP(hipMalloc(&fcount, 8));
+    CC_HIP(hipMemsetAsync(fcount, 0, 8, st));
+    f = CcForest{d_forest, fcount};
+  }
   CC_HIP(hipEventCreate(&e0));
   CC_HIP(hipEventCreate(&e1));
   CC_HIP(hipMalloc(&heavy, n * 4));
@@ -233,7 +260,7 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   CC_HIP(hipEventRecord(e0, st));
   k_cc_init<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
   k_cc_init<<<1, 64, 0, st>>>(nheavy, 1);   // nheavy = 0
-  k_cc_sample<<<grid_for_cc(n), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges, n, d_labels);
+  k_cc_sample<<<grid_for_cc(n), BLOCK, 0, st>>>(dev_g->offset, dev_g->edges, n, d_labels, f);
   if (has_in) {   // elect the most frequent root of the probes
     k_cc_probe<<<(CC_PROBES + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(d_labels, n, roots);
     CC_HIP(hipGetLastError());
@@ -251,17 +278,22 @@ int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *
   }
   k_cc_light<<<grid_for_cc(n), BLOCK, 0, st>>>(
       dev_g->offset, dev_g->edges, has_in ? dev_g->in_offset : nullptr,
-      has_in ? dev_g->in_edges : nullptr, n, CC_SAMPLE, giant, d_labels, heavy, nheavy);
+      has_in ? dev_g->in_edges : nullptr, n, CC_SAMPLE, giant, d_labels, heavy, nheavy, f);
   k_cc_heavy<<<grid_for_cc(n * WAVE / 16), BLOCK, 0, st>>>(
       dev_g->offset, dev_g->edges, has_in ? dev_g->in_offset : nullptr,
-      has_in ? dev_g->in_edges : nullptr, CC_SAMPLE, heavy, nheavy, d_labels);
+      has_in ? dev_g->in_edges : nullptr, CC_SAMPLE, heavy, nheavy, d_labels, f);
   k_cc_labels<<<grid_for_cc(n), BLOCK, 0, st>>>(d_labels, n);
   CC_HIP(hipGetLastError());
   CC_HIP(hipEventRecord(e1, st));
   CC_HIP(hipStreamSynchronize(st));
   CC_HIP(hipEventElapsedTime(&t, e0, e1));
   if (ms) *ms = t;
+  if (fcount) {
+    CC_HIP(hipMemcpy(&hfc, fcount, 8, hipMemcpyDeviceToHost));
+    *forest_count = hfc;
+  }
 done:
+  if (fcount) (void)hipFree(fcount);
   if (heavy) (void)hipFree(heavy);
   if (nheavy) (void)hipFree(nheavy);
   if (roots) (void)hipFree(roots);
@@ -271,17 +303,33 @@ done:
   return rc;
 }
 
-int psc_connectivity(const psc_graph *g, uint32_t *labels) {
+int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms) {
+  return cc_device(dev_g, d_labels, nullptr, nullptr, ms);
+}
+
+int psc_spanning_forest_device(const psc_graph *dev_g, uint32_t *d_labels, uint32_t *d_forest,
+                               uint64_t *count, double *ms) {
+  if (!count || (dev_g && dev_g->n > 1 && !d_forest)) {
+    psc::set_error("psc_spanning_forest_device: bad arguments");
+    return PSC_ERR_INVALID;
+  }
+  return cc_device(dev_g, d_labels, dev_g && dev_g->n > 1 ? d_forest : nullptr, count, ms);
+}
+
+static int cc_host(const psc_graph *g, uint32_t *labels, uint32_t *forest, uint64_t *count) {
   using namespace psc;
-  if (!g || (g->n && (!labels || !g->offset)) || (g->m && !g->edges)) {
+  if (!g || (g->n && (!labels || !g->offset)) || (g->m && !g->edges) ||
+      (forest && !count)) {
     set_error("psc_connectivity: bad arguments");
     return PSC_ERR_INVALID;
   }
   const uint64_t n = g->n, m = g->m;
   int rc = PSC_OK;
   uint64_t *off = nullptr, *ioff = nullptr;
-  uint32_t *adj = nullptr, *iadj = nullptr, *lab = nullptr;
+  uint32_t *adj = nullptr, *iadj = nullptr, *lab = nullptr, *fst = nullptr;
   psc_graph dg{};
+  const bool want_forest = forest && g->n > 1;
+  if (count) *count = 0;
   const bool has_in = m && g->in_offset && g->in_edges;
   if (n == 0) return PSC_OK;
   CC_HIP(hipMalloc(&off, (n + 1) * 8));
@@ -295,16 +343,33 @@ int psc_connectivity(const psc_graph *g, uint32_t *labels) {
     CC_HIP(hipMemcpy(ioff, g->in_offset, (n + 1) * 8, hipMemcpyHostToDevice));
     CC_HIP(hipMemcpy(iadj, g->in_edges, m * 4, hipMemcpyHostToDevice));
   }
+  if (want_forest) CC_HIP(hipMalloc(&fst, (n - 1) * 8));
   dg = psc_graph{n, m, off, adj, ioff, iadj};
-  rc = psc_connectivity_device(&dg, lab, nullptr);
+  rc = cc_device(&dg, lab, fst, count, nullptr);
   if (rc == PSC_OK) CC_HIP(hipMemcpy(labels, lab, n * 4, hipMemcpyDeviceToHost));
+  if (rc == PSC_OK && want_forest && *count)
+    CC_HIP(hipMemcpy(forest, fst, *count * 8, hipMemcpyDeviceToHost));
 done:
   if (off) (void)hipFree(off);
   if (adj) (void)hipFree(adj);
   if (ioff) (void)hipFree(ioff);
   if (iadj) (void)hipFree(iadj);
   if (lab) (void)hipFree(lab);
+  if (fst) (void)hipFree(fst);
   return rc;
+}
+
+int psc_connectivity(const psc_graph *g, uint32_t *labels) {
+  return cc_host(g, labels, nullptr, nullptr);
+}
+
+int psc_spanning_forest(const psc_graph *g, uint32_t *labels, uint32_t *forest,
+                        uint64_t *count) {
+  if (!count || (g && g->n > 1 && !forest)) {
+    psc::set_error("psc_spanning_forest: bad arguments");
+    return PSC_ERR_INVALID;
+  }
+  return cc_host(g, labels, forest, count);
 }
 
 }  // extern "C"

@@ -73,3 +73,34 @@ def test_cc_device_transpose_random(psc, oracle):
         finally:
             d.free()
         assert np.array_equal(got, oracle.cc(h))
+
+
+def _check_forest(psc, oracle, g, labels, forest):
+    expect = oracle.cc(g)
+    assert np.array_equal(labels, expect)
+    n_comp = np.unique(expect).size
+    assert forest.shape == (g.n - n_comp, 2)
+    # every forest arc is an arc of g
+    src = np.repeat(np.arange(g.n, dtype=np.uint64), np.diff(g.offset.astype(np.int64)))
+    arcs = set(zip(src.tolist(), g.edges.tolist()))
+    assert all((int(u), int(v)) in arcs for u, v in forest)
+    # n - c arcs that join every component: a spanning forest (no cycles)
+    f = psc.Graph.from_edges(g.n, forest[:, 0], forest[:, 1])
+    assert np.array_equal(oracle.cc(f), expect)
+
+
+@pytest.mark.parametrize("n,m,seed", [(1, 0, 1), (2, 1, 2), (5000, 4000, 4), (20000, 60000, 5)])
+def test_spanning_forest_host(psc, oracle, n, m, seed):
+    g = random_digraph(psc, n, m, seed, self_loops=True, dups=True)
+    labels, forest = psc.spanning_forest(g)
+    _check_forest(psc, oracle, g, labels, forest)
+
+
+def test_spanning_forest_device(psc, oracle):
+    dg = psc.DeviceGraph.rmat(16, 4 << 16, seed=9)
+    try:
+        h = dg.to_host()
+        labels, forest = psc.spanning_forest(dg)
+    finally:
+        dg.free()
+    _check_forest(psc, oracle, h, labels, forest)
