@@ -171,6 +171,7 @@ constexpr uint32_t LE_STAGE = 256;   // new pairs staged per wave before a flush
 constexpr uint64_t LE_HEAVY = 4096;  // longer lists are spread over the grid
 constexpr uint32_t LE_HSEG = 2048;   // heavy pairs per LDS prefix segment
 static_assert(LE_HSEG % BLOCK == 0, "a block builds the segment prefix in equal shares");
+constexpr uint64_t LE_GROW = 1ull << 28;   // entries a degree bound may add to the buffers
 
 // the staged pairs of a wave go to the next frontier and to the list at
 // lbase + their frontier slot, with one atomic for all of them
@@ -505,11 +506,10 @@ struct LeRun {
         if (r > 1) nlist += h.nf;   // pairs the previous sub-round added
         if (h.nf == 0 || h.deg == 0) break;
         // the frontier's out-degree sum bounds the pairs this sub-round
-        // adds; when it does not fit the buffers as they are, the arcs
-        // that pass the tests are counted for a tight bound
+        // adds; the buffers grow to it up to LE_GROW entries, past that
+        // the arcs that pass the tests are counted for a tight bound
         uint64_t bound = h.deg;
-        if (bound > front[cf ^ 1].cap / 8 || nlist + bound > lkey.cap / 8 ||
-            nlist + bound > ld.cap / 4) {
+        if (bound > LE_GROW || nlist + bound > LE_GROW + lkey.cap / 8) {
           k_le_count<<<gwave, BLOCK, 0, st>>>(front[cf].as<unsigned long long>(),
                                               ctl.as<LeCtl>(), g->offset, g->edges,
                                               dist.as<uint32_t>(), bmin.as<uint32_t>(), r);
