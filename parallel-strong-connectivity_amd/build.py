@@ -33,7 +33,7 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wextra",
 
 HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip", "cc.hip",
                "le_list.hip"]
-CXX_SOURCES = ["graph_host.cpp", "serve.cpp"]
+CXX_SOURCES = ["graph_host.cpp"]
 
 
 def _run(cmd):

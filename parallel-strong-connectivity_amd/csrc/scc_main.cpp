@@ -114,8 +114,19 @@ int main(int argc, char **argv) {
   printf("average multi_search time %g\n", multi / rep);
   printf("average hash table resize time %g\n", resize / rep);
   if (P.has("-status")) {   // src/scc.cpp:13-29
+    // labels are (internal & 0xffffffff) - 1 in [0, n] (or minimum vertex
+    // ids with -canonical), so they index n + 1 counters directly; the
+    // reference's 31-bit mask (src/scc.cpp:17) would fold labels of graphs
+    // with n >= 2^31 together
     std::vector<uint32_t> flag(g->n + 1, 0);
-    for (uint64_t i = 0; i < g->n; i++) flag[label[i] & 0x7FFFFFFFu]++;
+    for (uint64_t i = 0; i < g->n; i++) {
+      if (label[i] > g->n) {
+        fprintf(stderr, "label %llu of vertex %llu outside [0, n]\n",
+                (unsigned long long)label[i], (unsigned long long)i);
+        return 1;
+      }
+      flag[label[i]]++;
+    }
     uint64_t nscc = 0, big = 0;
     for (uint32_t f : flag) {
       nscc += f != 0;

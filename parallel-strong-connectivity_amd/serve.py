@@ -120,27 +120,30 @@ class Server:
         i = 0
         total = len(reqs)
         while i < total:
+            if self.errors:
+                return
             now = time.perf_counter() - t0
             if reqs[i].arrival > now:
                 time.sleep(reqs[i].arrival - now)
                 continue
             group = []
+            err = None
             with self.cv:
                 while i < total and reqs[i].arrival <= time.perf_counter() - t0:
                     r = reqs[i]
                     need = est_bytes(r.kind, r.n)
                     if need > self.budget:
-                        self.errors.append(RuntimeError(
-                            f"request {r.rid} needs {need} B, over the "
-                            f"{self.budget} B budget"))
-                        self.finished += 1
-                        i += 1
-                        self.cv.notify_all()
-                        continue
+                        err = RuntimeError(f"request {r.rid} needs {need} B, over the "
+                                           f"{self.budget} B budget")
+                        break
                     if self.in_use + need > self.budget:
                         if group:
                             break
-                        self.cv.wait()      # wait for memory to free up
+                        # wait for memory to free up; a failed lane ends
+                        # the run (its error is raised by serve())
+                        self.cv.wait(timeout=0.5)
+                        if self.errors:
+                            return
                         continue
                     self.in_use += need
                     self.peak_in_use = max(self.peak_in_use, self.in_use)
@@ -148,6 +151,13 @@ class Server:
                     i += 1
                     if len(group) >= self.max_admit:
                         break
+            if err is not None:
+                with self.cv:
+                    for _, need in group:
+                        self.in_use -= need
+                    self.finished += len(group)
+                self._abort(err, total)
+                return
             if not group:
                 continue
             tg = time.perf_counter()
@@ -157,13 +167,12 @@ class Server:
                     np.array([r.n for r, _ in group], np.uint64),
                     seeds=np.array([r.seed for r, _ in group], np.uint64))
             except Exception as e:
-                self.errors.append(e)
                 with self.cv:
                     for _, need in group:
                         self.in_use -= need
                     self.finished += len(group)
-                    self.cv.notify_all()
-                continue
+                self._abort(e, total)
+                return
             shared = _Pool(db, len(group))
             adm = time.perf_counter() - t0
             self.phase["generate"] += adm + t0 - tg
@@ -187,16 +196,44 @@ class Server:
             take.append(item)
         return take
 
+    def _release(self, items):
+        """Give back the pool references and the budget of finished (or
+        failed) items."""
+        for _, pool, _, _ in items:
+            pool.release()
+        with self.cv:
+            for _, _, _, need in items:
+                self.in_use -= need
+            self.finished += len(items)
+            self.cv.notify_all()
+
+    def _abort(self, err, total):
+        """A failed lane ends the run: the error is kept for serve(), every
+        waiting request gives its memory back, the other threads stop."""
+        with self.cv:
+            self.errors.append(err)
+            waiting, self.ready = self.ready, []
+        self._release(waiting)
+        with self.cv:
+            self.finished = total
+            self.cv.notify_all()
+
+    def _keep(self, r):
+        k = self.keep_labels
+        return bool(k) if isinstance(k, bool) else r.rid in k
+
     def _lane_loop(self, total, t0):
         psc = self.psc
         handle = None
-        pinned = psc.PinnedLabels()
+        pinned = None
+        items = []
         try:
+            pinned = psc.PinnedLabels()
             while True:
                 with self.cv:
-                    while not self.ready and self.finished < total:
+                    while not self.ready and self.finished < total and not self.errors:
                         self.cv.wait(timeout=0.5)
-                    if not self.ready and self.finished >= total:
+                    if self.errors or (not self.ready and self.finished >= total):
                         return
                     if not self.ready:
                         continue
@@ -214,54 +251,56 @@ class Server:
                         if not self.ready:
                             continue
                     items = self._take()
-                start = time.perf_counter() - t0
-                views = [pool.db.view(j) for _, pool, j, _ in items]
-                arr = (psc.CGraph * len(views))(*[v[0] for v in views])
-                vb = np.array([v[1] for v in views], dtype=np.uint64)
-                eb = np.array([v[2] for v in views], dtype=np.uint64)
-                vbp, ebp = psc._ptr(vb, psc.u64p), psc._ptr(eb, psc.u64p)
-                if handle is None:
-                    h = C.c_void_p()
-                    psc._check("psc_batch_create_from_graphs",
-                               psc.lib.psc_batch_create_from_graphs(
-                                   arr, len(views), vbp, ebp, C.byref(h)))
-                    handle = psc.BatchSCC._wrap(h)
-                else:
-                    psc._check("psc_batch_rebind",
-                               psc.lib.psc_batch_rebind(handle._h, arr, len(views),
-                                                        vbp, ebp))
-                    handle._refresh()
-                t1 = time.perf_counter()
-                betas = np.array([r.beta for r, _, _, _ in items], dtype=np.float64)
-                labels = handle.run(beta_per_graph=betas, tau=self.tau,
-                                    dense_div=self.dense_div, timing=0,
-                                    out=pinned.view(handle.n))
-                done = time.perf_counter() - t0
-                scc_s = handle.stats["ms_total"] / 1e3
-                self.phase["gather"] += t1 - t0 - start
-                self.calls["batches"] += 1
-                self.phase["scc"] += scc_s
-                self.phase["labels"] += done + t0 - t1 - scc_s
-                parts = handle.split(labels)
-                for (r, pool, j, need), lab in zip(items, parts):
-                    r.started, r.done, r.batch = start, done, len(items)
-                    if self.keep_labels:
-                        r.labels = lab.copy()
-                    pool.release()
-                with self.cv:
-                    for _, _, _, need in items:
-                        self.in_use -= need
-                    self.finished += len(items)
-                    self.cv.notify_all()
+                handle = self._run_batch(handle, items, pinned, t0)
+                done, items = items, []
+                self._release(done)
         except Exception as e:   # surfaced by serve()
-            self.errors.append(e)
-            with self.cv:
-                self.finished = total
-                self.cv.notify_all()
+            failed, items = items, []
+            self._release(failed)
+            self._abort(e, total)
         finally:
             if handle is not None:
                 handle.close()
-            pinned.free()
+            if pinned is not None:
+                pinned.free()
+
+    def _run_batch(self, handle, items, pinned, t0):
+        """Bind the lane's handle to the items' graphs (views into their
+        admission batches), run the batched SCC, hand out the labels."""
+        psc = self.psc
+        start = time.perf_counter() - t0
+        views = [pool.db.view(j) for _, pool, j, _ in items]
+        arr = (psc.CGraph * len(views))(*[v[0] for v in views])
+        vb = np.array([v[1] for v in views], dtype=np.uint64)
+        eb = np.array([v[2] for v in views], dtype=np.uint64)
+        vbp, ebp = psc._ptr(vb, psc.u64p), psc._ptr(eb, psc.u64p)
+        if handle is None:
+            h = C.c_void_p()
+            psc._check("psc_batch_create_from_graphs",
+                       psc.lib.psc_batch_create_from_graphs(
+                           arr, len(views), vbp, ebp, C.byref(h)))
+            handle = psc.BatchSCC._wrap(h)
+        else:
+            psc._check("psc_batch_rebind",
+                       psc.lib.psc_batch_rebind(handle._h, arr, len(views), vbp, ebp))
+            handle._refresh()
+        t1 = time.perf_counter()
+        betas = np.array([r.beta for r, _, _, _ in items], dtype=np.float64)
+        labels = handle.run(beta_per_graph=betas, tau=self.tau,
+                            dense_div=self.dense_div, timing=0,
+                            out=pinned.view(handle.n))
+        done = time.perf_counter() - t0
+        scc_s = handle.stats["ms_total"] / 1e3
+        self.phase["gather"] += t1 - t0 - start
+        self.calls["batches"] += 1
+        self.phase["scc"] += scc_s
+        self.phase["labels"] += done + t0 - t1 - scc_s
+        parts = handle.split(labels)
+        for (r, _, _, _), lab in zip(items, parts):
+            r.started, r.done, r.batch = start, done, len(items)
+            if self._keep(r):
+                r.labels = lab.copy()
+        return handle
 
     def serve(self, reqs):
         """Run the request stream; returns a summary dict."""
