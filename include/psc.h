@@ -143,6 +143,31 @@ int psc_run(psc_handle *h, const psc_params *p, uint64_t *labels_out,
 int psc_run_device(psc_handle *h, const psc_params *p, uint64_t *d_labels_out,
                    psc_stats *stats);
 
+/* Allocation-free SCC.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  psc_create_device + psc_run_device
+ * in one call, with no handle: the graph (device pointers) and d_labels_out
+ * (n uint64, device) are the caller's; every working buffer comes from a
+ * fixed per-device scratch array of PSC_NOALLOC_SCRATCH_BYTES, held from the
+ * device's first call until the process ends.  The call makes no device or
+ * pinned-host allocation and creates no stream or event: it runs on `stream`
+ * (a hipStream_t; NULL = the null stream) and returns after synchronising
+ * it.  Labels are those psc_run_device returns for the same graph and
+ * params; p->timing and p->capture are ignored (ms_total is host time).
+ * Graphs with n > PSC_NOALLOC_MAX_N or m > PSC_NOALLOC_MAX_M are rejected
+ * with PSC_ERR_INVALID, a run whose tables outgrow the scratch returns
+ * PSC_ERR_NOMEM; both leave a message in psc_last_error().  Calls on one
+ * device share the scratch array: calls that overlap on one device race. */
+#define PSC_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define PSC_NOALLOC_MAX_N (1ull << 20)
+#define PSC_NOALLOC_MAX_M (1ull << 24)
+int psc_run_noalloc(const psc_graph *dev_g, const psc_params *p, uint64_t *d_labels_out,
+                    void *stream, psc_stats *stats);
+/* Size of the scratch array, and the most of it any call on the current
+ * device has used so far. */
+uint64_t psc_noalloc_scratch_bytes(void);
+uint64_t psc_noalloc_scratch_peak(void);
+
 /* SCC_status (src/scc.cpp:13-29) computed on the device from labels in
  * [0, n] held in device memory: number of SCCs and the largest size. */
 int psc_status_device(psc_handle *h, const uint64_t *d_labels,

@@ -140,6 +140,10 @@ _sig("psc_dev_make_directed", C.c_int, C.c_uint64, C.c_uint64, C.c_void_p, C.c_v
      C.POINTER(C.POINTER(CDevGraph)))
 _sig("psc_dev_graph_download", C.c_int, C.POINTER(CDevGraph),
      C.POINTER(C.POINTER(CHostGraph)))
+_sig("psc_run_noalloc", C.c_int, C.POINTER(CGraph), C.POINTER(CParams), C.c_void_p,
+     C.c_void_p, C.POINTER(CStats))
+_sig("psc_noalloc_scratch_bytes", C.c_uint64)
+_sig("psc_noalloc_scratch_peak", C.c_uint64)
 
 
 class CBatchDesc(C.Structure):
@@ -590,6 +594,22 @@ class SCC:
             self.close()
         except Exception:
             pass
+
+
+def scc_noalloc(graph, d_labels, stream=None, beta=1.5, local_reach=True, local_scc=True,
+                **kw):
+    """psc_run_noalloc: SCC of a device graph (DeviceGraph or CGraph of
+    device pointers) into d_labels (device address, n uint64) with every
+    working buffer from the library's fixed scratch array; returns the
+    stats."""
+    g = graph.cgraph() if hasattr(graph, "cgraph") else graph
+    p = default_params(beta=beta, local_reach=int(local_reach),
+                       local_scc=int(local_scc), **kw)
+    st = CStats()
+    _check("psc_run_noalloc", lib.psc_run_noalloc(C.byref(g), C.byref(p),
+                                                  C.c_void_p(d_labels),
+                                                  C.c_void_p(stream), C.byref(st)))
+    return st.as_dict()
 
 
 def scc(graph, **kw):

@@ -32,7 +32,7 @@ CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wextra",
              f"-I{INCLUDE}", f"-I{CSRC}"]
 
 HIP_SOURCES = ["scc_engine.hip", "tail.hip", "gen_device.hip", "batch.hip", "cc.hip",
-               "le_list.hip"]
+               "le_list.hip", "noalloc.hip"]
 CXX_SOURCES = ["graph_host.cpp"]
 
 

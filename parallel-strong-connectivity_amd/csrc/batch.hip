@@ -508,7 +508,7 @@ int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
 int psc_batch_rebind(psc_handle *h, const psc_graph *graphs, uint64_t count,
                      const uint64_t *id_base, const uint64_t *edge_base) {
   if (!h || !graphs || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
-  if (!h->eng.owned_graph && h->eng.stream) {
+  if (!h->eng.owned_graph && h->eng.ready) {
     psc::set_error("psc_batch_rebind: handle borrows its graph");
     return PSC_ERR_INVALID;
   }

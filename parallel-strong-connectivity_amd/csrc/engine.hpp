@@ -44,10 +44,21 @@ extern thread_local std::string g_last_error;
 void pool_setup();
 hipStream_t pool_free_stream();
 
+// A fixed device region that an engine's buffers are carved from instead of
+// the pool (bump allocation, 256-byte aligned; emptied when the engine is
+// bound to a graph).  A buffer that grows takes a new piece; the old one is
+// not reused before the next binding.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0, peak = 0;
+  void *take(size_t bytes);
+};
+
 struct DevBuf {
   void *ptr = nullptr;
   size_t cap = 0;
   hipStream_t *owner = nullptr;   // the stream that orders its (re)allocation
+  Arena *const *arena = nullptr;  // the owning engine's arena (null: pool)
   int ensure(size_t bytes);
   void release();
   template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -78,6 +89,15 @@ struct Engine {
   uint64_t n = 0, m = 0;
   int device = 0, num_cu = 1;
   hipStream_t stream = nullptr;
+  bool ready = false;       // setup() done
+  // Arena mode (the allocation-free entry point): every device buffer comes
+  // from `arena`, the stream is the caller's, host staging is the engine's
+  // own (pageable) memory and no event is created: the engine makes no
+  // device, pinned-host, stream or event allocation.
+  Arena *arena = nullptr;
+  unsigned long long *labels_ext = nullptr;   // arena mode: the output labels
+  static constexpr size_t HOST_STAGE = 8192;
+  alignas(64) unsigned char host_stage[HOST_STAGE];
   hipEvent_t ev[8] = {};
   void *h_pinned = nullptr;
   bool owned_graph = false;

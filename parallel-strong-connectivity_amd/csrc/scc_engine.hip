@@ -1788,8 +1788,28 @@ hipStream_t pool_free_stream() {
   return g_free_stream;
 }
 
+void *Arena::take(size_t bytes) {
+  const size_t at = (used + 255) & ~(size_t)255;
+  if (at > cap || bytes > cap - at) return nullptr;
+  used = at + bytes;
+  peak = std::max(peak, used);
+  return base + at;
+}
+
 int DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return PSC_OK;
+  if (arena && *arena) {
+    Arena *a = *arena;
+    void *p = a->take(bytes);
+    if (!p) {
+      set_error("scratch exhausted: a buffer of " + std::to_string(bytes) + " B does not fit (" +
+                std::to_string(a->used) + " of " + std::to_string(a->cap) + " B in use)");
+      return PSC_ERR_NOMEM;
+    }
+    ptr = p;
+    cap = bytes;
+    return PSC_OK;
+  }
   pool_setup();
   hipStream_t st = owner ? *owner : nullptr;
   if (ptr) {
@@ -1816,23 +1836,30 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (ptr) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
+  if (ptr && !(arena && *arena)) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
   ptr = nullptr;
   cap = 0;
 }
 
 Engine::Engine() {
-  for (DevBuf *b : all_bufs()) b->owner = &stream;
-  for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->owner = &stream;
+  for (DevBuf *b : all_bufs()) {
+    b->owner = &stream;
+    b->arena = &arena;
+  }
+  for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) {
+    b->owner = &stream;
+    b->arena = &arena;
+  }
 }
 
 Engine::~Engine() {
-  if (stream) (void)hipStreamSynchronize(stream);
+  if (ready) (void)hipStreamSynchronize(stream);
   for (auto &g : gcache) (void)hipGraphExecDestroy(g.second);
   gcache.clear();
   for (DevBuf *b : all_bufs()) b->release();
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
+  if (arena) return;   // stream and host staging belong to the caller / the engine
   if (stream) (void)hipStreamSynchronize(stream);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (h_gpin) (void)hipHostFree(h_gpin);
@@ -1865,15 +1892,25 @@ int Engine::download_labels(uint64_t *dst) {
 }
 
 int Engine::setup() {
-  if (stream) return PSC_OK;
-  pool_setup();
+  if (ready) return PSC_OK;
   PSC_HIP(hipGetDevice(&device));
   hipDeviceProp_t prop;
   PSC_HIP(hipGetDeviceProperties(&prop, device));
   num_cu = prop.multiProcessorCount;
-  PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
-  PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  if (arena) {
+    // host staging in the engine itself: polls and descriptors [0, 4096),
+    // per-graph regions [4096, 7168), remainder descriptors [7168, 8192)
+    h_pinned = host_stage;
+    h_gpin = reinterpret_cast<unsigned long long *>(host_stage + 4096);
+    h_gpin_cap = 3072 / 8;
+    tail_pin = host_stage + 7168;
+    tail_pin_cap = 1024;
+  } else {
+    pool_setup();
+    PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
+    PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+  }
   int occ = 0;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
                                                        BLOCK, 0));
@@ -1886,6 +1923,7 @@ int Engine::setup() {
   grid_heavy = std::max(1, occ) * num_cu;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_pull, BLOCK, 0));
   grid_pull = std::max(1, std::min(occ, 2)) * num_cu;
+  ready = true;
   return PSC_OK;
 }
 
@@ -1909,9 +1947,18 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   // events, pinned staging, launch shapes and device buffers persist and
   // buffers only grow
   PSC_TRY(setup());
+  ms_dev_valid = false;   // descriptors may move
 
   const uint64_t nn = std::max<uint64_t>(n, 1);
   const uint64_t words = (nn + 31) / 32;
+  if (arena) {
+    // every buffer is carved again for this graph; the labels are the
+    // caller's output array
+    arena->used = 0;
+    for (DevBuf *b : all_bufs()) b->ptr = nullptr, b->cap = 0;
+    labels.ptr = labels_ext;
+    labels.cap = labels_ext ? nn * 8 : 0;
+  }
   PSC_TRY(labels.ensure(nn * 8));
   for (int s = 0; s < 2; ++s) {
     PSC_TRY(vis[s].ensure(words * 4));
@@ -1945,22 +1992,19 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   // it is one coalesced 4-byte load instead of an offset pair and a line of
   // the adjacency array
   {
-    DevBuf deg32;
-    deg32.owner = &stream;
-    PSC_TRY(deg32.ensure(nn * 4));
+    // the round front (n entries, unused until the rounds) holds the degrees
+    uint32_t *deg32 = front.as<uint32_t>();
     const uint64_t blocks = std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 65536);
     for (int s = 0; s < 2; ++s) {
       PSC_TRY(rhead[s].ensure(nn * 4));
       // a pull of search s reads the other CSR; its candidates' in-neighbours
       // are reached along the search's own CSR, whose degree ranks them
-      k_degrees32<<<(unsigned)blocks, BLOCK, 0, stream>>>(s == 0 ? d_off : d_ioff, n,
-                                                          deg32.as<uint32_t>());
+      k_degrees32<<<(unsigned)blocks, BLOCK, 0, stream>>>(s == 0 ? d_off : d_ioff, n, deg32);
       k_pull_heads<<<(unsigned)blocks, BLOCK, 0, stream>>>(
-          s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, deg32.as<uint32_t>(), n,
+          s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, deg32, n,
           rhead[s].as<uint32_t>());
       PSC_HIP(hipGetLastError());
     }
-    deg32.release();
   }
 
   ReachSearch rs[2];
@@ -2036,6 +2080,12 @@ int Engine::scan_tiles(uint64_t nt, unsigned long long *total, unsigned long lon
 int Engine::gpin(size_t count) {
   const size_t want = count * NREG;
   if (want <= h_gpin_cap) return PSC_OK;
+  if (arena) {
+    set_error("allocation-free run: per-graph staging holds " +
+              std::to_string(h_gpin_cap / NREG) + " entries, " + std::to_string(count) +
+              " needed");
+    return PSC_ERR_INVALID;
+  }
   PSC_HIP(hipStreamSynchronize(stream));
   if (h_gpin) (void)hipHostFree(h_gpin);
   h_gpin = nullptr;
@@ -2556,14 +2606,15 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(gpin(2 * (G + 1)));
     PSC_HIP(hipMemcpyAsync(region(2), nstart, 16 * (G + 1), hipMemcpyDeviceToHost,
                            stream));
-    PSC_HIP(hipEventRecord(ev[7], stream));
+    if (ev[7]) PSC_HIP(hipEventRecord(ev[7], stream));
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
   const bool trace_first = p.timing >= 3;   // the level report wants nnz first
   bool have_starts = false, counted = false;
   auto read_starts = [&]() -> int {
     if (have_starts) return PSC_OK;
-    PSC_HIP(hipEventSynchronize(ev[7]));
+    if (ev[7]) PSC_HIP(hipEventSynchronize(ev[7]));
+    else PSC_HIP(hipStreamSynchronize(stream));   // arena mode: no events
     std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
     nnz_total = h_starts[G];
     have_starts = true;
