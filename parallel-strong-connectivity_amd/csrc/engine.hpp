@@ -59,6 +59,7 @@ struct DevBuf {
   size_t cap = 0;
   hipStream_t *owner = nullptr;   // the stream that orders its (re)allocation
   Arena *const *arena = nullptr;  // the owning engine's arena (null: pool)
+  bool carved = false;            // a piece of the engine's set-up block
   int ensure(size_t bytes);
   void release();
   template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -85,6 +86,17 @@ struct KProf {
   double bytes = 0;
 };
 
+// A handle's stream with its timing events and small pinned staging
+// buffer; destroyed handles return theirs to a process-wide cache.
+struct StreamKit {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  void *pinned = nullptr;
+};
+bool kit_put(const StreamKit &k);
+bool kit_take(int device, StreamKit *k);
+
 struct Engine {
   uint64_t n = 0, m = 0;
   int device = 0, num_cu = 1;
@@ -105,7 +117,8 @@ struct Engine {
   const uint32_t *d_adj = nullptr, *d_iadj = nullptr;
   DevBuf g_off, g_adj, g_ioff, g_iadj;   // owned copies (psc_create)
   DevBuf labels, vis[2], rq[2][2], hv[2], hc[2];
-  DevBuf rhead[2];   // first pull-direction neighbour of each vertex, per search
+  DevBuf rhead[2];   // pull head of each vertex, per search (k_pull_heads)
+  DevBuf hubs;       // degree histogram and hub table of the head selection
   DevBuf centers, front, tiles, tile_off;
   DevBuf trim_off;   // exclusive non-trimmed count per tile (kept past selects)
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
@@ -148,6 +161,10 @@ struct Engine {
   ~Engine();
   Engine();
   std::vector<DevBuf *> all_bufs();
+  // set-up blocks: the buffers a binding needs, allocated together (one
+  // pool allocation instead of one per buffer) and freed with the engine
+  std::vector<void *> setup_blocks;
+  int ensure_all(const std::vector<std::pair<DevBuf *, size_t>> &req);
   // one-time device set-up: stream, events, pinned staging, launch shapes
   int setup();
   int init(const psc_graph *dg, bool owned, uint32_t ngraphs = 1,

@@ -744,44 +744,238 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
   }
 }
 
-// deg[v] = off[v + 1] - off[v], saturated to 32 bits.
-__global__ __launch_bounds__(BLOCK) void k_degrees32(const uint64_t *__restrict__ off,
-                                                     uint64_t n, uint32_t *deg) {
-  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
-       v += (uint64_t)gridDim.x * BLOCK)
-    deg[v] = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
-}
-
-// head[v] = the entry of v's list with the largest degree in the other
-// direction (`deg`), among its first HUB_SCAN entries, the earliest on
-// ties (v itself for an empty list).  A vertex of large degree is reached
-// early, so most pull candidates are decided by their head, and the heads
-// concentrate on few vertices whose visited words stay cached.
+// ------------------------------------------------------------ pull heads
+// Each vertex gets a head: one neighbour in the pull direction of large
+// degree in the search's own direction.  A vertex of large degree is
+// reached early, so most pull candidates are decided by their head, and the
+// heads concentrate on few vertices whose visited words stay cached.  The
+// head is the neighbour of largest degree among the graph's top hubs (at
+// most HUB_MAX vertices of largest degree) found in the first HUB_SCAN
+// entries of the list, else the neighbour of largest degree among the first
+// HUB_FALLBACK entries (earliest on ties; v itself for an empty list or a
+// trimmed v, whose head is never read).  Hub membership is an LDS hash
+// probe, so the only random global reads are the few fallback degrees.
 constexpr uint32_t HUB_SCAN = 128;
-__global__ __launch_bounds__(BLOCK) void k_pull_heads(const uint64_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ adj,
-                                                      const uint32_t *__restrict__ deg,
-                                                      uint64_t n, uint32_t *head) {
-  constexpr uint32_t K = 8;   // list entries (and their degrees) in flight
+constexpr uint32_t HUB_FALLBACK = 8;
+constexpr uint32_t HUB_MAX = 2048;
+constexpr uint32_t HUB_SLOTS = 2 * HUB_MAX;   // open addressing, load <= 1/2
+constexpr uint32_t HUB_BINS = 256;            // log-scale degree histogram
+constexpr uint32_t HUB_EMPTY = 0xFFFFFFFFu;
+constexpr int HEAD_BLOCK = 1024;
+// layout of the hub scratch (u32): histogram, threshold, keys, degrees
+constexpr uint32_t HUB_WORDS = HUB_BINS + 2 + 2 * HUB_SLOTS;
+
+// 8 bins per octave: degrees below 16 get bins 0..15, larger ones
+// 8 e + (the 3 bits after the leading one), e = floor(log2 d) >= 4.
+__device__ __forceinline__ uint32_t hub_bin(uint32_t d) {
+  if (d < 16) return d;
+  const uint32_t e = 31 - __clz(d);
+  return 8 * e + ((d >> (e - 3)) & 7);
+}
+__device__ __forceinline__ uint32_t hub_bin_floor(uint32_t b) {   // smallest d in bin b
+  if (b < 16) return b;
+  const uint32_t e = b / 8;
+  return (8 + (b & 7)) << (e - 3);
+}
+__device__ __forceinline__ uint32_t hub_hash(uint32_t v) {
+  return (v * 0x9E3779B1u) >> (32 - 12);   // log2(HUB_SLOTS) bits
+}
+static_assert(HUB_SLOTS == 1u << 12, "hub_hash yields 12 bits");
+
+// deg[v] = off[v + 1] - off[v] (saturated to 32 bits) and the histogram of
+// the non-zero degrees over HUB_BINS log-scale bins (zeroed beforehand).
+// Both searches are set up in the same launches: blockIdx.y selects the
+// search, whose own CSR ranks the heads and whose opposite CSR is pulled.
+struct HeadDir {
+  const uint64_t *own_off;    // degrees in the search's direction
+  const uint64_t *pull_off;   // the pull direction's CSR
+  const uint32_t *pull_adj;
+  uint32_t *deg;              // n degrees (scratch)
+  uint32_t *hub;              // HUB_WORDS: histogram, threshold, table
+  uint32_t *head;             // n heads (out)
+};
+struct HeadDirs {
+  HeadDir d[2];
+};
+
+__global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
+  const HeadDir &D = hd.d[blockIdx.y];
+  const uint64_t *__restrict__ off = D.own_off;
+  uint32_t *deg = D.deg, *hist = D.hub;
+  __shared__ uint32_t h[HUB_BINS];
+  for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK) h[i] = 0;
+  __syncthreads();
   for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t b = off[v];
-    const uint32_t d = (uint32_t)min(off[v + 1] - b, (uint64_t)HUB_SCAN);
-    uint32_t best = (uint32_t)v, bd = 0;
-    for (uint32_t t0 = 0; t0 < d; t0 += K) {
-      uint32_t u[K], du[K];
-#pragma unroll
-      for (uint32_t k = 0; k < K; ++k) u[k] = t0 + k < d ? adj[b + t0 + k] : 0u;
-#pragma unroll
-      for (uint32_t k = 0; k < K; ++k) du[k] = t0 + k < d ? deg[u[k]] : 0u;
-#pragma unroll
-      for (uint32_t k = 0; k < K; ++k)
-        if (t0 + k < d && (t0 + k == 0 || du[k] > bd)) {
-          best = u[k];
-          bd = du[k];
-        }
+    const uint32_t d = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
+    deg[v] = d;
+    if (d) atomicAdd(&h[hub_bin(d)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// One block: the hub threshold T = the floor of the lowest bin such that at
+// most HUB_MAX vertices have degree >= T (no vertex qualifies when the top
+// bin alone holds more), and an empty hub table.
+__global__ void k_hub_select(HeadDirs hd) {
+  uint32_t *hub = hd.d[blockIdx.y].hub;
+  uint32_t *hist = hub, *thr = hub + HUB_BINS, *keys = hub + HUB_BINS + 2;
+  for (uint32_t i = threadIdx.x; i < HUB_SLOTS; i += blockDim.x) keys[i] = HUB_EMPTY;
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0, t = 0xFFFFFFFFu;
+    for (int b = HUB_BINS - 1; b >= 0; --b) {
+      if (acc + hist[b] > HUB_MAX) break;
+      acc += hist[b];
+      if (hist[b]) t = hub_bin_floor((uint32_t)b);
     }
-    head[v] = best;
+    thr[0] = t;
+    thr[1] = acc;
+  }
+}
+
+// Vertices of degree >= T enter the hub table (key = id, value = degree).
+__global__ __launch_bounds__(BLOCK) void k_hub_insert(HeadDirs hd, uint64_t n) {
+  const uint32_t *__restrict__ deg = hd.d[blockIdx.y].deg;
+  uint32_t *hub = hd.d[blockIdx.y].hub;
+  // the degrees >= t fill exactly the selected bins: at most HUB_MAX
+  // vertices, so every insert finds an empty slot
+  const uint32_t t = hub[HUB_BINS];
+  if (hub[HUB_BINS + 1] == 0) return;
+  uint32_t *keys = hub + HUB_BINS + 2, *vals = keys + HUB_SLOTS;
+  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t d = deg[v];
+    if (d < t || d == 0) continue;
+    uint32_t s = hub_hash((uint32_t)v);
+    while (atomicCAS(&keys[s], HUB_EMPTY, (uint32_t)v) != HUB_EMPTY) s = (s + 1) & (HUB_SLOTS - 1);
+    vals[s] = d;
+  }
+}
+
+// head[v] for every vertex (see the rule above).  `off` / `adj` is the pull
+// direction's CSR, `deg` the degrees in the search's own direction (zero
+// for a vertex that cannot lie on a path of this search).  A wave takes 64
+// consecutive vertices and walks the concatenation of their scanned entries
+// in 64-entry windows, HEAD_WIN windows' loads in flight (lane e finds its
+// vertex by binary search over the wave's degree scan), so lists of very
+// different lengths cost their own entries only.  Hub hits raise an LDS
+// maximum per vertex of (hub degree, earliest position, neighbour); the
+// vertices left without a hub then walk their first HUB_FALLBACK entries the
+// same way, ranked by the looked-up degree.
+constexpr int HEAD_WIN = 4;
+
+__device__ __forceinline__ unsigned long long head_key(uint32_t d, uint32_t pos, uint32_t u) {
+  // degree saturated to 24 bits; ties go to the earlier position
+  return ((unsigned long long)min(d, 0xFFFFFFu) << 40) |
+         ((unsigned long long)(255u - pos) << 32) | u;
+}
+
+__device__ __forceinline__ int head_owner(const uint32_t *sc, uint32_t e) {
+  int lo = 0, hi = WAVE - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sc[mid] > e) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t n) {
+  const HeadDir &D = hd.d[blockIdx.y];
+  const uint64_t *__restrict__ off = D.pull_off;
+  const uint32_t *__restrict__ adj = D.pull_adj;
+  const uint32_t *__restrict__ deg = D.deg;
+  const uint32_t *__restrict__ hub = D.hub;
+  uint32_t *head = D.head;
+  constexpr int HW = HEAD_BLOCK / WAVE;
+  __shared__ uint32_t keys[HUB_SLOTS], vals[HUB_SLOTS];
+  __shared__ uint32_t sc_s[HW][WAVE];
+  __shared__ uint64_t st_s[HW][WAVE];
+  __shared__ unsigned long long best_s[HW][WAVE];
+  const uint32_t *gk = hub + HUB_BINS + 2;
+  for (uint32_t i = threadIdx.x; i < HUB_SLOTS; i += HEAD_BLOCK) {
+    keys[i] = gk[i];
+    vals[i] = gk[HUB_SLOTS + i];
+  }
+  __syncthreads();
+  const int w = threadIdx.x / WAVE, l = lane();
+  uint32_t *sc = sc_s[w];
+  uint64_t *st = st_s[w];
+  unsigned long long *best = best_s[w];
+  const uint64_t nwaves = (uint64_t)gridDim.x * HW;
+  for (uint64_t v0 = (blockIdx.x * (uint64_t)HW + w) * WAVE; v0 < n; v0 += nwaves * WAVE) {
+    const uint64_t v = v0 + l;
+    uint64_t b = 0;
+    uint32_t d = 0;
+    if (v < n) {
+      b = off[v];
+      // a vertex without edges in the search's own direction, or without a
+      // pull list, is trimmed: it is never a pull candidate
+      d = deg[v] ? (uint32_t)min(off[v + 1] - b, (uint64_t)HUB_SCAN) : 0u;
+    }
+    uint32_t inc = wave_incl_scan(d);
+    uint32_t total = __shfl(inc, WAVE - 1);
+    sc[l] = inc;
+    st[l] = b;
+    best[l] = 0;
+    wave_sync();
+    // hub hits among the first HUB_SCAN entries
+    for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
+      uint32_t u[HEAD_WIN], pos[HEAD_WIN];
+      int own[HEAD_WIN];
+#pragma unroll
+      for (int j = 0; j < HEAD_WIN; ++j) {
+        const uint32_t e = base + j * WAVE + l;
+        own[j] = -1;
+        u[j] = HUB_EMPTY;
+        pos[j] = 0;
+        if (e < total) {
+          own[j] = head_owner(sc, e);
+          pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
+          u[j] = adj[st[own[j]] + pos[j]];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < HEAD_WIN; ++j) {
+        if (own[j] < 0) continue;
+        uint32_t s = hub_hash(u[j]), key;
+        while ((key = keys[s]) != HUB_EMPTY && key != u[j]) s = (s + 1) & (HUB_SLOTS - 1);
+        if (key == u[j]) atomicMax(&best[own[j]], head_key(vals[s], pos[j], u[j]));
+      }
+    }
+    wave_sync();
+    // fallback: vertices without a hub rank their first HUB_FALLBACK entries
+    const uint32_t f = best[l] ? 0u : min(d, HUB_FALLBACK);
+    inc = wave_incl_scan(f);
+    total = __shfl(inc, WAVE - 1);
+    wave_sync();
+    sc[l] = inc;
+    wave_sync();
+    for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
+      uint32_t u[HEAD_WIN], du[HEAD_WIN], pos[HEAD_WIN];
+      int own[HEAD_WIN];
+#pragma unroll
+      for (int j = 0; j < HEAD_WIN; ++j) {
+        const uint32_t e = base + j * WAVE + l;
+        own[j] = -1;
+        u[j] = 0;
+        pos[j] = 0;
+        if (e < total) {
+          own[j] = head_owner(sc, e);
+          pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
+          u[j] = adj[st[own[j]] + pos[j]];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < HEAD_WIN; ++j) du[j] = own[j] >= 0 ? deg[u[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < HEAD_WIN; ++j)
+        if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
+    }
+    wave_sync();
+    if (v < n) head[v] = d ? (uint32_t)best[l] : (uint32_t)v;
+    wave_sync();
   }
 }
 
@@ -1813,7 +2007,9 @@ int DevBuf::ensure(size_t bytes) {
   pool_setup();
   hipStream_t st = owner ? *owner : nullptr;
   if (ptr) {
-    hipError_t e = hipFreeAsync(ptr, st);
+    // a piece of a set-up block stays with the block
+    hipError_t e = carved ? hipSuccess : hipFreeAsync(ptr, st);
+    carved = false;
     ptr = nullptr;
     // a buffer that has to grow again gets headroom
     bytes = std::max(bytes, cap + cap / 2);
@@ -1836,7 +2032,8 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (ptr && !(arena && *arena)) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
+  if (ptr && !carved && !(arena && *arena)) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
+  carved = false;
   ptr = nullptr;
   cap = 0;
 }
@@ -1860,13 +2057,81 @@ Engine::~Engine() {
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
   if (arena) return;   // stream and host staging belong to the caller / the engine
+  for (void *b : setup_blocks) (void)hipFreeAsync(b, stream);
   if (stream) (void)hipStreamSynchronize(stream);
-  if (h_pinned) (void)hipHostFree(h_pinned);
   if (h_gpin) (void)hipHostFree(h_gpin);
   if (tail_pin) (void)hipHostFree(tail_pin);
-  for (auto &e : ev) if (e) (void)hipEventDestroy(e);
   for (auto &e : pev) (void)hipEventDestroy(e);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (!stream) return;
+  // the stream, its events and the small pinned buffer go back to the
+  // process's cache for the next handle on this device
+  StreamKit k{device, stream, {}, h_pinned};
+  for (int i = 0; i < 8; ++i) k.ev[i] = ev[i];
+  if (!kit_put(k)) {
+    if (h_pinned) (void)hipHostFree(h_pinned);
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+// Streams, their timing events and the small pinned staging buffer of
+// destroyed handles, kept for reuse: creating a stream costs the runtime
+// up to milliseconds, which a new handle (a serving lane's, a batch
+// group's) would otherwise pay on every graph.
+namespace {
+std::mutex g_kit_mu;
+std::vector<StreamKit> g_kits;
+constexpr size_t KIT_CACHE = 32;
+}  // namespace
+
+bool kit_put(const StreamKit &k) {
+  std::lock_guard<std::mutex> g(g_kit_mu);
+  if (g_kits.size() >= KIT_CACHE) return false;
+  g_kits.push_back(k);
+  return true;
+}
+
+bool kit_take(int device, StreamKit *k) {
+  std::lock_guard<std::mutex> g(g_kit_mu);
+  for (size_t i = g_kits.size(); i-- > 0;)
+    if (g_kits[i].device == device) {
+      *k = g_kits[i];
+      g_kits.erase(g_kits.begin() + i);
+      return true;
+    }
+  return false;
+}
+
+int Engine::ensure_all(const std::vector<std::pair<DevBuf *, size_t>> &req) {
+  size_t total = 0, fresh = 0;
+  bool all_empty = true;
+  for (const auto &r : req)
+    if (r.second > r.first->cap) {
+      if (r.first->ptr) all_empty = false;
+      total += (r.second + 255) & ~(size_t)255;
+      ++fresh;
+    }
+  if (arena || !all_empty || fresh < 2) {
+    for (const auto &r : req) PSC_TRY(r.first->ensure(r.second));
+    return PSC_OK;
+  }
+  pool_setup();
+  void *blk = nullptr;
+  hipError_t e = hipMallocAsync(&blk, total, stream);
+  if (e != hipSuccess) {
+    set_error("hipMallocAsync(" + std::to_string(total) + " B): " + hipGetErrorString(e));
+    return PSC_ERR_NOMEM;
+  }
+  setup_blocks.push_back(blk);
+  char *at = static_cast<char *>(blk);
+  for (const auto &r : req)
+    if (r.second > r.first->cap) {
+      r.first->ptr = at;
+      r.first->cap = r.second;
+      r.first->carved = true;
+      at += (r.second + 255) & ~(size_t)255;
+    }
+  return PSC_OK;
 }
 
 std::vector<DevBuf *> Engine::all_bufs() {
@@ -1877,7 +2142,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &hubs, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta, &tail_spec, &tail_trace};
 }
 
@@ -1907,9 +2172,16 @@ int Engine::setup() {
     tail_pin_cap = 1024;
   } else {
     pool_setup();
-    PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
-    PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+    StreamKit k;
+    if (kit_take(device, &k)) {
+      stream = k.stream;
+      for (int i = 0; i < 8; ++i) ev[i] = k.ev[i];
+      h_pinned = k.pinned;
+    } else {
+      PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
+      PSC_HIP(hipHostMalloc(&h_pinned, 4096));
+    }
   }
   int occ = 0;
   PSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_reach_step,
@@ -1959,32 +2231,38 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     labels.ptr = labels_ext;
     labels.cap = labels_ext ? nn * 8 : 0;
   }
-  PSC_TRY(labels.ensure(nn * 8));
-  for (int s = 0; s < 2; ++s) {
-    PSC_TRY(vis[s].ensure(words * 4));
-    PSC_TRY(rq[s][0].ensure(nn * 4));
-    PSC_TRY(rq[s][1].ensure(nn * 4));
+  {
+    std::vector<std::pair<DevBuf *, size_t>> req;
+    req.emplace_back(&labels, nn * 8);
     const uint64_t hcap = std::min<uint64_t>(nn, m / std::min(HEAVY_DEG, PULL_HEAVY) + 1);
-    PSC_TRY(hv[s].ensure(hcap * 4));
-    PSC_TRY(hc[s].ensure(hcap * 8));
+    for (int s = 0; s < 2; ++s) {
+      req.emplace_back(&vis[s], words * 4);
+      req.emplace_back(&rq[s][0], nn * 4);
+      req.emplace_back(&rq[s][1], nn * 4);
+      req.emplace_back(&hv[s], hcap * 4);
+      req.emplace_back(&hc[s], hcap * 8);
+      req.emplace_back(&rhead[s], nn * 4);
+    }
+    req.emplace_back(&centers, nn * 4);
+    req.emplace_back(&front, nn * 4);
+    const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
+    req.emplace_back(&tiles, ntiles * 4);
+    req.emplace_back(&trim_bm, words * 4);
+    req.emplace_back(&tile_off, ntiles * 8);
+    req.emplace_back(&trim_off, ntiles * 8);
+    req.emplace_back(&reach_desc, sizeof(ReachSearch) * 2);
+    req.emplace_back(&reach_ctl, sizeof(ReachCtl));
+    req.emplace_back(&ms_desc, sizeof(MSSearch) * 2);
+    req.emplace_back(&ms_ctl, sizeof(MSCtl));
+    req.emplace_back(&misc, 256);
+    req.emplace_back(&scratch32, (nn + G) * 4);
+    for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) req.emplace_back(b, (G + 1) * 8);
+    req.emplace_back(&gA, 2 * (G + 1) * 8);   // nstart, then zstart
+    req.emplace_back(&gid_d, (G + 1) * 4);
+    req.emplace_back(&first_trim, (G + 1) * 4);
+    req.emplace_back(&hubs, 2 * HUB_WORDS * 4);
+    PSC_TRY(ensure_all(req));
   }
-  PSC_TRY(centers.ensure(nn * 4));
-  PSC_TRY(front.ensure(nn * 4));
-  const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
-  PSC_TRY(tiles.ensure(ntiles * 4));
-  PSC_TRY(trim_bm.ensure(words * 4));
-  PSC_TRY(tile_off.ensure(ntiles * 8));
-  PSC_TRY(trim_off.ensure(ntiles * 8));
-  PSC_TRY(reach_desc.ensure(sizeof(ReachSearch) * 2));
-  PSC_TRY(reach_ctl.ensure(sizeof(ReachCtl)));
-  PSC_TRY(ms_desc.ensure(sizeof(MSSearch) * 2));
-  PSC_TRY(ms_ctl.ensure(sizeof(MSCtl)));
-  PSC_TRY(misc.ensure(256));
-  PSC_TRY(scratch32.ensure((nn + G) * 4));
-  for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) PSC_TRY(b->ensure((G + 1) * 8));
-  PSC_TRY(gA.ensure(2 * (G + 1) * 8));   // nstart, then zstart
-  PSC_TRY(gid_d.ensure((G + 1) * 4));
-  PSC_TRY(first_trim.ensure((G + 1) * 4));
   PSC_TRY(gpin(G + 2));
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
   // per-graph state read by every pull level: the first neighbour of each
@@ -1992,19 +2270,29 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   // it is one coalesced 4-byte load instead of an offset pair and a line of
   // the adjacency array
   {
-    // the round front (n entries, unused until the rounds) holds the degrees
-    uint32_t *deg32 = front.as<uint32_t>();
-    const uint64_t blocks = std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 65536);
+    // a pull of search s reads the other CSR; its candidates' in-neighbours
+    // are reached along the search's own CSR, whose degree ranks them.  The
+    // round front and the centers (n entries each, unused until the first
+    // labelling) hold the degrees.
+    HeadDirs hd;
     for (int s = 0; s < 2; ++s) {
-      PSC_TRY(rhead[s].ensure(nn * 4));
-      // a pull of search s reads the other CSR; its candidates' in-neighbours
-      // are reached along the search's own CSR, whose degree ranks them
-      k_degrees32<<<(unsigned)blocks, BLOCK, 0, stream>>>(s == 0 ? d_off : d_ioff, n, deg32);
-      k_pull_heads<<<(unsigned)blocks, BLOCK, 0, stream>>>(
-          s == 0 ? d_ioff : d_off, s == 0 ? d_iadj : d_adj, deg32, n,
-          rhead[s].as<uint32_t>());
-      PSC_HIP(hipGetLastError());
+      hd.d[s].own_off = s == 0 ? d_off : d_ioff;
+      hd.d[s].pull_off = s == 0 ? d_ioff : d_off;
+      hd.d[s].pull_adj = s == 0 ? d_iadj : d_adj;
+      hd.d[s].deg = (s == 0 ? front : centers).as<uint32_t>();
+      hd.d[s].hub = hubs.as<uint32_t>() + s * HUB_WORDS;
+      hd.d[s].head = rhead[s].as<uint32_t>();
     }
+    const unsigned blocks = (unsigned)std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 2048);
+    const unsigned hblocks = (unsigned)std::max<uint64_t>(
+        1, std::min<uint64_t>((nn + HEAD_BLOCK - 1) / HEAD_BLOCK, (uint64_t)num_cu));
+    for (int s = 0; s < 2; ++s)
+      k_fill_u32<<<1, BLOCK, 0, stream>>>(hd.d[s].hub, HUB_BINS, 0u);
+    k_degrees32<<<dim3(blocks, 2), BLOCK, 0, stream>>>(hd, n);
+    k_hub_select<<<dim3(1, 2), BLOCK, 0, stream>>>(hd);
+    k_hub_insert<<<dim3(blocks, 2), BLOCK, 0, stream>>>(hd, n);
+    k_pull_heads<<<dim3(hblocks, 2), HEAD_BLOCK, 0, stream>>>(hd, n);
+    PSC_HIP(hipGetLastError());
   }
 
   ReachSearch rs[2];
