@@ -791,6 +791,7 @@ struct HeadDir {
   const uint64_t *pull_off;   // the pull direction's CSR
   const uint32_t *pull_adj;
   uint32_t *deg;              // n degrees (scratch)
+  uint8_t *lg;                // n log-scale degree bins (hub_bin), scratch
   uint32_t *hub;              // HUB_WORDS: histogram, threshold, table
   uint32_t *head;             // n heads (out)
 };
@@ -802,14 +803,17 @@ __global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
   const HeadDir &D = hd.d[blockIdx.y];
   const uint64_t *__restrict__ off = D.own_off;
   uint32_t *deg = D.deg, *hist = D.hub;
+  uint8_t *lg = D.lg;
   __shared__ uint32_t h[HUB_BINS];
   for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK) h[i] = 0;
   __syncthreads();
   for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t d = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
+    const uint32_t bin = hub_bin(d);
     deg[v] = d;
-    if (d) atomicAdd(&h[hub_bin(d)], 1u);
+    lg[v] = (uint8_t)bin;
+    if (d) atomicAdd(&h[bin], 1u);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK)
@@ -863,7 +867,9 @@ __global__ __launch_bounds__(BLOCK) void k_hub_insert(HeadDirs hd, uint64_t n) {
 // different lengths cost their own entries only.  Hub hits raise an LDS
 // maximum per vertex of (hub degree, earliest position, neighbour); the
 // vertices left without a hub then walk their first HUB_FALLBACK entries the
-// same way, ranked by the looked-up degree.
+// same way, ranked by the neighbours' log-scale degree bins (one byte per
+// vertex: a quarter of the degree array's lines, so the random lookups
+// mostly hit the caches).
 constexpr int HEAD_WIN = 4;
 
 __device__ __forceinline__ unsigned long long head_key(uint32_t d, uint32_t pos, uint32_t u) {
@@ -968,7 +974,7 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
         }
       }
 #pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j) du[j] = own[j] >= 0 ? deg[u[j]] : 0u;
+      for (int j = 0; j < HEAD_WIN; ++j) du[j] = own[j] >= 0 ? D.lg[u[j]] : 0u;
 #pragma unroll
       for (int j = 0; j < HEAD_WIN; ++j)
         if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
@@ -2280,6 +2286,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
       hd.d[s].pull_off = s == 0 ? d_ioff : d_off;
       hd.d[s].pull_adj = s == 0 ? d_iadj : d_adj;
       hd.d[s].deg = (s == 0 ? front : centers).as<uint32_t>();
+      hd.d[s].lg = rq[s][0].as<uint8_t>();   // frontier buffer, free until the runs
       hd.d[s].hub = hubs.as<uint32_t>() + s * HUB_WORDS;
       hd.d[s].head = rhead[s].as<uint32_t>();
     }
