@@ -80,13 +80,13 @@ typedef struct psc_params {
                            windows hold one pivot position each runs
                            as bitmap reaches (same labels as the
                            multi-search); 0: always multi-search      */
-  int32_t capture;      /* 1: the level batches enqueued between host
-                           polls are captured once into hipGraphs and
-                           replayed; 0 (default): plain launches.  Only
-                           for a process whose other threads make no
-                           HIP calls while a handle runs (no serving
-                           lanes): capture is a stream mode that other
-                           threads' allocating calls can disturb      */
+  int32_t capture;      /* 1 (default): the level batches enqueued
+                           between host polls are captured once into
+                           hipGraphs and replayed; 0: plain launches.
+                           A capture excludes the library's allocating
+                           calls in every other thread for its
+                           duration (the labels are the same either
+                           way)                                       */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).
@@ -375,6 +375,31 @@ int psc_batch_create_from_graphs(const psc_graph *dev_graphs, uint64_t count,
 int psc_batch_rebind(psc_handle *h, const psc_graph *dev_graphs, uint64_t count,
                      const uint64_t *id_base, const uint64_t *edge_base);
 
+/* Serving lanes (no reference counterpart).  psc_lane_create makes a
+ * re-bindable handle that carves every device buffer -- the union copy of
+ * the graphs psc_batch_rebind gathers, the working set, the tables -- from
+ * one region of arena_bytes allocated here, so the lane holds exactly that
+ * much device memory whatever it runs.  A rebind or run that does not fit
+ * returns PSC_ERR_NOMEM ("scratch exhausted") and the lane stays usable for
+ * a smaller binding.  psc_batch_layout_bytes gives the arena bytes a
+ * binding of num_graphs graphs with n vertices and m edges in all takes
+ * before its first run (the run adds its round buffers). */
+int psc_lane_create(uint64_t arena_bytes, psc_handle **out);
+uint64_t psc_batch_layout_bytes(uint64_t n, uint64_t m, uint64_t num_graphs);
+
+/* Device bytes a handle holds (its arena, or its buffers), and the most of
+ * its arena ever in use (= held for a handle without arena). */
+int psc_handle_device_bytes(psc_handle *h, uint64_t *held, uint64_t *peak);
+
+/* Free and total device memory as the driver reports them. */
+int psc_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* The current device's stream-ordered memory pool, which handles and device
+ * batches allocate from: bytes it holds from the driver and bytes of live
+ * allocations; psc_pool_trim gives unused memory back, keeping keep_bytes. */
+int psc_pool_usage(uint64_t *reserved, uint64_t *used);
+int psc_pool_trim(uint64_t keep_bytes);
+
 /* Number of graphs of a handle (1 for psc_create) and their vertex prefix
  * (num_graphs + 1 entries). */
 uint64_t psc_handle_graphs(psc_handle *h);
@@ -409,6 +434,11 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
                       const uint64_t *n, uint64_t seed, const uint64_t *seeds,
                       psc_dev_batch **out);
 void psc_dev_batch_free(psc_dev_batch *db);
+/* Device bytes psc_dev_gen_batch takes for the same kinds and sizes:
+ * *keep = what the returned batch holds, *peak = all it allocates at once
+ * while generating (keep included; pool-granular). */
+int psc_dev_gen_batch_bytes(uint64_t num_graphs, const uint32_t *kind, const uint64_t *n,
+                            uint64_t *keep, uint64_t *peak);
 /* The union graph of a device batch (device pointers, global ids; for a
  * one-graph batch these are the graph's local ids). */
 int psc_dev_batch_union(const psc_dev_batch *db, psc_graph *out);

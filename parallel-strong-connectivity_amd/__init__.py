@@ -172,6 +172,13 @@ _sig("psc_dev_gen_batch", C.c_int, C.c_uint64, u32p, u64p, C.c_uint64, u64p,
 _sig("psc_dev_batch_free", None, C.c_void_p)
 _sig("psc_dev_batch_info", C.c_int, C.c_void_p, u64p, u64p, u64p, u32p)
 _sig("psc_batch_create_dev", C.c_int, C.c_void_p, C.POINTER(C.c_void_p))
+_sig("psc_lane_create", C.c_int, C.c_uint64, C.POINTER(C.c_void_p))
+_sig("psc_batch_layout_bytes", C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64)
+_sig("psc_handle_device_bytes", C.c_int, C.c_void_p, u64p, u64p)
+_sig("psc_device_mem_info", C.c_int, u64p, u64p)
+_sig("psc_pool_usage", C.c_int, u64p, u64p)
+_sig("psc_pool_trim", C.c_int, C.c_uint64)
+_sig("psc_dev_gen_batch_bytes", C.c_int, C.c_uint64, u32p, u64p, u64p, u64p)
 _sig("psc_dev_batch_extract", C.c_int, C.c_void_p, C.c_uint64,
      C.POINTER(C.POINTER(CDevGraph)))
 

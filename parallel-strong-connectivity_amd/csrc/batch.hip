@@ -104,6 +104,7 @@ __global__ void k_rebase_one(const uint64_t *loff, const uint32_t *ladj,
 
 template <class T>
 int dev_copy_in(const T *src, size_t count, int on_device, T **dst) {
+  psc::GateGuard gate;
   *dst = nullptr;
   if (hipMalloc(dst, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
     psc::set_error("hipMalloc (batch input)");
@@ -125,10 +126,12 @@ int dev_copy_in(const T *src, size_t count, int on_device, T **dst) {
 // is released.  Frees are ordered on a library stream.
 template <class T>
 static hipError_t pool_alloc(T **p, size_t bytes, hipStream_t st) {
+  psc::GateGuard gate;
   return hipMallocAsync(reinterpret_cast<void **>(p), std::max<size_t>(bytes, 8), st);
 }
 
 static void pool_free(void *p) {
+  psc::GateGuard gate;
   if (p) (void)hipFreeAsync(p, psc::pool_free_stream());
 }
 
@@ -345,6 +348,7 @@ struct psc_dev_batch {
 extern "C" {
 
 int psc_batch_create(const psc_batch_desc *b, int on_device, psc_handle **out) {
+  psc::GateGuard gate;
   if (!b || !out || b->num_graphs == 0 || b->num_graphs > 0xFFFFFFF0ull)
     return PSC_ERR_INVALID;
   *out = nullptr;
@@ -452,6 +456,7 @@ static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count,
   psc::Engine &e = h->eng;
   const uint64_t ntot = vb[G], mtot = eb[G];
   int rc = e.setup();
+  if (!rc && e.arena) e.arena_reset();   // a lane carves the new binding afresh
   if (!rc) rc = e.g_off.ensure((ntot + 1) * 8);
   if (!rc) rc = e.g_ioff.ensure((ntot + 1) * 8);
   if (!rc) rc = e.g_adj.ensure(std::max<uint64_t>(mtot, 1) * 4);
@@ -492,6 +497,7 @@ static int gather_graphs(psc_handle *h, const psc_graph *graphs, uint64_t count,
 int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
                                  const uint64_t *id_base, const uint64_t *edge_base,
                                  psc_handle **out) {
+  psc::GateGuard gate;
   if (!graphs || !out || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
   *out = nullptr;
   auto *h = new (std::nothrow) psc_handle;
@@ -507,12 +513,133 @@ int psc_batch_create_from_graphs(const psc_graph *graphs, uint64_t count,
 
 int psc_batch_rebind(psc_handle *h, const psc_graph *graphs, uint64_t count,
                      const uint64_t *id_base, const uint64_t *edge_base) {
+  psc::GateGuard gate;
   if (!h || !graphs || count == 0 || count > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
-  if (!h->eng.owned_graph && h->eng.ready) {
+  if (!h->eng.owned_graph && h->eng.ready && !h->eng.lane) {
     psc::set_error("psc_batch_rebind: handle borrows its graph");
     return PSC_ERR_INVALID;
   }
   return gather_graphs(h, graphs, count, id_base, edge_base);
+}
+
+// A serving lane: a re-bindable handle whose every device buffer -- the
+// graphs' union copy, the working set, the tables -- is carved from one
+// region of arena_bytes allocated here, so the lane holds exactly that much
+// device memory whatever it runs.  A binding or run that does not fit
+// fails with PSC_ERR_NOMEM and leaves the lane usable.
+int psc_lane_create(uint64_t arena_bytes, psc_handle **out) {
+  psc::GateGuard gate;
+  if (!out || arena_bytes < (1u << 20)) return PSC_ERR_INVALID;
+  *out = nullptr;
+  auto *h = new (std::nothrow) psc_handle;
+  if (!h) return PSC_ERR_NOMEM;
+  psc::Engine &e = h->eng;
+  int rc = e.setup();
+  void *p = nullptr;
+  if (!rc && hipMalloc(&p, arena_bytes) != hipSuccess) {
+    psc::set_error("psc_lane_create: hipMalloc(" + std::to_string(arena_bytes) + " B) failed");
+    rc = PSC_ERR_NOMEM;
+  }
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  e.lane_arena.base = static_cast<char *>(p);
+  e.lane_arena.cap = arena_bytes;
+  e.arena = &e.lane_arena;
+  e.lane = true;
+  e.owned_graph = true;
+  e.voff_h.assign(2, 0);   // no graph bound yet
+  *out = h;
+  return PSC_OK;
+}
+
+// Device bytes a handle holds: its arena, or its pool buffers, set-up
+// blocks and owned graph copy; *peak gets the most of an arena ever in use
+// (the held bytes for a pool handle).
+int psc_handle_device_bytes(psc_handle *h, uint64_t *held, uint64_t *peak) {
+  if (!h) return PSC_ERR_INVALID;
+  psc::Engine &e = h->eng;
+  uint64_t b = 0, pk = 0;
+  if (e.arena) {
+    b = e.arena->cap;
+    pk = e.arena->peak;
+  } else {
+    for (psc::DevBuf *d : e.all_bufs())
+      if (!d->carved) b += d->cap;
+    b += e.setup_bytes;
+    if (e.owned_graph)
+      for (psc::DevBuf *d : {&e.g_off, &e.g_adj, &e.g_ioff, &e.g_iadj}) b += d->cap;
+    pk = b;
+  }
+  if (held) *held = b;
+  if (peak) *peak = pk;
+  return PSC_OK;
+}
+
+// Arena bytes a lane binding of G graphs with n vertices and m edges in all
+// takes before its first run: the graphs' union copy and the working set
+// of init_requests, each piece 256-byte aligned as the arena carves it.
+uint64_t psc_batch_layout_bytes(uint64_t n, uint64_t m, uint64_t num_graphs) {
+  auto a = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  psc::Engine e;
+  uint64_t t = 2 * a((n + 1) * 8) + 2 * a(std::max<uint64_t>(m, 1) * 4);
+  for (const auto &r : e.init_requests(n, m, std::max<uint64_t>(num_graphs, 1))) t += a(r.second);
+  return t;
+}
+
+// The device's stream-ordered pool, which the library's handles and device
+// batches allocate from: bytes it holds from the driver (reserved) and
+// bytes of live allocations (used).  Freed memory stays reserved for reuse
+// until psc_pool_trim gives it back.
+int psc_pool_usage(uint64_t *reserved, uint64_t *used) {
+  psc::GateGuard gate;
+  int dev = 0;
+  hipMemPool_t pool;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+    psc::set_error("psc_pool_usage: no default memory pool");
+    return PSC_ERR_HIP;
+  }
+  uint64_t r = 0, u = 0;
+  if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &r) != hipSuccess ||
+      hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &u) != hipSuccess) {
+    psc::set_error("psc_pool_usage: hipMemPoolGetAttribute failed");
+    return PSC_ERR_HIP;
+  }
+  if (reserved) *reserved = r;
+  if (used) *used = u;
+  return PSC_OK;
+}
+
+// Give the pool's unused memory back to the driver, keeping keep_bytes
+// reserved (frees already ordered on the library's free stream complete
+// first).
+int psc_pool_trim(uint64_t keep_bytes) {
+  psc::GateGuard gate;
+  int dev = 0;
+  hipMemPool_t pool;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) {
+    psc::set_error("psc_pool_trim: no default memory pool");
+    return PSC_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(psc::pool_free_stream());
+  if (hipMemPoolTrimTo(pool, keep_bytes) != hipSuccess) {
+    psc::set_error("psc_pool_trim: hipMemPoolTrimTo failed");
+    return PSC_ERR_HIP;
+  }
+  return PSC_OK;
+}
+
+// Free and total device memory as the driver reports them (hipMemGetInfo).
+int psc_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) {
+    psc::set_error("hipMemGetInfo failed");
+    return PSC_ERR_HIP;
+  }
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return PSC_OK;
 }
 
 // Graph g of a device batch as a view into its union (global ids from
@@ -540,7 +667,13 @@ uint64_t psc_handle_graphs(psc_handle *h) { return h ? h->eng.G : 0; }
 
 int psc_handle_vbase(psc_handle *h, uint64_t *vbase_out) {
   if (!h || !vbase_out) return PSC_ERR_INVALID;
-  std::memcpy(vbase_out, h->eng.voff_h.data(), (h->eng.G + 1) * 8);
+  const psc::Engine &e = h->eng;
+  // a lane that was never bound holds no graph: one empty graph
+  if (e.voff_h.size() < (size_t)e.G + 1) {
+    std::memset(vbase_out, 0, ((size_t)e.G + 1) * 8);
+    return PSC_OK;
+  }
+  std::memcpy(vbase_out, e.voff_h.data(), ((size_t)e.G + 1) * 8);
   return PSC_OK;
 }
 
@@ -605,17 +738,28 @@ int psc_batch_spec(uint64_t num_graphs, double lo_log2, double hi_log2,
 // seeds is NULL.  Graphs are built in groups of at most 2^30 edge keys
 // (generate, radix sort, de-duplicate, both CSR directions); the groups'
 // CSRs are concatenated into the union.
-int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
-                      const uint64_t *nreq, uint64_t seed,
-                      const uint64_t *seeds, psc_dev_batch **out) {
-  if (!out || !kind || !nreq || num_graphs == 0 || num_graphs > 0xFFFFFFF0ull)
-    return PSC_ERR_INVALID;
-  *out = nullptr;
+constexpr uint32_t GEN_KNN_K = 5;
+constexpr uint64_t GEN_GROUP_KEYS = 1ull << 30;
+
+// What a batch generation builds and allocates: per graph its description
+// and key count, the vertex prefix, the key count of the largest group of
+// graphs (groups of at most GEN_GROUP_KEYS keys are generated one after the
+// other), the total key count and the largest k-NN cell count of a group.
+struct GenPlan {
+  std::vector<GenGraph> gg;
+  std::vector<uint64_t> vb, nkeys;
+  uint64_t max_group = 0, total_keys = 0, max_cells = 0;
+};
+
+static int gen_plan(uint64_t num_graphs, const uint32_t *kind, const uint64_t *nreq,
+                    uint64_t seed, const uint64_t *seeds, GenPlan *P) {
   const uint32_t G = (uint32_t)num_graphs;
-  const uint32_t KNN_K = 5;
-  const uint64_t GROUP_KEYS = 1ull << 30;
-  std::vector<GenGraph> gg(G);
-  std::vector<uint64_t> vb(G + 1, 0), nkeys_g(G);
+  const uint32_t KNN_K = GEN_KNN_K;
+  P->gg.assign(G, GenGraph{});
+  P->vb.assign(G + 1, 0);
+  P->nkeys.assign(G, 0);
+  std::vector<GenGraph> &gg = P->gg;
+  std::vector<uint64_t> &vb = P->vb, &nkeys_g = P->nkeys;
   for (uint32_t g = 0; g < G; ++g) {
     GenGraph x{};
     x.kind = kind[g];
@@ -651,20 +795,67 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
     psc::set_error("batch too large: total vertices must be < 2^32-1");
     return PSC_ERR_INVALID;
   }
-  uint64_t max_group = 0;   // keys of the largest group
-  {
-    uint64_t acc = 0;
-    for (uint32_t g = 0; g < G; ++g) {
-      if (acc && acc + nkeys_g[g] > GROUP_KEYS) {
-        max_group = std::max(max_group, acc);
-        acc = 0;
-      }
-      acc += nkeys_g[g];
+  uint64_t acc = 0, cells = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    if (acc && acc + nkeys_g[g] > GEN_GROUP_KEYS) {   // the groups of psc_dev_gen_batch
+      P->max_group = std::max(P->max_group, acc);
+      P->max_cells = std::max(P->max_cells, cells);
+      acc = cells = 0;
     }
-    max_group = std::max(max_group, acc);
+    acc += nkeys_g[g];
+    if (gg[g].kind == KIND_KNN) cells += (uint64_t)gg[g].cells1 * gg[g].cells1 * gg[g].cells1;
   }
-  uint64_t total_keys = 0;
-  for (uint32_t g = 0; g < G; ++g) total_keys += nkeys_g[g];
+  P->max_group = std::max(P->max_group, acc);
+  P->max_cells = std::max(P->max_cells, cells);
+  for (uint32_t g = 0; g < G; ++g) P->total_keys += nkeys_g[g];
+  return PSC_OK;
+}
+
+// Device bytes of psc_dev_gen_batch for the same description: *keep = what
+// the returned batch holds, *peak = everything allocated at once while it
+// is built (the keep part included), each allocation rounded up to the
+// pool's 2 MiB granularity.
+int psc_dev_gen_batch_bytes(uint64_t num_graphs, const uint32_t *kind, const uint64_t *nreq,
+                            uint64_t *keep, uint64_t *peak) {
+  if (!kind || !nreq || num_graphs == 0 || num_graphs > 0xFFFFFFF0ull) return PSC_ERR_INVALID;
+  GenPlan P;
+  const int pr = gen_plan(num_graphs, kind, nreq, 0, nullptr, &P);
+  if (pr) return pr;
+  const uint64_t G = num_graphs, ntot = P.vb[G], mg = std::max<uint64_t>(P.max_group, 1);
+  auto r = [](uint64_t b) { return (std::max<uint64_t>(b, 8) + (2u << 20) - 1) & ~(uint64_t)((2u << 20) - 1); };
+  size_t tb = 0, tb2 = 0;
+  unsigned long long *nk = nullptr;
+  if (rocprim::radix_sort_keys(nullptr, tb, nk, nk, mg, 0, 64, hipStream_t(0)) != hipSuccess ||
+      rocprim::unique(nullptr, tb2, nk, nk, nk, mg, rocprim::equal_to<unsigned long long>(),
+                      hipStream_t(0)) != hipSuccess) {
+    psc::set_error("rocprim temporary size query failed");
+    return PSC_ERR_HIP;
+  }
+  const uint64_t k = 2 * r((ntot + 1) * 8) + 2 * r(std::max<uint64_t>(P.total_keys, 1) * 4);
+  const uint64_t t = 2 * r(mg * 8) + r(8) + r(G * sizeof(GenGraph)) + 2 * r((G + 1) * 8) +
+                     r(std::max(tb, tb2)) + (P.max_cells ? r((P.max_cells + 1) * 8) : 0);
+  if (keep) *keep = k;
+  if (peak) *peak = k + t;
+  return PSC_OK;
+}
+
+int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
+                      const uint64_t *nreq, uint64_t seed,
+                      const uint64_t *seeds, psc_dev_batch **out) {
+  if (!out || !kind || !nreq || num_graphs == 0 || num_graphs > 0xFFFFFFF0ull)
+    return PSC_ERR_INVALID;
+  *out = nullptr;
+  GenPlan P;
+  {
+    const int pr = gen_plan(num_graphs, kind, nreq, seed, seeds, &P);
+    if (pr) return pr;
+  }
+  const uint32_t G = (uint32_t)num_graphs;
+  const uint32_t KNN_K = GEN_KNN_K;
+  const uint64_t GROUP_KEYS = GEN_GROUP_KEYS;
+  std::vector<GenGraph> &gg = P.gg;
+  std::vector<uint64_t> &vb = P.vb, &nkeys_g = P.nkeys;
+  const uint64_t max_group = P.max_group, total_keys = P.total_keys;
   const uint64_t ntot = vb[G];
   int rc = PSC_OK;
   hipStream_t st = nullptr;
@@ -679,7 +870,10 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
   std::vector<uint64_t> eb(G + 1, 0);
   uint64_t *d_vb = nullptr, *d_eb = nullptr;
   psc::pool_setup();
-  B_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  {
+    psc::GateGuard gate;
+    B_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  }
   B_HIP(pool_alloc(&keys, std::max<uint64_t>(max_group, 1) * 8, st));
   B_HIP(pool_alloc(&tmp, std::max<uint64_t>(max_group, 1) * 8, st));
   B_HIP(pool_alloc(&cnt, 8, st));
@@ -736,7 +930,10 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
     if (!kgv.empty()) {
       B_HIP(hipMemcpyAsync(d_g, kgv.data(), kgv.size() * sizeof(GenGraph),
                            hipMemcpyHostToDevice, st));
-      if (cstart) (void)hipFreeAsync(cstart, st);
+      if (cstart) {
+        psc::GateGuard gate;
+        (void)hipFreeAsync(cstart, st);
+      }
       cstart = nullptr;
       B_HIP(pool_alloc(&cstart, (ncells + 1) * 8, st));
       // cell keys in the k-NN key region (it holds 5 slots per point)
@@ -804,13 +1001,16 @@ int psc_dev_gen_batch(uint64_t num_graphs, const uint32_t *kind,
 done:
   delete db;
   if (st) (void)hipStreamSynchronize(st);
-  for (void *p : {(void *)d_g, (void *)keys, (void *)tmp, (void *)cnt, (void *)cstart,
-                  (void *)d_vb, (void *)d_eb, (void *)off, (void *)ioff, (void *)E,
-                  (void *)iE, temp})
-    if (p) (void)hipFreeAsync(p, st);
-  if (st) {
-    (void)hipStreamSynchronize(st);
-    (void)hipStreamDestroy(st);
+  {
+    psc::GateGuard gate;
+    for (void *p : {(void *)d_g, (void *)keys, (void *)tmp, (void *)cnt, (void *)cstart,
+                    (void *)d_vb, (void *)d_eb, (void *)off, (void *)ioff, (void *)E,
+                    (void *)iE, temp})
+      if (p) (void)hipFreeAsync(p, st);
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   }
   return rc;
 }
@@ -818,6 +1018,7 @@ done:
 // The caller guarantees that no work using the batch is still in flight
 // (every handle that read it has returned from its run / rebind).
 void psc_dev_batch_free(psc_dev_batch *db) {
+  psc::GateGuard gate;
   if (!db) return;
   pool_free((void *)db->g.offset);
   pool_free((void *)db->g.edges);
@@ -838,6 +1039,7 @@ int psc_dev_batch_info(const psc_dev_batch *db, uint64_t *num_graphs,
 
 // Handle over a device batch in place (the union is borrowed).
 int psc_batch_create_dev(const psc_dev_batch *db, psc_handle **out) {
+  psc::GateGuard gate;
   if (!db || !out) return PSC_ERR_INVALID;
   *out = nullptr;
   auto *h = new (std::nothrow) psc_handle;
@@ -854,6 +1056,7 @@ int psc_batch_create_dev(const psc_dev_batch *db, psc_handle **out) {
 // Graph g of a device batch as a stand-alone device graph with local ids
 // (for the one-graph-per-launch path).
 int psc_dev_batch_extract(const psc_dev_batch *db, uint64_t g, psc_dev_graph **out) {
+  psc::GateGuard gate;
   if (!db || !out || g >= db->num_graphs) return PSC_ERR_INVALID;
   *out = nullptr;
   const uint64_t v0 = db->vbase[g], n = db->vbase[g + 1] - v0;

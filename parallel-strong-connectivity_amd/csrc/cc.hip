@@ -304,11 +304,13 @@ done:
 }
 
 int psc_connectivity_device(const psc_graph *dev_g, uint32_t *d_labels, double *ms) {
+  psc::GateGuard gate;
   return cc_device(dev_g, d_labels, nullptr, nullptr, ms);
 }
 
 int psc_spanning_forest_device(const psc_graph *dev_g, uint32_t *d_labels, uint32_t *d_forest,
                                uint64_t *count, double *ms) {
+  psc::GateGuard gate;
   if (!count || (dev_g && dev_g->n > 1 && !d_forest)) {
     psc::set_error("psc_spanning_forest_device: bad arguments");
     return PSC_ERR_INVALID;
@@ -360,11 +362,13 @@ done:
 }
 
 int psc_connectivity(const psc_graph *g, uint32_t *labels) {
+  psc::GateGuard gate;
   return cc_host(g, labels, nullptr, nullptr);
 }
 
 int psc_spanning_forest(const psc_graph *g, uint32_t *labels, uint32_t *forest,
                         uint64_t *count) {
+  psc::GateGuard gate;
   if (!count || (g && g->n > 1 && !forest)) {
     psc::set_error("psc_spanning_forest: bad arguments");
     return PSC_ERR_INVALID;

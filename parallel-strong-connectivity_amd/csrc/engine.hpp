@@ -38,6 +38,30 @@ extern thread_local std::string g_last_error;
   } while (0)
 
 
+// Capture gate.  A stream in capture mode must not see the runtime's
+// allocation, free, stream / event creation and destruction, graph
+// instantiation or device-wide synchronisation happen in other threads of
+// the process at the same time (a serving process whose lanes create, bind
+// and free while one lane captured its level batches ended in a host
+// segmentation fault inside the runtime).  Every such call of this library
+// holds the gate shared (GateGuard, re-entrant per thread); capturing and
+// instantiating a level batch holds it exclusively (CaptureGuard), after
+// the calls in flight have left and before new ones enter.
+void gate_enter();
+void gate_leave();
+struct GateGuard {
+  GateGuard() { gate_enter(); }
+  ~GateGuard() { gate_leave(); }
+  GateGuard(const GateGuard &) = delete;
+  GateGuard &operator=(const GateGuard &) = delete;
+};
+struct CaptureGuard {
+  CaptureGuard();
+  ~CaptureGuard();
+  CaptureGuard(const CaptureGuard &) = delete;
+  CaptureGuard &operator=(const CaptureGuard &) = delete;
+};
+
 // Device memory comes from the device's stream-ordered pool, allocated
 // and freed in the order of the owning handle's stream: growing a buffer
 // never waits for other streams (a plain hipFree waits for the device).
@@ -102,12 +126,19 @@ struct Engine {
   int device = 0, num_cu = 1;
   hipStream_t stream = nullptr;
   bool ready = false;       // setup() done
-  // Arena mode (the allocation-free entry point): every device buffer comes
-  // from `arena`, the stream is the caller's, host staging is the engine's
-  // own (pageable) memory and no event is created: the engine makes no
-  // device, pinned-host, stream or event allocation.
+  // Arena mode: every device buffer comes from `arena` (re-carved at each
+  // binding, arena_reset).  A serving lane owns its arena (lane_arena, one
+  // allocation made when the lane is created); the allocation-free entry
+  // point uses the library's static scratch and also sets host_static: the
+  // stream is then the caller's, host staging is the engine's own (pageable)
+  // memory and no event is created, so the engine makes no device,
+  // pinned-host, stream or event allocation at all.
   Arena *arena = nullptr;
-  unsigned long long *labels_ext = nullptr;   // arena mode: the output labels
+  bool host_static = false;
+  Arena lane_arena;
+  bool lane = false;        // a serving lane: re-bindable, graphs copied in
+  unsigned long long *labels_ext = nullptr;   // allocation-free: the output labels
+  void arena_reset();
   static constexpr size_t HOST_STAGE = 8192;
   alignas(64) unsigned char host_stage[HOST_STAGE];
   hipEvent_t ev[8] = {};
@@ -164,7 +195,9 @@ struct Engine {
   // set-up blocks: the buffers a binding needs, allocated together (one
   // pool allocation instead of one per buffer) and freed with the engine
   std::vector<void *> setup_blocks;
+  uint64_t setup_bytes = 0;
   int ensure_all(const std::vector<std::pair<DevBuf *, size_t>> &req);
+  std::vector<std::pair<DevBuf *, size_t>> init_requests(uint64_t n, uint64_t m, uint64_t G);
   // one-time device set-up: stream, events, pinned staging, launch shapes
   int setup();
   int init(const psc_graph *dg, bool owned, uint32_t ngraphs = 1,

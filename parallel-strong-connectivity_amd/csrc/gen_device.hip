@@ -14,6 +14,7 @@
 #include <rocprim/device/device_select.hpp>
 
 #include "../../include/psc.h"
+#include "engine.hpp"
 #include "gen_common.hpp"
 
 namespace psc {
@@ -210,6 +211,7 @@ extern "C" {
 
 int psc_dev_gen_rmat(uint32_t scale, uint64_t samples, double a, double b,
                      double c, uint64_t seed, int permute, psc_dev_graph **out) {
+  psc::GateGuard gate;
   if (!out || scale < 1 || scale > 31 || a < 0 || b < 0 || c < 0 ||
       a + b + c > 1.0) {
     psc::set_error("psc_dev_gen_rmat: bad arguments");
@@ -232,6 +234,7 @@ done:
 
 int psc_dev_gen_grid(uint32_t rows, uint32_t cols, double p_both,
                      uint64_t seed, psc_dev_graph **out) {
+  psc::GateGuard gate;
   if (!out || rows == 0 || cols == 0 ||
       (uint64_t)rows * cols >= 0xFFFFFFFFull) {
     psc::set_error("psc_dev_gen_grid: bad arguments");
@@ -255,6 +258,7 @@ done:
 
 int psc_dev_make_directed(uint64_t n, uint64_t m, const uint64_t *d_offset,
                           const uint32_t *d_edges, psc_dev_graph **out) {
+  psc::GateGuard gate;
   if (!out || !d_offset || (m && !d_edges) || n >= 0xFFFFFFFFull) {
     psc::set_error("psc_dev_make_directed: bad arguments");
     return PSC_ERR_INVALID;
@@ -338,6 +342,7 @@ done:
 }
 
 void psc_dev_graph_free(psc_dev_graph *dg) {
+  psc::GateGuard gate;
   if (!dg) return;
   (void)hipFree((void *)dg->g.offset);
   (void)hipFree((void *)dg->g.edges);
@@ -347,6 +352,7 @@ void psc_dev_graph_free(psc_dev_graph *dg) {
 }
 
 int psc_dev_graph_download(const psc_dev_graph *dg, psc_host_graph **out) {
+  psc::GateGuard gate;
   if (!dg || !out) return PSC_ERR_INVALID;
   *out = nullptr;
   const uint64_t n = dg->g.n, m = dg->g.m;

@@ -602,6 +602,7 @@ extern "C" {
 
 int psc_le_lists_device(const psc_graph *dev_g, const uint32_t *d_order, double beta,
                         uint32_t **d_rows, uint64_t *count, double *ms) {
+  psc::GateGuard gate;
   using namespace psc;
   if (!dev_g || !d_rows || !count || (dev_g->n && (!dev_g->offset || !d_order)) ||
       (dev_g->m && !dev_g->edges) || dev_g->n >= 0xFFFFFFFFull || !(beta >= 1.0) ||
@@ -619,6 +620,7 @@ int psc_le_lists_device(const psc_graph *dev_g, const uint32_t *d_order, double 
 
 int psc_le_lists(const psc_graph *g, const uint32_t *order, double beta, uint32_t **rows,
                  uint64_t *count) {
+  psc::GateGuard gate;
   using namespace psc;
   if (!g || !rows || !count || (g->n && (!g->offset || !order)) || (g->m && !g->edges)) {
     set_error("psc_le_lists: bad arguments");
@@ -661,6 +663,7 @@ int psc_le_lists(const psc_graph *g, const uint32_t *order, double beta, uint32_
 void psc_le_free(uint32_t *rows) { std::free(rows); }
 
 int psc_le_free_device(uint32_t *d_rows) {
+  psc::GateGuard gate;
   using namespace psc;
   if (d_rows) LE_HIP(hipFree(d_rows));
   return PSC_OK;

@@ -54,6 +54,7 @@ void setup_device(Device &d) {
   d.arena.cap = SCRATCH_BYTES;
   d.eng = new psc::Engine;
   d.eng->arena = &d.arena;
+  d.eng->host_static = true;
   d.rc = d.eng->setup();
   if (d.rc) d.err = psc::g_last_error;
 }
@@ -104,6 +105,7 @@ int psc_run_noalloc(const psc_graph *dev_g, const psc_params *p, uint64_t *d_lab
   psc::Engine &e = *d.eng;
   e.stream = static_cast<hipStream_t>(stream);
   e.labels_ext = reinterpret_cast<unsigned long long *>(d_labels_out);
+  e.arena_reset();
   int rc = e.init(dev_g, false);
   psc_stats tmp;
   if (!rc) rc = e.run(q, e.labels_ext, stats ? stats : &tmp);

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <condition_variable>
 #include <mutex>
 #include <vector>
 
@@ -1967,10 +1968,46 @@ static int grid_for(uint64_t n, int per_block = BLOCK, int cap = 65536) {
   return (int)g;
 }
 
+// ---------------------------------------------------------- capture gate
+namespace {
+std::mutex g_gate_mu;
+std::condition_variable g_gate_cv;
+int g_gate_readers = 0;
+bool g_gate_writer = false;
+thread_local int t_gate_depth = 0;
+}  // namespace
+
+void gate_enter() {
+  if (t_gate_depth++ > 0) return;
+  std::unique_lock<std::mutex> l(g_gate_mu);
+  g_gate_cv.wait(l, [] { return !g_gate_writer; });
+  ++g_gate_readers;
+}
+
+void gate_leave() {
+  if (--t_gate_depth > 0) return;
+  std::lock_guard<std::mutex> l(g_gate_mu);
+  if (--g_gate_readers == 0) g_gate_cv.notify_all();
+}
+
+CaptureGuard::CaptureGuard() {
+  std::unique_lock<std::mutex> l(g_gate_mu);
+  g_gate_cv.wait(l, [] { return !g_gate_writer; });
+  g_gate_writer = true;   // new entries wait from here on
+  g_gate_cv.wait(l, [] { return g_gate_readers == 0; });
+}
+
+CaptureGuard::~CaptureGuard() {
+  std::lock_guard<std::mutex> l(g_gate_mu);
+  g_gate_writer = false;
+  g_gate_cv.notify_all();
+}
+
 static std::once_flag g_pool_once;
 static hipStream_t g_free_stream = nullptr;
 
 void pool_setup() {
+  GateGuard gate;
   std::call_once(g_pool_once, [] {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return;
@@ -2011,6 +2048,7 @@ int DevBuf::ensure(size_t bytes) {
     return PSC_OK;
   }
   pool_setup();
+  GateGuard gate;
   hipStream_t st = owner ? *owner : nullptr;
   if (ptr) {
     // a piece of a set-up block stays with the block
@@ -2038,7 +2076,10 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (ptr && !carved && !(arena && *arena)) (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
+  if (ptr && !carved && !(arena && *arena)) {
+    GateGuard gate;
+    (void)hipFreeAsync(ptr, owner ? *owner : nullptr);
+  }
   carved = false;
   ptr = nullptr;
   cap = 0;
@@ -2056,13 +2097,18 @@ Engine::Engine() {
 }
 
 Engine::~Engine() {
+  GateGuard gate;
   if (ready) (void)hipStreamSynchronize(stream);
   for (auto &g : gcache) (void)hipGraphExecDestroy(g.second);
   gcache.clear();
   for (DevBuf *b : all_bufs()) b->release();
   if (owned_graph)
     for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->release();
-  if (arena) return;   // stream and host staging belong to the caller / the engine
+  if (host_static) return;   // stream and host staging belong to the caller / the engine
+  if (lane_arena.base) {
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(lane_arena.base);
+  }
   for (void *b : setup_blocks) (void)hipFreeAsync(b, stream);
   if (stream) (void)hipStreamSynchronize(stream);
   if (h_gpin) (void)hipHostFree(h_gpin);
@@ -2122,6 +2168,7 @@ int Engine::ensure_all(const std::vector<std::pair<DevBuf *, size_t>> &req) {
     return PSC_OK;
   }
   pool_setup();
+  GateGuard gate;
   void *blk = nullptr;
   hipError_t e = hipMallocAsync(&blk, total, stream);
   if (e != hipSuccess) {
@@ -2129,6 +2176,7 @@ int Engine::ensure_all(const std::vector<std::pair<DevBuf *, size_t>> &req) {
     return PSC_ERR_NOMEM;
   }
   setup_blocks.push_back(blk);
+  setup_bytes += total;
   char *at = static_cast<char *>(blk);
   for (const auto &r : req)
     if (r.second > r.first->cap) {
@@ -2164,11 +2212,12 @@ int Engine::download_labels(uint64_t *dst) {
 
 int Engine::setup() {
   if (ready) return PSC_OK;
+  GateGuard gate;
   PSC_HIP(hipGetDevice(&device));
   hipDeviceProp_t prop;
   PSC_HIP(hipGetDeviceProperties(&prop, device));
   num_cu = prop.multiProcessorCount;
-  if (arena) {
+  if (host_static) {
     // host staging in the engine itself: polls and descriptors [0, 4096),
     // per-graph regions [4096, 7168), remainder descriptors [7168, 8192)
     h_pinned = host_stage;
@@ -2205,6 +2254,45 @@ int Engine::setup() {
   return PSC_OK;
 }
 
+// The device buffers a binding of n vertices, m edges and G graphs needs
+// before its first run (the run itself sizes the round buffers: pivot
+// bitmap, tables, logs, remainder copy).
+std::vector<std::pair<DevBuf *, size_t>> Engine::init_requests(uint64_t n_, uint64_t m_,
+                                                               uint64_t G_) {
+  const uint64_t nn = std::max<uint64_t>(n_, 1);
+  const uint64_t words = (nn + 31) / 32;
+  std::vector<std::pair<DevBuf *, size_t>> req;
+  req.emplace_back(&labels, nn * 8);
+  const uint64_t hcap = std::min<uint64_t>(nn, m_ / std::min(HEAVY_DEG, PULL_HEAVY) + 1);
+  for (int s = 0; s < 2; ++s) {
+    req.emplace_back(&vis[s], words * 4);
+    req.emplace_back(&rq[s][0], nn * 4);
+    req.emplace_back(&rq[s][1], nn * 4);
+    req.emplace_back(&hv[s], hcap * 4);
+    req.emplace_back(&hc[s], hcap * 8);
+    req.emplace_back(&rhead[s], nn * 4);
+  }
+  req.emplace_back(&centers, nn * 4);
+  req.emplace_back(&front, nn * 4);
+  const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
+  req.emplace_back(&tiles, ntiles * 4);
+  req.emplace_back(&trim_bm, words * 4);
+  req.emplace_back(&tile_off, ntiles * 8);
+  req.emplace_back(&trim_off, ntiles * 8);
+  req.emplace_back(&reach_desc, sizeof(ReachSearch) * 2);
+  req.emplace_back(&reach_ctl, sizeof(ReachCtl));
+  req.emplace_back(&ms_desc, sizeof(MSSearch) * 2);
+  req.emplace_back(&ms_ctl, sizeof(MSCtl));
+  req.emplace_back(&misc, 256);
+  req.emplace_back(&scratch32, (nn + G_) * 4);
+  for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) req.emplace_back(b, (G_ + 1) * 8);
+  req.emplace_back(&gA, 2 * (G_ + 1) * 8);   // nstart, then zstart
+  req.emplace_back(&gid_d, (G_ + 1) * 4);
+  req.emplace_back(&first_trim, (G_ + 1) * 4);
+  req.emplace_back(&hubs, 2 * HUB_WORDS * 4);
+  return req;
+}
+
 int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
                  const uint64_t *voff) {
   n = dg->n;
@@ -2229,46 +2317,11 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
 
   const uint64_t nn = std::max<uint64_t>(n, 1);
   const uint64_t words = (nn + 31) / 32;
-  if (arena) {
-    // every buffer is carved again for this graph; the labels are the
-    // caller's output array
-    arena->used = 0;
-    for (DevBuf *b : all_bufs()) b->ptr = nullptr, b->cap = 0;
+  if (arena && labels_ext) {   // the caller's output array holds the labels
     labels.ptr = labels_ext;
-    labels.cap = labels_ext ? nn * 8 : 0;
+    labels.cap = nn * 8;
   }
-  {
-    std::vector<std::pair<DevBuf *, size_t>> req;
-    req.emplace_back(&labels, nn * 8);
-    const uint64_t hcap = std::min<uint64_t>(nn, m / std::min(HEAVY_DEG, PULL_HEAVY) + 1);
-    for (int s = 0; s < 2; ++s) {
-      req.emplace_back(&vis[s], words * 4);
-      req.emplace_back(&rq[s][0], nn * 4);
-      req.emplace_back(&rq[s][1], nn * 4);
-      req.emplace_back(&hv[s], hcap * 4);
-      req.emplace_back(&hc[s], hcap * 8);
-      req.emplace_back(&rhead[s], nn * 4);
-    }
-    req.emplace_back(&centers, nn * 4);
-    req.emplace_back(&front, nn * 4);
-    const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
-    req.emplace_back(&tiles, ntiles * 4);
-    req.emplace_back(&trim_bm, words * 4);
-    req.emplace_back(&tile_off, ntiles * 8);
-    req.emplace_back(&trim_off, ntiles * 8);
-    req.emplace_back(&reach_desc, sizeof(ReachSearch) * 2);
-    req.emplace_back(&reach_ctl, sizeof(ReachCtl));
-    req.emplace_back(&ms_desc, sizeof(MSSearch) * 2);
-    req.emplace_back(&ms_ctl, sizeof(MSCtl));
-    req.emplace_back(&misc, 256);
-    req.emplace_back(&scratch32, (nn + G) * 4);
-    for (DevBuf *b : {&voff_d, &gB, &gC, &gD, &rr_plab}) req.emplace_back(b, (G + 1) * 8);
-    req.emplace_back(&gA, 2 * (G + 1) * 8);   // nstart, then zstart
-    req.emplace_back(&gid_d, (G + 1) * 4);
-    req.emplace_back(&first_trim, (G + 1) * 4);
-    req.emplace_back(&hubs, 2 * HUB_WORDS * 4);
-    PSC_TRY(ensure_all(req));
-  }
+  PSC_TRY(ensure_all(init_requests(n, m, G)));
   PSC_TRY(gpin(G + 2));
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
   // per-graph state read by every pull level: the first neighbour of each
@@ -2372,15 +2425,30 @@ int Engine::scan_tiles(uint64_t nt, unsigned long long *total, unsigned long lon
 // Region 0 serves polls; each uploaded array of a round has its own region.
 // Every round polls the device after its uploads, so a region is never
 // rewritten while its previous copy is still pending.
+// Empty the arena for a new binding.  The fixed-size control blocks are
+// carved first, so they keep their addresses from binding to binding (the
+// keys of captured level batches hold them).
+void Engine::arena_reset() {
+  arena->used = 0;
+  for (DevBuf *b : all_bufs()) b->ptr = nullptr, b->cap = 0, b->carved = false;
+  for (DevBuf *b : {&g_off, &g_adj, &g_ioff, &g_iadj}) b->ptr = nullptr, b->cap = 0;
+  (void)reach_desc.ensure(sizeof(ReachSearch) * 2);
+  (void)reach_ctl.ensure(sizeof(ReachCtl));
+  (void)ms_desc.ensure(sizeof(MSSearch) * 2);
+  (void)ms_ctl.ensure(sizeof(MSCtl));
+  (void)misc.ensure(256);
+}
+
 int Engine::gpin(size_t count) {
   const size_t want = count * NREG;
   if (want <= h_gpin_cap) return PSC_OK;
-  if (arena) {
+  if (host_static) {
     set_error("allocation-free run: per-graph staging holds " +
               std::to_string(h_gpin_cap / NREG) + " entries, " + std::to_string(count) +
               " needed");
     return PSC_ERR_INVALID;
   }
+  GateGuard gate;
   PSC_HIP(hipStreamSynchronize(stream));
   if (h_gpin) (void)hipHostFree(h_gpin);
   h_gpin = nullptr;
@@ -2442,6 +2510,7 @@ int Engine::replay(const BatchKey &k, F enqueue) {
       break;
     }
   if (!ex) {
+    CaptureGuard capture;   // no allocation or free in any thread meanwhile
     hipGraph_t graph = nullptr;
     PSC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
     enqueue();
@@ -3212,6 +3281,7 @@ int Engine::prof_reset(int level) {
 void Engine::prof_begin(int kind) {
   if (prof_level < 2) return;
   if (pev_used + 2 > pev.size()) {
+    GateGuard gate;
     for (int i = 0; i < 256; ++i) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;
@@ -3291,7 +3361,7 @@ void psc_params_default(psc_params *p) {
   p->timing = 1;
   p->tail_max = 65536;
   p->pivot_reach = 1;
-  p->capture = 0;
+  p->capture = 1;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }
@@ -3314,6 +3384,7 @@ static int validate_host_graph(const psc_graph *g) {
 }
 
 int psc_create(const psc_graph *g, psc_handle **out) {
+  psc::GateGuard gate;
   if (!out) return PSC_ERR_INVALID;
   *out = nullptr;
   int r = validate_host_graph(g);
@@ -3374,6 +3445,7 @@ int psc_create(const psc_graph *g, psc_handle **out) {
 }
 
 int psc_create_device(const psc_graph *dg, psc_handle **out) {
+  psc::GateGuard gate;
   if (!out) return PSC_ERR_INVALID;
   *out = nullptr;
   int r = validate_host_graph(dg);
@@ -3441,9 +3513,13 @@ int psc_kernel_profile(psc_handle *h, psc_kernel_prof *out, int cap) {
   return k;
 }
 
-void psc_destroy(psc_handle *h) { delete h; }
+void psc_destroy(psc_handle *h) {
+  psc::GateGuard gate;
+  delete h;
+}
 
 int psc_device_alloc(size_t bytes, void **ptr) {
+  psc::GateGuard gate;
   if (!ptr) return PSC_ERR_INVALID;
   *ptr = nullptr;
   hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
@@ -3455,6 +3531,7 @@ int psc_device_alloc(size_t bytes, void **ptr) {
 }
 
 int psc_device_free(void *ptr) {
+  psc::GateGuard gate;
   hipError_t e = hipFree(ptr);
   if (e != hipSuccess) {
     psc::set_error(std::string("hipFree: ") + hipGetErrorString(e));
@@ -3470,6 +3547,7 @@ uint64_t psc_pivot_perm(uint64_t n, uint64_t seed, uint64_t i, int inverse) {
 }
 
 int psc_host_alloc(size_t bytes, void **ptr) {
+  psc::GateGuard gate;
   if (!ptr) return PSC_ERR_INVALID;
   *ptr = nullptr;
   hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1);
@@ -3482,6 +3560,7 @@ int psc_host_alloc(size_t bytes, void **ptr) {
 }
 
 int psc_host_free(void *ptr) {
+  psc::GateGuard gate;
   hipError_t e = hipHostFree(ptr);
   if (e != hipSuccess) {
     psc::set_error(std::string("hipHostFree: ") + hipGetErrorString(e));
@@ -3491,6 +3570,7 @@ int psc_host_free(void *ptr) {
 }
 
 int psc_device_copy(void *dst, const void *src, size_t bytes) {
+  psc::GateGuard gate;
   hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
   if (e != hipSuccess) {
     psc::set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));

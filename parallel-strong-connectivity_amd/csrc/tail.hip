@@ -657,10 +657,11 @@ int blocks(uint64_t n, int per = 256, int cap = 16384) {
 
 int Engine::tail_pin_ensure(size_t bytes) {
   if (bytes <= tail_pin_cap) return PSC_OK;
-  if (arena) {
+  if (host_static) {
     set_error("allocation-free run: remainder descriptors exceed the staging area");
     return PSC_ERR_INVALID;
   }
+  GateGuard gate;
   PSC_HIP(hipStreamSynchronize(stream));
   if (tail_pin) (void)hipHostFree(tail_pin);
   tail_pin = nullptr;

@@ -82,19 +82,19 @@ def test_batch_2048_graphs(psc, oracle):
     db.free()
 
 
-@pytest.mark.skip(reason="the serving budget is an estimate: this run exhausted device "
-                         "memory under it; re-enabled with exact device accounting")
 def test_serve_10000_requests(psc, oracle):
     import psc_amd.serve as sv
     reqs = sv.poisson_requests(psc, 10000, 2000.0, lo_log2=10, hi_log2=24, seed=11)
     rng = np.random.default_rng(11)
     sample = set(rng.choice(len(reqs), 256, replace=False).tolist())
-    srv = sv.Server(psc, budget_bytes=200 << 30, lanes=4, mode="batched",
+    srv = sv.Server(psc, budget_bytes=200 * 10**9, lanes=4, mode="batched",
                     keep_labels=sample)
     summary = srv.serve(reqs)
     print(summary, flush=True)
     assert summary["requests"] == 10000
-    assert summary["peak_pool_GB"] <= summary["budget_GB"]
+    assert summary["peak_reserved_GB"] <= summary["budget_GB"]
+    assert (summary["peak_device_used_GB"] - summary["device_used_before_GB"]
+            <= summary["budget_GB"])
     assert summary["mean_batch"] > 1.0
     tarjan = sorted(sample)[::8]
     for r in reqs:

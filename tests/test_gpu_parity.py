@@ -35,9 +35,9 @@ PARAM_SETS = [
     {"tail_max": 0, "pivot_reach": 0},
     # every host round single-pivot (windows never grow): all bitmap reaches
     {"tail_max": 0, "beta": 1.0},
-    # level batches replayed from hipGraphs instead of launched one by one
-    {"capture": 1},
-    {"tail_max": 0, "capture": 1},
+    # level batches launched one by one instead of replayed from hipGraphs
+    {"capture": 0},
+    {"tail_max": 0, "capture": 0},
     # remainders of up to 100 vertices in one launch, larger ones on the host
     {"tail_max": 100},
 ]

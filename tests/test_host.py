@@ -203,12 +203,13 @@ def test_cli_bad_file_fails_cleanly(tmp_path):
 
 
 def test_serve_rejects_over_budget_without_gpu(psc):
-    """Admission control runs on the host: a request larger than the HBM
-    budget is refused before any device work."""
+    """Admission control runs on the host: a request whose binding does not
+    fit a lane's arena (sized by the library's own layout count) is refused
+    before any device work."""
     import psc_amd.serve as sv
     reqs = sv.poisson_requests(psc, 3, 0.0, lo_log2=12, hi_log2=13, seed=1)
-    srv = sv.Server(psc, budget_bytes=1 << 16, lanes=2)
-    with pytest.raises(RuntimeError, match="over the"):
+    srv = sv.Server(psc, budget_bytes=1 << 30, lanes=2, lane_bytes=1 << 20)
+    with pytest.raises(RuntimeError, match="does not fit a lane"):
         srv.serve(reqs)
 
 
