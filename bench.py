@@ -56,6 +56,8 @@ def bench_grid(args, psc):
     taus = [1 << k for k in range(18)] if args.taus is None else args.taus
     for tau in taus:
         prm = dict(beta=1.5, local_reach=True, local_scc=True, tau=tau, timing=1)
+        if args.deep_vgc:   # a wave may spend its whole tau budget serially
+            prm["vgc_batches"] = max(64, 4 * tau // 64)
         s.run(**prm)
         ts = []
         for _ in range(args.steps):
@@ -64,6 +66,7 @@ def bench_grid(args, psc):
             ts.append(time.perf_counter() - t)
         ms = float(np.median(ts)) * 1e3
         rows.append({"tau": tau, "ms": round(ms, 3),
+                     "vgc_batches": prm.get("vgc_batches", 64),
                      "gedges_s": round(dg.m / (ms * 1e-3) / 1e9, 4),
                      "levels": [st["levels_fwd"], st["levels_bwd"]],
                      "ms_first": round(st["ms_first"], 3),
@@ -170,8 +173,10 @@ def bench_batch(args, psc):
 
 def bench_serve(args, psc):
     """C5: Poisson SCC requests (config-4 graph mix, n log-uniform in
-    [2^10, 2^hi]) under an HBM budget, continuous batching vs one request
-    per launch; p50/p99 latency and graphs/s per arrival rate."""
+    [2^10, 2^hi]) under a device budget, continuous batching vs one request
+    per lane run; p50/p99 latency and graphs/s per arrival rate (0 = every
+    request at t = 0).  Each request's graph is generated on the device at
+    admission (its ingest), which is part of the served time."""
     import psc_amd.autotune as at
     import psc_amd.serve as sv
     table = at.load_table()
@@ -180,9 +185,9 @@ def bench_serve(args, psc):
         for mode in ("batched", "plain"):
             reqs = sv.poisson_requests(psc, args.requests, rate, 10, args.hi_log2,
                                        seed=11, table=table)
-            srv = sv.Server(psc, budget_bytes=int(args.budget_gb * 2**30),
+            srv = sv.Server(psc, budget_bytes=int(args.budget_gb * 1e9),
                             lanes=args.lanes, mode=mode,
-                            max_batch_vertices=1 << args.max_batch_log2)
+                            max_batch_vertices=1 << args.max_batch_log2, progress_s=20.0)
             summ = srv.serve(reqs)
             summ["rate"] = rate
             rows.append(summ)
@@ -191,8 +196,8 @@ def bench_serve(args, psc):
     plain = [r for r in rows if r["mode"] == "plain" and r["rate"] == best["rate"]][0]
     print(json.dumps({"metric": "SCC serving graphs/s and p50/p99 latency",
                       "value": best["graphs_per_s"], "unit": "graphs/s",
-                      "higher_is_better": True, "p50_ms": best["p50_ms"],
-                      "p99_ms": best["p99_ms"],
+                      "higher_is_better": True, "rate": best["rate"],
+                      "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
                       "plain_graphs_per_s": plain["graphs_per_s"],
                       "plain_p50_ms": plain["p50_ms"], "plain_p99_ms": plain["p99_ms"],
                       "requests": args.requests, "hi_log2": args.hi_log2,
@@ -335,13 +340,16 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=1024)
     ap.add_argument("--grid-side", type=int, default=2048)
     ap.add_argument("--taus", type=int, nargs="*", default=None)
+    ap.add_argument("--deep-vgc", action="store_true",
+                    help="grid sweep: let a wave's local search run its whole tau "
+                         "budget (vgc_batches = 4 tau / 64) instead of 64 batches")
     ap.add_argument("--graphs", type=int, default=2048)
     ap.add_argument("--hi-log2", type=float, default=None)
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--tarjan-checks", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--requests", type=int, default=1000)
-    ap.add_argument("--rates", type=float, nargs="*", default=[50.0, 200.0, 1000.0])
+    ap.add_argument("--requests", type=int, default=10000)
+    ap.add_argument("--rates", type=float, nargs="*", default=[100.0, 150.0, 200.0, 0.0])
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-batch-log2", type=int, default=27,
                     help="serving: vertex cap of one lane's batch")

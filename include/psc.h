@@ -87,6 +87,11 @@ typedef struct psc_params {
                            calls in every other thread for its
                            duration (the labels are the same either
                            way)                                       */
+  uint64_t vgc_batches; /* serial 64-vertex batches one wave's local
+                           (VGC) search may run per level launch
+                           before its new vertices go to the shared
+                           frontier; 0 = default (64).  Bounds how much
+                           of tau a lone wave spends serially         */
 } psc_params;
 
 /* Breakdown of one psc_run (src/scc.hpp:94-114 timers plus counters).

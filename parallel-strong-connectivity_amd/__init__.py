@@ -53,7 +53,8 @@ class CParams(C.Structure):
                 ("seed", C.c_uint64), ("table_init", C.c_uint64),
                 ("canonical", C.c_int32), ("timing", C.c_int32),
                 ("tail_max", C.c_uint64),
-                ("pivot_reach", C.c_int32), ("capture", C.c_int32)]
+                ("pivot_reach", C.c_int32), ("capture", C.c_int32),
+                ("vgc_batches", C.c_uint64)]
 
 
 class CStats(C.Structure):

@@ -153,6 +153,7 @@ struct Engine {
   DevBuf centers, front, tiles, tile_off;
   DevBuf trim_off;   // exclusive non-trimmed count per tile (kept past selects)
   DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
+  DevBuf trim_static;   // the same set, kept when single-pivot rounds reuse trim_bm
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
   DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta,
@@ -184,7 +185,7 @@ struct Engine {
   uint32_t round_levels_hint = 0; // levels of the previous single-pivot round
   uint64_t nnz_total = 0;         // non-trimmed vertices of the current run
   uint64_t syncs = 0;
-  unsigned char h_reach[256];
+  unsigned char h_reach[512];
   unsigned char h_ms[256];
   unsigned char h_ms_dev[256];   // descriptors last copied to the device
   bool ms_dev_valid = false;

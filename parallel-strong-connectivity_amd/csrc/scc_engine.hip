@@ -542,6 +542,16 @@ __device__ __forceinline__ int reach_mode(const ReachSlot *c,
   return (unv * PULL_VERTEX_COST + R.n / PULL_WALK_DIV < push) ? MODE_PULL : MODE_PUSH;
 }
 
+// Frontier entries a pull level stores.  Once a pull level has found
+// pull_store_cap vertices, the next level pulls whatever else it finds
+// (reach_mode: nf + nf * avg_deg >= thresh, with one to spare for the
+// rounding of the degree estimate), and a pull level never reads its
+// frontier: the entries past the cap are counted but not stored.
+__device__ __forceinline__ uint64_t pull_store_cap(const ReachSearch &R) {
+  const double t = ceil((double)R.thresh / (1.0 + R.avg_deg)) + 1.0;
+  return t < (double)R.n ? (uint64_t)t : (uint64_t)R.n;
+}
+
 // K counters summed over the block, one atomic per counter and block.
 // Every thread of the block must call it (it synchronises the block).
 template <int K>
@@ -672,7 +682,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
         while (head < tail) {
           // a wave's serial batches per launch are capped: past the cap its
           // new vertices go to the next frontier, where every wave shares them
-          if (++nbatch > VGC_MAX_BATCHES) budget = 0;
+          if (++nbatch > (uint32_t)vgc) budget = 0;
           const uint32_t k = min(64u, tail - head);
           const bool valid = (uint32_t)l < k;
           const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
@@ -798,6 +808,8 @@ struct HeadDir {
 };
 struct HeadDirs {
   HeadDir d[2];
+  const uint32_t *trim;                // trimmed vertices
+  const unsigned long long *nzoff;     // non-trimmed vertices before each tile
 };
 
 __global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
@@ -981,7 +993,18 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
         if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
     }
     wave_sync();
-    if (v < n) head[v] = d ? (uint32_t)best[l] : (uint32_t)v;
+    // the non-trimmed rank of v: the tile's prefix, the clear bits of the
+    // tile's words before this wave's two, then the lanes below in them
+    const uint64_t tile = v0 / SEL_TILE, w0 = v0 >> 5, tw0 = tile * (SEL_TILE / 32);
+    const uint64_t wl = tw0 + l;
+    const uint32_t before = wl < w0 ? (uint32_t)__popc(~hd.trim[wl]) : 0u;
+    const uint32_t tw_a = hd.trim[w0], tw_b = w0 + 1 < (n + 31) / 32 ? hd.trim[w0 + 1] : ~0u;
+    const uint32_t mine = l < 32 ? tw_a : tw_b;
+    const uint32_t below = l < 32 ? (uint32_t)__popc(~tw_a & ((1u << l) - 1u))
+                                  : (uint32_t)__popc(~tw_a) +
+                                        (uint32_t)__popc(~tw_b & ((1u << (l - 32)) - 1u));
+    const uint64_t rank = hd.nzoff[tile] + wave_sum(before) + below;
+    if (v < n && !((mine >> (l & 31)) & 1u)) head[rank] = (uint32_t)best[l];
     wave_sync();
   }
 }
@@ -1007,7 +1030,10 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
 // longer than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 constexpr int PULL_PAIRS = 2;
 constexpr int PULL_CHUNKS = 8;  // chunks whose candidates one pass may hold
-constexpr int PULL_HEADS = 4;   // head tests per lane in flight
+#ifndef PSC_PULL_HEADS
+#define PSC_PULL_HEADS 4
+#endif
+constexpr int PULL_HEADS = PSC_PULL_HEADS;   // head tests per lane in flight
 constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
 
 struct PullQueue {   // per-wave LDS queue of undecided vertices
@@ -1024,7 +1050,7 @@ template <class Out>
 __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
                                            uint32_t *sc, uint64_t *st, uint32_t *fl,
                                            Out &out, uint32_t *qsize, uint32_t *qn,
-                                           unsigned long long &n_edges,
+                                           uint64_t qcap, unsigned long long &n_edges,
                                            unsigned long long &n_vis) {
   const int l = lane();
   const uint32_t k = min(q.n, (uint32_t)WAVE);
@@ -1072,7 +1098,7 @@ __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
   }
   q.n = rest;
   wave_sync();
-  out.put(found, v, qsize, qn, R.n);
+  out.put(found, v, qsize, qn, qcap);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reach_pull(
@@ -1088,12 +1114,20 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   __shared__ uint16_t cand_s[WPB][WAVE * 32];
   __shared__ uint32_t fw_s[WPB][PULL_CHUNKS * WAVE];   // found bits per listed chunk
   __shared__ uint64_t ck_s[WPB][PULL_CHUNKS];          // the listed chunks
+  // per listed chunk: its trimmed words, the non-trimmed vertices before
+  // each word inside the chunk, and before the chunk (head ranks)
+  __shared__ uint32_t tw_s[WPB][PULL_CHUNKS * WAVE];
+  __shared__ uint16_t np_s[WPB][PULL_CHUNKS * WAVE];
+  __shared__ unsigned long long nb_s[WPB][PULL_CHUNKS];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *sc = sc_s[w], *fl = fl_s[w];
   uint64_t *st = st_s[w];
   uint16_t *cand = cand_s[w];
   uint32_t *fw = fw_s[w];
   uint64_t *ck = ck_s[w];
+  uint32_t *tw = tw_s[w];
+  uint16_t *np = np_s[w];
+  unsigned long long *nb = nb_s[w];
   const int ci = L % 3, ni = (L + 1) % 3;
   const int p = L & 1;
   int s_lo, s_hi;
@@ -1109,6 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     if (reach_mode(cur, R) != MODE_PULL) continue;
     if (gw == 0 && l == 0) nxt->from_pull = 1;   // first wave of this search
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
+    const uint64_t qcap = pull_store_cap(R);
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
@@ -1139,8 +1174,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           const uint32_t idx = cb + j * WAVE + l;
           todo[j] = idx < ncand;
           e[j] = todo[j] ? cand[idx] : 0u;
-          vx[j] = ck[e[j] >> 11] * (uint64_t)(WAVE * 32) + (e[j] & 2047u);
-          hd[j] = todo[j] ? R.rhead[vx[j]] : 0u;
+          const uint32_t sl = e[j] >> 11, wo = (e[j] & 2047u) >> 5;
+          vx[j] = ck[sl] * (uint64_t)(WAVE * 32) + (e[j] & 2047u);
+          const uint64_t rank = nb[sl] + np[sl * WAVE + wo] +
+                                (uint32_t)__popc(~tw[sl * WAVE + wo] & ((1u << (e[j] & 31)) - 1u));
+          hd[j] = todo[j] ? R.rhead[rank] : 0u;
         }
 #pragma unroll
         for (int j = 0; j < PULL_HEADS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
@@ -1157,7 +1195,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           // found bits gather in the chunks' LDS words
           if (hit) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += hit;
-          out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, R.n);
+          out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, qcap);
         }
       }
       wave_sync();
@@ -1225,11 +1263,11 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           q.n += (uint32_t)__popcll(qm);
           if (found) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += found;
-          out.put(found, (uint32_t)v[j], &nxt->qsize, qn, R.n);
+          out.put(found, (uint32_t)v[j], &nxt->qsize, qn, qcap);
         }
         wave_sync();
         while (q.n >= WAVE)
-          pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
+          pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
       }
       wave_sync();
       for (uint32_t k = 0; k < nck; ++k) {
@@ -1243,19 +1281,26 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     // (visited | trimmed) word of this lane in chunk c; the next chunk's is
     // loaded before a chunk is listed (in a pull level a vertex's bit is
     // set only by its own processing, so an early load reads the same word)
-    auto skip_word = [&](uint64_t c) -> uint32_t {
+    // (the trimmed word comes with it: the skip set of a first-reach level,
+    // a separate load in a single-pivot round)
+    const bool own_trim = R.tstat == R.trim;
+    auto skip_word = [&](uint64_t c, uint32_t *ts) -> uint32_t {
       const uint64_t wi = c * WAVE + l;
       uint32_t sk = ~0u;
+      *ts = ~0u;
       if (c < chunks && wi < words) {
-        sk = R.vis[wi] | R.trim[wi];
+        const uint32_t tr = R.trim[wi];
+        *ts = own_trim ? tr : R.tstat[wi];
+        sk = R.vis[wi] | tr;
         if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
       }
       return sk;
     };
-    uint32_t sk_next = skip_word(gw);
+    uint32_t ts_next;
+    uint32_t sk_next = skip_word(gw, &ts_next);
     for (uint64_t c = gw; c < chunks; c += nw) {
-      const uint32_t sk = sk_next;
-      sk_next = skip_word(c + nw);
+      const uint32_t sk = sk_next, ts = ts_next;
+      sk_next = skip_word(c + nw, &ts_next);
       ++n_chunks;
       // the chunk's candidates (clear bits), appended to the list in order;
       // chunks with few candidates share one pass of both phases
@@ -1272,15 +1317,23 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           cand[at++] = (uint16_t)((nck << 11) | (l * 32 + (__ffs(tt) - 1)));
       }
       fw[nck * WAVE + l] = 0;
-      if (l == 0) ck[nck] = c;
+      tw[nck * WAVE + l] = ts;
+      {
+        const uint32_t nz = (uint32_t)__popc(~ts);
+        np[nck * WAVE + l] = (uint16_t)(wave_incl_scan(nz) - nz);
+      }
+      if (l == 0) {
+        ck[nck] = c;
+        nb[nck] = R.nzoff[c];
+      }
       ncand += total;
       ++nck;
       wave_sync();
       if (ncand >= WAVE * PULL_HEADS) process();
     }
     if (ncand) process();
-    while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, n_edges, n_vis);
-    out.flush(&nxt->qsize, qn, R.n);
+    while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
+    out.flush(&nxt->qsize, qn, qcap);
     unsigned long long *const cp[5] = {&C->cnt[s][R_PULL_EDGES], &C->cnt[s][R_PULL_SCANS],
                                        &C->cnt[s][R_PULL_VIS], &nxt->cum_vis,
                                        &C->cnt[s][R_PULL_UNITS]};
@@ -1334,6 +1387,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
     const ReachSearch R = S[s];
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
     const bool pull = reach_mode(cur, R) == MODE_PULL;
+    const uint64_t qcap = pull ? pull_store_cap(R) : R.n;
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
     // a pull remainder starts after the PULL_PROBE probed edges
@@ -1395,10 +1449,10 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
             ++n_app;
           }
         }
-        out.put(mine, u, &nxt->qsize, qn, R.n);
+        out.put(mine, u, &nxt->qsize, qn, qcap);
       }
     }
-    out.flush(&nxt->qsize, qn, R.n);
+    out.flush(&nxt->qsize, qn, qcap);
     unsigned long long *const cp[6] = {&nxt->degsum, &C->cnt[s][R_HEAVY_EDGES],
                                        &C->cnt[s][R_HEAVY_VIS], &nxt->cum_vis,
                                        &C->cnt[s][R_HEAVY_APP], &C->cnt[s][R_HEAVY_ITEMS]};
@@ -1648,7 +1702,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
       uint64_t ebudget = budget * VGC_EDGE_MULT;
       wave_sync();
       while (head < tail) {
-        if (++nbatch > VGC_MAX_BATCHES) budget = 0;   // as in k_reach_step
+        if (++nbatch > (uint32_t)vgc) budget = 0;   // as in k_reach_step
         const uint32_t k = min(64u, tail - head);
         const bool valid = (uint32_t)l < k;
         const unsigned long long pr = valid ? ring[(head + l) & (MS_RING - 1)] : 0ull;
@@ -2196,7 +2250,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &hubs, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &hubs, &trim_static, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta, &tail_spec, &tail_trace};
 }
 
@@ -2277,6 +2331,7 @@ std::vector<std::pair<DevBuf *, size_t>> Engine::init_requests(uint64_t n_, uint
   const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
   req.emplace_back(&tiles, ntiles * 4);
   req.emplace_back(&trim_bm, words * 4);
+  req.emplace_back(&trim_static, words * 4);
   req.emplace_back(&tile_off, ntiles * 8);
   req.emplace_back(&trim_off, ntiles * 8);
   req.emplace_back(&reach_desc, sizeof(ReachSearch) * 2);
@@ -2343,6 +2398,18 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
       hd.d[s].hub = hubs.as<uint32_t>() + s * HUB_WORDS;
       hd.d[s].head = rhead[s].as<uint32_t>();
     }
+    hd.trim = trim_static.as<uint32_t>();
+    hd.nzoff = trim_off.as<unsigned long long>();
+    // the trimmed vertices (zero in- or out-degree) and the non-trimmed
+    // count before each tile: heads are stored by non-trimmed rank, so the
+    // pull levels read them densely (the run recomputes the same prefix)
+    const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
+    if (n) {
+      k_trim_count<<<(unsigned)ntiles, BLOCK, 0, stream>>>(
+          d_off, d_ioff, n, words, tiles.as<uint32_t>(), vis[0].as<uint32_t>(),
+          vis[1].as<uint32_t>(), trim_static.as<uint32_t>(), misc.as<unsigned long long>() + 2);
+      PSC_TRY(scan_tiles(ntiles, misc.as<unsigned long long>(), trim_off.as<unsigned long long>()));
+    }
     const unsigned blocks = (unsigned)std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 2048);
     const unsigned hblocks = (unsigned)std::max<uint64_t>(
         1, std::min<uint64_t>((nn + HEAD_BLOCK - 1) / HEAD_BLOCK, (uint64_t)num_cu));
@@ -2364,6 +2431,8 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     rs[s].rhead = rhead[s].as<uint32_t>();
     rs[s].vis = vis[s].as<uint32_t>();
     rs[s].trim = trim_bm.as<uint32_t>();
+    rs[s].tstat = trim_static.as<uint32_t>();
+    rs[s].nzoff = trim_off.as<unsigned long long>();
     rs[s].q[0] = rq[s][0].as<uint32_t>();
     rs[s].q[1] = rq[s][1].as<uint32_t>();
     rs[s].hv = hv[s].as<uint32_t>();
@@ -2554,6 +2623,13 @@ int Engine::poll(void *dst, const void *src, size_t bytes) {
   return PSC_OK;
 }
 
+// Serial 64-vertex batches a wave may run in one launch of a VGC local
+// search (psc_params.vgc_batches, default VGC_MAX_BATCHES).
+static int vgc_batch_cap(const psc_params &p) {
+  const uint64_t b = p.vgc_batches ? p.vgc_batches : VGC_MAX_BATCHES;
+  return (int)std::min<uint64_t>(b, 1u << 30);
+}
+
 // First round: fused forward + backward reach from P[0] (src/scc.hpp:316-339).
 // Levels are enqueued in batches of K without host round trips; kernels of
 // levels past the end exit at once.  The host reads the done flags after
@@ -2571,6 +2647,9 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     rs[s].avg_deg = nnz_total ? (double)m / (double)nnz_total : 0.0;
     rs[s].nnz = nnz_total;   // a single-pivot round counts it on the device
     rs[s].excl = rr ? trim_bm.as<uint32_t>() : nullptr;
+    // a single-pivot round overwrites trim_bm with its exclusions; head
+    // ranks then come from the kept copy of the trimmed set
+    rs[s].tstat = rr ? trim_static.as<uint32_t>() : trim_bm.as<uint32_t>();
   }
   char *pin = (char *)h_pinned;
   std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
@@ -2614,6 +2693,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
   prof_end();
   const uint32_t total_waves = (uint32_t)grid_step * WPB;
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
+  // the kernels' VGC argument: 0 = off, else a wave's serial batch cap
+  const int vgc_reach = p.local_reach ? vgc_batch_cap(p) : 0;
   uint32_t L = 0;
   // the previous run's level count (same graph: one poll, no idle levels;
   // the poll itself shows that the level after the batch has no frontier).
@@ -2633,8 +2714,8 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
       for (int k = 0; k < K; ++k) {
         const uint32_t Lk = L0 + (uint32_t)k;
         prof_begin(KK_STEP);
-        k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, Lk, tau,
-                                                        p.local_reach, total_waves);
+        k_reach_step<<<grid_step, BLOCK, 0, stream>>>(S, C, 2, Lk, tau, vgc_reach,
+                                                        total_waves);
         prof_end();
         prof_begin(KK_PULL);
         k_reach_pull<<<grid_pull, BLOCK, 0, stream>>>(S, C, 2, Lk,
@@ -2647,7 +2728,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
       }
     };
     if (p.capture && !trace && prof_level < 2) {
-      const BatchKey key{0, L, (uint32_t)K, tau, total_waves, p.local_reach, grid_pull,
+      const BatchKey key{0, L, (uint32_t)K, tau, total_waves, vgc_reach, grid_pull,
                          grid_heavy, S, C, nullptr};
       PSC_TRY(replay(key, [&] { levels(key.L0); }));
     } else {
@@ -2805,6 +2886,7 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
   MSCtl *C = ms_ctl.as<MSCtl>();
   int want[2] = {1, 1};
   const uint32_t tau = (uint32_t)std::min<uint64_t>(p.tau ? p.tau : 1, 1u << 30);
+  const int vgc_ms = p.local_scc ? vgc_batch_cap(p) : 0;
   const uint32_t total_waves = (uint32_t)grid_ms * WPB;
   int attempt = 0;
   while (want[0] || want[1]) {
@@ -2849,8 +2931,7 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
           const uint32_t Lk = L0 + (uint32_t)k;
           prof_begin(KK_MS_EXPAND);
           k_ms_expand<<<grid_ms, BLOCK, 0, stream>>>(
-              S, C, 2, Lk, labels.as<unsigned long long>(), tau, p.local_scc,
-              total_waves);
+              S, C, 2, Lk, labels.as<unsigned long long>(), tau, vgc_ms, total_waves);
           prof_end();
           prof_begin(KK_MS_HEAVY);
           k_ms_heavy<<<grid_heavy, BLOCK, 0, stream>>>(
@@ -2860,7 +2941,7 @@ int Engine::multi_search(const psc_params &p, uint64_t nfront, uint32_t A,
         }
       };
       if (p.capture && prof_level < 2) {
-        const BatchKey key{1, L, (uint32_t)K, tau, total_waves, p.local_scc, grid_heavy,
+        const BatchKey key{1, L, (uint32_t)K, tau, total_waves, vgc_ms, grid_heavy,
                            0, S, C, labels.ptr};
         PSC_TRY(replay(key, [&] { levels(key.L0); }));
       } else {
@@ -3362,6 +3443,7 @@ void psc_params_default(psc_params *p) {
   p->tail_max = 65536;
   p->pivot_reach = 1;
   p->capture = 1;
+  p->vgc_batches = 0;
 }
 
 const char *psc_last_error(void) { return psc::g_last_error.c_str(); }

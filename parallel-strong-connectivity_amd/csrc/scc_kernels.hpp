@@ -49,9 +49,14 @@ struct ReachSearch {
   const uint32_t *adj;
   const uint64_t *roff;  // pull direction CSR (transpose)
   const uint32_t *radj;
-  const uint32_t *rhead; // radj[roff[v]] per vertex (v itself if no in-edge)
+  const uint32_t *rhead; // pull head of each non-trimmed vertex, by rank
   uint32_t *vis;         // visited bitmap, ceil(n/32) words
-  const uint32_t *trim;  // trimmed-vertex bitmap (skipped by pull levels)
+  const uint32_t *trim;  // vertices a pull level skips: the trimmed ones, or
+                         // the excluded ones of a single-pivot round
+  const uint32_t *tstat; // trimmed vertices (zero in- or out-degree); the
+                         // rank of a non-trimmed v is nzoff[v / 2048] + its
+                         // clear bits before v in its 2048-vertex tile
+  const unsigned long long *nzoff;   // non-trimmed vertices before each tile
   const uint32_t *excl;  // vertices outside the searched subproblems (push
                          // skips them), or nullptr
   uint32_t *q[2];        // frontier ping-pong, n entries each

@@ -279,7 +279,10 @@ struct TailArgs {
 };
 constexpr uint32_t TAIL_TRACE = 64;
 
-// One workgroup per graph: all rounds of its remainder.
+// One workgroup per graph: all rounds of its remainder.  The host orders the
+// graphs of a launch by decreasing work, and workgroups are dispatched in
+// order as CUs free up, so the longest remainders start first and the short
+// ones fill in behind them.
 __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   __shared__ uint32_t wsum[TAIL_BLOCK / WAVE], wsum2[TAIL_BLOCK / WAVE];
   __shared__ unsigned int cnt_q[2], cnt_log[2], flag_full;
@@ -813,6 +816,11 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   std::vector<TailGraph> launch;
   std::vector<TailState> hs;
   while (!pending.empty()) {
+    // largest remainders first (positions left plus compact edges estimate
+    // a graph's rounds)
+    std::stable_sort(pending.begin(), pending.end(), [&](size_t x, size_t y) {
+      return tgs[x].nr - tgs[x].cur + h_e[tgs[x].g] > tgs[y].nr - tgs[y].cur + h_e[tgs[y].g];
+    });
     launch.clear();
     uint64_t slots = 0, lslots = 0;
     for (size_t i : pending) {

@@ -131,7 +131,7 @@ class Server:
     def __init__(self, psc, budget_bytes=200 << 30, lanes=4, mode="batched",
                  lane_bytes=None, max_batch_vertices=1 << 27, tau=512, dense_div=20,
                  keep_labels=False, target_batch=16, max_wait_s=0.002,
-                 max_admit=256, capture=1):
+                 max_admit=256, capture=1, progress_s=0.0):
         self.psc = psc
         self.max_admit = max_admit
         self.target_batch = target_batch
@@ -146,6 +146,7 @@ class Server:
         self.dense_div = dense_div
         self.capture = capture
         self.keep_labels = keep_labels
+        self.progress_s = progress_s   # > 0: a progress line on stderr this often
         self.cv = threading.Condition()
         self.ready = []          # (request, _Pool, index in the pool)
         self.reserved = 0        # bytes of the budget in use (exact)
@@ -449,6 +450,16 @@ class Server:
         adm.start()
         for th in lanes:
             th.start()
+        if self.progress_s > 0:   # a progress line every progress_s seconds
+            import sys
+            alive = [adm] + lanes
+            while alive:
+                deadline = time.perf_counter() + self.progress_s
+                for th in alive:
+                    th.join(timeout=max(0.0, deadline - time.perf_counter()))
+                alive = [th for th in alive if th.is_alive()]
+                print(f"serve[{self.mode}]: {self.finished}/{len(reqs)} done after "
+                      f"{time.perf_counter() - t0:.0f}s", file=sys.stderr, flush=True)
         adm.join()
         for th in lanes:
             th.join()
