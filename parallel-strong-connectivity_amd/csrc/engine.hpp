@@ -117,6 +117,10 @@ struct StreamKit {
   hipStream_t stream = nullptr;
   hipEvent_t ev[8] = {};
   void *pinned = nullptr;
+  void *gpin = nullptr;    // per-graph staging and its capacity (u64 entries)
+  size_t gpin_cap = 0;
+  void *tpin = nullptr;    // remainder descriptors and their capacity (bytes)
+  size_t tpin_cap = 0;
 };
 bool kit_put(const StreamKit &k);
 bool kit_take(int device, StreamKit *k);
@@ -152,8 +156,14 @@ struct Engine {
   DevBuf hubs;       // degree histogram and hub table of the head selection
   DevBuf centers, front, tiles, tile_off;
   DevBuf trim_off;   // exclusive non-trimmed count per tile (kept past selects)
-  DevBuf trim_bm;   // trimmed vertices (zero in- or out-degree), one bit each
-  DevBuf trim_static;   // the same set, kept when single-pivot rounds reuse trim_bm
+  DevBuf trim_bm;   // exclusions of a single-pivot round (per-run scratch)
+  DevBuf trim_static;   // trimmed vertices (zero in- or out-degree), one bit each
+  // dead ends of the two searches (no out-edge / no in-edge), two bitmaps:
+  // a run's visited bitmaps start as copies of them
+  DevBuf dead;
+  // per-graph non-trimmed / trimmed vertices before each graph (nstart,
+  // then zstart, G + 1 each), fixed by the binding like the bitmaps above
+  std::vector<uint64_t> starts_h;
   DevBuf pbits, sort_tmp;    // remaining vertices by pivot position; scan scratch
   // single-launch rounds of small remainders (tail.hip)
   DevBuf tail_cnt, tail_lab, tail_off, tail_adj, tail_table, tail_lists, tail_meta,

@@ -244,6 +244,20 @@ __global__ __launch_bounds__(BLOCK) void k_trim_count(
   }
 }
 
+// A run's first step: both visited bitmaps start as their search's dead-end
+// set (dead holds the forward one, then the backward one), and the run's
+// contains() probe counter starts at zero.
+__global__ void k_run_start(const uint32_t *__restrict__ dead, uint64_t words,
+                            uint32_t *vis_f, uint32_t *vis_b,
+                            unsigned long long *run_zero) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *run_zero = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    vis_f[i] = dead[i];
+    vis_b[i] = dead[words + i];
+  }
+}
+
 // Per-graph starts: nstart[g] = non-trimmed vertices before voff[g] (the
 // tile prefix plus a recount of the tile's head, one wave per graph) and
 // zstart[g] = voff[g] - nstart[g], for g in [0, G].
@@ -761,13 +775,24 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
 // reached early, so most pull candidates are decided by their head, and the
 // heads concentrate on few vertices whose visited words stay cached.  The
 // head is the neighbour of largest degree among the graph's top hubs (at
-// most HUB_MAX vertices of largest degree) found in the first HUB_SCAN
-// entries of the list, else the neighbour of largest degree among the first
+// most HUB_MAX vertices of largest degree) found in the first HUB_SCAN1
+// entries of the list; a list without a hub there takes the same maximum
+// over entries [HUB_SCAN1, HUB_SCAN); a list without a hub in its first
+// HUB_SCAN entries takes the neighbour of largest degree among its first
 // HUB_FALLBACK entries (earliest on ties; v itself for an empty list or a
 // trimmed v, whose head is never read).  Hub membership is an LDS hash
 // probe, so the only random global reads are the few fallback degrees.
+// On a skewed graph most lists meet a hub within HUB_SCAN1 entries, so the
+// second scan reads few lists.
+#ifndef PSC_HUB_SCAN1
+#define PSC_HUB_SCAN1 16
+#endif
+#ifndef PSC_HUB_FALLBACK
+#define PSC_HUB_FALLBACK 4
+#endif
+constexpr uint32_t HUB_SCAN1 = PSC_HUB_SCAN1;
 constexpr uint32_t HUB_SCAN = 128;
-constexpr uint32_t HUB_FALLBACK = 8;
+constexpr uint32_t HUB_FALLBACK = PSC_HUB_FALLBACK;
 constexpr uint32_t HUB_MAX = 2048;
 constexpr uint32_t HUB_SLOTS = 2 * HUB_MAX;   // open addressing, load <= 1/2
 constexpr uint32_t HUB_BINS = 256;            // log-scale degree histogram
@@ -922,54 +947,77 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   unsigned long long *best = best_s[w];
-  const uint64_t nwaves = (uint64_t)gridDim.x * HW;
-  for (uint64_t v0 = (blockIdx.x * (uint64_t)HW + w) * WAVE; v0 < n; v0 += nwaves * WAVE) {
+  // the pull list's length is the other search's own degree
+  const uint32_t *__restrict__ pdeg = hd.d[1 - blockIdx.y].deg;
+  const uint64_t stride = (uint64_t)gridDim.x * HW * WAVE;
+  // a wave's groups of 64 consecutive vertices follow each other with a
+  // stride; a group's list starts and degrees are loaded one group ahead
+  uint64_t v0 = (blockIdx.x * (uint64_t)HW + w) * WAVE;
+  uint64_t nb = 0;
+  uint32_t ndg = 0, npd = 0;
+  if (v0 + l < n) {
+    nb = off[v0 + l];
+    ndg = deg[v0 + l];
+    npd = pdeg[v0 + l];
+  }
+  for (; v0 < n; v0 += stride) {
     const uint64_t v = v0 + l;
-    uint64_t b = 0;
-    uint32_t d = 0;
-    if (v < n) {
-      b = off[v];
-      // a vertex without edges in the search's own direction, or without a
-      // pull list, is trimmed: it is never a pull candidate
-      d = deg[v] ? (uint32_t)min(off[v + 1] - b, (uint64_t)HUB_SCAN) : 0u;
+    const uint64_t b = nb;
+    // a vertex without edges in the search's own direction, or without a
+    // pull list, is trimmed: it is never a pull candidate
+    const uint32_t d = v < n && ndg ? min(npd, HUB_SCAN) : 0u;
+    if (v + stride < n) {
+      nb = off[v + stride];
+      ndg = deg[v + stride];
+      npd = pdeg[v + stride];
     }
-    uint32_t inc = wave_incl_scan(d);
-    uint32_t total = __shfl(inc, WAVE - 1);
-    sc[l] = inc;
-    st[l] = b;
     best[l] = 0;
-    wave_sync();
-    // hub hits among the first HUB_SCAN entries
-    for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
-      uint32_t u[HEAD_WIN], pos[HEAD_WIN];
-      int own[HEAD_WIN];
+    // hub hits among entries [p0, p0 + c) of each list (c per lane), as
+    // 64-entry windows over the concatenated lists (lane e finds its list
+    // by binary search over the wave's count scan), HEAD_WIN windows'
+    // loads in flight
+    auto scan_hubs = [&](uint32_t p0, uint32_t c) {
+      const uint32_t inc = wave_incl_scan(c);
+      const uint32_t total = __shfl(inc, WAVE - 1);
+      wave_sync();
+      sc[l] = inc;
+      st[l] = b + p0;
+      wave_sync();
+      for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
+        uint32_t u[HEAD_WIN], pos[HEAD_WIN];
+        int own[HEAD_WIN];
 #pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j) {
-        const uint32_t e = base + j * WAVE + l;
-        own[j] = -1;
-        u[j] = HUB_EMPTY;
-        pos[j] = 0;
-        if (e < total) {
-          own[j] = head_owner(sc, e);
-          pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
-          u[j] = adj[st[own[j]] + pos[j]];
+        for (int j = 0; j < HEAD_WIN; ++j) {
+          const uint32_t e = base + j * WAVE + l;
+          own[j] = -1;
+          u[j] = HUB_EMPTY;
+          pos[j] = 0;
+          if (e < total) {
+            own[j] = head_owner(sc, e);
+            pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
+            u[j] = adj[st[own[j]] + pos[j]];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < HEAD_WIN; ++j) {
+          if (own[j] < 0) continue;
+          uint32_t s = hub_hash(u[j]), key;
+          while ((key = keys[s]) != HUB_EMPTY && key != u[j]) s = (s + 1) & (HUB_SLOTS - 1);
+          if (key == u[j]) atomicMax(&best[own[j]], head_key(vals[s], p0 + pos[j], u[j]));
         }
       }
-#pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j) {
-        if (own[j] < 0) continue;
-        uint32_t s = hub_hash(u[j]), key;
-        while ((key = keys[s]) != HUB_EMPTY && key != u[j]) s = (s + 1) & (HUB_SLOTS - 1);
-        if (key == u[j]) atomicMax(&best[own[j]], head_key(vals[s], pos[j], u[j]));
-      }
-    }
-    wave_sync();
+      wave_sync();
+    };
+    scan_hubs(0, min(d, HUB_SCAN1));
+    const uint32_t d2 = best[l] || d <= HUB_SCAN1 ? 0u : d - HUB_SCAN1;
+    if (ballot(d2 > 0)) scan_hubs(HUB_SCAN1, d2);
     // fallback: vertices without a hub rank their first HUB_FALLBACK entries
     const uint32_t f = best[l] ? 0u : min(d, HUB_FALLBACK);
-    inc = wave_incl_scan(f);
-    total = __shfl(inc, WAVE - 1);
+    uint32_t inc = wave_incl_scan(f);
+    uint32_t total = __shfl(inc, WAVE - 1);
     wave_sync();
     sc[l] = inc;
+    st[l] = b;
     wave_sync();
     for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
       uint32_t u[HEAD_WIN], du[HEAD_WIN], pos[HEAD_WIN];
@@ -987,7 +1035,8 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
         }
       }
 #pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j) du[j] = own[j] >= 0 ? D.lg[u[j]] : 0u;
+      for (int j = 0; j < HEAD_WIN; ++j)
+        du[j] = own[j] >= 0 && HUB_FALLBACK > 1 ? D.lg[u[j]] : 0u;
 #pragma unroll
       for (int j = 0; j < HEAD_WIN; ++j)
         if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
@@ -2165,25 +2214,25 @@ Engine::~Engine() {
   }
   for (void *b : setup_blocks) (void)hipFreeAsync(b, stream);
   if (stream) (void)hipStreamSynchronize(stream);
-  if (h_gpin) (void)hipHostFree(h_gpin);
-  if (tail_pin) (void)hipHostFree(tail_pin);
   for (auto &e : pev) (void)hipEventDestroy(e);
-  if (!stream) return;
-  // the stream, its events and the small pinned buffer go back to the
+  // the stream, its events and the pinned staging buffers go back to the
   // process's cache for the next handle on this device
-  StreamKit k{device, stream, {}, h_pinned};
+  StreamKit k{device, stream, {}, h_pinned, h_gpin, h_gpin_cap, tail_pin, tail_pin_cap};
   for (int i = 0; i < 8; ++i) k.ev[i] = ev[i];
-  if (!kit_put(k)) {
+  if (!stream || !kit_put(k)) {
+    if (h_gpin) (void)hipHostFree(h_gpin);
+    if (tail_pin) (void)hipHostFree(tail_pin);
+    if (!stream) return;
     if (h_pinned) (void)hipHostFree(h_pinned);
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream);
   }
 }
 
-// Streams, their timing events and the small pinned staging buffer of
-// destroyed handles, kept for reuse: creating a stream costs the runtime
-// up to milliseconds, which a new handle (a serving lane's, a batch
-// group's) would otherwise pay on every graph.
+// Streams, their timing events and the pinned staging buffers of destroyed
+// handles, kept for reuse: creating a stream or pinning host memory costs
+// the runtime up to milliseconds, which a new handle (a serving lane's, a
+// batch group's) would otherwise pay on every graph.
 namespace {
 std::mutex g_kit_mu;
 std::vector<StreamKit> g_kits;
@@ -2250,7 +2299,7 @@ std::vector<DevBuf *> Engine::all_bufs() {
           &ms_ctl, &misc, &table[0], &table[1], &log[0], &log[1],
           &mq[0][0], &mq[0][1], &mq[1][0], &mq[1][1], &mhp[0], &mhp[1],
           &mhc[0], &mhc[1], &scratch32, &voff_d, &gA, &gB, &gC, &gD,
-          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &hubs, &trim_static, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
+          &gid_d, &rnd, &rr_plab, &first_trim, &rhead[0], &rhead[1], &hubs, &trim_static, &dead, &tail_cnt, &tail_lab, &tail_off, &tail_adj, &tail_table,
           &tail_lists, &tail_meta, &tail_spec, &tail_trace};
 }
 
@@ -2286,6 +2335,10 @@ int Engine::setup() {
       stream = k.stream;
       for (int i = 0; i < 8; ++i) ev[i] = k.ev[i];
       h_pinned = k.pinned;
+      h_gpin = static_cast<unsigned long long *>(k.gpin);
+      h_gpin_cap = k.gpin_cap;
+      tail_pin = k.tpin;
+      tail_pin_cap = k.tpin_cap;
     } else {
       PSC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       for (auto &e : ev) PSC_HIP(hipEventCreate(&e));
@@ -2332,6 +2385,7 @@ std::vector<std::pair<DevBuf *, size_t>> Engine::init_requests(uint64_t n_, uint
   req.emplace_back(&tiles, ntiles * 4);
   req.emplace_back(&trim_bm, words * 4);
   req.emplace_back(&trim_static, words * 4);
+  req.emplace_back(&dead, words * 8);
   req.emplace_back(&tile_off, ntiles * 8);
   req.emplace_back(&trim_off, ntiles * 8);
   req.emplace_back(&reach_desc, sizeof(ReachSearch) * 2);
@@ -2377,7 +2431,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     labels.cap = nn * 8;
   }
   PSC_TRY(ensure_all(init_requests(n, m, G)));
-  PSC_TRY(gpin(G + 2));
+  PSC_TRY(gpin(2 * (G + 1)));   // per-graph starts (init), windows (runs)
   PSC_TRY(upload(voff_d, voff_h.data(), G + 1, 1));
   // per-graph state read by every pull level: the first neighbour of each
   // vertex in the pull direction, gathered once here so that a pull probe of
@@ -2400,15 +2454,25 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     }
     hd.trim = trim_static.as<uint32_t>();
     hd.nzoff = trim_off.as<unsigned long long>();
-    // the trimmed vertices (zero in- or out-degree) and the non-trimmed
-    // count before each tile: heads are stored by non-trimmed rank, so the
-    // pull levels read them densely (the run recomputes the same prefix)
+    // Trimming (src/scc.hpp:295-306) depends on the graph alone, so the
+    // binding does it once: the trimmed vertices (zero in- or out-degree),
+    // the dead ends of each search, the non-trimmed count before each tile
+    // and before each graph.  Heads are stored by non-trimmed rank, so the
+    // pull levels read them densely.
     const uint64_t ntiles = (nn + SEL_TILE - 1) / SEL_TILE;
+    unsigned long long *nstart = gA.as<unsigned long long>();
+    starts_h.assign(2 * (G + 1), 0);
     if (n) {
+      unsigned long long *tot = misc.as<unsigned long long>();
       k_trim_count<<<(unsigned)ntiles, BLOCK, 0, stream>>>(
-          d_off, d_ioff, n, words, tiles.as<uint32_t>(), vis[0].as<uint32_t>(),
-          vis[1].as<uint32_t>(), trim_static.as<uint32_t>(), misc.as<unsigned long long>() + 2);
-      PSC_TRY(scan_tiles(ntiles, misc.as<unsigned long long>(), trim_off.as<unsigned long long>()));
+          d_off, d_ioff, n, words, tiles.as<uint32_t>(), dead.as<uint32_t>(),
+          dead.as<uint32_t>() + words, trim_static.as<uint32_t>(), tot + 2);
+      PSC_TRY(scan_tiles(ntiles, tot, trim_off.as<unsigned long long>()));
+      k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
+          d_off, d_ioff, n, voff_d.as<unsigned long long>(), G,
+          trim_off.as<unsigned long long>(), tot, nstart, nstart + (G + 1));
+      PSC_TRY(gpin(2 * (G + 1)));
+      PSC_HIP(hipMemcpyAsync(region(2), nstart, 16 * (G + 1), hipMemcpyDeviceToHost, stream));
     }
     const unsigned blocks = (unsigned)std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 2048);
     const unsigned hblocks = (unsigned)std::max<uint64_t>(
@@ -2430,7 +2494,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     rs[s].radj = s == 0 ? d_iadj : d_adj;
     rs[s].rhead = rhead[s].as<uint32_t>();
     rs[s].vis = vis[s].as<uint32_t>();
-    rs[s].trim = trim_bm.as<uint32_t>();
+    rs[s].trim = trim_static.as<uint32_t>();
     rs[s].tstat = trim_static.as<uint32_t>();
     rs[s].nzoff = trim_off.as<unsigned long long>();
     rs[s].q[0] = rq[s][0].as<uint32_t>();
@@ -2443,6 +2507,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
   }
   std::memcpy(h_reach, rs, sizeof(rs));
   PSC_HIP(hipStreamSynchronize(stream));
+  if (n) std::memcpy(starts_h.data(), region(2), 16 * (G + 1));
   return PSC_OK;
 }
 
@@ -2646,10 +2711,11 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     // frontier vertices are non-trimmed: their mean degree is m / nnz
     rs[s].avg_deg = nnz_total ? (double)m / (double)nnz_total : 0.0;
     rs[s].nnz = nnz_total;   // a single-pivot round counts it on the device
+    // a single-pivot round skips its exclusions (trim_bm); the first reach
+    // skips the trimmed vertices; head ranks always count the trimmed set
     rs[s].excl = rr ? trim_bm.as<uint32_t>() : nullptr;
-    // a single-pivot round overwrites trim_bm with its exclusions; head
-    // ranks then come from the kept copy of the trimmed set
-    rs[s].tstat = rr ? trim_static.as<uint32_t>() : trim_bm.as<uint32_t>();
+    rs[s].trim = rr ? trim_bm.as<uint32_t>() : trim_static.as<uint32_t>();
+    rs[s].tstat = trim_static.as<uint32_t>();
   }
   char *pin = (char *)h_pinned;
   std::memcpy(pin + PIN_REACH, rs, sizeof(rs));
@@ -2662,7 +2728,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
   if (!rr) {
     const uint64_t words = (std::max<uint64_t>(n, 1) + 31) / 32;
     k_reach_sources<<<grid_for((uint64_t)G * WAVE), BLOCK, 0, stream>>>(
-        S, C, 2, trim_bm.as<uint32_t>(), words, trim_off.as<unsigned long long>(),
+        S, C, 2, trim_static.as<uint32_t>(), words, trim_off.as<unsigned long long>(),
         (std::max<uint64_t>(n, 1) + SEL_TILE - 1) / SEL_TILE, gA.as<unsigned long long>(),
         G, p.seed, m);
   } else {
@@ -3024,49 +3090,26 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   // non-trimmed vertices per tile and writes the trimmed and dead-end
   // bitmaps; NZ (the non-trimmed vertices in order) is never listed: the
   // first pivot is found by rank, every label is written after the reach
+  // the binding trimmed every graph (trim_static, dead, trim_off and the
+  // per-graph starts); a run starts its searches' visited bitmaps as the
+  // dead-end sets and clears its probe counter
   unsigned long long *nstart = gA.as<unsigned long long>();
   unsigned long long *zstart = nstart + (G + 1);
-  std::vector<uint64_t> h_starts(2 * (G + 1), 0);
   if (n) {
     const uint64_t words = (n + 31) / 32;
-    const uint64_t nt = (n + SEL_TILE - 1) / SEL_TILE;
-    unsigned long long *tot = misc.as<unsigned long long>();
     prof_begin(KK_MISC);
-    k_trim_count<<<(unsigned)nt, BLOCK, 0, stream>>>(d_off, d_ioff, n, words,
-                                                    tiles.as<uint32_t>(),
-                                                    vis[0].as<uint32_t>(),
-                                                    vis[1].as<uint32_t>(),
-                                                    trim_bm.as<uint32_t>(),
-                                                    misc.as<unsigned long long>() + 2);
-    PSC_TRY(scan_tiles(nt, tot, trim_off.as<unsigned long long>()));
-    k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
-        d_off, d_ioff, n, voffd, G, trim_off.as<unsigned long long>(), tot, nstart,
-        zstart);
+    k_run_start<<<grid_for(words, BLOCK, 4096), BLOCK, 0, stream>>>(
+        dead.as<uint32_t>(), words, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(),
+        misc.as<unsigned long long>() + 2);
     prof_end();
-    kbytes[KK_MISC] += 32.0 * n + 3 * n / 8.0;
+    kbytes[KK_MISC] += 16.0 * words;
+    st->bytes_front += 16 * words;
     PSC_HIP(hipGetLastError());
-    // the per-graph starts travel to pinned memory while the first reach
-    // runs (its kernels take the counts from the device); the host waits
-    // for this copy only, not for the reach
-    PSC_TRY(gpin(2 * (G + 1)));
-    PSC_HIP(hipMemcpyAsync(region(2), nstart, 16 * (G + 1), hipMemcpyDeviceToHost,
-                           stream));
-    if (ev[7]) PSC_HIP(hipEventRecord(ev[7], stream));
   }
   if (p.timing) PSC_HIP(hipEventRecord(ev[1], stream));
-  const bool trace_first = p.timing >= 3;   // the level report wants nnz first
-  bool have_starts = false, counted = false;
-  auto read_starts = [&]() -> int {
-    if (have_starts) return PSC_OK;
-    if (ev[7]) PSC_HIP(hipEventSynchronize(ev[7]));
-    else PSC_HIP(hipStreamSynchronize(stream));   // arena mode: no events
-    std::memcpy(h_starts.data(), region(2), 16 * (G + 1));
-    nnz_total = h_starts[G];
-    have_starts = true;
-    return PSC_OK;
-  };
-  if (n && trace_first) PSC_TRY(read_starts());
-  const uint64_t *h_nstart = h_starts.data(), *h_zstart = h_starts.data() + G + 1;
+  bool counted = false;
+  nnz_total = starts_h[G];
+  const uint64_t *h_nstart = starts_h.data(), *h_zstart = starts_h.data() + G + 1;
   // per-graph label counters: trimmed get 1..nz_g, the first SCC nz_g + 1,
   // the rounds' labels start at nz_g + 2
   std::vector<uint64_t> label_off(G), label_next(G);
@@ -3086,7 +3129,6 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   // reach went on, it is enqueued again after the last batch (it rewrites
   // everything it writes; the trim's tile offsets live in their own buffer).
   auto label_pass = [&]() -> int {
-    PSC_TRY(read_starts());
     const uint64_t nnz = h_nstart[G];
     set_label_offs();
     // final values go straight into the labels unless a graph with
@@ -3106,7 +3148,7 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_fill_u32<<<grid_for(pwords, BLOCK, 4096), BLOCK, 0, stream>>>(pbits.as<uint32_t>(),
                                                                    pwords, 0u);
     k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
-        trim_bm.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
+        trim_static.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
         gC.as<unsigned long long>(), posv, pbits.as<uint32_t>(), direct_final ? 1 : 0,
         first_trim.as<uint32_t>());
@@ -3152,7 +3194,6 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
   const uint64_t nnz = h_nstart[G], nz = n - nnz;
   nnz_total = nnz;
   st->n_trimmed = nz;
-  st->bytes_csr += 16 * n;
   uint64_t n_rem = 0;
   std::vector<uint64_t> h_cstart(G + 1, 0);
   if (nnz) {
@@ -3354,7 +3395,7 @@ int Engine::prof_reset(int level) {
     kbytes[k] = 0;
   }
   // the contains() probe counter (misc[2]) is cleared by the run's first
-  // kernel (k_trim_count); an empty graph has no kernel
+  // kernel (k_run_start); an empty graph has no kernel
   if (n == 0) (void)hipMemsetAsync(misc.as<unsigned long long>() + 2, 0, 8, stream);
   return PSC_OK;
 }

@@ -276,14 +276,25 @@ struct TailArgs {
   // timing >= 3: per round of the first graph, (start tick, window,
   // seeds, search levels), TAIL_TRACE rounds at most; else null
   unsigned long long *trace;
+  uint32_t first;              // descriptor / state index of block 0
 };
 constexpr uint32_t TAIL_TRACE = 64;
+// Table slots of a graph whose pair tables, logs and frontiers live in LDS
+// (two tables of TAIL_LDS_CAP u64 and six lists of TAIL_LDS_CAP / 2 u32).
+// Every table, log and frontier access of its rounds is then an LDS access;
+// only the compact graph and its labels are read from memory.  A graph whose
+// rounds overflow them is relaunched with tables in memory.
+constexpr uint64_t TAIL_LDS_CAP = 2048;
 
 // One workgroup per graph: all rounds of its remainder.  The host orders the
 // graphs of a launch by decreasing work, and workgroups are dispatched in
 // order as CUs free up, so the longest remainders start first and the short
-// ones fill in behind them.
+// ones fill in behind them.  LT: tables, logs and frontiers in LDS (cap =
+// TAIL_LDS_CAP), else in the per-graph regions of A.table / A.lists.
+template <bool LT>
 __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
+  __shared__ unsigned long long lt_tab[LT ? 2 * TAIL_LDS_CAP : 1];
+  __shared__ uint32_t lt_list[LT ? 3 * TAIL_LDS_CAP : 1];
   __shared__ uint32_t wsum[TAIL_BLOCK / WAVE], wsum2[TAIL_BLOCK / WAVE];
   __shared__ unsigned int cnt_q[2], cnt_log[2], flag_full;
   __shared__ unsigned int front_n, seed_n;
@@ -294,11 +305,12 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   __shared__ uint32_t st_s[TAIL_BLOCK / WAVE][WAVE];
   __shared__ uint32_t el_s[TAIL_BLOCK / WAVE][WAVE];
   __shared__ unsigned long long lw_s[TAIL_BLOCK / WAVE][WAVE];
-  const TailGraph tg = A.graphs[blockIdx.x];
+  const uint32_t gi = A.first + blockIdx.x;
+  const TailGraph tg = A.graphs[gi];
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
   const uint64_t base = tg.base, nr = tg.nr, half = tg.cap / 2, mask = tg.cap - 1;
-  unsigned long long *const T0 = A.table + tg.tbase, *const T1 = T0 + tg.cap;
-  uint32_t *const L0 = A.lists + tg.lbase;   // slot indices of inserted pairs
+  unsigned long long *const T0 = LT ? lt_tab : A.table + tg.tbase, *const T1 = T0 + tg.cap;
+  uint32_t *const L0 = LT ? lt_list : A.lists + tg.lbase;   // slot indices of inserted pairs
   uint32_t *const L1 = L0 + half;
   // frontier ping-pong of each direction
   uint32_t *const qa0 = L0 + 2 * half, *const qb0 = L0 + 3 * half;
@@ -312,9 +324,13 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   unsigned long long rounds = 0, ins = 0, edges = 0;
   uint32_t nlog0 = 0, nlog1 = 0;
   bool full = false, handover = false;
+  if (LT) {   // LDS tables start empty (memory ones are cleared by the host)
+    for (uint32_t i = tid; i < 2 * TAIL_LDS_CAP; i += TAIL_BLOCK) T0[i] = EMPTY_SLOT;
+    __syncthreads();
+  }
   while (cur < nr && !full && !handover) {
     const uint64_t end = min(cur + step, nr);
-    const bool tr = A.trace && blockIdx.x == 0 && tid == 0 && rounds < TAIL_TRACE;
+    const bool tr = A.trace && gi == 0 && tid == 0 && rounds < TAIL_TRACE;
     if (tr) {
       A.trace[4 * rounds] = wall_clock64() - t0;
       A.trace[4 * rounds + 1] = end - cur;
@@ -577,7 +593,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     }
     __syncthreads();
   }
-  if (full) {
+  if (full && !LT) {
     // slots of a stopped round are not all logged: clear the whole region
     for (uint64_t i = tid; i < 2 * tg.cap; i += TAIL_BLOCK) T0[i] = EMPTY_SLOT;
   }
@@ -589,7 +605,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   }
   __syncthreads();
   if (tid == 0) {
-    A.state[blockIdx.x] = TailState{cur, step, rounds, full ? 1ull : 0ull, 0, 0,
+    A.state[gi] = TailState{cur, step, rounds, full ? 1ull : 0ull, 0, 0,
                                     wall_clock64() - t0};
     A.lab_off[tg.g] = loff;
   }
@@ -805,10 +821,17 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     t.cap = cap;
     t.cap_max = std::min<uint64_t>(TAIL_MAX_SLOTS, cap * 64);
   }
+  // A graph with few compact edges (its pairs per round are bounded by its
+  // special positions) runs with its tables in LDS first.
+  // (Forced small tables, p.table_init, apply to the LDS tables too: the
+  // overflow of both kinds is then exercised.)
+  std::vector<char> lds(nt, 0);
+  for (size_t i = 0; i < nt; ++i) lds[i] = h_e[tgs[i].g] <= TAIL_LDS_CAP / 4;
   // Launch the graphs still pending; a graph whose tables overflowed stops
-  // at the start of that round and is relaunched with 4x the slots, up to
-  // 64x its first size and TAIL_MAX_SLOTS (then the host-driven rounds
-  // finish it).
+  // at the start of that round and is relaunched: from LDS to memory tables
+  // of its computed size, from memory tables with 4x the slots, up to 64x
+  // its first size and TAIL_MAX_SLOTS (then the host-driven rounds finish
+  // it).
   std::vector<size_t> pending(nt);
   for (size_t i = 0; i < nt; ++i) pending[i] = i;
   std::vector<uint64_t> rounds(nt, 0);
@@ -816,19 +839,27 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   std::vector<TailGraph> launch;
   std::vector<TailState> hs;
   while (!pending.empty()) {
-    // largest remainders first (positions left plus compact edges estimate
-    // a graph's rounds)
+    // LDS graphs, then the others; within each, largest remainders first
+    // (positions left plus compact edges estimate a graph's rounds)
     std::stable_sort(pending.begin(), pending.end(), [&](size_t x, size_t y) {
+      if (lds[x] != lds[y]) return lds[x] > lds[y];
       return tgs[x].nr - tgs[x].cur + h_e[tgs[x].g] > tgs[y].nr - tgs[y].cur + h_e[tgs[y].g];
     });
     launch.clear();
     uint64_t slots = 0, lslots = 0;
+    size_t nlds = 0;
     for (size_t i : pending) {
       TailGraph t = tgs[i];
-      t.tbase = slots;   // two tables of cap slots
-      t.lbase = lslots;   // six lists of cap / 2 u32
-      slots += 2 * t.cap;
-      lslots += 3 * t.cap;
+      if (lds[i]) {
+        t.cap = p.table_init ? std::min<uint64_t>(t.cap, TAIL_LDS_CAP) : TAIL_LDS_CAP;
+        t.tbase = t.lbase = 0;
+        ++nlds;
+      } else {
+        t.tbase = slots;   // two tables of cap slots
+        t.lbase = lslots;   // six lists of cap / 2 u32
+        slots += 2 * t.cap;
+        lslots += 3 * t.cap;
+      }
       launch.push_back(t);
     }
     const size_t nl = launch.size();
@@ -863,7 +894,10 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       PSC_TRY(tail_trace.ensure((4 * TAIL_TRACE + 1) * 8));
       a.trace = tail_trace.as<unsigned long long>();
     }
-    k_ms_tail<<<(unsigned)nl, TAIL_BLOCK, 0, stream>>>(a);
+    a.first = 0;
+    if (nlds) k_ms_tail<true><<<(unsigned)nlds, TAIL_BLOCK, 0, stream>>>(a);
+    a.first = (uint32_t)nlds;
+    if (nl > nlds) k_ms_tail<false><<<(unsigned)(nl - nlds), TAIL_BLOCK, 0, stream>>>(a);
     // labels as of every completed round, enqueued before the poll (a
     // relaunch continues from the compact labels; the host rounds and the
     // finalize read these)
@@ -912,7 +946,10 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       rounds[i] += hs[k].rounds;
       if (hs[k].full) {
         ++st->resizes;
-        if (t.cap * 4 <= t.cap_max) {
+        if (lds[i]) {   // memory tables of the computed size
+          lds[i] = 0;
+          again.push_back(i);
+        } else if (t.cap * 4 <= t.cap_max) {
           t.cap *= 4;
           again.push_back(i);
         }
