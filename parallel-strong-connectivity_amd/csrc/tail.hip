@@ -37,11 +37,13 @@ namespace psc {
 
 namespace {
 
+// Workgroup of a remainder: half of its waves expand the forward search,
+// half the backward one.  Windows are scanned in parts of 32 positions a
+// thread (one bit mask).  A remainder small enough to live in LDS (mode 2
+// below) runs with two waves: its rounds are short, and barriers of two
+// waves cost little.
 constexpr int TAIL_BLOCK = 1024;
-constexpr uint32_t HALF_WAVES = TAIL_BLOCK / WAVE / 2;   // waves per search direction
-// Windows are scanned in parts of TAIL_PART positions, so that a thread's
-// share of a part is at most 32 positions (one bit mask).
-constexpr uint64_t TAIL_PART = 32ull * TAIL_BLOCK;
+constexpr int TAIL_SMALL_BLOCK = 128;
 // Remainder positions (finished vertices included) a graph may have here:
 // every round scans its window's positions.
 constexpr uint64_t TAIL_MAX_POS = 1ull << 20;
@@ -185,42 +187,65 @@ __device__ __forceinline__ int tail_insert(unsigned long long *T, uint64_t mask,
 // singleton SCC.  A round therefore reads the done flags of its window's
 // special positions only, and labels the isolated ones by arithmetic.
 //
-// One part [c0, c1) of a round's window (c1 - c0 <= TAIL_PART): thread t
+// One part [c0, c1) of a round's window (at most 32 positions a thread): thread t
 // owns a contiguous segment of at most 32 positions starting at span
 // position p0 (its bits come from one or two bitmap words).  done marks
 // its special positions that are finished now; u0 / ns0 are the
 // block-wide exclusive prefixes of unfinished positions / unfinished
 // special positions (seeds), tot / nstot their totals.  Every thread of
 // the block calls it (it synchronises the block).
+// Block-wide exclusive scan of one value per thread (NB threads); *total =
+// the sum.  ws holds NB / WAVE entries; every thread of the block calls it.
+template <int NB>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *ws, uint32_t *total) {
+  const uint32_t w = threadIdx.x / WAVE;
+  const uint32_t inc = wave_incl_scan(x);
+  __syncthreads();   // earlier readers of ws are done
+  if ((threadIdx.x & (WAVE - 1)) == WAVE - 1) ws[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, t = 0;
+  for (uint32_t k = 0; k < NB / WAVE; ++k) {
+    before += k < w ? ws[k] : 0u;
+    t += ws[k];
+  }
+  *total = t;
+  return before + inc - x;
+}
+
 struct SpecWord {
   uint64_t p0;                 // span position of bit 0
   uint32_t inwin, spec, done;  // bit masks
   uint32_t u0, tot, ns0, nstot;
 };
 
-__device__ __forceinline__ SpecWord tail_spec_part(const uint32_t *specbm,
-                                                   const unsigned long long *lab,
-                                                   uint64_t base, uint64_t c0, uint64_t c1,
+// The bits of positions [p, p + len) (len <= 32) of a bitmap.
+__device__ __forceinline__ uint32_t bits_at(const uint32_t *bm, uint64_t p, uint32_t len) {
+  if (!len) return 0u;
+  const uint64_t wd = p >> 5;
+  const uint32_t sh = (uint32_t)(p & 31);
+  uint64_t bits = bm[wd];
+  if (sh + len > 32) bits |= (uint64_t)bm[wd + 1] << 32;
+  return (uint32_t)(bits >> sh) & (len >= 32 ? ~0u : ((1u << len) - 1u));
+}
+
+// specw(p, len): special bits of span positions [p, p + len); labp(p): the
+// label of special span position p.
+template <int NB, class SpecF, class LabF>
+__device__ __forceinline__ SpecWord tail_spec_part(SpecF specw, LabF labp, uint64_t base,
+                                                   uint64_t c0, uint64_t c1,
                                                    uint32_t *wsum, uint32_t *wsum2) {
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
-  const uint64_t W = c1 - c0, per = (W + TAIL_BLOCK - 1) / TAIL_BLOCK;   // <= 32
+  const uint64_t W = c1 - c0, per = (W + NB - 1) / NB;   // <= 32
   const uint64_t s0 = min((uint64_t)tid * per, W), s1 = min(s0 + per, W);
   SpecWord r;
   r.p0 = base + c0 + s0;
   const uint32_t len = (uint32_t)(s1 - s0);
   r.inwin = len >= 32 ? ~0u : ((1u << len) - 1u);
-  r.spec = 0;
-  if (len) {
-    const uint64_t wd = r.p0 >> 5;
-    const uint32_t sh = (uint32_t)(r.p0 & 31);
-    uint64_t bits = specbm[wd];
-    if (sh + len > 32) bits |= (uint64_t)specbm[wd + 1] << 32;
-    r.spec = (uint32_t)(bits >> sh) & r.inwin;
-  }
+  r.spec = specw(r.p0, len);
   r.done = 0;
   for (uint32_t m = r.spec; m; m &= m - 1) {
     const int b = __ffs(m) - 1;
-    if (lab[r.p0 + b] & TOP_BIT) r.done |= 1u << b;
+    if (labp(r.p0 + b) & TOP_BIT) r.done |= 1u << b;
   }
   const uint32_t unf = (uint32_t)__popc(r.inwin & ~r.done);
   const uint32_t sd = (uint32_t)__popc(r.spec & ~r.done);
@@ -233,7 +258,7 @@ __device__ __forceinline__ SpecWord tail_spec_part(const uint32_t *specbm,
   uint32_t before = 0, befores = 0;
   r.tot = 0;
   r.nstot = 0;
-  for (uint32_t x = 0; x < TAIL_BLOCK / WAVE; ++x) {
+  for (uint32_t x = 0; x < NB / WAVE; ++x) {
     before += x < w ? wsum[x] : 0u;
     r.tot += wsum[x];
     befores += x < w ? wsum2[x] : 0u;
@@ -285,26 +310,47 @@ constexpr uint32_t TAIL_TRACE = 64;
 // only the compact graph and its labels are read from memory.  A graph whose
 // rounds overflow them is relaunched with tables in memory.
 constexpr uint64_t TAIL_LDS_CAP = 2048;
+// A graph of at most TAIL_LG_POS remainder positions, TAIL_LG_VERTS special
+// positions and TAIL_LG_EDGES compact edge entries (both directions) also
+// keeps its special positions in LDS: their labels and compact lists under
+// dense ids (rank among the special positions), so its searches and its
+// partition touch no memory at all.  The labels go back at the end.
+constexpr uint32_t TAIL_LG_POS = 32768;
+constexpr uint32_t TAIL_LG_VERTS = 1024;
+constexpr uint32_t TAIL_LG_EDGES = 2048;
 
 // One workgroup per graph: all rounds of its remainder.  The host orders the
 // graphs of a launch by decreasing work, and workgroups are dispatched in
 // order as CUs free up, so the longest remainders start first and the short
-// ones fill in behind them.  LT: tables, logs and frontiers in LDS (cap =
-// TAIL_LDS_CAP), else in the per-graph regions of A.table / A.lists.
-template <bool LT>
-__global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
+// ones fill in behind them.  MODE 0: tables, logs and frontiers in the
+// per-graph regions of A.table / A.lists; 1: in LDS (cap <= TAIL_LDS_CAP);
+// 2: in LDS, and the special positions too (dense ids).  A graph that does
+// not fit mode 2 stops before its first round with full = 2.
+template <int MODE, int NB>
+__global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
+  constexpr bool LT = MODE >= 1, LG = MODE == 2;
+  constexpr uint32_t HALF_WAVES = NB / WAVE / 2;   // waves per search direction
+  constexpr uint64_t TAIL_PART = 32ull * NB;        // window part (<= 32 positions a thread)
   __shared__ unsigned long long lt_tab[LT ? 2 * TAIL_LDS_CAP : 1];
   __shared__ uint32_t lt_list[LT ? 3 * TAIL_LDS_CAP : 1];
-  __shared__ uint32_t wsum[TAIL_BLOCK / WAVE], wsum2[TAIL_BLOCK / WAVE];
+  // mode 2: special bits of the graph's positions, specials before each
+  // word, and per dense id: position, label, out- then in-offsets, lists
+  __shared__ uint32_t lg_sw[LG ? TAIL_LG_POS / 32 : 1];
+  __shared__ uint16_t lg_rp[LG ? TAIL_LG_POS / 32 : 1];
+  __shared__ uint32_t lg_pos[LG ? TAIL_LG_VERTS : 1];
+  __shared__ unsigned long long lg_lab[LG ? TAIL_LG_VERTS : 1];
+  __shared__ uint32_t lg_off[LG ? 2 * (TAIL_LG_VERTS + 1) : 1];
+  __shared__ uint32_t lg_adj[LG ? TAIL_LG_EDGES : 1];
+  __shared__ uint32_t wsum[NB / WAVE], wsum2[NB / WAVE];
   __shared__ unsigned int cnt_q[2], cnt_log[2], flag_full;
   __shared__ unsigned int front_n, seed_n;
   __shared__ unsigned long long round_edges;
   // per-wave local-search ring and window scratch
-  __shared__ uint32_t ring_s[TAIL_BLOCK / WAVE][TAIL_RING];
-  __shared__ uint32_t sc_s[TAIL_BLOCK / WAVE][WAVE];
-  __shared__ uint32_t st_s[TAIL_BLOCK / WAVE][WAVE];
-  __shared__ uint32_t el_s[TAIL_BLOCK / WAVE][WAVE];
-  __shared__ unsigned long long lw_s[TAIL_BLOCK / WAVE][WAVE];
+  __shared__ uint32_t ring_s[NB / WAVE][TAIL_RING];
+  __shared__ uint32_t sc_s[NB / WAVE][WAVE];
+  __shared__ uint32_t st_s[NB / WAVE][WAVE];
+  __shared__ uint32_t el_s[NB / WAVE][WAVE];
+  __shared__ unsigned long long lw_s[NB / WAVE][WAVE];
   const uint32_t gi = A.first + blockIdx.x;
   const TailGraph tg = A.graphs[gi];
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
@@ -315,7 +361,8 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   // frontier ping-pong of each direction
   uint32_t *const qa0 = L0 + 2 * half, *const qb0 = L0 + 3 * half;
   uint32_t *const qa1 = L0 + 4 * half, *const qb1 = L0 + 5 * half;
-  unsigned long long *lab = A.tlab;
+  // labels of the searched vertices: dense ids (mode 2) or span positions
+  unsigned long long *const lab = LG ? lg_lab : A.tlab;
   const uint32_t tau = A.tau;
   const int vgc = A.vgc;
   uint64_t cur = tg.cur, step = tg.step;
@@ -323,11 +370,110 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   const unsigned long long t0 = wall_clock64();
   unsigned long long rounds = 0, ins = 0, edges = 0;
   uint32_t nlog0 = 0, nlog1 = 0;
-  bool full = false, handover = false;
+  bool full = false, handover = false, lg_fail = false;
   if (LT) {   // LDS tables start empty (memory ones are cleared by the host)
-    for (uint32_t i = tid; i < 2 * TAIL_LDS_CAP; i += TAIL_BLOCK) T0[i] = EMPTY_SLOT;
+    for (uint32_t i = tid; i < 2 * TAIL_LDS_CAP; i += NB) T0[i] = EMPTY_SLOT;
     __syncthreads();
   }
+  // dense id of the special local position q (mode 2)
+  auto rank = [&](uint64_t q) -> uint32_t {
+    const uint32_t wd = (uint32_t)(q >> 5);
+    return lg_rp[wd] + (uint32_t)__popc(lg_sw[wd] & ((1u << (q & 31)) - 1u));
+  };
+  uint32_t lg_n = 0;
+  if (LG) {
+    // special bits by local position (WPT words per thread, in order), the
+    // specials before each word, then per dense id its position, label and
+    // degrees; then the offsets (out-lists first, then in-lists, one array)
+    // and the lists with neighbours by dense id
+    constexpr uint32_t WPT = TAIL_LG_POS / 32 / NB, VPT = TAIL_LG_VERTS / NB;
+    const uint32_t nw = (uint32_t)((nr + 31) / 32);
+    const bool fits = nr <= TAIL_LG_POS;
+    uint32_t c = 0;
+    if (fits)
+      for (uint32_t k = 0; k < WPT; ++k) {
+        const uint32_t wd = tid * WPT + k;
+        uint32_t bits = 0;
+        if (wd < nw)
+          bits = bits_at(A.spec, base + 32ull * wd, (uint32_t)min<uint64_t>(32, nr - 32ull * wd));
+        lg_sw[wd] = bits;
+        c += (uint32_t)__popc(bits);
+      }
+    uint32_t total = 0;
+    uint32_t d = block_excl_scan<NB>(c, wsum, &total);
+    lg_n = total;
+    lg_fail = !fits || total > TAIL_LG_VERTS;
+    if (!lg_fail)
+      for (uint32_t k = 0; k < WPT; ++k) {
+        const uint32_t wd = tid * WPT + k;
+        if (wd >= nw) break;
+        lg_rp[wd] = (uint16_t)d;
+        for (uint32_t m = lg_sw[wd]; m; m &= m - 1, ++d) {
+          const uint32_t q = 32 * wd + (uint32_t)(__ffs(m) - 1);
+          const uint64_t p = base + q;
+          lg_pos[d] = q;
+          lg_lab[d] = A.tlab[p];
+          lg_off[d] = A.toff[p + 1] - A.toff[p];
+          lg_off[TAIL_LG_VERTS + 1 + d] = A.tioff[p + 1] - A.tioff[p];
+        }
+      }
+    __syncthreads();
+    if (!lg_fail) {
+      uint32_t so = 0, si = 0;
+      for (uint32_t k = 0; k < VPT; ++k) {
+        const uint32_t dd = tid * VPT + k;
+        if (dd < lg_n) {
+          so += lg_off[dd];
+          si += lg_off[TAIL_LG_VERTS + 1 + dd];
+        }
+      }
+      uint32_t to = 0, ti = 0;
+      uint32_t ro = block_excl_scan<NB>(so, wsum, &to);
+      uint32_t ri = block_excl_scan<NB>(si, wsum2, &ti);
+      lg_fail = to + ti > TAIL_LG_EDGES;
+      if (!lg_fail) {
+        ri += to;
+        // each thread rewrites the degrees it read into offsets
+        for (uint32_t k = 0; k < VPT; ++k) {
+          const uint32_t dd = tid * VPT + k;
+          if (dd >= lg_n) break;
+          const uint32_t a = lg_off[dd], b = lg_off[TAIL_LG_VERTS + 1 + dd];
+          lg_off[dd] = ro;
+          lg_off[TAIL_LG_VERTS + 1 + dd] = ri;
+          ro += a;
+          ri += b;
+        }
+        if (tid == 0) {   // end sentinels
+          lg_off[lg_n] = to;
+          lg_off[TAIL_LG_VERTS + 1 + lg_n] = to + ti;
+        }
+      }
+      __syncthreads();
+      // every compact neighbour is special
+      if (!lg_fail)
+        for (uint32_t dd = tid; dd < lg_n; dd += NB) {
+          const uint64_t p = base + lg_pos[dd];
+          for (int dir = 0; dir < 2; ++dir) {
+            const uint32_t *o = dir ? A.tioff : A.toff;
+            const uint32_t *a = dir ? A.tiadj : A.tadj;
+            uint32_t at = lg_off[dir * (TAIL_LG_VERTS + 1) + dd];
+            for (uint32_t e = o[p]; e < o[p + 1]; ++e) lg_adj[at++] = rank(a[e] - base);
+          }
+        }
+      __syncthreads();
+    }
+    full = lg_fail;
+  }
+  // one part of a round's window (tail_spec_part), from LDS in mode 2
+  auto spec_part = [&](uint64_t c0, uint64_t c1) -> SpecWord {
+    if (LG)
+      return tail_spec_part<NB>(
+          [&](uint64_t p, uint32_t len) { return bits_at(lg_sw, p - base, len); },
+          [&](uint64_t p) { return lg_lab[rank(p - base)]; }, base, c0, c1, wsum, wsum2);
+    return tail_spec_part<NB>(
+        [&](uint64_t p, uint32_t len) { return bits_at(A.spec, p, len); },
+        [&](uint64_t p) { return A.tlab[p]; }, base, c0, c1, wsum, wsum2);
+  };
   while (cur < nr && !full && !handover) {
     const uint64_t end = min(cur + step, nr);
     const bool tr = A.trace && gi == 0 && tid == 0 && rounds < TAIL_TRACE;
@@ -354,14 +500,13 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       // are seeded into the tables.
       uint32_t fb = 0, kb = 0;
       for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
-        const SpecWord sw = tail_spec_part(A.spec, lab, base, c0, min(c0 + TAIL_PART, end),
-                                           wsum, wsum2);
+        const SpecWord sw = spec_part(c0, min(c0 + TAIL_PART, end));
         sw_keep = sw;   // the commit reuses a one-part window's scan
         const uint32_t seeds = sw.spec & ~sw.done, unf = sw.inwin & ~sw.done;
         uint32_t k = kb + sw.ns0;
         for (uint32_t mm = seeds; mm; mm &= mm - 1) {
           const int b = __ffs(mm) - 1;
-          const uint32_t v = (uint32_t)(sw.p0 + b);
+          const uint32_t v = LG ? rank(sw.p0 + b - base) : (uint32_t)(sw.p0 + b);
           const uint32_t i = fb + sw.u0 + (uint32_t)__popc(unf & ((1u << b) - 1u));
           const uint32_t lb = (uint32_t)(loff + i);
           uint64_t q0 = 0, q1 = 0;
@@ -392,7 +537,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     // searches instead of their sum
     nlog0 = nlog1 = 0;
     if (nseed && !flag_full) {
-      for (uint32_t i = tid; i < nseed; i += TAIL_BLOCK) {
+      for (uint32_t i = tid; i < nseed; i += NB) {
         qa0[i] = L0[i];
         qa1[i] = L1[i];
       }
@@ -402,8 +547,8 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       const uint32_t wd = w - (uint32_t)dw * HALF_WAVES;
       unsigned long long *const Td = dw ? T1 : T0;
       uint32_t *const Ld = dw ? L1 : L0;
-      const uint32_t *const eo = dw ? A.tioff : A.toff;
-      const uint32_t *const ea = dw ? A.tiadj : A.tadj;
+      const uint32_t *const eo = LG ? lg_off + dw * (TAIL_LG_VERTS + 1) : (dw ? A.tioff : A.toff);
+      const uint32_t *const ea = LG ? lg_adj : (dw ? A.tiadj : A.tadj);
       __syncthreads();
       while ((nq0 || nq1) && !flag_full) {
         ++slevels;
@@ -549,22 +694,20 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
       {
         uint32_t fb = 0;
         for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
-          const SpecWord sw = end - cur <= TAIL_PART
-                                  ? sw_keep
-                                  : tail_spec_part(A.spec, lab, base, c0,
-                                                   min(c0 + TAIL_PART, end), wsum, wsum2);
+          const SpecWord sw = end - cur <= TAIL_PART ? sw_keep
+                                                     : spec_part(c0, min(c0 + TAIL_PART, end));
           const uint32_t unf = sw.inwin & ~sw.done;
           for (uint32_t mm = sw.inwin & ~sw.spec; mm; mm &= mm - 1) {
             const int b = __ffs(mm) - 1;
             const uint32_t i = fb + sw.u0 + (uint32_t)__popc(unf & ((1u << b) - 1u));
-            lab[sw.p0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
+            A.tlab[sw.p0 + b] = (unsigned long long)(loff + i) | TOP_BIT;
           }
           fb += sw.tot;
         }
       }
       // ---- partition (src/scc.hpp:437-464): a pair in both tables is in
       // the pivot's SCC; labels take the maximum
-      for (uint32_t i = tid; i < nlog0; i += TAIL_BLOCK) {
+      for (uint32_t i = tid; i < nlog0; i += NB) {
         const unsigned long long pr = T0[L0[i]];
         const uint32_t v = pair_vertex(pr), lb = pair_label(pr);
         uint32_t probes = 0;
@@ -573,7 +716,7 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
                                                      : (unsigned long long)lb;
         if (lab[v] < val) atomicMax(&lab[v], val);
       }
-      for (uint32_t i = tid; i < nlog1; i += TAIL_BLOCK) {
+      for (uint32_t i = tid; i < nlog1; i += NB) {
         const unsigned long long pr = T1[L1[i]];
         const unsigned long long val = pair_label(pr);
         const uint32_t v = pair_vertex(pr);
@@ -588,14 +731,17 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
     __syncthreads();
     // ---- empty the slots this round used
     if (!full) {
-      for (uint32_t i = tid; i < nlog0; i += TAIL_BLOCK) T0[L0[i]] = EMPTY_SLOT;
-      for (uint32_t i = tid; i < nlog1; i += TAIL_BLOCK) T1[L1[i]] = EMPTY_SLOT;
+      for (uint32_t i = tid; i < nlog0; i += NB) T0[L0[i]] = EMPTY_SLOT;
+      for (uint32_t i = tid; i < nlog1; i += NB) T1[L1[i]] = EMPTY_SLOT;
     }
     __syncthreads();
   }
   if (full && !LT) {
     // slots of a stopped round are not all logged: clear the whole region
-    for (uint64_t i = tid; i < 2 * tg.cap; i += TAIL_BLOCK) T0[i] = EMPTY_SLOT;
+    for (uint64_t i = tid; i < 2 * tg.cap; i += NB) T0[i] = EMPTY_SLOT;
+  }
+  if (LG && !lg_fail) {   // the special positions' labels as of the last round
+    for (uint32_t d = tid; d < lg_n; d += NB) A.tlab[base + lg_pos[d]] = lg_lab[d];
   }
   ins = wave_sum(ins);
   edges = wave_sum(edges);
@@ -605,8 +751,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ms_tail(TailArgs A) {
   }
   __syncthreads();
   if (tid == 0) {
-    A.state[gi] = TailState{cur, step, rounds, full ? 1ull : 0ull, 0, 0,
-                                    wall_clock64() - t0};
+    // full: 1 = a table, log or frontier overflowed, 2 = the graph does not
+    // fit mode 2 (no round ran)
+    A.state[gi] = TailState{cur, step, rounds, lg_fail ? 2ull : full ? 1ull : 0ull, 0, 0,
+                            wall_clock64() - t0};
     A.lab_off[tg.g] = loff;
   }
 }
@@ -822,11 +970,17 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     t.cap_max = std::min<uint64_t>(TAIL_MAX_SLOTS, cap * 64);
   }
   // A graph with few compact edges (its pairs per round are bounded by its
-  // special positions) runs with its tables in LDS first.
+  // special positions) runs with its tables in LDS first (lds 1), and with
+  // its special positions in LDS too when it may fit them (lds 2; the
+  // kernel checks the count of special positions).
   // (Forced small tables, p.table_init, apply to the LDS tables too: the
-  // overflow of both kinds is then exercised.)
+  // overflow of every kind is then exercised.)
   std::vector<char> lds(nt, 0);
-  for (size_t i = 0; i < nt; ++i) lds[i] = h_e[tgs[i].g] <= TAIL_LDS_CAP / 4;
+  for (size_t i = 0; i < nt; ++i) {
+    const uint64_t e = h_e[tgs[i].g];
+    lds[i] = e > TAIL_LDS_CAP / 4 ? 0
+             : tgs[i].nr <= TAIL_LG_POS && 2 * e <= TAIL_LG_EDGES ? 2 : 1;
+  }
   // Launch the graphs still pending; a graph whose tables overflowed stops
   // at the start of that round and is relaunched: from LDS to memory tables
   // of its computed size, from memory tables with 4x the slots, up to 64x
@@ -847,13 +1001,15 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     });
     launch.clear();
     uint64_t slots = 0, lslots = 0;
-    size_t nlds = 0;
+    size_t nmode[3] = {0, 0, 0};
+    std::vector<char> lmode;   // the launch's mode per graph
     for (size_t i : pending) {
       TailGraph t = tgs[i];
+      ++nmode[(int)lds[i]];
+      lmode.push_back(lds[i]);
       if (lds[i]) {
         t.cap = p.table_init ? std::min<uint64_t>(t.cap, TAIL_LDS_CAP) : TAIL_LDS_CAP;
         t.tbase = t.lbase = 0;
-        ++nlds;
       } else {
         t.tbase = slots;   // two tables of cap slots
         t.lbase = lslots;   // six lists of cap / 2 u32
@@ -894,10 +1050,13 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       PSC_TRY(tail_trace.ensure((4 * TAIL_TRACE + 1) * 8));
       a.trace = tail_trace.as<unsigned long long>();
     }
+    // the launch lists mode 2 graphs, then mode 1, then mode 0
     a.first = 0;
-    if (nlds) k_ms_tail<true><<<(unsigned)nlds, TAIL_BLOCK, 0, stream>>>(a);
-    a.first = (uint32_t)nlds;
-    if (nl > nlds) k_ms_tail<false><<<(unsigned)(nl - nlds), TAIL_BLOCK, 0, stream>>>(a);
+    if (nmode[2]) k_ms_tail<2, TAIL_SMALL_BLOCK><<<(unsigned)nmode[2], TAIL_SMALL_BLOCK, 0, stream>>>(a);
+    a.first = (uint32_t)nmode[2];
+    if (nmode[1]) k_ms_tail<1, TAIL_BLOCK><<<(unsigned)nmode[1], TAIL_BLOCK, 0, stream>>>(a);
+    a.first = (uint32_t)(nmode[2] + nmode[1]);
+    if (nmode[0]) k_ms_tail<0, TAIL_BLOCK><<<(unsigned)nmode[0], TAIL_BLOCK, 0, stream>>>(a);
     // labels as of every completed round, enqueued before the poll (a
     // relaunch continues from the compact labels; the host rounds and the
     // finalize read these)
@@ -944,7 +1103,10 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       cur[t.g] = hs[k].cur;
       stepg[t.g] = hs[k].step;
       rounds[i] += hs[k].rounds;
-      if (hs[k].full) {
+      if (hs[k].full == 2) {   // more special positions than mode 2 holds
+        lds[i] = 1;
+        again.push_back(i);
+      } else if (hs[k].full) {
         ++st->resizes;
         if (lds[i]) {   // memory tables of the computed size
           lds[i] = 0;
@@ -981,10 +1143,10 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
         const size_t k = ord[r];
         const TailGraph &t = launch[k];
         std::fprintf(stderr,
-                     "tail graph %u: remainder %llu compact edges %llu cap %llu rounds %llu "
-                     "%s %.1f us\n",
+                     "tail graph %u: remainder %llu compact edges %llu cap %llu mode %d "
+                     "rounds %llu %s %.1f us\n",
                      t.g, (unsigned long long)t.nr, (unsigned long long)h_e[t.g],
-                     (unsigned long long)t.cap, hs[k].rounds,
+                     (unsigned long long)t.cap, (int)lmode[k], hs[k].rounds,
                      hs[k].full ? "overflow" : (hs[k].cur < t.nr ? "handed over" : "done"),
                      hs[k].ticks / 100.0);
       }
