@@ -301,7 +301,9 @@ struct TailArgs {
   // timing >= 3: per round of the first graph, (start tick, window,
   // seeds, search levels), TAIL_TRACE rounds at most; else null
   unsigned long long *trace;
-  uint32_t first;              // descriptor / state index of block 0
+  uint32_t first;              // descriptor / state index of the launch's first graph
+  uint32_t count;              // graphs of the launch
+  unsigned int *queue;         // work counter of a persistent launch (zero on entry)
 };
 constexpr uint32_t TAIL_TRACE = 64;
 // Table slots of a graph whose pair tables, logs and frontiers live in LDS
@@ -351,7 +353,8 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
   __shared__ uint32_t st_s[NB / WAVE][WAVE];
   __shared__ uint32_t el_s[NB / WAVE][WAVE];
   __shared__ unsigned long long lw_s[NB / WAVE][WAVE];
-  const uint32_t gi = A.first + blockIdx.x;
+  // one graph of the launch, by its descriptor / state index
+  auto run_graph = [&](const uint32_t gi) {
   const TailGraph tg = A.graphs[gi];
   const uint32_t tid = threadIdx.x, w = tid / WAVE;
   const uint64_t base = tg.base, nr = tg.nr, half = tg.cap / 2, mask = tg.cap - 1;
@@ -757,6 +760,22 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
                             wall_clock64() - t0};
     A.lab_off[tg.g] = loff;
   }
+  };
+  if constexpr (NB == TAIL_SMALL_BLOCK) {
+    // a persistent grid: each workgroup takes the launch's graphs from its
+    // work counter, in the host's order (largest remainders first)
+    __shared__ uint32_t item_s;
+    for (;;) {
+      __syncthreads();   // the previous graph's LDS use and item reads are done
+      if (threadIdx.x == 0) item_s = atomicAdd(A.queue, 1u);
+      __syncthreads();
+      const uint32_t item = item_s;
+      if (item >= A.count) break;
+      run_graph(A.first + item);
+    }
+  } else {
+    run_graph(A.first + blockIdx.x);   // one workgroup per graph
+  }
 }
 
 // Unfinished vertices of each described graph at positions [cur, nr) of
@@ -808,8 +827,9 @@ __global__ void k_tail_writeback(const uint32_t *centers, uint64_t nrem,
 }
 
 __global__ void k_tail_fill_u64(unsigned long long *p, uint64_t n,
-                                unsigned long long *zero2) {
+                                unsigned long long *zero2, unsigned int *queue) {
   if (blockIdx.x == 0 && threadIdx.x < 2) zero2[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = EMPTY_SLOT;
@@ -1028,8 +1048,9 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     PSC_HIP(hipMemcpyAsync(d_tg, tail_pin, nl * sizeof(TailGraph), hipMemcpyHostToDevice,
                            stream));
     prof_begin(KK_TAIL);
+    unsigned int *d_queue = reinterpret_cast<unsigned int *>(d_go + 1);
     k_tail_fill_u64<<<blocks(slots), 256, 0, stream>>>(tail_table.as<unsigned long long>(),
-                                                       slots, d_stats);
+                                                       slots, d_stats, d_queue);
     TailArgs a;
     a.graphs = d_tg;
     a.state = d_ts;
@@ -1050,12 +1071,21 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
       PSC_TRY(tail_trace.ensure((4 * TAIL_TRACE + 1) * 8));
       a.trace = tail_trace.as<unsigned long long>();
     }
-    // the launch lists mode 2 graphs, then mode 1, then mode 0
+    // the launch lists mode 2 graphs, then mode 1, then mode 0; mode 2 runs
+    // as a persistent grid (one workgroup per CU, LDS-bound) over the work
+    // counter, the others one workgroup per graph
+    a.queue = d_queue;
     a.first = 0;
-    if (nmode[2]) k_ms_tail<2, TAIL_SMALL_BLOCK><<<(unsigned)nmode[2], TAIL_SMALL_BLOCK, 0, stream>>>(a);
+    a.count = (uint32_t)nmode[2];
+    if (nmode[2])
+      k_ms_tail<2, TAIL_SMALL_BLOCK>
+          <<<(unsigned)std::min<size_t>(nmode[2], (size_t)num_cu), TAIL_SMALL_BLOCK, 0,
+             stream>>>(a);
     a.first = (uint32_t)nmode[2];
+    a.count = (uint32_t)nmode[1];
     if (nmode[1]) k_ms_tail<1, TAIL_BLOCK><<<(unsigned)nmode[1], TAIL_BLOCK, 0, stream>>>(a);
     a.first = (uint32_t)(nmode[2] + nmode[1]);
+    a.count = (uint32_t)nmode[0];
     if (nmode[0]) k_ms_tail<0, TAIL_BLOCK><<<(unsigned)nmode[0], TAIL_BLOCK, 0, stream>>>(a);
     // labels as of every completed round, enqueued before the poll (a
     // relaunch continues from the compact labels; the host rounds and the
