@@ -97,7 +97,7 @@ def bench_batch(args, psc):
     kind, n = psc.batch_spec(G, 10, args.hi_log2, seed=4)
     seeds = np.array([psc.graph_seed(4, g) for g in range(G)], dtype=np.uint64)
     table = at.load_table()
-    groups = at.group_by_class(n, table, max_groups=args.lanes)
+    groups = at.group_by_class(n, table, max_groups=args.lanes, family=kind)
     t = time.perf_counter()
     dbs = [psc.DeviceBatch(kind[ix], n[ix], seeds=seeds[ix]) for ix, _, _ in groups]
     gen_s = time.perf_counter() - t
@@ -161,8 +161,8 @@ def bench_batch(args, psc):
            "graphs": G, "n_total": n_tot, "m_total": m_tot,
            "batched_gedges_s": round(m_tot / wall / 1e9, 3),
            "labels_identical": mism == 0, "tarjan_checked": checked,
-           "groups": [{"classes": cs, "graphs": int(len(ix)), "tau": p["tau"],
-                       "dense_div": p["dense_div"]} for ix, p, cs in groups],
+           "groups": [{"classes": cs, "family": int(kind[ix[0]]), "graphs": int(len(ix)),
+                       "tau": p["tau"], "dense_div": p["dense_div"]} for ix, p, cs in groups],
            "gen_seconds": round(gen_s, 2)}
     print(json.dumps(out), flush=True)
     for b in bss:

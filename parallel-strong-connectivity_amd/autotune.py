@@ -98,10 +98,29 @@ def tune(psc, lo=10, hi=24, per_class=24, seed=77, log=print, path=TABLE_PATH):
     return table
 
 
-def group_by_class(n, table, max_groups=4):
+def group_by_class(n, table, max_groups=4, family=None):
     """Partition graph indices into at most max_groups groups of adjacent
     size classes, balancing total vertices; each group runs with the
-    parameters of its dominant class."""
+    parameters of its dominant class.
+
+    family (optional, one id per graph, e.g. the generator of a batch
+    graph): graphs of different families go to different groups, at most
+    max(1, max_groups // families) class groups per family.  A handle
+    switches between push and pull per level for all its graphs together,
+    and runs as many levels as its deepest graph: a skewed graph (RMAT) that
+    is done in a few pull-heavy levels would otherwise ride along the many
+    levels of a lattice or k-NN graph."""
+    n = np.asarray(n)
+    if family is not None:
+        family = np.asarray(family)
+        fams = sorted(set(family.tolist()))
+        per = max(1, max_groups // len(fams))
+        out = []
+        for f in fams:
+            fi = np.flatnonzero(family == f)
+            for ix, prm, cs in group_by_class(n[fi], table, max_groups=per):
+                out.append((fi[ix], prm, cs))
+        return out
     cls = np.array([size_class(x) for x in n])
     classes = sorted(set(cls.tolist()))
     weight = {c: float(np.sum(n[cls == c])) for c in classes}

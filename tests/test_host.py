@@ -228,6 +228,20 @@ def test_autotune_grouping(psc):
         assert np.allclose(prm["betas"], [2.0 if c < 15 else 1.5 for c in cls])
 
 
+def test_autotune_grouping_by_family(psc):
+    import psc_amd.autotune as at
+    kind, n = psc.batch_spec(600, 10, 20, seed=5)
+    table = {c: {"tau": 64, "beta": 1.5, "dense_div": 20} for c in range(10, 21)}
+    groups = at.group_by_class(n, table, max_groups=6, family=kind)
+    idx = np.sort(np.concatenate([g[0] for g in groups]))
+    assert np.array_equal(idx, np.arange(600))
+    assert len(groups) <= 6
+    for ix, prm, cs in groups:
+        assert len(set(kind[ix].tolist())) == 1          # one family per group
+        assert set(at.size_class(x) for x in n[ix]) <= set(cs)
+        assert len(prm["betas"]) == len(ix)
+
+
 def test_batch_spec_distribution(psc):
     kind, n = psc.batch_spec(4000, 10, 20, seed=4)
     assert set(np.unique(kind).tolist()) == {0, 1, 2}
