@@ -115,6 +115,9 @@ def bench_batch(args, psc):
     wall = float(np.median(walls))
     _, labs = at.run_groups(bss, prms, labels=True, timing=1)
     log(f"batched: {wall * 1e3:.2f} ms for {G} graphs ({G / wall:.1f} graphs/s)")
+    if args.levels:   # per-level and remainder-round reports on stderr, group by group
+        for b, prm in zip(bss, prms):
+            at.run_groups([b], [prm], timing=3)
     for (ix, prm, cs), b in zip(groups, bss):
         st = b.stats
         log(f"  group {cs[0]}..{cs[-1]}: {len(ix)} graphs n={b.n} "

@@ -92,28 +92,53 @@ __device__ __forceinline__ unsigned long long wave_append64(
 // written to the global list in bulk: one atomic per flush instead of one
 // per 64-lane step, and the global stores of a flush are coalesced.  `n` is
 // wave-uniform.  Entries at or beyond `cap` are dropped and reported.
+//
+// A list whose readers never look past `cap` entries (capped = true) stops
+// reserving once a flush of the wave has seen the counter pass `cap` (the
+// counter only grows): later entries are only counted, and finish() adds
+// their number with one atomic.  Flush reservations are device-scope
+// atomics on one counter and serialise; a level that finds millions of
+// vertices past the cap would otherwise make one per CAP - WAVE entries for
+// nothing.
 template <class T, int CAP>
 struct WaveOut {
   T *buf;
   uint32_t n;
-  __device__ __forceinline__ void init(T *b) {
+  uint32_t over;   // entries counted past the cap (capped lists)
+  bool capped, past;
+  __device__ __forceinline__ void init(T *b, bool cap_only = false) {
     buf = b;
     n = 0;
+    over = 0;
+    capped = cap_only;
+    past = false;
   }
   // returns false if the global list overflowed `cap`
   template <class Ctr>
   __device__ __forceinline__ bool flush(Ctr *ctr, T *dst, unsigned long long cap) {
     if (n == 0) return true;
+    if (past) {
+      over += n;
+      n = 0;
+      return false;
+    }
     wave_sync();
     Ctr base = 0;
     if (lane() == 0) base = atomicAdd(ctr, (Ctr)n);
     base = __shfl(base, 0);
+    past = capped && (unsigned long long)base + n >= cap;
     for (uint32_t i = lane(); i < n; i += WAVE)
       if ((unsigned long long)base + i < cap) dst[base + i] = buf[i];
     const bool ok = (unsigned long long)base + n <= cap;
     n = 0;
     wave_sync();
     return ok;
+  }
+  // a capped list's entries counted past the cap join the counter
+  template <class Ctr>
+  __device__ __forceinline__ void finish(Ctr *ctr) {
+    if (over && lane() == 0) atomicAdd(ctr, (Ctr)over);
+    over = 0;
   }
   template <class Ctr>
   __device__ __forceinline__ bool put(bool want, T v, Ctr *ctr, T *dst,

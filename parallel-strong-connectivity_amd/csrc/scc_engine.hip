@@ -556,6 +556,31 @@ __device__ __forceinline__ int reach_mode(const ReachSlot *c,
   return (unv * PULL_VERTEX_COST + R.n / PULL_WALK_DIV < push) ? MODE_PULL : MODE_PUSH;
 }
 
+// A dense pull level (the reference's criterion) right after push levels
+// tests heads only and leaves every head miss to the next level, which then
+// pulls completely: most of those vertices are found there by their head,
+// and the rest would be scanned there again anyway.  Reached sets do not
+// depend on the level a vertex is found at; the search cannot end while
+// misses are deferred.  Such a level runs in k_reach_step (head_level).
+__device__ __forceinline__ bool pull_defers(const ReachSlot *c, const ReachSearch &R) {
+  return !c->from_pull && !c->deferred && (uint64_t)c->qsize + c->degsum >= R.thresh;
+}
+
+// Index of the j-th set bit of y (j < popc(y)), by halving.
+__device__ __forceinline__ uint32_t select_bit(uint32_t y, uint32_t j) {
+  uint32_t pos = 0, c;
+  c = (uint32_t)__popc(y & 0xFFFFu);
+  if (j >= c) { j -= c; y >>= 16; pos += 16; }
+  c = (uint32_t)__popc(y & 0xFFu);
+  if (j >= c) { j -= c; y >>= 8; pos += 8; }
+  c = (uint32_t)__popc(y & 0xFu);
+  if (j >= c) { j -= c; y >>= 4; pos += 4; }
+  c = (uint32_t)__popc(y & 0x3u);
+  if (j >= c) { j -= c; y >>= 2; pos += 2; }
+  if (j >= (y & 1u)) pos += 1;
+  return pos;
+}
+
 // Frontier entries a pull level stores.  Once a pull level has found
 // pull_store_cap vertices, the next level pulls whatever else it finds
 // (reach_mode: nf + nf * avg_deg >= thresh, with one to spare for the
@@ -625,6 +650,125 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
   }
 }
 
+// Head-only pull level of one search (pull_defers): every unvisited
+// non-trimmed vertex tests its head (one in-neighbour of large degree, see
+// k_pull_heads) and is found if the head is visited; a miss is left to the
+// next level, which pulls completely.  That level is forced (`deferred`)
+// even without misses, so it never reads this level's frontier: the found
+// vertices are only counted (one atomic per block), not stored -- a
+// frontier flush is a device-scope atomic on one counter, and a level that
+// finds tens of millions of vertices would serialise on tens of thousands
+// of them.  A wave walks 2048-vertex chunks (one (visited | skipped) word
+// per lane, the next chunk's loaded ahead) and takes the chunk's
+// non-trimmed vertices in rank order, 64 consecutive ranks per step, so the
+// heads (stored by rank) are one coalesced load; lane k of a step finds its
+// vertex by binary search over the chunk's per-word counts in LDS and a bit
+// select in the word, and vertices already visited load nothing.
+// HEAD_STEPS steps keep their loads in flight together.  Found bits gather
+// in LDS and go out with one atomic per word.  `lds` holds 4 * WAVE words of
+// the wave's own LDS.  Every thread of the block calls it (block-wide
+// counter reduction).
+constexpr int HEAD_STEPS = 4;
+
+__device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
+                                           uint32_t *lds, uint64_t gw, uint64_t nw,
+                                           unsigned long long *cnt_s) {
+  const int l = lane();
+  if (gw == 0 && l == 0) {
+    nxt->from_pull = 1;
+    nxt->deferred = 1;
+  }
+  const uint64_t words = ((uint64_t)R.n + 31) / 32;
+  const uint64_t chunks = (words + WAVE - 1) / WAVE;
+  const uint32_t tail_bits = R.n & 31;
+  const bool own_trim = R.tstat == R.trim;
+  uint32_t *cnt = lds, *skw = lds + WAVE, *nzw = lds + 2 * WAVE, *fw = lds + 3 * WAVE;
+  uint32_t n_test = 0, n_vis = 0, n_chunks = 0;   // per wave: < 2^32
+  // this lane's (visited | skipped) word of chunk c, and its trimmed word
+  auto load = [&](uint64_t c, uint32_t *ts) -> uint32_t {
+    const uint64_t wi = c * WAVE + l;
+    uint32_t sk = ~0u;
+    *ts = ~0u;
+    if (c < chunks && wi < words) {
+      const uint32_t tr = R.trim[wi];
+      *ts = own_trim ? tr : R.tstat[wi];
+      sk = R.vis[wi] | tr;
+      if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
+    }
+    return sk;
+  };
+  uint32_t ts_next;
+  uint32_t sk_next = load(gw, &ts_next);
+  unsigned long long r0_next = gw < chunks ? R.nzoff[gw] : 0ull;
+  for (uint64_t c = gw; c < chunks; c += nw) {
+    const uint32_t sk = sk_next, ts = ts_next;
+    const unsigned long long r0 = r0_next;
+    sk_next = load(c + nw, &ts_next);
+    r0_next = c + nw < chunks ? R.nzoff[c + nw] : 0ull;
+    ++n_chunks;
+    if (!ballot(sk != ~0u)) continue;
+    const uint32_t nz = (uint32_t)__popc(~ts);
+    const uint32_t inc = wave_incl_scan(nz);
+    const uint32_t total = __shfl(inc, WAVE - 1);
+    wave_sync();   // the previous chunk's readers are done
+    cnt[l] = inc;
+    skw[l] = sk;
+    nzw[l] = ~ts;
+    fw[l] = 0;
+    wave_sync();
+    // pos: the vertex's offset in the chunk, bit 31 set if it is tested;
+    // the heads of the next HEAD_STEPS steps are loaded while the visited
+    // words of the current ones are (software pipelined)
+    uint32_t hd[HEAD_STEPS], pos[HEAD_STEPS];
+    auto issue = [&](uint32_t base) {
+#pragma unroll
+      for (int j = 0; j < HEAD_STEPS; ++j) {
+        const uint32_t k = base + j * WAVE + l;
+        pos[j] = 0;
+        if (k < total) {
+          uint32_t lo = 0, hi = WAVE - 1;   // first word with cnt > k
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cnt[mid] > k) hi = mid; else lo = mid + 1;
+          }
+          const uint32_t bt = select_bit(nzw[lo], k - (lo ? cnt[lo - 1] : 0u));
+          pos[j] = lo * 32 + bt | (((skw[lo] >> bt) & 1u) ? 0u : 1u << 31);
+        }
+        hd[j] = (pos[j] >> 31) ? R.rhead[r0 + k] : 0u;
+      }
+    };
+    issue(0);
+    for (uint32_t base = 0; base < total; base += WAVE * HEAD_STEPS) {
+      uint32_t hw[HEAD_STEPS], ph[HEAD_STEPS], pp[HEAD_STEPS];
+#pragma unroll
+      for (int j = 0; j < HEAD_STEPS; ++j) {
+        hw[j] = (pos[j] >> 31) ? R.vis[hd[j] >> 5] : 0u;
+        ph[j] = hd[j];
+        pp[j] = pos[j];
+      }
+      if (base + WAVE * HEAD_STEPS < total) issue(base + WAVE * HEAD_STEPS);
+#pragma unroll
+      for (int j = 0; j < HEAD_STEPS; ++j) {
+        // v's own bit is clear, so an empty list's head (v) misses
+        const bool todo = pp[j] >> 31;
+        const uint32_t o = pp[j] & 2047u;
+        const bool hit = todo && ((hw[j] >> (ph[j] & 31)) & 1u);
+        n_test += todo;
+        n_vis += hit;
+        if (hit) atomicOr(&fw[o >> 5], 1u << (o & 31));
+      }
+    }
+    wave_sync();
+    if (fw[l]) atomicOr(&R.vis[c * WAVE + l], fw[l]);
+  }
+  const uint32_t wv = wave_sum(n_vis);   // the next level's frontier size
+  if (l == 0 && wv) atomicAdd(&nxt->qsize, wv);
+  unsigned long long *const cp[4] = {&cnt_s[R_HEAD_TESTS], &cnt_s[R_HEAD_VIS], &nxt->cum_vis,
+                                     &cnt_s[R_HEAD_UNITS]};
+  const unsigned long long cv[4] = {n_test, n_vis, n_vis, l == 0 ? n_chunks : 0ull};
+  block_add<4>(cp, cv);
+}
+
 // One BFS level of both searches.  Push levels (src/reach.hpp:114-171):
 // wave w takes a contiguous share of the frontier, S seeds at a time, keeps
 // a queue of up to REACH_RING vertices in LDS and expands it with
@@ -634,13 +778,11 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // lasts and the ring has room; the rest are staged in LDS and spilled to
 // the next frontier in bulk.  Vertices with more than HEAVY_DEG edges go to
 // the heavy list (k_reach_heavy).
-// Pull levels (src/reach.hpp:173-190): every unvisited vertex scans its
-// in-edges until it meets a visited vertex; testing "visited" instead of
-// "in the previous frontier" keeps the reachable set identical.  Long
-// remainders (> PULL_HEAVY edges after the first probes) are chunked onto
-// the heavy list.
+// Pull levels (src/reach.hpp:173-190) run in k_reach_pull, except the
+// head-only ones (pull_defers), which run here (head_level): their waves
+// need little LDS and few registers, so this kernel's occupancy suits them.
 __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
-    const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
+    const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
   __shared__ uint32_t out_s[WPB][OUT_CAP];
@@ -765,6 +907,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
                                          &nxt->cum_vis, &C->cnt[s][R_PUSH_APP]};
       const unsigned long long cv[6] = {dsum, n_edges, n_exp, n_vis, n_vis, n_app};
       block_add<6>(cp, cv);
+    } else if (mode == MODE_PULL && pull_defers(cur, R)) {
+      head_level(R, nxt, ring, gw, nw, C->cnt[s]);
     }
   }
 }
@@ -1077,6 +1221,7 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
 // left undecided queue their remaining in-edges in LDS; a full batch of 64
 // queued vertices is scanned as load-balanced 64-edge windows.  Remainders
 // longer than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
+// Head-only levels (pull_defers) are k_reach_step's (head_level).
 constexpr int PULL_PAIRS = 2;
 constexpr int PULL_CHUNKS = 8;  // chunks whose candidates one pass may hold
 #ifndef PSC_PULL_HEADS
@@ -1151,7 +1296,7 @@ __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reach_pull(
-    const ReachSearch *S, ReachCtl *C, int ns, uint32_t L, uint32_t total_waves) {
+    const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t total_waves) {
   __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
@@ -1183,29 +1328,22 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   uint64_t gw, nw;
   xcd_split(ns, w, &s_lo, &s_hi, &gw, &nw, &total_waves);
   WaveOut<uint32_t, OUT_CAP> out;
-  out.init(out_s[w]);
+  out.init(out_s[w], true);   // read only up to pull_store_cap entries
   PullQueue q{qv_s[w], qb_s[w], qd_s[w], 0};
   for (int s = s_lo; s < s_hi; ++s) {
     ReachSlot *cur = &C->slot[s][ci];
     ReachSlot *nxt = &C->slot[s][ni];
     const ReachSearch R = S[s];
-    if (reach_mode(cur, R) != MODE_PULL) continue;
+    // head-only levels run in k_reach_step
+    if (reach_mode(cur, R) != MODE_PULL || pull_defers(cur, R)) continue;
     if (gw == 0 && l == 0) nxt->from_pull = 1;   // first wave of this search
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
     const uint64_t qcap = pull_store_cap(R);
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
-    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0, n_def = 0;
+    unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0;
     uint32_t ncand = 0, nck = 0;
-    // A dense pull level (the reference's criterion) right after push
-    // levels leaves the in-list remainders of its undecided candidates to
-    // the next level, which then pulls completely: most of them are found
-    // there by their head, and the rest would be scanned there again
-    // anyway.  Reached sets do not depend on the level a vertex is found
-    // at; the search cannot end while remainders are deferred.
-    const bool defer = !cur->from_pull && !cur->deferred &&
-                       (uint64_t)cur->qsize + cur->degsum >= R.thresh;
     // Both phases over the listed candidates (of up to PULL_CHUNKS chunks),
     // then the found bits of the chunks' words go out, one atomic per word.
     auto process = [&]() {
@@ -1248,12 +1386,6 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         }
       }
       wave_sync();
-      // a deferring level leaves every head miss to the next level, which
-      // scans it completely
-      if (defer) {
-        n_def += nmiss;
-        nmiss = 0;
-      }
       // phase 2: the misses load their offsets and probe in-edges
       // 1..PULL_PROBE, PULL_PAIRS per lane
       for (uint32_t cb = 0; cb < nmiss; cb += WAVE * PULL_PAIRS) {
@@ -1292,11 +1424,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
             found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
           const uint64_t k = min(deg[j], (uint64_t)PULL_PROBE);
           n_edges += k;
-          uint64_t rem = found ? 0 : deg[j] - k;
-          if (defer && rem) {
-            ++n_def;
-            rem = 0;
-          }
+          const uint64_t rem = found ? 0 : deg[j] - k;
           // long remainders go to the heavy list (k_reach_heavy, chunked),
           // the rest to the wave's queue
           const bool heavy = rem > PULL_HEAVY;
@@ -1383,13 +1511,13 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     if (ncand) process();
     while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, qcap);
+    out.finish(&nxt->qsize);
     unsigned long long *const cp[5] = {&C->cnt[s][R_PULL_EDGES], &C->cnt[s][R_PULL_SCANS],
                                        &C->cnt[s][R_PULL_VIS], &nxt->cum_vis,
                                        &C->cnt[s][R_PULL_UNITS]};
     const unsigned long long cv[5] = {n_edges, n_scan, n_vis, n_vis,
                                       l == 0 ? n_chunks : 0ull};
     block_add<5>(cp, cv);
-    if (defer && __syncthreads_or(n_def != 0) && threadIdx.x == 0) atomicOr(&nxt->deferred, 1u);
   }
 }
 
@@ -1399,7 +1527,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
 // 2048).  Pull level: the in-edge remainder of an unvisited vertex with a
 // long in-list; the first chunk to meet a visited in-neighbour sets the
 // vertex's bit and appends it, the other chunks of that vertex stop early.
-__global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *S,
+__global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *__restrict__ S,
                                                        ReachCtl *C, int ns,
                                                        uint32_t L,
                                                        uint32_t total_waves) {
@@ -2822,12 +2950,12 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
         std::fprintf(stderr,
                      "level %u %s %s nf=%u degsum=%llu heavy=%llu/%llu next=%u "
                      "push_exp=%llu push_edges=%llu pull_scans=%llu pull_edges=%llu "
-                     "heavy_edges=%llu wall=%.1fus\n",
+                     "head_tests=%llu heavy_edges=%llu wall=%.1fus\n",
                      lv, s ? "bwd" : "fwd", mode, c.qsize, c.degsum,
                      c.heavy >> H_CNT_SHIFT, c.heavy & H_CHUNK_MASK,
                      hc_.slot[s][(lv + 1) % 3].qsize, a[R_PUSH_EXP] - b[R_PUSH_EXP],
                      a[R_PUSH_EDGES] - b[R_PUSH_EDGES], a[R_PULL_SCANS] - b[R_PULL_SCANS],
-                     a[R_PULL_EDGES] - b[R_PULL_EDGES],
+                     a[R_PULL_EDGES] - b[R_PULL_EDGES], a[R_HEAD_TESTS] - b[R_HEAD_TESTS],
                      a[R_HEAVY_EDGES] - b[R_HEAVY_EDGES], us);
       }
       prev_ = hc_;
@@ -2853,12 +2981,13 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     st->sub_rounds += std::max(lvl[0], lvl[1]);
     for (int s = 0; s < 2; ++s)
       rr->reached[s] = hc_.cnt[s][R_PUSH_VIS] + hc_.cnt[s][R_HEAVY_VIS] +
-                       hc_.cnt[s][R_PULL_VIS];
+                       hc_.cnt[s][R_PULL_VIS] + hc_.cnt[s][R_HEAD_VIS];
   }
   hint = std::max(lvl[0], lvl[1]);
   for (int s = 0; s < 2; ++s) {
     const unsigned long long *c = hc_.cnt[s];
-    st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES];
+    st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
+                         c[R_HEAD_TESTS];
     // push: offsets of expanded vertices, edge ids, visited test per edge,
     // visited write per new vertex, frontier read/append, degree of appended
     const double push = 16.0 * c[R_PUSH_EXP] + 5.0 * c[R_PUSH_EDGES] +
@@ -2870,16 +2999,21 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     // vertices
     const double pull = 512.0 * c[R_PULL_UNITS] + 16.0 * c[R_PULL_SCANS] +
                         5.0 * c[R_PULL_EDGES] + 5.0 * c[R_PULL_VIS];
-    kbytes[KK_STEP] += push;
+    // head-only pull levels (run by the step kernel): the same words per
+    // chunk, 4 B head + visited test per tested vertex, new vertices
+    const double head = 512.0 * c[R_HEAD_UNITS] + 5.0 * c[R_HEAD_TESTS] +
+                        5.0 * c[R_HEAD_VIS];
+    kbytes[KK_STEP] += push + head;
     kbytes[KK_PULL] += pull;
     kbytes[KK_HEAVY] += heavy;
     st->bytes_csr += 16 * (c[R_PUSH_EXP] + c[R_PULL_SCANS]) +
-                     4 * (c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES]);
+                     4 * (c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
+                          c[R_HEAD_TESTS]);
     st->bytes_label += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
-                       c[R_PUSH_VIS] + c[R_HEAVY_VIS] + c[R_PULL_VIS] +
-                       512 * c[R_PULL_UNITS];
+                       c[R_HEAD_TESTS] + c[R_PUSH_VIS] + c[R_HEAVY_VIS] + c[R_PULL_VIS] +
+                       c[R_HEAD_VIS] + 512 * (c[R_PULL_UNITS] + c[R_HEAD_UNITS]);
     st->bytes_front += 4 * (c[R_PUSH_APP] + c[R_HEAVY_APP] + c[R_PULL_VIS] +
-                            c[R_PUSH_EXP]) +
+                            c[R_HEAD_VIS] + c[R_PUSH_EXP]) +
                        16 * (c[R_PUSH_APP] + c[R_HEAVY_APP]);
   }
   return PSC_OK;

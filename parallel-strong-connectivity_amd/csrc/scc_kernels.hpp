@@ -87,6 +87,7 @@ enum : int {
   R_PUSH_EXP, R_PUSH_EDGES, R_PUSH_VIS, R_PUSH_APP,
   R_HEAVY_EDGES, R_HEAVY_VIS, R_HEAVY_APP, R_HEAVY_ITEMS,
   R_PULL_SCANS, R_PULL_EDGES, R_PULL_VIS, R_PULL_UNITS,
+  R_HEAD_TESTS, R_HEAD_VIS, R_HEAD_UNITS,   // head-only pull levels
   NRC
 };
 
