@@ -1224,6 +1224,7 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
 // Head-only levels (pull_defers) are k_reach_step's (head_level).
 constexpr int PULL_PAIRS = 2;
 constexpr int PULL_CHUNKS = 8;  // chunks whose candidates one pass may hold
+constexpr int PULL_OUT_CAP = 512;   // per-wave staged frontier entries
 #ifndef PSC_PULL_HEADS
 #define PSC_PULL_HEADS 4
 #endif
@@ -1297,7 +1298,7 @@ __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
 
 __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t total_waves) {
-  __shared__ uint32_t out_s[WPB][OUT_CAP];
+  __shared__ uint32_t out_s[WPB][PULL_OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
   __shared__ uint32_t fl_s[WPB][WAVE];
@@ -1327,7 +1328,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   int s_lo, s_hi;
   uint64_t gw, nw;
   xcd_split(ns, w, &s_lo, &s_hi, &gw, &nw, &total_waves);
-  WaveOut<uint32_t, OUT_CAP> out;
+  WaveOut<uint32_t, PULL_OUT_CAP> out;
   out.init(out_s[w], true);   // read only up to pull_store_cap entries
   PullQueue q{qv_s[w], qb_s[w], qd_s[w], 0};
   for (int s = s_lo; s < s_hi; ++s) {
