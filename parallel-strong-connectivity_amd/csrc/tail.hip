@@ -320,6 +320,9 @@ constexpr uint64_t TAIL_LDS_CAP = 2048;
 constexpr uint32_t TAIL_LG_POS = 32768;
 constexpr uint32_t TAIL_LG_VERTS = 1024;
 constexpr uint32_t TAIL_LG_EDGES = 2048;
+// lg_pos entries: the local position, and bit 31 once the position is known
+// to have been finished before its own round
+constexpr uint32_t LG_PREDONE = 1u << 31, LG_POS_MASK = LG_PREDONE - 1;
 
 // One workgroup per graph: all rounds of its remainder.  The host orders the
 // graphs of a launch by decreasing work, and workgroups are dispatched in
@@ -370,6 +373,8 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
   const int vgc = A.vgc;
   uint64_t cur = tg.cur, step = tg.step;
   unsigned long long loff = A.lab_off[tg.g];
+  const uint64_t c0 = cur;                   // the launch's first position
+  const unsigned long long loff0 = loff;     // and its label offset
   const unsigned long long t0 = wall_clock64();
   unsigned long long rounds = 0, ins = 0, edges = 0;
   uint32_t nlog0 = 0, nlog1 = 0;
@@ -384,6 +389,8 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
     return lg_rp[wd] + (uint32_t)__popc(lg_sw[wd] & ((1u << (q & 31)) - 1u));
   };
   uint32_t lg_n = 0;
+  // dense id of the first special position at or after q (q <= nr)
+  auto rank_end = [&](uint64_t q) -> uint32_t { return q >= nr ? lg_n : rank(q); };
   if (LG) {
     // special bits by local position (WPT words per thread, in order), the
     // specials before each word, then per dense id its position, label and
@@ -494,7 +501,50 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
       round_edges = 0;
     }
     __syncthreads();
-    {
+    if (LG) {
+      // Mode 2: only the window's special positions are read (dense ids
+      // [d0, d1), in position order).  Every position of the window that was
+      // unfinished at the round's start is a pivot, in order: pivot q gets
+      // loff + (q - cur) - (specials of the window before q that were
+      // finished before this round).  The seeds are the unfinished special
+      // positions; the isolated ones (no compact edges) take their labels
+      // from the same count after the last round (they are never finished
+      // by another pivot, so nothing before then reads them).
+      const uint32_t d0 = rank_end(cur), d1 = rank_end(end);
+      uint32_t kb = 0, npre = 0;
+      for (uint32_t x0 = d0; x0 < d1; x0 += NB) {
+        const uint32_t d = x0 + tid;
+        uint32_t pre = 0, sd = 0;
+        if (d < d1) {
+          pre = (lg_lab[d] & TOP_BIT) ? 1u : 0u;
+          sd = 1u - pre;
+        }
+        uint32_t tot = 0;   // pre-done count << 16 | seed count (<= NB each)
+        const uint32_t ex = block_excl_scan<NB>((pre << 16) | sd, wsum, &tot);
+        if (sd) {
+          const uint32_t q = lg_pos[d] & LG_POS_MASK;
+          const uint32_t lb = (uint32_t)(loff + (q - cur) - (npre + (ex >> 16)));
+          const uint32_t k = kb + (ex & 0xFFFFu);
+          uint64_t q0 = 0, q1 = 0;
+          const int r0 = tail_insert(T0, mask, d, lb, &q0);
+          const int r1 = tail_insert(T1, mask, d, lb, &q1);
+          if (r0 < 0 || r1 < 0 || k >= half) {
+            flag_full = 1;
+          } else {
+            L0[k] = (uint32_t)q0;
+            L1[k] = (uint32_t)q1;
+          }
+        }
+        if (pre) lg_pos[d] |= LG_PREDONE;
+        npre += tot >> 16;
+        kb += tot & 0xFFFFu;
+      }
+      if (tid == 0) {
+        front_n = (uint32_t)(end - cur) - npre;
+        seed_n = kb;
+      }
+      __syncthreads();
+    } else {
       // Unfinished vertices of the window, part by part: pivot i of the
       // round gets label loff + i.  A pivot without compact edges in either
       // direction is its own SCC this round (nothing reaches it, it reaches
@@ -693,8 +743,9 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
       // isolated pivots: their own SCC, (loff + i) | TOP_BIT (written only
       // now: a stopped round leaves every label as it was).  The window's
       // parts are scanned again; only isolated positions are written, and
-      // only special positions are read.
-      {
+      // only special positions are read.  (Mode 2 writes them after the
+      // last round.)
+      if (!LG) {
         uint32_t fb = 0;
         for (uint64_t c0 = cur; c0 < end; c0 += TAIL_PART) {
           const SpecWord sw = end - cur <= TAIL_PART ? sw_keep
@@ -743,8 +794,50 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
     // slots of a stopped round are not all logged: clear the whole region
     for (uint64_t i = tid; i < 2 * tg.cap; i += NB) T0[i] = EMPTY_SLOT;
   }
-  if (LG && !lg_fail) {   // the special positions' labels as of the last round
-    for (uint32_t d = tid; d < lg_n; d += NB) A.tlab[base + lg_pos[d]] = lg_lab[d];
+  if (LG && !lg_fail) {
+    // the isolated positions of the completed rounds, [c0, cur): pivot q
+    // has loff0 + (q - c0) - (specials in [c0, q) finished before their own
+    // round), the running pivot index over all rounds of this launch; the
+    // count comes from an exclusive prefix of those flags over dense ids
+    // [rank(c0), rank(cur)), kept in lg_adj (the lists are no longer read)
+    const uint32_t dc0 = rank_end(c0), dcur = rank_end(cur);
+    constexpr uint32_t VPT = TAIL_LG_VERTS / NB;
+    uint32_t cnt = 0;
+    for (uint32_t k = 0; k < VPT; ++k) {
+      const uint32_t d = dc0 + tid * VPT + k;
+      if (d < dcur) cnt += lg_pos[d] >> 31;
+    }
+    uint32_t tot = 0;
+    uint32_t run = block_excl_scan<NB>(cnt, wsum, &tot);
+    uint32_t *const pdx = lg_adj;
+    for (uint32_t k = 0; k < VPT; ++k) {
+      const uint32_t d = dc0 + tid * VPT + k;
+      if (d < dcur) {
+        pdx[d - dc0] = run;
+        run += lg_pos[d] >> 31;
+      }
+    }
+    if (tid == 0) pdx[dcur - dc0] = tot;
+    __syncthreads();
+    // a bitmap word of positions per thread: its specials' flags as a mask
+    // (flags are set only for specials in [c0, cur)), one prefix read
+    const uint64_t w_end = cur > c0 ? (cur + 31) / 32 : 0;
+    for (uint64_t w = c0 / 32 + tid; w < w_end; w += NB) {
+      const uint32_t bits = lg_sw[w], r0 = lg_rp[w];
+      uint32_t pdm = 0, d = r0;
+      for (uint32_t m = bits; m; m &= m - 1, ++d)
+        if (lg_pos[d] >> 31) pdm |= m & (~m + 1u);
+      const uint32_t pd0 = r0 >= dc0 ? pdx[r0 - dc0] : 0u;
+      const uint64_t q0 = 32 * w;
+      for (uint32_t b = 0; b < 32; ++b) {
+        const uint64_t q = q0 + b;
+        if (q < c0 || q >= cur || ((bits >> b) & 1u)) continue;
+        const uint32_t pd = pd0 + (uint32_t)__popc(pdm & ((1u << b) - 1u));
+        A.tlab[base + q] = (unsigned long long)(loff0 + (q - c0) - pd) | TOP_BIT;
+      }
+    }
+    // the special positions' labels as of the last round
+    for (uint32_t d = tid; d < lg_n; d += NB) A.tlab[base + (lg_pos[d] & LG_POS_MASK)] = lg_lab[d];
   }
   ins = wave_sum(ins);
   edges = wave_sum(edges);
