@@ -669,6 +669,7 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // the wave's own LDS.  Every thread of the block calls it (block-wide
 // counter reduction).
 constexpr int HEAD_STEPS = 4;
+constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
 
 __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
                                            uint32_t *lds, uint64_t gw, uint64_t nw,
@@ -781,7 +782,7 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
 // Pull levels (src/reach.hpp:173-190) run in k_reach_pull, except the
 // head-only ones (pull_defers), which run here (head_level): their waves
 // need little LDS and few registers, so this kernel's occupancy suits them.
-__global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
+__global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
     const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
@@ -859,43 +860,67 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
           ebudget = ebudget > total ? ebudget - total : 0;
           if (!ebudget) budget = 0;
           wave_sync();
-          for (uint32_t base = 0; base < total; base += WAVE) {
-            const uint32_t e = base + l;
-            bool found = false;
-            uint32_t v = 0;
-            if (e < total) {
-              int lo = 0, hi = WAVE - 1;
-              while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (sc[mid] > e) hi = mid; else lo = mid + 1;
+          // PUSH_WIN 64-edge windows at a time: their edge loads, visited
+          // tests, claims and degree loads are each in flight together (a
+          // local search is a chain of such steps, so each step's latency
+          // counts)
+          for (uint32_t base = 0; base < total; base += WAVE * PUSH_WIN) {
+            uint32_t v[PUSH_WIN], w[PUSH_WIN];
+            bool found[PUSH_WIN];
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) {
+              const uint32_t e = base + x * WAVE + l;
+              found[x] = e < total;
+              v[x] = 0;
+              if (found[x]) {
+                int lo = 0, hi = WAVE - 1;
+                while (lo < hi) {
+                  const int mid = (lo + hi) >> 1;
+                  if (sc[mid] > e) hi = mid; else lo = mid + 1;
+                }
+                const uint32_t ex = lo ? sc[lo - 1] : 0u;
+                v[x] = R.adj[st[lo] + (e - ex)];
+                ++n_edges;
               }
-              const uint32_t ex = lo ? sc[lo - 1] : 0u;
-              v = R.adj[st[lo] + (e - ex)];
-              found = reach_claim(R, v);
-              ++n_edges;
             }
-            const uint64_t fm = ballot(found);
-            if (!fm) continue;
-            n_vis += found;
-            uint64_t dv = 0;
-            if (found) dv = R.off[v + 1] - R.off[v];
-            // VGC (src/reach.hpp:125-154): sparse new vertices stay in the
-            // wave's queue while the budget lasts and the ring has room
-            const bool want = found && vgc && dv <= VGC_MAX_DEG;
-            const uint64_t lm = ballot(want);
-            const uint32_t nwant = (uint32_t)__popcll(lm);
-            const uint32_t room = REACH_RING - (tail - head);
-            const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
-            const bool loc = want && lanes_below(lm) < take;
-            if (loc) ring[(tail + lanes_below(lm)) & (REACH_RING - 1)] = v;
-            tail += take;
-            budget -= take;
-            const bool glob = found && !loc;
-            if (glob) {
-              dsum += dv;
-              ++n_app;
+            // claim (reach_claim): outside the searched subproblems never,
+            // else the visited bit once
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) {
+              w[x] = found[x] ? R.vis[v[x] >> 5] : 0u;
+              if (found[x] && R.excl) w[x] |= R.excl[v[x] >> 5];
             }
-            out.put(glob, v, &nxt->qsize, qn, R.n);
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) {
+              const uint32_t bit = 1u << (v[x] & 31);
+              found[x] = found[x] && !(w[x] & bit) && !(atomicOr(&R.vis[v[x] >> 5], bit) & bit);
+            }
+            uint64_t dv[PUSH_WIN];
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) dv[x] = found[x] ? R.off[v[x] + 1] - R.off[v[x]] : 0;
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) {
+              const uint64_t fm = ballot(found[x]);
+              if (!fm) continue;
+              n_vis += found[x];
+              // VGC (src/reach.hpp:125-154): sparse new vertices stay in the
+              // wave's queue while the budget lasts and the ring has room
+              const bool want = found[x] && vgc && dv[x] <= VGC_MAX_DEG;
+              const uint64_t lm = ballot(want);
+              const uint32_t nwant = (uint32_t)__popcll(lm);
+              const uint32_t room = REACH_RING - (tail - head);
+              const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
+              const bool loc = want && lanes_below(lm) < take;
+              if (loc) ring[(tail + lanes_below(lm)) & (REACH_RING - 1)] = v[x];
+              tail += take;
+              budget -= take;
+              const bool glob = found[x] && !loc;
+              if (glob) {
+                dsum += dv[x];
+                ++n_app;
+              }
+              out.put(glob, v[x], &nxt->qsize, qn, R.n);
+            }
           }
           wave_sync();
         }
@@ -935,7 +960,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_reach_step(
 #define PSC_HUB_FALLBACK 4
 #endif
 constexpr uint32_t HUB_SCAN1 = PSC_HUB_SCAN1;
-constexpr uint32_t HUB_SCAN = 128;
+constexpr uint32_t HUB_SCAN = 64;
 constexpr uint32_t HUB_FALLBACK = PSC_HUB_FALLBACK;
 constexpr uint32_t HUB_MAX = 2048;
 constexpr uint32_t HUB_SLOTS = 2 * HUB_MAX;   // open addressing, load <= 1/2
