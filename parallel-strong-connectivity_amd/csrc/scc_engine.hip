@@ -566,21 +566,6 @@ __device__ __forceinline__ bool pull_defers(const ReachSlot *c, const ReachSearc
   return !c->from_pull && !c->deferred && (uint64_t)c->qsize + c->degsum >= R.thresh;
 }
 
-// Index of the j-th set bit of y (j < popc(y)), by halving.
-__device__ __forceinline__ uint32_t select_bit(uint32_t y, uint32_t j) {
-  uint32_t pos = 0, c;
-  c = (uint32_t)__popc(y & 0xFFFFu);
-  if (j >= c) { j -= c; y >>= 16; pos += 16; }
-  c = (uint32_t)__popc(y & 0xFFu);
-  if (j >= c) { j -= c; y >>= 8; pos += 8; }
-  c = (uint32_t)__popc(y & 0xFu);
-  if (j >= c) { j -= c; y >>= 4; pos += 4; }
-  c = (uint32_t)__popc(y & 0x3u);
-  if (j >= c) { j -= c; y >>= 2; pos += 2; }
-  if (j >= (y & 1u)) pos += 1;
-  return pos;
-}
-
 // Frontier entries a pull level stores.  Once a pull level has found
 // pull_store_cap vertices, the next level pulls whatever else it finds
 // (reach_mode: nf + nf * avg_deg >= thresh, with one to spare for the
@@ -661,18 +646,20 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // of them.  A wave walks 2048-vertex chunks (one (visited | skipped) word
 // per lane, the next chunk's loaded ahead) and takes the chunk's
 // non-trimmed vertices in rank order, 64 consecutive ranks per step, so the
-// heads (stored by rank) are one coalesced load; lane k of a step finds its
-// vertex by binary search over the chunk's per-word counts in LDS and a bit
-// select in the word, and vertices already visited load nothing.
+// heads (stored by rank) are one coalesced load; each lane first lists its
+// word's non-trimmed vertices at their ranks in an LDS map (offset in the
+// chunk, and whether the vertex is tested), so lane k of a step reads its
+// vertex from the map, and vertices already visited load nothing.
 // HEAD_STEPS steps keep their loads in flight together.  Found bits gather
-// in LDS and go out with one atomic per word.  `lds` holds 4 * WAVE words of
-// the wave's own LDS.  Every thread of the block calls it (block-wide
-// counter reduction).
+// in LDS and go out with one atomic per word.  The map's two halves
+// (1024 entries each) and `aux` (WAVE words) are the wave's own LDS.  Every
+// thread of the block calls it (block-wide counter reduction).
 constexpr int HEAD_STEPS = 4;
 constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
 
 __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
-                                           uint32_t *lds, uint64_t gw, uint64_t nw,
+                                           uint16_t *map_lo, uint16_t *map_hi,
+                                           uint32_t *aux, uint64_t gw, uint64_t nw,
                                            unsigned long long *cnt_s) {
   const int l = lane();
   if (gw == 0 && l == 0) {
@@ -683,7 +670,7 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
   const uint64_t chunks = (words + WAVE - 1) / WAVE;
   const uint32_t tail_bits = R.n & 31;
   const bool own_trim = R.tstat == R.trim;
-  uint32_t *cnt = lds, *skw = lds + WAVE, *nzw = lds + 2 * WAVE, *fw = lds + 3 * WAVE;
+  uint32_t *fw = aux;   // found bits of the chunk's words
   uint32_t n_test = 0, n_vis = 0, n_chunks = 0;   // per wave: < 2^32
   // this lane's (visited | skipped) word of chunk c, and its trimmed word
   auto load = [&](uint64_t c, uint32_t *ts) -> uint32_t {
@@ -697,6 +684,11 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
       if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
     }
     return sk;
+  };
+  // the chunk's map: rank in the chunk -> offset in the chunk, bit 15 set
+  // if that vertex is tested (2048 entries over two halves)
+  auto map_at = [&](uint32_t k) -> uint32_t {
+    return k < 1024 ? map_lo[k] : map_hi[k - 1024];
   };
   uint32_t ts_next;
   uint32_t sk_next = load(gw, &ts_next);
@@ -712,9 +704,16 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
     const uint32_t inc = wave_incl_scan(nz);
     const uint32_t total = __shfl(inc, WAVE - 1);
     wave_sync();   // the previous chunk's readers are done
-    cnt[l] = inc;
-    skw[l] = sk;
-    nzw[l] = ~ts;
+    {
+      // each lane lists its word's non-trimmed vertices at their ranks
+      uint32_t at = inc - nz;
+      for (uint32_t m = ~ts; m; m &= m - 1) {
+        const uint32_t b = (uint32_t)(__ffs(m) - 1);
+        const uint16_t e = (uint16_t)((l * 32 + b) | (((sk >> b) & 1u) ? 0u : 0x8000u));
+        if (at < 1024) map_lo[at] = e; else map_hi[at - 1024] = e;
+        ++at;
+      }
+    }
     fw[l] = 0;
     wave_sync();
     // pos: the vertex's offset in the chunk, bit 31 set if it is tested;
@@ -727,13 +726,8 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
         const uint32_t k = base + j * WAVE + l;
         pos[j] = 0;
         if (k < total) {
-          uint32_t lo = 0, hi = WAVE - 1;   // first word with cnt > k
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (cnt[mid] > k) hi = mid; else lo = mid + 1;
-          }
-          const uint32_t bt = select_bit(nzw[lo], k - (lo ? cnt[lo - 1] : 0u));
-          pos[j] = lo * 32 + bt | (((skw[lo] >> bt) & 1u) ? 0u : 1u << 31);
+          const uint32_t e = map_at(k);
+          pos[j] = (e & 2047u) | ((e & 0x8000u) << 16);
         }
         hd[j] = (pos[j] >> 31) ? R.rhead[r0 + k] : 0u;
       }
@@ -933,7 +927,9 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
       const unsigned long long cv[6] = {dsum, n_edges, n_exp, n_vis, n_vis, n_app};
       block_add<6>(cp, cv);
     } else if (mode == MODE_PULL && pull_defers(cur, R)) {
-      head_level(R, nxt, ring, gw, nw, C->cnt[s]);
+      head_level(R, nxt, reinterpret_cast<uint16_t *>(ring),
+                 reinterpret_cast<uint16_t *>(out_s[w]), reinterpret_cast<uint32_t *>(st), gw,
+                 nw, C->cnt[s]);
     }
   }
 }
