@@ -119,6 +119,52 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
   }
 }
 
+// Exclusive scan of the per-tile counts of a select in one block (counts
+// made elsewhere, e.g. by atomics while the items are placed): tile_off
+// gets the prefixes, *count_dev the total, and with starts1 (a single
+// graph) the graph's starts {0, total}.  nt <= 2^20 (1024 counts a thread).
+constexpr int TSCAN_BLOCK = 1024;
+constexpr uint64_t TSCAN_MAX = (uint64_t)TSCAN_BLOCK * TSCAN_BLOCK;
+__global__ __launch_bounds__(TSCAN_BLOCK) void k_tile_scan(const uint32_t *cnt, uint64_t nt,
+                                                            unsigned long long *tile_off,
+                                                            unsigned long long *count_dev,
+                                                            unsigned long long *starts1) {
+  __shared__ unsigned long long ws[TSCAN_BLOCK / WAVE];
+  const uint64_t per = (nt + TSCAN_BLOCK - 1) / TSCAN_BLOCK;
+  const uint64_t b = threadIdx.x * per, e = min(nt, b + per);
+  unsigned long long sum = 0;
+  for (uint64_t i = b; i < e; ++i) sum += cnt[i];
+  const unsigned long long inc = wave_incl_scan(sum);
+  const int w = threadIdx.x / WAVE;
+  if (lane() == WAVE - 1) ws[w] = inc;
+  __syncthreads();
+  unsigned long long run = inc - sum, tot = 0;
+  for (int x = 0; x < TSCAN_BLOCK / WAVE; ++x) {
+    run += x < w ? ws[x] : 0ull;
+    tot += ws[x];
+  }
+  for (uint64_t i = b; i < e; ++i) {
+    tile_off[i] = run;
+    run += cnt[i];
+  }
+  if (threadIdx.x == 0) {
+    *count_dev = tot;
+    if (starts1) {
+      starts1[0] = 0;
+      starts1[1] = tot;
+    }
+  }
+}
+
+// Zero two arrays in one launch.
+__global__ void k_zero2(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+    if (i < na) a[i] = 0u;
+    else b[i - na] = 0u;
+  }
+}
+
 // Stable select of a short list in one block of SMALL_BLOCK threads (one
 // launch instead of three); each thread issues the predicates of
 // SMALL_ITEMS items before ranking them.  With wpre given (the list is the
@@ -1670,13 +1716,16 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *__rest
 // perm_g^-1(j - nstart[g]) with j its rank among the non-trimmed vertices
 // (NZ index): posv[pos] = v and bit pos of pbits set, so compacting pbits
 // in order yields the remaining vertices in P order without materialising
-// or sorting P.  Ranks come from the trim's tile prefix and ballots.
+// or sorting P; each placement also counts in its SEL_TILE tile of
+// positions (pos_tiles), so the compaction needs no counting pass.  Ranks
+// come from the trim's tile prefix and ballots.
 __global__ __launch_bounds__(BLOCK) void k_first_label(
     const uint32_t *__restrict__ trim_bm, uint64_t n, const unsigned long long *tile_off,
     const unsigned long long *voff, uint32_t G, const unsigned long long *zstart,
     const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
-    uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim) {
+    uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim,
+    uint32_t *pos_tiles) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
@@ -1750,6 +1799,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(rr[j] - b0);
     posv[pos] = (uint32_t)i;
     atomicOr(&pbits[pos >> 5], 1u << (pos & 31));
+    atomicAdd(&pos_tiles[pos / SEL_TILE], 1u);   // the compaction's tile counts
   }
 }
 
@@ -3300,27 +3350,45 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     PSC_TRY(pbits.ensure(pwords * 4));
     uint32_t *posv = front.as<uint32_t>();   // front is free until the rounds
     const uint64_t nt = (n + SEL_TILE - 1) / SEL_TILE;
+    // the remaining vertices are counted per tile of positions while they
+    // are placed (the select's counts), into `tiles`, zeroed with the
+    // position bitmap
+    const uint64_t st_tiles = (std::max<uint64_t>(nnz, 1) + SEL_TILE - 1) / SEL_TILE;
     prof_begin(KK_MISC);
-    k_fill_u32<<<grid_for(pwords, BLOCK, 4096), BLOCK, 0, stream>>>(pbits.as<uint32_t>(),
-                                                                   pwords, 0u);
+    k_zero2<<<grid_for(pwords + st_tiles, BLOCK, 4096), BLOCK, 0, stream>>>(
+        pbits.as<uint32_t>(), pwords, tiles.as<uint32_t>(), st_tiles);
     k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
         trim_static.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
         gC.as<unsigned long long>(), posv, pbits.as<uint32_t>(), direct_final ? 1 : 0,
-        first_trim.as<uint32_t>());
+        first_trim.as<uint32_t>(), tiles.as<uint32_t>());
     prof_end();
     PSC_HIP(hipGetLastError());
     if (nnz) {
       // the remaining count stays on the device: the per-graph starts
       // (cstart[G] = the count) come back with the poll
       unsigned long long *rem_d = misc.as<unsigned long long>();
-      PSC_TRY(select(SelBits{pbits.as<uint32_t>(), posv}, nnz, centers.as<uint32_t>(),
-                     nullptr, nullptr, rem_d));
-      prof_begin(KK_MISC);
-      k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
-          centers.as<uint32_t>(), 0, rem_d, voffd, G, nullptr, G,
-          gB.as<unsigned long long>());
+      prof_begin(KK_SELECT);
+      if (st_tiles <= TSCAN_MAX) {
+        k_tile_scan<<<1, TSCAN_BLOCK, 0, stream>>>(tiles.as<uint32_t>(), st_tiles,
+                                                   tile_off.as<unsigned long long>(), rem_d,
+                                                   G == 1 ? gB.as<unsigned long long>() : nullptr);
+      } else {
+        PSC_TRY(scan_tiles(st_tiles, rem_d));
+        if (G == 1) k_segment_starts<<<1, 64, 0, stream>>>(centers.as<uint32_t>(), 0, rem_d, voffd,
+                                                         G, nullptr, G, gB.as<unsigned long long>());
+      }
+      k_sel_scatter<SelBits><<<(unsigned)st_tiles, BLOCK, 0, stream>>>(
+          SelBits{pbits.as<uint32_t>(), posv}, nnz, tile_off.as<unsigned long long>(),
+          centers.as<uint32_t>(), nullptr);
       prof_end();
+      if (G > 1) {
+        prof_begin(KK_MISC);
+        k_segment_starts<<<grid_for(G + 1), BLOCK, 0, stream>>>(
+            centers.as<uint32_t>(), 0, rem_d, voffd, G, nullptr, G,
+            gB.as<unsigned long long>());
+        prof_end();
+      }
       PSC_HIP(hipMemcpyAsync(region(3), gB.ptr, 8 * (G + 1), hipMemcpyDeviceToHost,
                              stream));
     }
