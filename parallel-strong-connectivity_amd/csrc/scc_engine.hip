@@ -122,9 +122,10 @@ __global__ __launch_bounds__(BLOCK) void k_sel_scatter(
 // Exclusive scan of the per-tile counts of a select in one block (counts
 // made elsewhere, e.g. by atomics while the items are placed): tile_off
 // gets the prefixes, *count_dev the total, and with starts1 (a single
-// graph) the graph's starts {0, total}.  nt <= 2^20 (1024 counts a thread).
+// graph) the graph's starts {0, total}.  For nt <= TSCAN_MAX (64 counts a
+// thread); longer count arrays take the library scan.
 constexpr int TSCAN_BLOCK = 1024;
-constexpr uint64_t TSCAN_MAX = (uint64_t)TSCAN_BLOCK * TSCAN_BLOCK;
+constexpr uint64_t TSCAN_MAX = (uint64_t)TSCAN_BLOCK * 64;
 __global__ __launch_bounds__(TSCAN_BLOCK) void k_tile_scan(const uint32_t *cnt, uint64_t nt,
                                                             unsigned long long *tile_off,
                                                             unsigned long long *count_dev,
