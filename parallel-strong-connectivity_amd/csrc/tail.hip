@@ -40,10 +40,11 @@ namespace {
 // Workgroup of a remainder: half of its waves expand the forward search,
 // half the backward one.  Windows are scanned in parts of 32 positions a
 // thread (one bit mask).  A remainder small enough to live in LDS (mode 2
-// below) runs with two waves: its rounds are short, and barriers of two
-// waves cost little.
+// below) runs with four waves (one workgroup per CU, LDS-bound): its rounds
+// are short chains of LDS steps (measured on RMAT 2^25's remainder: 111 us
+// with two waves, 100 with four, 103 with eight, 163 with one).
 constexpr int TAIL_BLOCK = 1024;
-constexpr int TAIL_SMALL_BLOCK = 128;
+constexpr int TAIL_SMALL_BLOCK = 256;
 // Remainder positions (finished vertices included) a graph may have here:
 // every round scans its window's positions.
 constexpr uint64_t TAIL_MAX_POS = 1ull << 20;
