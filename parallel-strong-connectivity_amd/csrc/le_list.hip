@@ -172,6 +172,16 @@ constexpr uint64_t LE_HEAVY = 4096;  // longer lists are spread over the grid
 constexpr uint32_t LE_HSEG = 2048;   // heavy pairs per LDS prefix segment
 static_assert(LE_HSEG % BLOCK == 0, "a block builds the segment prefix in equal shares");
 constexpr uint64_t LE_GROW = 1ull << 28;   // entries a degree bound may add to the buffers
+// The bound above, or PSC_LE_GROW from the environment: a test sets it tiny
+// so that every sub-round takes the counted bound (k_le_count).
+static uint64_t le_grow_limit() {
+  const char *e = std::getenv("PSC_LE_GROW");
+  if (e && *e) {
+    const unsigned long long v = std::strtoull(e, nullptr, 10);
+    if (v > 0) return v;
+  }
+  return LE_GROW;
+}
 
 // the staged pairs of a wave go to the next frontier and to the list at
 // lbase + their frontier slot, with one atomic for all of them
@@ -484,6 +494,7 @@ struct LeRun {
     LE_HIP(front[1].ensure(std::max<uint64_t>(2 * n, 1024) * 8, 0, st));
     LE_HIP(bmin.ensure(n * 4, 0, st));
     uint64_t nlist = 0, finished = 0, step = 1;
+    const uint64_t grow = le_grow_limit();
     while (finished < n) {
       // batch [finished, finished + k) (:264-272; the first batch is the
       // single BFS of :248-255)
@@ -509,7 +520,7 @@ struct LeRun {
         // adds; the buffers grow to it up to LE_GROW entries, past that
         // the arcs that pass the tests are counted for a tight bound
         uint64_t bound = h.deg;
-        if (bound > LE_GROW || nlist + bound > LE_GROW + lkey.cap / 8) {
+        if (bound > grow || nlist + bound > grow + lkey.cap / 8) {
           k_le_count<<<gwave, BLOCK, 0, st>>>(front[cf].as<unsigned long long>(),
                                               ctl.as<LeCtl>(), g->offset, g->edges,
                                               dist.as<uint32_t>(), bmin.as<uint32_t>(), r);

@@ -66,3 +66,16 @@ def test_le_lists_rejects_bad_input(psc):
     g.edges[1] = 9   # endpoint >= n, caught before any kernel indexes with it
     with pytest.raises(psc.PscError):
         psc.le_lists(g, np.arange(3, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("n,m,seed,beta", [(3000, 9000, 4, 1.5), (2000, 30000, 6, 4.0)])
+def test_le_lists_counted_bound(psc, oracle, monkeypatch, n, m, seed, beta):
+    """With the growth limit at one entry, every sub-round sizes its buffers
+    by the counting pass (k_le_count) instead of the degree bound; the rows
+    must not change."""
+    monkeypatch.setenv("PSC_LE_GROW", "1")
+    g = random_digraph(psc, n, m, seed, self_loops=True, dups=True)
+    order = psc.random_order(n, seed)
+    print("le_lists counted bound", n, m, seed, beta, flush=True)
+    got = psc.le_lists(g, order, beta=beta)
+    assert np.array_equal(got, oracle.le_lists(g, order))
