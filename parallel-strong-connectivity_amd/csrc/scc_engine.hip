@@ -801,6 +801,9 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
       }
     }
     wave_sync();
+    // an atomic, not a plain store: the bit reaches the other XCDs' waves
+    // within this level (a plain store stays in this XCD's L2 until the
+    // kernel ends), so later head tests there can hit it
     if (fw[l]) atomicOr(&R.vis[c * WAVE + l], fw[l]);
   }
   const uint32_t wv = wave_sum(n_vis);   // the next level's frontier size
@@ -1279,14 +1282,17 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
 // The vertex range is walked by bitmap: a wave loads 64 words of
 // (visited | trimmed) with one coalesced load (2048 vertices) and lists the
 // chunk's candidates (clear bits) in LDS; chunks with few candidates are
-// listed together (up to PULL_CHUNKS) before both phases run over the list.
+// listed together (up to PULL_CHUNKS).  The words of the wave's next
+// PULL_PF chunks are in flight while the current ones are listed.
 // Phase 1: every candidate tests its head (one in-neighbour of large degree,
 // from the per-vertex head array, coalesced), PULL_HEADS x 64 at a time.
-// Phase 2: the misses, compacted in the list, take PULL_PAIRS x 64 at a time
-// with their loads in flight together: in-offsets, the first PULL_PROBE
-// in-edges, their visited words.  Found vertices set their bits in LDS
-// copies of their chunks' words, flushed with one atomic per word.  Vertices
-// left undecided queue their remaining in-edges in LDS; a full batch of 64
+// Phase 2: the misses gather in a per-wave LDS list across chunks and are
+// taken PULL_PAIRS x 64 at a time (the rest once the wave's range is
+// walked), with their loads in flight together: in-offsets, the first
+// PULL_PROBE in-edges, their visited words.  Vertices found in phase 1 set
+// their bits in LDS copies of their chunks' words, flushed with one atomic
+// per word; phase 2 and the drains set theirs directly.  Vertices left
+// undecided queue their remaining in-edges in LDS; a full batch of 64
 // queued vertices is scanned as load-balanced 64-edge windows.  Remainders
 // longer than PULL_HEAVY go to the heavy list (k_reach_heavy) instead.
 // Head-only levels (pull_defers) are k_reach_step's (head_level).
@@ -1298,6 +1304,12 @@ constexpr int PULL_OUT_CAP = 512;   // per-wave staged frontier entries
 #endif
 constexpr int PULL_HEADS = PSC_PULL_HEADS;   // head tests per lane in flight
 constexpr int PULL_QCAP = WAVE * (PULL_PAIRS + 1);
+// miss list: fewer than one phase-2 batch left over plus one head batch
+constexpr int PULL_MCAP = WAVE * (PULL_PAIRS + PULL_HEADS);
+#ifndef PSC_PULL_PF
+#define PSC_PULL_PF 4
+#endif
+constexpr int PULL_PF = PSC_PULL_PF;   // chunks whose words are loaded ahead
 
 struct PullQueue {   // per-wave LDS queue of undecided vertices
   uint32_t *v;
@@ -1375,6 +1387,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   __shared__ uint32_t qd_s[WPB][PULL_QCAP];
   // listed candidates: chunk slot << 11 | offset in the chunk
   __shared__ uint16_t cand_s[WPB][WAVE * 32];
+  __shared__ uint32_t miss_s[WPB][PULL_MCAP];          // phase-2 vertices
   __shared__ uint32_t fw_s[WPB][PULL_CHUNKS * WAVE];   // found bits per listed chunk
   __shared__ uint64_t ck_s[WPB][PULL_CHUNKS];          // the listed chunks
   // per listed chunk: its trimmed words, the non-trimmed vertices before
@@ -1382,10 +1395,18 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
   __shared__ uint32_t tw_s[WPB][PULL_CHUNKS * WAVE];
   __shared__ uint16_t np_s[WPB][PULL_CHUNKS * WAVE];
   __shared__ unsigned long long nb_s[WPB][PULL_CHUNKS];
+  // the group of PULL_PF chunks being listed: skip and trimmed words, and
+  // the non-trimmed vertices before each chunk
+  __shared__ uint32_t gs_s[WPB][PULL_PF * WAVE];
+  __shared__ uint32_t gt_s[WPB][PULL_PF * WAVE];
+  __shared__ unsigned long long gn_s[WPB][PULL_PF];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *sc = sc_s[w], *fl = fl_s[w];
   uint64_t *st = st_s[w];
   uint16_t *cand = cand_s[w];
+  uint32_t *ml = miss_s[w];
+  uint32_t *gs = gs_s[w], *gt = gt_s[w];
+  unsigned long long *gn = gn_s[w];
   uint32_t *fw = fw_s[w];
   uint64_t *ck = ck_s[w];
   uint32_t *tw = tw_s[w];
@@ -1411,16 +1432,79 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
+    const bool own_trim = R.tstat == R.trim;
     unsigned long long n_edges = 0, n_scan = 0, n_vis = 0, n_chunks = 0;
-    uint32_t ncand = 0, nck = 0;
-    // Both phases over the listed candidates (of up to PULL_CHUNKS chunks),
-    // then the found bits of the chunks' words go out, one atomic per word.
+    uint32_t ncand = 0, nck = 0, nmiss = 0;
+    // phase 2 over the first cnt (at most WAVE * PULL_PAIRS) misses of the
+    // list: in-offsets, then the first PULL_PROBE in-edges, then their
+    // visited words; the rest of the list moves to the front
+    auto probe = [&](uint32_t cnt) {
+      uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
+      bool todo[PULL_PAIRS];
+#pragma unroll
+      for (int j = 0; j < PULL_PAIRS; ++j) {
+        const uint32_t idx = j * WAVE + l;
+        todo[j] = idx < cnt;
+        v[j] = todo[j] ? ml[idx] : 0u;
+        b[j] = todo[j] ? R.roff[v[j]] : 0ull;
+        deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
+      }
+      uint32_t nv[PULL_PAIRS][PULL_PROBE];
+#pragma unroll
+      for (int j = 0; j < PULL_PAIRS; ++j) {
+        deg[j] -= b[j];
+        n_scan += todo[j];
+#pragma unroll
+        for (uint32_t t = 0; t < PULL_PROBE; ++t)
+          nv[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
+      }
+      uint32_t wd[PULL_PAIRS][PULL_PROBE];
+#pragma unroll
+      for (int j = 0; j < PULL_PAIRS; ++j)
+#pragma unroll
+        for (uint32_t t = 0; t < PULL_PROBE; ++t)
+          wd[j][t] = t < deg[j] ? R.vis[nv[j][t] >> 5] : 0u;
+#pragma unroll
+      for (int j = 0; j < PULL_PAIRS; ++j) {
+        bool found = false;
+#pragma unroll
+        for (uint32_t t = 0; t < PULL_PROBE; ++t)
+          found |= t < deg[j] && ((wd[j][t] >> (nv[j][t] & 31)) & 1u);
+        const uint64_t k = min(deg[j], (uint64_t)PULL_PROBE);
+        n_edges += k;
+        const uint64_t rem = found ? 0 : deg[j] - k;
+        // long remainders go to the heavy list (k_reach_heavy, chunked),
+        // the rest to the wave's queue
+        const bool heavy = rem > PULL_HEAVY;
+        heavy_append(R, cur, heavy, (uint32_t)v[j], rem);
+        const bool qd = rem > 0 && !heavy;
+        const uint64_t qm = ballot(qd);
+        if (qd) {
+          const uint32_t at = q.n + lanes_below(qm);
+          q.v[at] = (uint32_t)v[j];
+          q.b[at] = b[j] + k;
+          q.deg[at] = (uint32_t)rem;
+        }
+        q.n += (uint32_t)__popcll(qm);
+        // the wave owns v in this level: no other lane sets its bit
+        if (found) atomicOr(&R.vis[v[j] >> 5], 1u << (v[j] & 31));
+        n_vis += found;
+        out.put(found, (uint32_t)v[j], &nxt->qsize, qn, qcap);
+      }
+      wave_sync();
+      const uint32_t rest = nmiss - cnt;
+      for (uint32_t i = l; i < rest; i += WAVE) ml[i] = ml[cnt + i];
+      nmiss = rest;
+      wave_sync();
+      while (q.n >= WAVE)
+        pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
+    };
+    // Phase 1 over the listed candidates (of up to PULL_CHUNKS chunks), the
+    // misses to the miss list, then the found bits of the chunks' words go
+    // out, one atomic per word.
     auto process = [&]() {
-      // phase 1: every candidate tests its head, PULL_HEADS per lane at a
-      // time (two dependent loads); the misses are compacted to the front
-      // of the list (the entries a batch writes lie at or below the
-      // batch's own, which are already in registers)
-      uint32_t nmiss = 0;
+      // every candidate tests its head, PULL_HEADS per lane at a time (two
+      // dependent loads); a full batch of misses goes through phase 2
       for (uint32_t cb = 0; cb < ncand; cb += WAVE * PULL_HEADS) {
         uint32_t e[PULL_HEADS], hd[PULL_HEADS], hw[PULL_HEADS];
         uint64_t vx[PULL_HEADS];
@@ -1438,7 +1522,6 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         }
 #pragma unroll
         for (int j = 0; j < PULL_HEADS; ++j) hw[j] = todo[j] ? R.vis[hd[j] >> 5] : 0u;
-        wave_sync();
 #pragma unroll
         for (int j = 0; j < PULL_HEADS; ++j) {
           // v's own bit is clear, so an empty list's head (v) misses
@@ -1446,77 +1529,19 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           const bool miss = todo[j] && !hit;
           n_edges += todo[j];
           const uint64_t mm = ballot(miss);
-          if (miss) cand[nmiss + lanes_below(mm)] = (uint16_t)e[j];
+          if (miss) ml[nmiss + lanes_below(mm)] = (uint32_t)vx[j];
           nmiss += (uint32_t)__popcll(mm);
           // found bits gather in the chunks' LDS words
           if (hit) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += hit;
           out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, qcap);
         }
-      }
-      wave_sync();
-      // phase 2: the misses load their offsets and probe in-edges
-      // 1..PULL_PROBE, PULL_PAIRS per lane
-      for (uint32_t cb = 0; cb < nmiss; cb += WAVE * PULL_PAIRS) {
-        uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
-        uint32_t e[PULL_PAIRS];
-        bool todo[PULL_PAIRS];
-#pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j) {
-          const uint32_t idx = cb + j * WAVE + l;
-          todo[j] = idx < nmiss;
-          e[j] = todo[j] ? cand[idx] : 0u;
-          v[j] = ck[e[j] >> 11] * (uint64_t)(WAVE * 32) + (e[j] & 2047u);
-          b[j] = todo[j] ? R.roff[v[j]] : 0ull;
-          deg[j] = todo[j] ? R.roff[v[j] + 1] : 0ull;
-        }
-        uint32_t nb[PULL_PAIRS][PULL_PROBE];
-#pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j) {
-          deg[j] -= b[j];
-          n_scan += todo[j];
-#pragma unroll
-          for (uint32_t t = 0; t < PULL_PROBE; ++t)
-            nb[j][t] = t < deg[j] ? R.radj[b[j] + t] : 0u;
-        }
-        uint32_t wd[PULL_PAIRS][PULL_PROBE];
-#pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j)
-#pragma unroll
-          for (uint32_t t = 0; t < PULL_PROBE; ++t)
-            wd[j][t] = t < deg[j] ? R.vis[nb[j][t] >> 5] : 0u;
-#pragma unroll
-        for (int j = 0; j < PULL_PAIRS; ++j) {
-          bool found = false;
-#pragma unroll
-          for (uint32_t t = 0; t < PULL_PROBE; ++t)
-            found |= t < deg[j] && ((wd[j][t] >> (nb[j][t] & 31)) & 1u);
-          const uint64_t k = min(deg[j], (uint64_t)PULL_PROBE);
-          n_edges += k;
-          const uint64_t rem = found ? 0 : deg[j] - k;
-          // long remainders go to the heavy list (k_reach_heavy, chunked),
-          // the rest to the wave's queue
-          const bool heavy = rem > PULL_HEAVY;
-          heavy_append(R, cur, heavy, (uint32_t)v[j], rem);
-          const bool qd = rem > 0 && !heavy;
-          const uint64_t qm = ballot(qd);
-          if (qd) {
-            const uint32_t at = q.n + lanes_below(qm);
-            q.v[at] = (uint32_t)v[j];
-            q.b[at] = b[j] + k;
-            q.deg[at] = (uint32_t)rem;
-          }
-          q.n += (uint32_t)__popcll(qm);
-          if (found) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
-          n_vis += found;
-          out.put(found, (uint32_t)v[j], &nxt->qsize, qn, qcap);
-        }
         wave_sync();
-        while (q.n >= WAVE)
-          pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
+        while (nmiss >= WAVE * PULL_PAIRS) probe(WAVE * PULL_PAIRS);
       }
       wave_sync();
       for (uint32_t k = 0; k < nck; ++k) {
+        // atomics make the bits visible to the other XCDs within the level
         const uint32_t f = fw[k * WAVE + l];
         if (f) atomicOr(&R.vis[ck[k] * WAVE + l], f);
       }
@@ -1524,60 +1549,82 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       nck = 0;
       wave_sync();
     };
-    // (visited | trimmed) word of this lane in chunk c; the next chunk's is
-    // loaded before a chunk is listed (in a pull level a vertex's bit is
-    // set only by its own processing, so an early load reads the same word)
-    // (the trimmed word comes with it: the skip set of a first-reach level,
-    // a separate load in a single-pivot round)
-    const bool own_trim = R.tstat == R.trim;
-    auto skip_word = [&](uint64_t c, uint32_t *ts) -> uint32_t {
+    // This lane's (visited | skipped) word of chunk c, its trimmed word,
+    // and the non-trimmed vertices before the chunk.  The words of the next
+    // PULL_PF chunks are loaded before a group of chunks is listed (in a
+    // pull level a vertex's bit is set only by its own processing, so an
+    // early load reads the same word).  The skip set is the trimmed set in
+    // a first-reach level, another set in a single-pivot round.
+    auto skip_word = [&](uint64_t c, uint32_t &ts, unsigned long long &nz) -> uint32_t {
       const uint64_t wi = c * WAVE + l;
       uint32_t sk = ~0u;
-      *ts = ~0u;
-      if (c < chunks && wi < words) {
-        const uint32_t tr = R.trim[wi];
-        *ts = own_trim ? tr : R.tstat[wi];
-        sk = R.vis[wi] | tr;
-        if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
+      ts = ~0u;
+      nz = 0;
+      if (c < chunks) {
+        nz = R.nzoff[c];
+        if (wi < words) {
+          const uint32_t tr = R.trim[wi];
+          ts = own_trim ? tr : R.tstat[wi];
+          sk = R.vis[wi] | tr;
+          if (wi == words - 1 && tail_bits) sk |= ~((1u << tail_bits) - 1);
+        }
       }
       return sk;
     };
-    uint32_t ts_next;
-    uint32_t sk_next = skip_word(gw, &ts_next);
-    for (uint64_t c = gw; c < chunks; c += nw) {
-      const uint32_t sk = sk_next, ts = ts_next;
-      sk_next = skip_word(c + nw, &ts_next);
-      ++n_chunks;
-      // the chunk's candidates (clear bits), appended to the list in order;
-      // chunks with few candidates share one pass of both phases
-      const uint32_t todo_w = ~sk;
-      const uint32_t cnt = (uint32_t)__popc(todo_w);
-      const uint32_t inc = wave_incl_scan(cnt);
-      const uint32_t total = __shfl(inc, WAVE - 1);
-      if (total == 0) continue;
-      if (ncand + total > WAVE * 32 || nck == PULL_CHUNKS) process();
+    uint32_t pf_sk[PULL_PF], pf_ts[PULL_PF];
+    unsigned long long pf_nz[PULL_PF];
+#pragma unroll
+    for (int i = 0; i < PULL_PF; ++i) pf_sk[i] = skip_word(gw + (uint64_t)i * nw, pf_ts[i], pf_nz[i]);
+    for (uint64_t c0 = gw; c0 < chunks; c0 += (uint64_t)PULL_PF * nw) {
+      // this group's words to LDS, the next group's loads issued
+#pragma unroll
+      for (int i = 0; i < PULL_PF; ++i) {
+        gs[i * WAVE + l] = pf_sk[i];
+        gt[i * WAVE + l] = pf_ts[i];
+        if (l == 0) gn[i] = pf_nz[i];
+      }
+#pragma unroll
+      for (int i = 0; i < PULL_PF; ++i)
+        pf_sk[i] = skip_word(c0 + (uint64_t)(PULL_PF + i) * nw, pf_ts[i], pf_nz[i]);
       wave_sync();
-      {
-        uint32_t at = ncand + inc - cnt;
-        for (uint32_t tt = todo_w; tt; tt &= tt - 1)
-          cand[at++] = (uint16_t)((nck << 11) | (l * 32 + (__ffs(tt) - 1)));
+      for (int i = 0; i < PULL_PF; ++i) {
+        const uint64_t c = c0 + (uint64_t)i * nw;
+        if (c >= chunks) break;
+        const uint32_t sk = gs[i * WAVE + l], ts = gt[i * WAVE + l];
+        ++n_chunks;
+        // the chunk's candidates (clear bits), appended to the list in
+        // order; chunks with few candidates share one pass of phase 1
+        const uint32_t todo_w = ~sk;
+        const uint32_t cnt = (uint32_t)__popc(todo_w);
+        const uint32_t inc = wave_incl_scan(cnt);
+        const uint32_t total = __shfl(inc, WAVE - 1);
+        if (total == 0) continue;
+        if (ncand + total > WAVE * 32 || nck == PULL_CHUNKS) process();
+        wave_sync();
+        {
+          uint32_t at = ncand + inc - cnt;
+          for (uint32_t tt = todo_w; tt; tt &= tt - 1)
+            cand[at++] = (uint16_t)((nck << 11) | (l * 32 + (__ffs(tt) - 1)));
+        }
+        fw[nck * WAVE + l] = 0;
+        tw[nck * WAVE + l] = ts;
+        {
+          const uint32_t nz = (uint32_t)__popc(~ts);
+          np[nck * WAVE + l] = (uint16_t)(wave_incl_scan(nz) - nz);
+        }
+        if (l == 0) {
+          ck[nck] = c;
+          nb[nck] = gn[i];
+        }
+        ncand += total;
+        ++nck;
+        wave_sync();
+        if (ncand >= WAVE * PULL_HEADS) process();
       }
-      fw[nck * WAVE + l] = 0;
-      tw[nck * WAVE + l] = ts;
-      {
-        const uint32_t nz = (uint32_t)__popc(~ts);
-        np[nck * WAVE + l] = (uint16_t)(wave_incl_scan(nz) - nz);
-      }
-      if (l == 0) {
-        ck[nck] = c;
-        nb[nck] = R.nzoff[c];
-      }
-      ncand += total;
-      ++nck;
       wave_sync();
-      if (ncand >= WAVE * PULL_HEADS) process();
     }
     if (ncand) process();
+    if (nmiss) probe(nmiss);
     while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
     out.flush(&nxt->qsize, qn, qcap);
     out.finish(&nxt->qsize);
