@@ -79,6 +79,9 @@ struct ReachSlot {
   // nonzero: the previous (pull) level left in-list remainders to this one,
   // which pulls, completely, whatever its frontier
   uint32_t deferred;
+  // nonzero: the previous (pull) level counted its frontier without storing
+  // it, so this level pulls if that frontier is not empty
+  uint32_t unstored;
   unsigned long long cum_vis;  // vertices visited before this level
 };
 
