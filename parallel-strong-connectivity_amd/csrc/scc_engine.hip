@@ -709,6 +709,8 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // (1024 entries each) and `aux` (WAVE words) are the wave's own LDS.  Every
 // thread of the block calls it (block-wide counter reduction).
 constexpr int HEAD_STEPS = 4;
+constexpr uint64_t RING_NO_OFF = ~0ull;
+static_assert(VGC_MAX_DEG < 16, "a queued degree fits 4 bits");
 constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
 
 __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
@@ -828,7 +830,9 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
 // binary search over the wave's inclusive degree scan).  Newly visited
 // sparse vertices stay in the queue while the VGC budget (tau per seed)
 // lasts and the ring has room; the rest are staged in LDS and spilled to
-// the next frontier in bulk.  Vertices with more than HEAVY_DEG edges go to
+// the next frontier in bulk.  A local search is a chain of dependent
+// steps, so each step keeps few memory round trips: a claim's offsets are
+// loaded with its atomic, and a queued vertex keeps them in the ring.  Vertices with more than HEAVY_DEG edges go to
 // the heavy list (k_reach_heavy).
 // Pull levels (src/reach.hpp:173-190) run in k_reach_pull, except the
 // head-only ones (pull_defers), which run here (head_level): their waves
@@ -837,11 +841,15 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
     const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
+  // per ring entry: (first edge << 4) | degree of a vertex queued by a
+  // claim, RING_NO_OFF for one taken from the frontier
+  __shared__ uint64_t ringo_s[WPB][REACH_RING];
   __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *ring = ring_s[w];
+  uint64_t *ringo = ringo_s[w];
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   const int ci = L % 3, ni = (L + 1) % 3, zi = (L + 2) % 3;
@@ -880,7 +888,10 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
       uint32_t nbatch = 0;
       while (g < gend) {
         const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
-        if ((uint32_t)l < cnt) ring[l] = (p ? R.q[1] : R.q[0])[g + l];
+        if ((uint32_t)l < cnt) {
+          ring[l] = (p ? R.q[1] : R.q[0])[g + l];
+          ringo[l] = RING_NO_OFF;
+        }
         g += cnt;
         uint32_t head = 0, tail = cnt;
         uint64_t budget = min((uint64_t)tau * cnt, wbud);
@@ -893,12 +904,19 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
           if (++nbatch > (uint32_t)vgc) budget = 0;
           const uint32_t k = min(64u, tail - head);
           const bool valid = (uint32_t)l < k;
-          const uint32_t u = valid ? ring[(head + l) & (REACH_RING - 1)] : 0u;
+          const uint32_t slot = (head + l) & (REACH_RING - 1);
+          const uint32_t u = valid ? ring[slot] : 0u;
+          const uint64_t ro = valid ? ringo[slot] : 0ull;
           head += k;
           uint64_t b = 0, deg = 0;
           if (valid) {
-            b = R.off[u];
-            deg = R.off[u + 1] - b;
+            if (ro == RING_NO_OFF) {
+              b = R.off[u];
+              deg = R.off[u + 1] - b;
+            } else {
+              b = ro >> 4;
+              deg = ro & 15u;
+            }
             ++n_exp;
           }
           const bool heavy = deg > HEAVY_DEG;
@@ -941,14 +959,20 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               w[x] = found[x] ? R.vis[v[x] >> 5] : 0u;
               if (found[x] && R.excl) w[x] |= R.excl[v[x] >> 5];
             }
+            // an unvisited target's offsets are loaded with its claim
+            uint64_t ob[PUSH_WIN], dv[PUSH_WIN];
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) {
+              found[x] = found[x] && !((w[x] >> (v[x] & 31)) & 1u);
+              ob[x] = found[x] ? R.off[v[x]] : 0;
+              dv[x] = found[x] ? R.off[v[x] + 1] : 0;
+            }
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
               const uint32_t bit = 1u << (v[x] & 31);
-              found[x] = found[x] && !(w[x] & bit) && !(atomicOr(&R.vis[v[x] >> 5], bit) & bit);
+              found[x] = found[x] && !(atomicOr(&R.vis[v[x] >> 5], bit) & bit);
+              dv[x] = found[x] ? dv[x] - ob[x] : 0;
             }
-            uint64_t dv[PUSH_WIN];
-#pragma unroll
-            for (int x = 0; x < PUSH_WIN; ++x) dv[x] = found[x] ? R.off[v[x] + 1] - R.off[v[x]] : 0;
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
               const uint64_t fm = ballot(found[x]);
@@ -962,7 +986,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               const uint32_t room = REACH_RING - (tail - head);
               const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
               const bool loc = want && lanes_below(lm) < take;
-              if (loc) ring[(tail + lanes_below(lm)) & (REACH_RING - 1)] = v[x];
+              if (loc) {
+                const uint32_t at = (tail + lanes_below(lm)) & (REACH_RING - 1);
+                ring[at] = v[x];
+                ringo[at] = (ob[x] << 4) | dv[x];
+              }
               tail += take;
               budget -= take;
               const bool glob = found[x] && !loc;
