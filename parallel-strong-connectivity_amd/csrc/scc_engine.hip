@@ -709,9 +709,15 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // (1024 entries each) and `aux` (WAVE words) are the wave's own LDS.  Every
 // thread of the block calls it (block-wide counter reduction).
 constexpr int HEAD_STEPS = 4;
+constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
 constexpr uint64_t RING_NO_OFF = ~0ull;
 static_assert(VGC_MAX_DEG < 16, "a queued degree fits 4 bits");
-constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
+constexpr int CLAIM_CACHE = 256;            // per-wave claim cache entries
+constexpr uint32_t CLAIM_EMPTY = ~0u;       // no vertex has this id
+__device__ __forceinline__ uint32_t claim_slot(uint32_t v) {
+  return (v * 0x9E3779B1u) >> (32 - 8);
+}
+static_assert(CLAIM_CACHE == 256, "claim_slot yields 8 bits");
 
 __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
                                            uint16_t *map_lo, uint16_t *map_hi,
@@ -847,9 +853,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
   __shared__ uint32_t out_s[WPB][OUT_CAP];
   __shared__ uint32_t sc_s[WPB][WAVE];
   __shared__ uint64_t st_s[WPB][WAVE];
+  __shared__ uint32_t cc_s[WPB][CLAIM_CACHE];   // the wave's recent claims
   const int w = threadIdx.x / WAVE, l = lane();
   uint32_t *ring = ring_s[w];
   uint64_t *ringo = ringo_s[w];
+  uint32_t *cc = cc_s[w];
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   const int ci = L % 3, ni = (L + 1) % 3, zi = (L + 2) % 3;
@@ -878,6 +886,8 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
     if (gw == 0 && l == 0 && cur->cum_vis) atomicAdd(&nxt->cum_vis, cur->cum_vis);
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
     if (mode == MODE_PUSH) {
+      for (int i = l; i < CLAIM_CACHE; i += WAVE) cc[i] = CLAIM_EMPTY;
+      wave_sync();
       // contiguous share of the frontier, S seeds at a time
       const uint64_t per = (nf + (uint64_t)total_waves - 1) / total_waves;
       const uint32_t sd = (uint32_t)min((uint64_t)64, per);
@@ -959,7 +969,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               w[x] = found[x] ? R.vis[v[x] >> 5] : 0u;
               if (found[x] && R.excl) w[x] |= R.excl[v[x] >> 5];
             }
-            // an unvisited target's offsets are loaded with its claim
+            // an unvisited target's offsets are loaded before its claim
             uint64_t ob[PUSH_WIN], dv[PUSH_WIN];
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
@@ -967,17 +977,18 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               ob[x] = found[x] ? R.off[v[x]] : 0;
               dv[x] = found[x] ? R.off[v[x] + 1] : 0;
             }
+            // the wave's claims of this launch by exact vertex id: a target
+            // that another lane or an earlier step of the wave took is
+            // dropped (an evicted entry only costs a repeated expansion)
 #pragma unroll
-            for (int x = 0; x < PUSH_WIN; ++x) {
-              const uint32_t bit = 1u << (v[x] & 31);
-              found[x] = found[x] && !(atomicOr(&R.vis[v[x] >> 5], bit) & bit);
-              dv[x] = found[x] ? dv[x] - ob[x] : 0;
-            }
+            for (int x = 0; x < PUSH_WIN; ++x)
+              if (found[x]) found[x] = atomicExch(&cc[claim_slot(v[x])], v[x]) != v[x];
+#pragma unroll
+            for (int x = 0; x < PUSH_WIN; ++x) dv[x] = found[x] ? dv[x] - ob[x] : 0;
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
               const uint64_t fm = ballot(found[x]);
               if (!fm) continue;
-              n_vis += found[x];
               // VGC (src/reach.hpp:125-154): sparse new vertices stay in the
               // wave's queue while the budget lasts and the ring has room
               const bool want = found[x] && vgc && dv[x] <= VGC_MAX_DEG;
@@ -986,14 +997,21 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               const uint32_t room = REACH_RING - (tail - head);
               const uint32_t take = (uint32_t)min((uint64_t)min(nwant, room), budget);
               const bool loc = want && lanes_below(lm) < take;
+              const uint32_t bit = 1u << (v[x] & 31);
+              // a local claim sets the bit without waiting for the old word
+              // (two waves may both expand a vertex they raced for); a claim
+              // for the next frontier is exact, so its entries stay unique
               if (loc) {
+                atomicOr(&R.vis[v[x] >> 5], bit);
                 const uint32_t at = (tail + lanes_below(lm)) & (REACH_RING - 1);
                 ring[at] = v[x];
                 ringo[at] = (ob[x] << 4) | dv[x];
               }
               tail += take;
               budget -= take;
-              const bool glob = found[x] && !loc;
+              bool glob = found[x] && !loc;
+              if (glob) glob = !(atomicOr(&R.vis[v[x] >> 5], bit) & bit);
+              n_vis += loc || glob;
               if (glob) {
                 dsum += dv[x];
                 ++n_app;
