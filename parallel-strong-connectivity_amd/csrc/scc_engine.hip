@@ -595,7 +595,7 @@ __device__ __forceinline__ int reach_mode(const ReachSlot *c,
   if (c->deferred) return MODE_PULL;
   const uint32_t nf = c->qsize;
   if (nf == 0) return MODE_NONE;
-  if (c->unstored) return MODE_PULL;
+  if (c->from_pull == PULL_UNSTORED) return MODE_PULL;
   unsigned long long ds = c->degsum;
   if (c->from_pull) ds = (unsigned long long)((double)nf * R.avg_deg);
   const uint64_t push = (uint64_t)nf + ds;
@@ -620,11 +620,11 @@ __device__ __forceinline__ bool pull_defers(const ReachSlot *c, const ReachSearc
 // rounding of the degree estimate), and a pull level never reads its
 // frontier: the entries past the cap are counted but not stored.  The
 // complete pull level after a head-only one stores none if it starts with
-// more unvisited vertices than the cap (`unstored`: the next level pulls
-// unless nothing was found): most of them are found there, by their heads,
-// so the next level would pull anyway.  Only that level: a level forced to
-// pull by `unstored` stores its frontier, so a search that finds few
-// vertices per level returns to push levels.
+// more unvisited vertices than the cap (from_pull = PULL_UNSTORED: the
+// next level pulls unless nothing was found): most of them are found
+// there, by their heads, so the next level would pull anyway.  Only that
+// level: a level forced to pull that way stores its frontier, so a search
+// that finds few vertices per level returns to push levels.
 __device__ __forceinline__ uint64_t pull_store_cap(const ReachSearch &R) {
   const double t = ceil((double)R.thresh / (1.0 + R.avg_deg)) + 1.0;
   return t < (double)R.n ? (uint64_t)t : (uint64_t)R.n;
@@ -1482,10 +1482,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const uint64_t qcap = pull_store_cap(R);
     const uint64_t unv = R.nnz > cur->cum_vis ? R.nnz - cur->cum_vis : 0;
     const bool unstored = cur->deferred && unv > qcap;
-    if (gw == 0 && l == 0) {   // first wave of this search
-      nxt->from_pull = 1;
-      if (unstored) nxt->unstored = 1;
-    }
+    if (gw == 0 && l == 0) nxt->from_pull = unstored ? PULL_UNSTORED : 1;   // first wave
     out.init(out_s[w], true);   // read only up to pull_store_cap entries
     out.past = unstored;        // or counted only
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
@@ -3123,7 +3120,7 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
         const double ds = c.from_pull ? (double)c.qsize * rs[s].avg_deg : (double)c.degsum;
         const double push = (double)c.qsize + ds;
         const double unv = nnz_total > c.cum_vis ? (double)(nnz_total - c.cum_vis) : 0.0;
-        const bool pull = c.unstored || push >= (double)thresh ||
+        const bool pull = c.from_pull == PULL_UNSTORED || push >= (double)thresh ||
                           unv * PULL_VERTEX_COST + (double)(n / PULL_WALK_DIV) < push;
         const char *mode = c.qsize == 0 ? "none" : (pull ? "pull" : "push");
         std::fprintf(stderr,

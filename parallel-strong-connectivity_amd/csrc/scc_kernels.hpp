@@ -69,19 +69,19 @@ struct ReachSearch {
   uint64_t nnz;          // non-trimmed vertices (all graphs)
 };
 
+constexpr uint32_t PULL_UNSTORED = 2;
 struct ReachSlot {
   uint32_t qsize;            // frontier size of this level
   uint32_t grab;             // seeds handed out
   unsigned long long degsum; // out-degree sum of the frontier
   unsigned long long heavy;  // (count << 40) | chunks
   unsigned long long hgrab;  // heavy chunks handed out
-  uint32_t from_pull;        // frontier produced by a pull level
+  // frontier produced by a pull level: 1 stored (up to the cap), 2
+  // counted only (PULL_UNSTORED), so this level pulls unless it is empty
+  uint32_t from_pull;
   // nonzero: the previous (pull) level left in-list remainders to this one,
   // which pulls, completely, whatever its frontier
   uint32_t deferred;
-  // nonzero: the previous (pull) level counted its frontier without storing
-  // it, so this level pulls if that frontier is not empty
-  uint32_t unstored;
   unsigned long long cum_vis;  // vertices visited before this level
 };
 
