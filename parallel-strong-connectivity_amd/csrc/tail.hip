@@ -419,15 +419,39 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
         const uint32_t wd = tid * WPT + k;
         if (wd >= nw) break;
         lg_rp[wd] = (uint16_t)d;
-        for (uint32_t m = lg_sw[wd]; m; m &= m - 1, ++d) {
-          const uint32_t q = 32 * wd + (uint32_t)(__ffs(m) - 1);
-          const uint64_t p = base + q;
-          lg_pos[d] = q;
-          lg_lab[d] = A.tlab[p];
-          lg_off[d] = A.toff[p + 1] - A.toff[p];
-          lg_off[TAIL_LG_VERTS + 1 + d] = A.tioff[p + 1] - A.tioff[p];
+        for (uint32_t m = lg_sw[wd]; m; m &= m - 1, ++d)
+          lg_pos[d] = 32 * wd + (uint32_t)(__ffs(m) - 1);
+      }
+    __syncthreads();
+    // then a dense id per thread, the loads of all of a thread's ids in
+    // flight together; the offsets stay in registers for the lists
+    uint32_t o0[VPT], o1[VPT], i0[VPT], i1[VPT];
+    if (!lg_fail) {
+      unsigned long long lb[VPT];
+#pragma unroll
+      for (uint32_t k = 0; k < VPT; ++k) {
+        const uint32_t dd = tid + k * NB;
+        lb[k] = 0;
+        o0[k] = o1[k] = i0[k] = i1[k] = 0;
+        if (dd < lg_n) {
+          const uint64_t p = base + lg_pos[dd];
+          lb[k] = A.tlab[p];
+          o0[k] = A.toff[p];
+          o1[k] = A.toff[p + 1];
+          i0[k] = A.tioff[p];
+          i1[k] = A.tioff[p + 1];
         }
       }
+#pragma unroll
+      for (uint32_t k = 0; k < VPT; ++k) {
+        const uint32_t dd = tid + k * NB;
+        if (dd < lg_n) {
+          lg_lab[dd] = lb[k];
+          lg_off[dd] = o1[k] - o0[k];
+          lg_off[TAIL_LG_VERTS + 1 + dd] = i1[k] - i0[k];
+        }
+      }
+    }
     __syncthreads();
     if (!lg_fail) {
       uint32_t so = 0, si = 0;
@@ -462,14 +486,14 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
       __syncthreads();
       // every compact neighbour is special
       if (!lg_fail)
-        for (uint32_t dd = tid; dd < lg_n; dd += NB) {
-          const uint64_t p = base + lg_pos[dd];
-          for (int dir = 0; dir < 2; ++dir) {
-            const uint32_t *o = dir ? A.tioff : A.toff;
-            const uint32_t *a = dir ? A.tiadj : A.tadj;
-            uint32_t at = lg_off[dir * (TAIL_LG_VERTS + 1) + dd];
-            for (uint32_t e = o[p]; e < o[p + 1]; ++e) lg_adj[at++] = rank(a[e] - base);
-          }
+#pragma unroll
+        for (uint32_t k = 0; k < VPT; ++k) {
+          const uint32_t dd = tid + k * NB;
+          if (dd >= lg_n) break;
+          uint32_t at = lg_off[dd];
+          for (uint32_t e = o0[k]; e < o1[k]; ++e) lg_adj[at++] = rank(A.tadj[e] - base);
+          at = lg_off[TAIL_LG_VERTS + 1 + dd];
+          for (uint32_t e = i0[k]; e < i1[k]; ++e) lg_adj[at++] = rank(A.tiadj[e] - base);
         }
       __syncthreads();
     }
@@ -820,21 +844,42 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
     }
     if (tid == 0) pdx[dcur - dc0] = tot;
     __syncthreads();
-    // a bitmap word of positions per thread: its specials' flags as a mask
-    // (flags are set only for specials in [c0, cur)), one prefix read
+    // per bitmap word of positions (a thread each): its specials' flags as
+    // a mask (flags are set only for specials in [c0, cur)) and the flags
+    // before the word, in the first table's LDS (no longer read)
+    uint32_t *const pdm_w = reinterpret_cast<uint32_t *>(lt_tab);
+    uint32_t *const pd0_w = pdm_w + TAIL_LG_POS / 32;
+    static_assert(2 * (TAIL_LG_POS / 32) * 4 <= 2 * TAIL_LDS_CAP * 8, "word flags fit the tables");
     const uint64_t w_end = cur > c0 ? (cur + 31) / 32 : 0;
     for (uint64_t w = c0 / 32 + tid; w < w_end; w += NB) {
       const uint32_t bits = lg_sw[w], r0 = lg_rp[w];
       uint32_t pdm = 0, d = r0;
       for (uint32_t m = bits; m; m &= m - 1, ++d)
         if (lg_pos[d] >> 31) pdm |= m & (~m + 1u);
-      const uint32_t pd0 = r0 >= dc0 ? pdx[r0 - dc0] : 0u;
-      const uint64_t q0 = 32 * w;
-      for (uint32_t b = 0; b < 32; ++b) {
-        const uint64_t q = q0 + b;
-        if (q < c0 || q >= cur || ((bits >> b) & 1u)) continue;
-        const uint32_t pd = pd0 + (uint32_t)__popc(pdm & ((1u << b) - 1u));
-        A.tlab[base + q] = (unsigned long long)(loff0 + (q - c0) - pd) | TOP_BIT;
+      pdm_w[w] = pdm;
+      pd0_w[w] = r0 >= dc0 ? pdx[r0 - dc0] : 0u;
+    }
+    __syncthreads();
+    // then a position per thread, so the label stores are coalesced, and
+    // EPI_STEPS positions' LDS reads in flight together
+    constexpr uint32_t EPI_STEPS = 8;
+    for (uint64_t q0 = c0 + tid; q0 < cur; q0 += NB * EPI_STEPS) {
+      uint32_t sw[EPI_STEPS], pd[EPI_STEPS], pm[EPI_STEPS];
+#pragma unroll
+      for (uint32_t u = 0; u < EPI_STEPS; ++u) {
+        const uint64_t q = q0 + u * NB;
+        const uint32_t w = q < cur ? (uint32_t)(q >> 5) : 0u;
+        sw[u] = lg_sw[w];
+        pd[u] = pd0_w[w];
+        pm[u] = pdm_w[w];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < EPI_STEPS; ++u) {
+        const uint64_t q = q0 + u * NB;
+        const uint32_t b = (uint32_t)(q & 31);
+        if (q >= cur || ((sw[u] >> b) & 1u)) continue;
+        const uint32_t p = pd[u] + (uint32_t)__popc(pm[u] & ((1u << b) - 1u));
+        A.tlab[base + q] = (unsigned long long)(loff0 + (q - c0) - p) | TOP_BIT;
       }
     }
     // the special positions' labels as of the last round
