@@ -952,7 +952,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               found[x] = e < total;
               v[x] = 0;
               if (found[x]) {
-                int lo = 0, hi = WAVE - 1;
+                int lo = 0, hi = (int)k - 1;   // lanes >= k hold no list
                 while (lo < hi) {
                   const int mid = (lo + hi) >> 1;
                   if (sc[mid] > e) hi = mid; else lo = mid + 1;
@@ -1396,7 +1396,7 @@ __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
   for (uint32_t base = 0; base < total; base += WAVE) {
     const uint32_t e = base + l;
     if (e < total) {
-      int lo = 0, hi = WAVE - 1;
+      int lo = 0, hi = (int)k - 1;   // lanes >= k hold no list
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (sc[mid] > e) hi = mid; else lo = mid + 1;
@@ -2101,7 +2101,7 @@ __global__ __launch_bounds__(BLOCK) void k_ms_expand(
           bool ins = false;
           unsigned long long np = 0;
           if (e < total) {
-            int lo = 0, hi = WAVE - 1;
+            int lo = 0, hi = (int)k - 1;   // lanes >= k hold no list
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
               if (sc[mid] > e) hi = mid; else lo = mid + 1;

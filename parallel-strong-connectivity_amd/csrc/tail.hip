@@ -684,7 +684,7 @@ __global__ __launch_bounds__(NB) void k_ms_tail(TailArgs A) {
               bool fresh = false;
               uint32_t x = 0, slot32 = 0, lbx = 0;
               if (e < total) {
-                int lo = 0, hi = WAVE - 1;
+                int lo = 0, hi = (int)k - 1;   // lanes >= k hold no list
                 while (lo < hi) {
                   const int mid = (lo + hi) >> 1;
                   if (sc[mid] > e) hi = mid; else lo = mid + 1;
