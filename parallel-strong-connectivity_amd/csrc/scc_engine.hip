@@ -843,7 +843,7 @@ __device__ __forceinline__ void head_level(const ReachSearch &R, ReachSlot *nxt,
 // Pull levels (src/reach.hpp:173-190) run in k_reach_pull, except the
 // head-only ones (pull_defers), which run here (head_level): their waves
 // need little LDS and few registers, so this kernel's occupancy suits them.
-__global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
+__global__ __launch_bounds__(BLOCK, 3) void k_reach_step(
     const ReachSearch *__restrict__ S, ReachCtl *C, int ns, uint32_t L, uint32_t tau,
     int vgc, uint32_t total_waves) {
   __shared__ uint32_t ring_s[WPB][REACH_RING];
@@ -896,6 +896,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
       unsigned long long n_edges = 0, n_exp = 0, n_vis = 0, n_app = 0, dsum = 0;
       uint64_t wbud = vgc ? (uint64_t)tau * VGC_WAVE_MULT : 0;   // per launch
       uint32_t nbatch = 0;
+      uint32_t sp_t = 1, sp_u = 1;   // the last pass's targets: tested, unvisited
       while (g < gend) {
         const uint32_t cnt = (uint32_t)min((uint64_t)sd, gend - g);
         if ((uint32_t)l < cnt) {
@@ -963,19 +964,30 @@ __global__ __launch_bounds__(BLOCK, 4) void k_reach_step(
               }
             }
             // claim (reach_claim): outside the searched subproblems never,
-            // else the visited bit once
+            // else the visited bit once.  During a local search whose last
+            // pass found at least half of its targets unvisited, every
+            // target's offsets are loaded with its visited word (one step
+            // less in the chain, a wasted load per visited target); else an
+            // unvisited target's after the test.
+            const bool spec = budget > 0 && 2 * sp_u >= sp_t;
+            uint64_t ob[PUSH_WIN], dv[PUSH_WIN];
+            sp_t = sp_u = 0;
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
               w[x] = found[x] ? R.vis[v[x] >> 5] : 0u;
               if (found[x] && R.excl) w[x] |= R.excl[v[x] >> 5];
+              ob[x] = spec && found[x] ? R.off[v[x]] : 0;
+              dv[x] = spec && found[x] ? R.off[v[x] + 1] : 0;
             }
-            // an unvisited target's offsets are loaded before its claim
-            uint64_t ob[PUSH_WIN], dv[PUSH_WIN];
 #pragma unroll
             for (int x = 0; x < PUSH_WIN; ++x) {
+              sp_t += (uint32_t)__popcll(ballot(found[x]));
               found[x] = found[x] && !((w[x] >> (v[x] & 31)) & 1u);
-              ob[x] = found[x] ? R.off[v[x]] : 0;
-              dv[x] = found[x] ? R.off[v[x] + 1] : 0;
+              sp_u += (uint32_t)__popcll(ballot(found[x]));
+              if (!spec) {
+                ob[x] = found[x] ? R.off[v[x]] : 0;
+                dv[x] = found[x] ? R.off[v[x] + 1] : 0;
+              }
             }
             // the wave's claims of this launch by exact vertex id: a target
             // that another lane or an earlier step of the wave took is
