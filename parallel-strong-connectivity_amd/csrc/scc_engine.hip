@@ -1385,12 +1385,15 @@ struct PullQueue {   // per-wave LDS queue of undecided vertices
 
 // Scan the first min(64, q.n) queued vertices' remaining in-edges as
 // 64-edge windows; a vertex meeting a visited in-neighbour is marked and
-// appended to the next frontier.
+// appended to the next frontier, or, when `resid` is set, each vertex that
+// met none is appended to the residual list (counted in *rctr).
 template <class Out>
 __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
                                            uint32_t *sc, uint64_t *st, uint32_t *fl,
                                            Out &out, uint32_t *qsize, uint32_t *qn,
-                                           uint64_t qcap, unsigned long long &n_edges,
+                                           uint64_t qcap, bool resid,
+                                           unsigned long long *rctr,
+                                           unsigned long long &n_edges,
                                            unsigned long long &n_vis) {
   const int l = lane();
   const uint32_t k = min(q.n, (uint32_t)WAVE);
@@ -1438,7 +1441,8 @@ __device__ __forceinline__ void pull_drain(const ReachSearch &R, PullQueue &q,
   }
   q.n = rest;
   wave_sync();
-  out.put(found, v, qsize, qn, qcap);
+  if (resid) out.put(own && d > 0 && !fl[l], v, rctr, qn, (uint64_t)R.n);
+  else out.put(found, v, qsize, qn, qcap);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_reach_pull(
@@ -1494,9 +1498,15 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     const uint64_t qcap = pull_store_cap(R);
     const uint64_t unv = R.nnz > cur->cum_vis ? R.nnz - cur->cum_vis : 0;
     const bool unstored = cur->deferred && unv > qcap;
+    // the level after an unstored one pulls the previous level's residual
+    // vertices (its frontier buffer) instead of walking the bitmap
+    const bool listed = cur->from_pull == PULL_UNSTORED;
     if (gw == 0 && l == 0) nxt->from_pull = unstored ? PULL_UNSTORED : 1;   // first wave
-    out.init(out_s[w], true);   // read only up to pull_store_cap entries
-    out.past = unstored;        // or counted only
+    // an unstored level's staged entries are its residual vertices (every
+    // candidate left unvisited or handed to the heavy list), counted in the
+    // next slot's degsum; its found vertices are only counted
+    unsigned long long *const rctr = &nxt->degsum;
+    out.init(out_s[w], !unstored);   // frontier: read only up to pull_store_cap
     const uint64_t words = ((uint64_t)R.n + 31) / 32;
     const uint64_t chunks = (words + WAVE - 1) / WAVE;
     const uint32_t tail_bits = R.n & 31;
@@ -1506,9 +1516,9 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     // phase 2 over the first cnt (at most WAVE * PULL_PAIRS) misses of the
     // list: in-offsets, then the first PULL_PROBE in-edges, then their
     // visited words; the rest of the list moves to the front
-    auto probe = [&](uint32_t cnt) {
+    auto probe = [&](uint32_t cnt) __attribute__((always_inline)) {
       uint64_t v[PULL_PAIRS], b[PULL_PAIRS], deg[PULL_PAIRS];
-      bool todo[PULL_PAIRS];
+      bool todo[PULL_PAIRS], rs[PULL_PAIRS];
 #pragma unroll
       for (int j = 0; j < PULL_PAIRS; ++j) {
         const uint32_t idx = j * WAVE + l;
@@ -1545,6 +1555,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         // the rest to the wave's queue
         const bool heavy = rem > PULL_HEAVY;
         heavy_append(R, cur, heavy, (uint32_t)v[j], rem);
+        rs[j] = todo[j] && (heavy || (!found && rem == 0));
         const bool qd = rem > 0 && !heavy;
         const uint64_t qm = ballot(qd);
         if (qd) {
@@ -1557,20 +1568,24 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
         // the wave owns v in this level: no other lane sets its bit
         if (found) atomicOr(&R.vis[v[j] >> 5], 1u << (v[j] & 31));
         n_vis += found;
-        out.put(found, (uint32_t)v[j], &nxt->qsize, qn, qcap);
+        if (!unstored) out.put(found, (uint32_t)v[j], &nxt->qsize, qn, qcap);
       }
+      if (unstored)
+#pragma unroll
+        for (int j = 0; j < PULL_PAIRS; ++j) out.put(rs[j], (uint32_t)v[j], rctr, qn, (uint64_t)R.n);
       wave_sync();
       const uint32_t rest = nmiss - cnt;
       for (uint32_t i = l; i < rest; i += WAVE) ml[i] = ml[cnt + i];
       nmiss = rest;
       wave_sync();
       while (q.n >= WAVE)
-        pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
+        pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, unstored, rctr, n_edges,
+                   n_vis);
     };
     // Phase 1 over the listed candidates (of up to PULL_CHUNKS chunks), the
     // misses to the miss list, then the found bits of the chunks' words go
     // out, one atomic per word.
-    auto process = [&]() {
+    auto process = [&]() __attribute__((always_inline)) {
       // every candidate tests its head, PULL_HEADS per lane at a time (two
       // dependent loads); a full batch of misses goes through phase 2
       for (uint32_t cb = 0; cb < ncand; cb += WAVE * PULL_HEADS) {
@@ -1602,7 +1617,7 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
           // found bits gather in the chunks' LDS words
           if (hit) atomicOr(&fw[(e[j] >> 11) * WAVE + ((e[j] & 2047u) >> 5)], 1u << (e[j] & 31));
           n_vis += hit;
-          out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, qcap);
+          if (!unstored) out.put(hit, (uint32_t)vx[j], &nxt->qsize, qn, qcap);
         }
         wave_sync();
         while (nmiss >= WAVE * PULL_PAIRS) probe(WAVE * PULL_PAIRS);
@@ -1639,11 +1654,32 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
       }
       return sk;
     };
+    if (listed) {
+      // the residual vertices, 64 a wave at a time: those still unvisited
+      // go straight to the miss list (their heads missed last level)
+      const uint32_t *res = p ? R.q[1] : R.q[0];
+      const uint64_t nres = cur->degsum;
+      for (uint64_t i0 = gw * WAVE; i0 < nres; i0 += nw * WAVE) {
+        const uint64_t i = i0 + l;
+        uint32_t v = 0;
+        bool c = false;
+        if (i < nres) {
+          v = res[i];
+          c = !bit_test(R.vis, v);
+        }
+        const uint64_t mm = ballot(c);
+        if (c) ml[nmiss + lanes_below(mm)] = v;
+        nmiss += (uint32_t)__popcll(mm);
+        wave_sync();
+        while (nmiss >= WAVE * PULL_PAIRS) probe(WAVE * PULL_PAIRS);
+      }
+    }
     uint32_t pf_sk[PULL_PF], pf_ts[PULL_PF];
     unsigned long long pf_nz[PULL_PF];
 #pragma unroll
-    for (int i = 0; i < PULL_PF; ++i) pf_sk[i] = skip_word(gw + (uint64_t)i * nw, pf_ts[i], pf_nz[i]);
-    for (uint64_t c0 = gw; c0 < chunks; c0 += (uint64_t)PULL_PF * nw) {
+    for (int i = 0; i < PULL_PF; ++i)
+      pf_sk[i] = skip_word(listed ? chunks : gw + (uint64_t)i * nw, pf_ts[i], pf_nz[i]);
+    for (uint64_t c0 = listed ? chunks : gw; c0 < chunks; c0 += (uint64_t)PULL_PF * nw) {
       // this group's words to LDS, the next group's loads issued
 #pragma unroll
       for (int i = 0; i < PULL_PF; ++i) {
@@ -1693,9 +1729,16 @@ __global__ __launch_bounds__(BLOCK) void k_reach_pull(
     }
     if (ncand) process();
     if (nmiss) probe(nmiss);
-    while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, n_edges, n_vis);
-    out.flush(&nxt->qsize, qn, qcap);
-    out.finish(&nxt->qsize);
+    while (q.n) pull_drain(R, q, sc, st, fl, out, &nxt->qsize, qn, qcap, unstored, rctr, n_edges,
+                   n_vis);
+    if (unstored) {
+      out.flush(rctr, qn, (uint64_t)R.n);
+      const unsigned long long nv = wave_sum(n_vis);   // the next level's frontier size
+      if (l == 0 && nv) atomicAdd(&nxt->qsize, (uint32_t)nv);
+    } else {
+      out.flush(&nxt->qsize, qn, qcap);
+      out.finish(&nxt->qsize);
+    }
     unsigned long long *const cp[5] = {&C->cnt[s][R_PULL_EDGES], &C->cnt[s][R_PULL_SCANS],
                                        &C->cnt[s][R_PULL_VIS], &nxt->cum_vis,
                                        &C->cnt[s][R_PULL_UNITS]};
@@ -1748,7 +1791,8 @@ __global__ __launch_bounds__(BLOCK) void k_reach_heavy(const ReachSearch *__rest
     const ReachSearch R = S[s];
     uint32_t *qn = (p ? R.q[0] : R.q[1]);
     const bool pull = reach_mode(cur, R) == MODE_PULL;
-    const uint64_t qcap = pull ? pull_store_cap(R) : R.n;
+    // an unstored pull level's frontier buffer holds its residual list
+    const uint64_t qcap = !pull ? R.n : nxt->from_pull == PULL_UNSTORED ? 0 : pull_store_cap(R);
     const uint64_t *off = pull ? R.roff : R.off;
     const uint32_t *adj = pull ? R.radj : R.adj;
     // a pull remainder starts after the PULL_PROBE probed edges
