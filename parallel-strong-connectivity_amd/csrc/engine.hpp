@@ -193,6 +193,7 @@ struct Engine {
   uint32_t ms_levels_hint = 2;   // sub-rounds of the previous multi-search
   uint32_t reach_levels_hint = 0; // levels of the previous first reach
   uint32_t round_levels_hint = 0; // levels of the previous single-pivot round
+  uint32_t reach_levels_prev = 0, round_levels_prev = 0;   // the last run's depth
   uint64_t nnz_total = 0;         // non-trimmed vertices of the current run
   uint64_t syncs = 0;
   unsigned char h_reach[512];

@@ -3204,7 +3204,9 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
       if (after_ran) *after_ran = ran;
       break;
     }
-    if (!trace) K = std::min(K * 2, 64);
+    // past a hinted batch the search is usually a level or two from its
+    // end: a short batch, then doubling
+    if (!trace) K = (hint0 && L == hint0) ? 2 : std::min(K * 2, 64);
   }
   if (!rr) {
     st->levels_fwd = lvl[0];
@@ -3215,7 +3217,15 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
       rr->reached[s] = hc_.cnt[s][R_PUSH_VIS] + hc_.cnt[s][R_HEAVY_VIS] +
                        hc_.cnt[s][R_PULL_VIS] + hc_.cnt[s][R_HEAD_VIS];
   }
-  hint = std::max(lvl[0], lvl[1]);
+  // the first reach's depth varies by a level from run to run (which
+  // vertices a level finds depends on timing): the hint is the deeper of
+  // the last two runs, so a run rarely outlasts its hinted batch
+  {
+    const uint32_t d = std::max(lvl[0], lvl[1]);
+    uint32_t &prev = rr ? round_levels_prev : reach_levels_prev;
+    hint = rr ? d : std::max(d, prev);
+    prev = d;
+  }
   for (int s = 0; s < 2; ++s) {
     const unsigned long long *c = hc_.cnt[s];
     st->edges_scanned += c[R_PUSH_EDGES] + c[R_HEAVY_EDGES] + c[R_PULL_EDGES] +
