@@ -194,6 +194,10 @@ struct Engine {
   uint32_t reach_levels_hint = 0; // levels of the previous first reach
   uint32_t round_levels_hint = 0; // levels of the previous single-pivot round
   uint32_t reach_levels_prev = 0, round_levels_prev = 0;   // the last run's depth
+  // set while the labelling is enqueued ahead of the poll that shows the
+  // first reach over: k_first_label checks level label_gate_level itself
+  bool label_gate = false;
+  uint32_t label_gate_level = 0;
   uint64_t nnz_total = 0;         // non-trimmed vertices of the current run
   uint64_t syncs = 0;
   unsigned char h_reach[512];

@@ -1885,8 +1885,17 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
     uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim,
-    uint32_t *pos_tiles) {
+    uint32_t *pos_tiles, const ReachCtl *gate, uint32_t gate_level) {
   __shared__ uint32_t wc[SEL_ITEMS][WPB];
+  // enqueued ahead of the poll that shows the reach over: if it is not
+  // over at gate_level, the pass is enqueued again later and this one
+  // does nothing
+  if (gate) {
+    const int gi = (int)(gate_level % 3);
+    for (int s = 0; s < 2; ++s)
+      if (!gate->done[s] && (gate->slot[s][gi].qsize != 0 || gate->slot[s][gi].deferred != 0))
+        return;
+  }
   const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
   const int w = threadIdx.x / WAVE;
   // every load the tile needs first: its prefix, the per-graph constants
@@ -3163,7 +3172,15 @@ int Engine::first_reach(const psc_params &p, psc_stats *st, ReachRound *rr,
     L += (uint32_t)K;
     PSC_HIP(hipGetLastError());
     const bool ran = after_levels && !trace && hint0 && L >= hint0;
-    if (ran) PSC_TRY(after_levels());
+    if (ran) {
+      // ahead of the poll: the labelling checks on the device that the
+      // reach is over at level L
+      label_gate = true;
+      label_gate_level = L;
+      const int e = after_levels();
+      label_gate = false;
+      PSC_TRY(e);
+    }
     PSC_TRY(poll(&hc_, C, sizeof(ReachCtl)));
     if (trace) {
       const auto now = std::chrono::steady_clock::now();
@@ -3531,7 +3548,8 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
         trim_static.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
         gC.as<unsigned long long>(), posv, pbits.as<uint32_t>(), direct_final ? 1 : 0,
-        first_trim.as<uint32_t>(), tiles.as<uint32_t>());
+        first_trim.as<uint32_t>(), tiles.as<uint32_t>(),
+        label_gate ? reach_ctl.as<ReachCtl>() : nullptr, label_gate_level);
     prof_end();
     PSC_HIP(hipGetLastError());
     if (nnz) {
