@@ -87,7 +87,8 @@ __global__ void k_tail_go(const TailState *ts, const TailGraph *tg, uint32_t nl,
 
 // lid[centers[j]] = j, compact labels, and the compact degrees: edges to
 // vertices that are unfinished and carry the same label (one wave per
-// remaining vertex, 64-edge windows).
+// remaining vertex, 64-edge windows of both directions at once, so the
+// vertex's chain is four dependent loads, not eight).
 __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
                                const unsigned long long *labels,
                                const uint64_t *off, const uint32_t *adj,
@@ -103,29 +104,30 @@ __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
        j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
     const uint32_t v = centers[j];
     const unsigned long long lv = labels[v];
+    const uint64_t o0 = off[v], o1 = off[v + 1], i0 = ioff[v], i1 = ioff[v + 1];
     if (l == 0) {
       lid[v] = (uint32_t)j;
       tlab[j] = lv;
     }
-    for (int dir = 0; dir < 2; ++dir) {
-      const uint64_t *o = dir ? ioff : off;
-      const uint32_t *a = dir ? iadj : adj;
-      const uint64_t e0 = o[v], e1 = o[v + 1];
-      uint32_t c = 0;
-      for (uint64_t e = e0; e < e1; e += WAVE) {
-        bool keep = false;
-        if (e + l < e1) {
-          const unsigned long long lx = labels[a[e + l]];
-          keep = !(lx & TOP_BIT) && lx == lv;
-        }
-        c += (uint32_t)__popcll(ballot(keep));
-      }
-      if (l == 0) (dir ? cnt_in : cnt_out)[j] = c;
+    uint32_t co = 0, ci = 0;
+    const uint64_t dmax = max(o1 - o0, i1 - i0);
+    for (uint64_t k = 0; k < dmax; k += WAVE) {
+      const uint64_t eo = o0 + k + l, ei = i0 + k + l;
+      const uint32_t xo = eo < o1 ? adj[eo] : 0u, xi = ei < i1 ? iadj[ei] : 0u;
+      const unsigned long long lo = eo < o1 ? labels[xo] : TOP_BIT;
+      const unsigned long long li = ei < i1 ? labels[xi] : TOP_BIT;
+      co += (uint32_t)__popcll(ballot(!(lo & TOP_BIT) && lo == lv));
+      ci += (uint32_t)__popcll(ballot(!(li & TOP_BIT) && li == lv));
+    }
+    if (l == 0) {
+      cnt_out[j] = co;
+      cnt_in[j] = ci;
     }
   }
 }
 
-// Compact adjacency in original order (ballot ranks within each window).
+// Compact adjacency in original order (ballot ranks within each window),
+// both directions' windows at once.
 __global__ void k_tail_fill(const uint32_t *centers, uint64_t nrem,
                             const unsigned long long *labels, const uint32_t *lid,
                             const uint64_t *off, const uint32_t *adj,
@@ -137,24 +139,21 @@ __global__ void k_tail_fill(const uint32_t *centers, uint64_t nrem,
        j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
     const uint32_t v = centers[j];
     const unsigned long long lv = labels[v];
-    for (int dir = 0; dir < 2; ++dir) {
-      const uint64_t *o = dir ? ioff : off;
-      const uint32_t *a = dir ? iadj : adj;
-      uint32_t *out = dir ? tiadj : tadj;
-      uint32_t at = (dir ? tioff : toff)[j];
-      const uint64_t e0 = o[v], e1 = o[v + 1];
-      for (uint64_t e = e0; e < e1; e += WAVE) {
-        bool keep = false;
-        uint32_t x = 0;
-        if (e + l < e1) {
-          x = a[e + l];
-          const unsigned long long lx = labels[x];
-          keep = !(lx & TOP_BIT) && lx == lv;
-        }
-        const uint64_t m = ballot(keep);
-        if (keep) out[at + lanes_below(m)] = lid[x];
-        at += (uint32_t)__popcll(m);
-      }
+    const uint64_t o0 = off[v], o1 = off[v + 1], i0 = ioff[v], i1 = ioff[v + 1];
+    uint32_t ato = toff[j], ati = tioff[j];
+    const uint64_t dmax = max(o1 - o0, i1 - i0);
+    for (uint64_t k = 0; k < dmax; k += WAVE) {
+      const uint64_t eo = o0 + k + l, ei = i0 + k + l;
+      const uint32_t xo = eo < o1 ? adj[eo] : 0u, xi = ei < i1 ? iadj[ei] : 0u;
+      const unsigned long long lo = eo < o1 ? labels[xo] : TOP_BIT;
+      const unsigned long long li = ei < i1 ? labels[xi] : TOP_BIT;
+      const bool ko = !(lo & TOP_BIT) && lo == lv, ki = !(li & TOP_BIT) && li == lv;
+      const uint32_t lo_id = ko ? lid[xo] : 0u, li_id = ki ? lid[xi] : 0u;
+      const uint64_t mo = ballot(ko), mi = ballot(ki);
+      if (ko) tadj[ato + lanes_below(mo)] = lo_id;
+      if (ki) tiadj[ati + lanes_below(mi)] = li_id;
+      ato += (uint32_t)__popcll(mo);
+      ati += (uint32_t)__popcll(mi);
     }
   }
 }
