@@ -1886,7 +1886,6 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
     uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim,
     uint32_t *pos_tiles, const ReachCtl *gate, uint32_t gate_level) {
-  __shared__ uint32_t wc[SEL_ITEMS][WPB];
   // enqueued ahead of the poll that shows the reach over: if it is not
   // over at gate_level, the pass is enqueued again later and this one
   // does nothing
@@ -1896,16 +1895,32 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
       if (!gate->done[s] && (gate->slot[s][gi].qsize != 0 || gate->slot[s][gi].deferred != 0))
         return;
   }
-  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
-  const int w = threadIdx.x / WAVE;
-  // every load the tile needs first: its prefix, the per-graph constants
-  // of a single graph, and the bitmap words (trimmed, both reaches)
+  // each wave labels its own SEG = SEL_TILE / WPB consecutive vertices of
+  // the tile, 64 at a time, with no barrier: its first rank is the tile's
+  // prefix plus the non-trimmed vertices of the waves before it (their
+  // trimmed words, one per lane)
+  constexpr uint32_t SEG = SEL_TILE / WPB;
+  static_assert(SEG == SEL_ITEMS * WAVE && SEG / 32 * (WPB - 1) <= WAVE, "segment layout");
+  const int w = threadIdx.x / WAVE, l = lane();
+  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE + (uint64_t)w * SEG;
+  // every load the segment needs first: the tile's prefix, the words of
+  // the earlier segments, the per-graph constants of a single graph, and
+  // the bitmap words (trimmed, both reaches)
   uint64_t r0 = tile_off[blockIdx.x];
+  uint32_t pw = ~0u;
+  {
+    const uint64_t wd = (uint64_t)blockIdx.x * (SEL_TILE / 32) + l;
+    if ((uint32_t)l < (uint32_t)w * (SEG / 32) && wd * 32 < n) {
+      pw = trim_bm[wd];
+      const uint64_t hi = n - wd * 32;   // positions of the word inside the graph
+      if (hi < 32) pw |= ~((1u << hi) - 1u);
+    }
+  }
   const unsigned long long z0 = zstart[0], lab_one = lab1[0];
   uint32_t tw[SEL_ITEMS], fw[SEL_ITEMS], bw[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    const uint64_t i = base + (uint64_t)j * WAVE + l;
     tw[j] = ~0u;
     fw[j] = bw[j] = 0;
     if (i < n) {
@@ -1914,31 +1929,18 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
       bw[j] = vb[i >> 5];
     }
   }
-  bool p[SEL_ITEMS];
-  uint32_t rk[SEL_ITEMS];
-#pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    p[j] = i < n && !((tw[j] >> (i & 31)) & 1u);
-    const uint64_t m = ballot(p[j]);
-    rk[j] = lanes_below(m);
-    if (lane() == 0) wc[j][w] = (uint32_t)__popcll(m);
-  }
-  __syncthreads();
+  r0 += wave_sum((uint32_t)__popc(~pw));
   uint32_t place = 0;            // items left to place by pivot position
   uint64_t rr[SEL_ITEMS];
+  bool p[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
-    uint32_t before = 0, round_tot = 0;
-#pragma unroll
-    for (int x = 0; x < WPB; ++x) {
-      before += x < w ? wc[j][x] : 0u;
-      round_tot += wc[j][x];
-    }
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    const uint64_t r = r0 + before + rk[j];
+    const uint64_t i = base + (uint64_t)j * WAVE + l;
+    p[j] = i < n && !((tw[j] >> (i & 31)) & 1u);
+    const uint64_t m = ballot(p[j]);
+    const uint64_t r = r0 + lanes_below(m);
     rr[j] = r;
-    r0 += round_tot;
+    r0 += (uint32_t)__popcll(m);
     if (i >= n) continue;
     const uint32_t g = G == 1 ? 0u : seg_of(voff, G, i);
     if (!p[j]) {
@@ -1962,7 +1964,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
 #pragma unroll 1
   for (uint32_t mm = place; mm; mm &= mm - 1) {
     const int j = __ffs(mm) - 1;
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+    const uint64_t i = base + (uint64_t)j * WAVE + l;
     const uint32_t g = G == 1 ? 0u : seg_of(voff, G, i);
     const uint64_t b0 = nstart[g];
     const uint64_t pos = b0 + PivotPerm(nstart[g + 1] - b0, seed).inverse(rr[j] - b0);
