@@ -1975,12 +1975,29 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
 }
 
 // Remaining vertices in pivot order: the set bits of pbits, in order.
-struct SelBits {
-  using T = uint32_t;
-  const uint32_t *pbits, *posv;
-  __device__ bool pred(uint64_t pos) const { return (pbits[pos >> 5] >> (pos & 31)) & 1u; }
-  __device__ uint32_t val(uint64_t pos) const { return posv[pos]; }
-};
+// The pivot-order compaction: sel[tile_off[t] + k] = posv of the k-th set
+// bit of position tile t.  One wave per tile, a bitmap word per lane (the
+// bits are sparse: only the vertices left after the first SCC are set).
+__global__ __launch_bounds__(BLOCK) void k_sel_bits(const uint32_t *__restrict__ pbits,
+                                                    const uint32_t *__restrict__ posv,
+                                                    uint64_t len,
+                                                    const unsigned long long *tile_off,
+                                                    uint32_t *sel) {
+  static_assert(SEL_TILE == 32 * WAVE, "one bitmap word per lane and tile");
+  const uint64_t t = blockIdx.x * (uint64_t)WPB + threadIdx.x / WAVE;
+  if (t * SEL_TILE >= len) return;
+  const int l = lane();
+  const uint64_t wd = t * WAVE + l;
+  uint32_t bits = 0;
+  if (wd * 32 < len) {
+    bits = pbits[wd];
+    const uint64_t hi = len - wd * 32;   // positions of the word below len
+    if (hi < 32) bits &= (1u << hi) - 1u;
+  }
+  const uint32_t c = (uint32_t)__popc(bits);
+  uint64_t r = tile_off[t] + (wave_incl_scan(c) - c);
+  for (uint32_t m = bits; m; m &= m - 1) sel[r++] = posv[wd * 32 + (uint32_t)(__ffs(m) - 1)];
+}
 
 // ----------------------------------------------------- multi-search kernels
 // Start of a multi-search attempt for the searches in `want` (bit s):
@@ -3568,9 +3585,9 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
         if (G == 1) k_segment_starts<<<1, 64, 0, stream>>>(centers.as<uint32_t>(), 0, rem_d, voffd,
                                                          G, nullptr, G, gB.as<unsigned long long>());
       }
-      k_sel_scatter<SelBits><<<(unsigned)st_tiles, BLOCK, 0, stream>>>(
-          SelBits{pbits.as<uint32_t>(), posv}, nnz, tile_off.as<unsigned long long>(),
-          centers.as<uint32_t>(), nullptr);
+      k_sel_bits<<<(unsigned)((st_tiles + WPB - 1) / WPB), BLOCK, 0, stream>>>(
+          pbits.as<uint32_t>(), posv, nnz, tile_off.as<unsigned long long>(),
+          centers.as<uint32_t>());
       prof_end();
       if (G > 1) {
         prof_begin(KK_MISC);
