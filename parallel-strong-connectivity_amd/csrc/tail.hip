@@ -85,75 +85,91 @@ __global__ void k_tail_go(const TailState *ts, const TailGraph *tg, uint32_t nl,
   if (threadIdx.x == 0) *go = all ? 1ull : 0ull;
 }
 
+// Remainder vertices are scanned by TAIL_GRP-lane groups, WAVE / TAIL_GRP
+// vertices a wave (their lists are short), the windows of both directions
+// at once, so a vertex's chain is four dependent loads.
+constexpr int TAIL_GRP = 16;
+static_assert(WAVE % TAIL_GRP == 0, "whole groups per wave");
+
 // lid[centers[j]] = j, compact labels, and the compact degrees: edges to
-// vertices that are unfinished and carry the same label (one wave per
-// remaining vertex, 64-edge windows of both directions at once, so the
-// vertex's chain is four dependent loads, not eight).
+// vertices that are unfinished and carry the same label.
 __global__ void k_tail_degrees(const uint32_t *centers, uint64_t nrem,
                                const unsigned long long *labels,
                                const uint64_t *off, const uint32_t *adj,
                                const uint64_t *ioff, const uint32_t *iadj,
                                uint32_t *cnt_out, uint32_t *cnt_in, uint32_t *lid,
                                unsigned long long *tlab) {
-  const int l = lane();
+  const int gl = lane() % TAIL_GRP, gs = lane() / TAIL_GRP * TAIL_GRP;
   if (blockIdx.x == 0 && threadIdx.x == 0) {   // scan sentinels
     cnt_out[nrem] = 0;
     cnt_in[nrem] = 0;
   }
-  for (uint64_t j = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; j < nrem;
-       j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
-    const uint32_t v = centers[j];
-    const unsigned long long lv = labels[v];
-    const uint64_t o0 = off[v], o1 = off[v + 1], i0 = ioff[v], i1 = ioff[v + 1];
-    if (l == 0) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / TAIL_GRP;
+  for (uint64_t j0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE * (WAVE / TAIL_GRP);
+       j0 < nrem; j0 += stride) {
+    const uint64_t j = j0 + lane() / TAIL_GRP;
+    const bool own = j < nrem;
+    const uint32_t v = own ? centers[j] : 0u;
+    const unsigned long long lv = own ? labels[v] : 0ull;
+    const uint64_t o0 = own ? off[v] : 0, o1 = own ? off[v + 1] : 0;
+    const uint64_t i0 = own ? ioff[v] : 0, i1 = own ? ioff[v + 1] : 0;
+    if (own && gl == 0) {
       lid[v] = (uint32_t)j;
       tlab[j] = lv;
     }
     uint32_t co = 0, ci = 0;
     const uint64_t dmax = max(o1 - o0, i1 - i0);
-    for (uint64_t k = 0; k < dmax; k += WAVE) {
-      const uint64_t eo = o0 + k + l, ei = i0 + k + l;
+    for (uint64_t k = 0; ballot(k < dmax); k += TAIL_GRP) {
+      const uint64_t eo = o0 + k + gl, ei = i0 + k + gl;
       const uint32_t xo = eo < o1 ? adj[eo] : 0u, xi = ei < i1 ? iadj[ei] : 0u;
       const unsigned long long lo = eo < o1 ? labels[xo] : TOP_BIT;
       const unsigned long long li = ei < i1 ? labels[xi] : TOP_BIT;
-      co += (uint32_t)__popcll(ballot(!(lo & TOP_BIT) && lo == lv));
-      ci += (uint32_t)__popcll(ballot(!(li & TOP_BIT) && li == lv));
+      const uint32_t mo = (uint32_t)(ballot(!(lo & TOP_BIT) && lo == lv) >> gs) & 0xFFFFu;
+      const uint32_t mi = (uint32_t)(ballot(!(li & TOP_BIT) && li == lv) >> gs) & 0xFFFFu;
+      co += (uint32_t)__popc(mo);
+      ci += (uint32_t)__popc(mi);
     }
-    if (l == 0) {
+    if (own && gl == 0) {
       cnt_out[j] = co;
       cnt_in[j] = ci;
     }
   }
 }
 
-// Compact adjacency in original order (ballot ranks within each window),
-// both directions' windows at once.
+// Compact adjacency in original order (ballot ranks within each group's
+// window), both directions' windows at once.
 __global__ void k_tail_fill(const uint32_t *centers, uint64_t nrem,
                             const unsigned long long *labels, const uint32_t *lid,
                             const uint64_t *off, const uint32_t *adj,
                             const uint64_t *ioff, const uint32_t *iadj,
                             const uint32_t *toff, const uint32_t *tioff,
                             uint32_t *tadj, uint32_t *tiadj) {
-  const int l = lane();
-  for (uint64_t j = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE; j < nrem;
-       j += (uint64_t)gridDim.x * blockDim.x / WAVE) {
-    const uint32_t v = centers[j];
-    const unsigned long long lv = labels[v];
-    const uint64_t o0 = off[v], o1 = off[v + 1], i0 = ioff[v], i1 = ioff[v + 1];
-    uint32_t ato = toff[j], ati = tioff[j];
+  const int gl = lane() % TAIL_GRP, gs = lane() / TAIL_GRP * TAIL_GRP;
+  const uint32_t below = (1u << gl) - 1u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / TAIL_GRP;
+  for (uint64_t j0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE * (WAVE / TAIL_GRP);
+       j0 < nrem; j0 += stride) {
+    const uint64_t j = j0 + lane() / TAIL_GRP;
+    const bool own = j < nrem;
+    const uint32_t v = own ? centers[j] : 0u;
+    const unsigned long long lv = own ? labels[v] : 0ull;
+    const uint64_t o0 = own ? off[v] : 0, o1 = own ? off[v + 1] : 0;
+    const uint64_t i0 = own ? ioff[v] : 0, i1 = own ? ioff[v + 1] : 0;
+    uint32_t ato = own ? toff[j] : 0u, ati = own ? tioff[j] : 0u;
     const uint64_t dmax = max(o1 - o0, i1 - i0);
-    for (uint64_t k = 0; k < dmax; k += WAVE) {
-      const uint64_t eo = o0 + k + l, ei = i0 + k + l;
+    for (uint64_t k = 0; ballot(k < dmax); k += TAIL_GRP) {
+      const uint64_t eo = o0 + k + gl, ei = i0 + k + gl;
       const uint32_t xo = eo < o1 ? adj[eo] : 0u, xi = ei < i1 ? iadj[ei] : 0u;
       const unsigned long long lo = eo < o1 ? labels[xo] : TOP_BIT;
       const unsigned long long li = ei < i1 ? labels[xi] : TOP_BIT;
       const bool ko = !(lo & TOP_BIT) && lo == lv, ki = !(li & TOP_BIT) && li == lv;
       const uint32_t lo_id = ko ? lid[xo] : 0u, li_id = ki ? lid[xi] : 0u;
-      const uint64_t mo = ballot(ko), mi = ballot(ki);
-      if (ko) tadj[ato + lanes_below(mo)] = lo_id;
-      if (ki) tiadj[ati + lanes_below(mi)] = li_id;
-      ato += (uint32_t)__popcll(mo);
-      ati += (uint32_t)__popcll(mi);
+      const uint32_t mo = (uint32_t)(ballot(ko) >> gs) & 0xFFFFu;
+      const uint32_t mi = (uint32_t)(ballot(ki) >> gs) & 0xFFFFu;
+      if (ko) tadj[ato + (uint32_t)__popc(mo & below)] = lo_id;
+      if (ki) tiadj[ati + (uint32_t)__popc(mi & below)] = li_id;
+      ato += (uint32_t)__popc(mo);
+      ati += (uint32_t)__popc(mi);
     }
   }
 }
@@ -1070,7 +1086,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   uint32_t *cnt_out = tail_cnt.as<uint32_t>(), *cnt_in = cnt_out + (n_rem + 1);
   uint32_t *lid = scratch32.as<uint32_t>();
   prof_begin(KK_TAIL);
-  k_tail_degrees<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
+  k_tail_degrees<<<blocks(n_rem * TAIL_GRP), 256, 0, stream>>>(
       cen, n_rem, lab, d_off, d_adj, d_ioff, d_iadj, cnt_out, cnt_in, lid,
       tail_lab.as<unsigned long long>());
   // one scan over both degree arrays (contiguous, each with a 0 sentinel):
@@ -1111,7 +1127,7 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
   uint32_t *tadj = tail_adj.as<uint32_t>(), *tiadj = tadj;
   PSC_TRY(tail_spec.ensure(((n_rem + 31) / 32 + 1) * 4));
   prof_begin(KK_TAIL);
-  k_tail_fill<<<blocks(n_rem * WAVE), 256, 0, stream>>>(
+  k_tail_fill<<<blocks(n_rem * TAIL_GRP), 256, 0, stream>>>(
       cen, n_rem, lab, lid, d_off, d_adj, d_ioff, d_iadj, toff, tioff, tadj, tiadj);
   k_tail_special<<<blocks(n_rem), 256, 0, stream>>>(tail_lab.as<unsigned long long>(), toff,
                                                     tioff, n_rem, tail_spec.as<uint32_t>());
