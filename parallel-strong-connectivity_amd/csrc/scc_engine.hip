@@ -1885,7 +1885,9 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
     const unsigned long long *nstart, uint64_t seed, const uint32_t *vf,
     const uint32_t *vb, unsigned long long *labels, const unsigned long long *lab1,
     uint32_t *posv, uint32_t *pbits, int direct, uint32_t *first_trim,
-    uint32_t *pos_tiles, const ReachCtl *gate, uint32_t gate_level) {
+    uint32_t *pos_tiles, const ReachCtl *gate, uint32_t gate_level,
+    unsigned long long lab_v, unsigned long long next_v, unsigned long long *lab_out,
+    unsigned long long *next_out) {
   // enqueued ahead of the poll that shows the reach over: if it is not
   // over at gate_level, the pass is enqueued again later and this one
   // does nothing
@@ -1916,7 +1918,13 @@ __global__ __launch_bounds__(BLOCK) void k_first_label(
       if (hi < 32) pw |= ~((1u << hi) - 1u);
     }
   }
-  const unsigned long long z0 = zstart[0], lab_one = lab1[0];
+  // a single graph's label offsets come by value (lab1 null) and are
+  // stored here for the later kernels
+  const unsigned long long z0 = zstart[0], lab_one = lab1 ? lab1[0] : lab_v;
+  if (!lab1 && blockIdx.x == 0 && threadIdx.x == 0) {
+    lab_out[0] = lab_v;
+    next_out[0] = next_v;
+  }
   uint32_t tw[SEL_ITEMS], fw[SEL_ITEMS], bw[SEL_ITEMS];
 #pragma unroll
   for (int j = 0; j < SEL_ITEMS; ++j) {
@@ -3550,8 +3558,11 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     for (uint32_t g = 0; g < G; ++g)
       if (h_nstart[g + 1] > h_nstart[g] && h_zstart[g + 1] == h_zstart[g])
         direct_final = false;
-    PSC_TRY(upload(gC, label_off.data(), G, 5));
-    PSC_TRY(upload(gD, label_next.data(), G, 4));
+    // one graph: its two offsets go to k_first_label as arguments
+    if (G > 1) {
+      PSC_TRY(upload(gC, label_off.data(), G, 5));
+      PSC_TRY(upload(gD, label_next.data(), G, 4));
+    }
     const uint64_t pwords = (std::max<uint64_t>(nnz, 1) + 31) / 32;
     PSC_TRY(pbits.ensure(pwords * 4));
     uint32_t *posv = front.as<uint32_t>();   // front is free until the rounds
@@ -3566,9 +3577,10 @@ int Engine::run(const psc_params &p, unsigned long long *d_out, psc_stats *st) {
     k_first_label<<<(unsigned)nt, BLOCK, 0, stream>>>(
         trim_static.as<uint32_t>(), n, trim_off.as<unsigned long long>(), voffd, G, zstart,
         nstart, p.seed, vis[0].as<uint32_t>(), vis[1].as<uint32_t>(), lab,
-        gC.as<unsigned long long>(), posv, pbits.as<uint32_t>(), direct_final ? 1 : 0,
-        first_trim.as<uint32_t>(), tiles.as<uint32_t>(),
-        label_gate ? reach_ctl.as<ReachCtl>() : nullptr, label_gate_level);
+        G > 1 ? gC.as<unsigned long long>() : nullptr, posv, pbits.as<uint32_t>(),
+        direct_final ? 1 : 0, first_trim.as<uint32_t>(), tiles.as<uint32_t>(),
+        label_gate ? reach_ctl.as<ReachCtl>() : nullptr, label_gate_level, label_off[0],
+        label_next[0], gC.as<unsigned long long>(), gD.as<unsigned long long>());
     prof_end();
     PSC_HIP(hipGetLastError());
     if (nnz) {
