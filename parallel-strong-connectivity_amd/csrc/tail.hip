@@ -85,6 +85,33 @@ __global__ void k_tail_go(const TailState *ts, const TailGraph *tg, uint32_t nl,
   if (threadIdx.x == 0) *go = all ? 1ull : 0ull;
 }
 
+// The writeback of the compact labels and, when every graph of the launch
+// finished its rounds (k_tail_go's test, made by every workgroup), the
+// finalize of the remaining vertices and of each graph's first trimmed
+// vertex (k_finalize_rest's values) in one pass.
+__global__ void k_tail_final(const uint32_t *centers, uint64_t nrem,
+                             const unsigned long long *tlab, unsigned long long *labels,
+                             const TailState *ts, const TailGraph *tg, uint32_t nl,
+                             const uint32_t *first_trim, const unsigned long long *zstart,
+                             uint32_t G, unsigned long long *go) {
+  bool ok = true;
+  for (uint32_t k = lane(); k < nl; k += WAVE) ok = ok && ts[k].cur == tg[k].nr && !ts[k].full;
+  const bool fin = ballot(!ok) == 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *go = fin ? 1ull : 0ull;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nrem + G;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < nrem) {
+      const unsigned long long l = tlab[i];
+      labels[centers[i]] = fin ? (l & VAL_MASK) - 1 : l;
+    } else if (fin) {
+      const uint64_t g = i - nrem;
+      if (zstart[g + 1] == zstart[g]) continue;   // no trimmed vertex
+      const uint32_t v = first_trim[g];
+      labels[v] = (labels[v] & VAL_MASK) - 1;
+    }
+  }
+}
+
 // Remainder vertices are scanned by TAIL_GRP-lane groups, WAVE / TAIL_GRP
 // vertices a wave (their lists are short), the windows of both directions
 // at once, so a vertex's chain is four dependent loads.
@@ -1241,23 +1268,34 @@ int Engine::ms_tail(const psc_params &p, const uint64_t *h_cstart, uint64_t n_re
     a.first = (uint32_t)(nmode[2] + nmode[1]);
     a.count = (uint32_t)nmode[0];
     if (nmode[0]) k_ms_tail<0, TAIL_BLOCK><<<(unsigned)nmode[0], TAIL_BLOCK, 0, stream>>>(a);
-    // labels as of every completed round, enqueued before the poll (a
-    // relaunch continues from the compact labels; the host rounds and the
-    // finalize read these)
-    k_tail_writeback<<<blocks(n_rem), 256, 0, stream>>>(
-        cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
-    prof_end();
-    PSC_HIP(hipGetLastError());
     // When every graph with rounds left is in this launch, the finalize
     // goes behind it before the poll, skipped on the device unless all of
     // them finished: the poll then returns with the labels final.
-    bool spec_fin = false;
+    bool spec_fin = false, all_in = true;
     {
       std::vector<char> in_launch(G, 0);
       for (const TailGraph &t : launch) in_launch[t.g] = 1;
-      bool all_in = true;
       for (uint32_t g = 0; g < G && all_in; ++g)
         if (!in_launch[g] && cur[g] < h_cstart[g + 1] - h_cstart[g]) all_in = false;
+    }
+    // labels as of every completed round, enqueued before the poll (a
+    // relaunch continues from the compact labels; the host rounds and the
+    // finalize read these); with the finalize when it can go with them
+    if (all_in && direct_final && final_rem == n_rem) {
+      k_tail_final<<<blocks(n_rem + G), 256, 0, stream>>>(
+          cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>(), d_ts,
+          d_tg, (uint32_t)nl, first_trim.as<uint32_t>(), gA.as<unsigned long long>() + (G + 1),
+          G, d_go);
+      if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));   // the finalize is in it
+      spec_fin = true;
+      all_in = false;   // nothing more to enqueue
+    } else {
+      k_tail_writeback<<<blocks(n_rem), 256, 0, stream>>>(
+          cen, n_rem, tail_lab.as<unsigned long long>(), labels.as<unsigned long long>());
+    }
+    prof_end();
+    PSC_HIP(hipGetLastError());
+    {
       if (all_in) {
         if (p.timing) PSC_HIP(hipEventRecord(ev[3], stream));
         prof_begin(KK_MISC);
