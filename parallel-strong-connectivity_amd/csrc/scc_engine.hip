@@ -708,7 +708,7 @@ __device__ __forceinline__ void heavy_append(const ReachSearch &R, ReachSlot *cu
 // in LDS and go out with one atomic per word.  The map's two halves
 // (1024 entries each) and `aux` (WAVE words) are the wave's own LDS.  Every
 // thread of the block calls it (block-wide counter reduction).
-constexpr int HEAD_STEPS = 4;
+constexpr int HEAD_STEPS = 6;
 constexpr int PUSH_WIN = 4;   // push windows whose loads are in flight together
 constexpr uint64_t RING_NO_OFF = ~0ull;
 static_assert(VGC_MAX_DEG < 16, "a queued degree fits 4 bits");
