@@ -1059,8 +1059,8 @@ __global__ __launch_bounds__(BLOCK, 3) void k_reach_step(
 // entries of the list; a list without a hub there takes the same maximum
 // over entries [HUB_SCAN1, HUB_SCAN); a list without a hub in its first
 // HUB_SCAN entries takes the neighbour of largest degree among its first
-// HUB_FALLBACK entries (earliest on ties; v itself for an empty list or a
-// trimmed v, whose head is never read).  Hub membership is an LDS hash
+// HUB_FALLBACK entries (earliest on ties; a trimmed v gets no head, since
+// heads are stored by non-trimmed rank).  Hub membership is an LDS hash
 // probe, so the only random global reads are the few fallback degrees.
 // On a skewed graph most lists meet a hub within HUB_SCAN1 entries, so the
 // second scan reads few lists.
@@ -1098,15 +1098,15 @@ __device__ __forceinline__ uint32_t hub_hash(uint32_t v) {
 }
 static_assert(HUB_SLOTS == 1u << 12, "hub_hash yields 12 bits");
 
-// deg[v] = off[v + 1] - off[v] (saturated to 32 bits) and the histogram of
-// the non-zero degrees over HUB_BINS log-scale bins (zeroed beforehand).
+// lg[v] = the log-scale bin of d = off[v + 1] - off[v] (hub_bin) and the
+// histogram of the non-zero degrees over the HUB_BINS bins (zeroed
+// beforehand).
 // Both searches are set up in the same launches: blockIdx.y selects the
 // search, whose own CSR ranks the heads and whose opposite CSR is pulled.
 struct HeadDir {
   const uint64_t *own_off;    // degrees in the search's direction
   const uint64_t *pull_off;   // the pull direction's CSR
   const uint32_t *pull_adj;
-  uint32_t *deg;              // n degrees (scratch)
   uint8_t *lg;                // n log-scale degree bins (hub_bin), scratch
   uint32_t *hub;              // HUB_WORDS: histogram, threshold, table
   uint32_t *head;             // n heads (out)
@@ -1120,7 +1120,7 @@ struct HeadDirs {
 __global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
   const HeadDir &D = hd.d[blockIdx.y];
   const uint64_t *__restrict__ off = D.own_off;
-  uint32_t *deg = D.deg, *hist = D.hub;
+  uint32_t *hist = D.hub;
   uint8_t *lg = D.lg;
   __shared__ uint32_t h[HUB_BINS];
   for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK) h[i] = 0;
@@ -1129,7 +1129,6 @@ __global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
        v += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t d = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
     const uint32_t bin = hub_bin(d);
-    deg[v] = d;
     lg[v] = (uint8_t)bin;
     if (d) atomicAdd(&h[bin], 1u);
   }
@@ -1158,22 +1157,40 @@ __global__ void k_hub_select(HeadDirs hd) {
 }
 
 // Vertices of degree >= T enter the hub table (key = id, value = degree).
+// T is the floor of a bin and bins grow with the degree, so d >= T exactly
+// when d's bin is at least T's: the scan reads the one-byte bins, 16 per
+// thread, and only the selected vertices' degrees.
 __global__ __launch_bounds__(BLOCK) void k_hub_insert(HeadDirs hd, uint64_t n) {
-  const uint32_t *__restrict__ deg = hd.d[blockIdx.y].deg;
-  uint32_t *hub = hd.d[blockIdx.y].hub;
+  const HeadDir &D = hd.d[blockIdx.y];
+  const uint64_t *__restrict__ off = D.own_off;
+  const uint8_t *__restrict__ lg = D.lg;
+  uint32_t *hub = D.hub;
   // the degrees >= t fill exactly the selected bins: at most HUB_MAX
   // vertices, so every insert finds an empty slot
   const uint32_t t = hub[HUB_BINS];
   if (hub[HUB_BINS + 1] == 0) return;
+  const uint32_t tb = hub_bin(t);   // >= 1: the histogram counts no zero degree
   uint32_t *keys = hub + HUB_BINS + 2, *vals = keys + HUB_SLOTS;
-  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
-       v += (uint64_t)gridDim.x * BLOCK) {
-    const uint32_t d = deg[v];
-    if (d < t || d == 0) continue;
+  auto insert = [&](uint64_t v) {
     uint32_t s = hub_hash((uint32_t)v);
     while (atomicCAS(&keys[s], HUB_EMPTY, (uint32_t)v) != HUB_EMPTY) s = (s + 1) & (HUB_SLOTS - 1);
-    vals[s] = d;
+    vals[s] = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
+  };
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK, nq = n / 16;
+  const uint64_t g = blockIdx.x * (uint64_t)BLOCK + threadIdx.x;
+  for (uint64_t q = g; q < nq; q += stride) {
+    const uint4 x = reinterpret_cast<const uint4 *>(lg)[q];
+    const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!wd[k]) continue;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (((wd[k] >> (8 * b)) & 0xFFu) >= tb) insert(q * 16 + 4 * k + b);
+    }
   }
+  for (uint64_t v = nq * 16 + g; v < n; v += stride)
+    if (lg[v] >= tb) insert(v);
 }
 
 // head[v] for every vertex (see the rule above).  `off` / `adj` is the pull
@@ -1205,11 +1222,11 @@ __device__ __forceinline__ int head_owner(const uint32_t *sc, uint32_t e) {
   return lo;
 }
 
-__global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t n) {
+__global__ __launch_bounds__(HEAD_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+void k_pull_heads(HeadDirs hd, uint64_t n) {
   const HeadDir &D = hd.d[blockIdx.y];
   const uint64_t *__restrict__ off = D.pull_off;
   const uint32_t *__restrict__ adj = D.pull_adj;
-  const uint32_t *__restrict__ deg = D.deg;
   const uint32_t *__restrict__ hub = D.hub;
   uint32_t *head = D.head;
   constexpr int HW = HEAD_BLOCK / WAVE;
@@ -1223,34 +1240,51 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
     vals[i] = gk[HUB_SLOTS + i];
   }
   __syncthreads();
-  const int w = threadIdx.x / WAVE, l = lane();
+  // wave-uniform in a scalar register, so the group's trim words and tile
+  // prefix are scalar loads
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), l = lane();
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   unsigned long long *best = best_s[w];
-  // the pull list's length is the other search's own degree
-  const uint32_t *__restrict__ pdeg = hd.d[1 - blockIdx.y].deg;
+  const uint32_t *__restrict__ trim = hd.trim;
   const uint64_t stride = (uint64_t)gridDim.x * HW * WAVE;
+  const uint64_t nwords = (n + 31) / 32;
   // a wave's groups of 64 consecutive vertices follow each other with a
-  // stride; a group's list starts and degrees are loaded one group ahead
+  // stride; a group's list starts and its two trim words (scalar loads) are
+  // loaded one group ahead (lane 63 also loads the next group's first
+  // start, the end of its own list)
   uint64_t v0 = (blockIdx.x * (uint64_t)HW + w) * WAVE;
-  uint64_t nb = 0;
-  uint32_t ndg = 0, npd = 0;
-  if (v0 + l < n) {
-    nb = off[v0 + l];
-    ndg = deg[v0 + l];
-    npd = pdeg[v0 + l];
-  }
+  uint64_t nb = 0, ne = 0;
+  uint32_t nta = ~0u, ntb = ~0u;
+  auto load_starts = [&](uint64_t g0) {
+    if (g0 >= n) return;
+    const uint64_t gw = g0 >> 5;
+    nta = trim[gw];
+    ntb = gw + 1 < nwords ? trim[gw + 1] : ~0u;
+    if (g0 + l <= n) nb = off[g0 + l];
+    if (l == WAVE - 1 && g0 + WAVE <= n) ne = off[g0 + WAVE];
+  };
+  load_starts(v0);
   for (; v0 < n; v0 += stride) {
     const uint64_t v = v0 + l;
-    const uint64_t b = nb;
+    // the tile's trim words before the group's own two (all ones past them,
+    // so they count nothing) and the tile's non-trimmed prefix, for the
+    // ranks: in flight while the lists are scanned
+    const uint64_t tile = v0 / SEL_TILE, gw = v0 >> 5, wl = tile * (SEL_TILE / 32) + l;
+    const uint32_t tw = wl < gw ? trim[wl] : ~0u;
+    const unsigned long long nz = hd.nzoff[tile];
+    const uint32_t tw_a = nta, tw_b = ntb;
     // a vertex without edges in the search's own direction, or without a
     // pull list, is trimmed: it is never a pull candidate
-    const uint32_t d = v < n && ndg ? min(npd, HUB_SCAN) : 0u;
-    if (v + stride < n) {
-      nb = off[v + stride];
-      ndg = deg[v + stride];
-      npd = pdeg[v + stride];
-    }
+    const uint32_t mine = l < 32 ? tw_a : tw_b;
+    const bool cand = v < n && !((mine >> (l & 31)) & 1u);
+    // the shuffle runs in every lane: a lane that reads an inactive lane
+    // gets 0, not its neighbour's start
+    const uint64_t nx = __shfl_down((unsigned long long)nb, 1);
+    const uint64_t end = l == WAVE - 1 ? ne : nx;
+    const uint32_t d = cand ? (uint32_t)min(end - nb, (uint64_t)HUB_SCAN) : 0u;
+    st[l] = nb;
+    load_starts(v0 + stride);
     best[l] = 0;
     // hub hits among entries [p0, p0 + c) of each list (c per lane), as
     // 64-entry windows over the concatenated lists (lane e finds its list
@@ -1261,7 +1295,6 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
       const uint32_t total = __shfl(inc, WAVE - 1);
       wave_sync();
       sc[l] = inc;
-      st[l] = b + p0;
       wave_sync();
       for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
         uint32_t u[HEAD_WIN], pos[HEAD_WIN];
@@ -1275,7 +1308,7 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
           if (e < total) {
             own[j] = head_owner(sc, e);
             pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
-            u[j] = adj[st[own[j]] + pos[j]];
+            u[j] = adj[st[own[j]] + p0 + pos[j]];
           }
         }
 #pragma unroll
@@ -1292,48 +1325,46 @@ __global__ __launch_bounds__(HEAD_BLOCK) void k_pull_heads(HeadDirs hd, uint64_t
     const uint32_t d2 = best[l] || d <= HUB_SCAN1 ? 0u : d - HUB_SCAN1;
     if (ballot(d2 > 0)) scan_hubs(HUB_SCAN1, d2);
     // fallback: vertices without a hub rank their first HUB_FALLBACK entries
+    // by the neighbours' log-scale degree bins
     const uint32_t f = best[l] ? 0u : min(d, HUB_FALLBACK);
-    uint32_t inc = wave_incl_scan(f);
-    uint32_t total = __shfl(inc, WAVE - 1);
-    wave_sync();
-    sc[l] = inc;
-    st[l] = b;
-    wave_sync();
-    for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
-      uint32_t u[HEAD_WIN], du[HEAD_WIN], pos[HEAD_WIN];
-      int own[HEAD_WIN];
+    if (ballot(f > 0)) {
+      const uint32_t inc = wave_incl_scan(f);
+      const uint32_t total = __shfl(inc, WAVE - 1);
+      wave_sync();
+      sc[l] = inc;
+      wave_sync();
+      for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
+        uint32_t u[HEAD_WIN], du[HEAD_WIN], pos[HEAD_WIN];
+        int own[HEAD_WIN];
 #pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j) {
-        const uint32_t e = base + j * WAVE + l;
-        own[j] = -1;
-        u[j] = 0;
-        pos[j] = 0;
-        if (e < total) {
-          own[j] = head_owner(sc, e);
-          pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
-          u[j] = adj[st[own[j]] + pos[j]];
+        for (int j = 0; j < HEAD_WIN; ++j) {
+          const uint32_t e = base + j * WAVE + l;
+          own[j] = -1;
+          u[j] = 0;
+          pos[j] = 0;
+          if (e < total) {
+            own[j] = head_owner(sc, e);
+            pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
+            u[j] = adj[st[own[j]] + pos[j]];
+          }
         }
+#pragma unroll
+        for (int j = 0; j < HEAD_WIN; ++j)
+          du[j] = own[j] >= 0 && HUB_FALLBACK > 1 ? D.lg[u[j]] : 0u;
+#pragma unroll
+        for (int j = 0; j < HEAD_WIN; ++j)
+          if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
       }
-#pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j)
-        du[j] = own[j] >= 0 && HUB_FALLBACK > 1 ? D.lg[u[j]] : 0u;
-#pragma unroll
-      for (int j = 0; j < HEAD_WIN; ++j)
-        if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
+      wave_sync();
     }
-    wave_sync();
     // the non-trimmed rank of v: the tile's prefix, the clear bits of the
     // tile's words before this wave's two, then the lanes below in them
-    const uint64_t tile = v0 / SEL_TILE, w0 = v0 >> 5, tw0 = tile * (SEL_TILE / 32);
-    const uint64_t wl = tw0 + l;
-    const uint32_t before = wl < w0 ? (uint32_t)__popc(~hd.trim[wl]) : 0u;
-    const uint32_t tw_a = hd.trim[w0], tw_b = w0 + 1 < (n + 31) / 32 ? hd.trim[w0 + 1] : ~0u;
-    const uint32_t mine = l < 32 ? tw_a : tw_b;
+    const uint32_t before = (uint32_t)__popc(~tw);
     const uint32_t below = l < 32 ? (uint32_t)__popc(~tw_a & ((1u << l) - 1u))
                                   : (uint32_t)__popc(~tw_a) +
                                         (uint32_t)__popc(~tw_b & ((1u << (l - 32)) - 1u));
-    const uint64_t rank = hd.nzoff[tile] + wave_sum(before) + below;
-    if (v < n && !((mine >> (l & 31)) & 1u)) head[rank] = (uint32_t)best[l];
+    const uint64_t rank = nz + wave_sum(before) + below;
+    if (cand) head[rank] = (uint32_t)best[l];
     wave_sync();
   }
 }
@@ -2843,7 +2874,6 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
       hd.d[s].own_off = s == 0 ? d_off : d_ioff;
       hd.d[s].pull_off = s == 0 ? d_ioff : d_off;
       hd.d[s].pull_adj = s == 0 ? d_iadj : d_adj;
-      hd.d[s].deg = (s == 0 ? front : centers).as<uint32_t>();
       hd.d[s].lg = rq[s][0].as<uint8_t>();   // frontier buffer, free until the runs
       hd.d[s].hub = hubs.as<uint32_t>() + s * HUB_WORDS;
       hd.d[s].head = rhead[s].as<uint32_t>();
