@@ -1234,6 +1234,7 @@ void k_pull_heads(HeadDirs hd, uint64_t n) {
   __shared__ uint32_t sc_s[HW][WAVE];
   __shared__ uint64_t st_s[HW][WAVE];
   __shared__ unsigned long long best_s[HW][WAVE];
+  __shared__ uint32_t fbu_s[HW][WAVE * HUB_FALLBACK];   // first entries of each list
   const uint32_t *gk = hub + HUB_BINS + 2;
   for (uint32_t i = threadIdx.x; i < HUB_SLOTS; i += HEAD_BLOCK) {
     keys[i] = gk[i];
@@ -1246,6 +1247,7 @@ void k_pull_heads(HeadDirs hd, uint64_t n) {
   uint32_t *sc = sc_s[w];
   uint64_t *st = st_s[w];
   unsigned long long *best = best_s[w];
+  uint32_t *fbu = fbu_s[w];
   const uint32_t *__restrict__ trim = hd.trim;
   const uint64_t stride = (uint64_t)gridDim.x * HW * WAVE;
   const uint64_t nwords = (n + 31) / 32;
@@ -1317,45 +1319,31 @@ void k_pull_heads(HeadDirs hd, uint64_t n) {
           uint32_t s = hub_hash(u[j]), key;
           while ((key = keys[s]) != HUB_EMPTY && key != u[j]) s = (s + 1) & (HUB_SLOTS - 1);
           if (key == u[j]) atomicMax(&best[own[j]], head_key(vals[s], p0 + pos[j], u[j]));
+          // the first pass keeps each list's first entries for the fallback
+          if (p0 == 0 && pos[j] < HUB_FALLBACK) fbu[own[j] * HUB_FALLBACK + pos[j]] = u[j];
         }
       }
       wave_sync();
     };
     scan_hubs(0, min(d, HUB_SCAN1));
-    const uint32_t d2 = best[l] || d <= HUB_SCAN1 ? 0u : d - HUB_SCAN1;
+    // fallback of a vertex without a hub in its first HUB_SCAN1 entries:
+    // the largest log-scale degree bin among its first HUB_FALLBACK entries
+    // (kept in LDS by the first pass), loaded before the second pass and
+    // taken if that pass finds no hub either
+    const bool nohub = best[l] == 0;
+    const uint32_t f = nohub ? min(d, HUB_FALLBACK) : 0u;
+    uint32_t fd[HUB_FALLBACK];
+#pragma unroll
+    for (uint32_t k = 0; k < HUB_FALLBACK; ++k)
+      fd[k] = k < f && HUB_FALLBACK > 1 ? D.lg[fbu[l * HUB_FALLBACK + k]] : 0u;
+    const uint32_t d2 = !nohub || d <= HUB_SCAN1 ? 0u : d - HUB_SCAN1;
     if (ballot(d2 > 0)) scan_hubs(HUB_SCAN1, d2);
-    // fallback: vertices without a hub rank their first HUB_FALLBACK entries
-    // by the neighbours' log-scale degree bins
-    const uint32_t f = best[l] ? 0u : min(d, HUB_FALLBACK);
-    if (ballot(f > 0)) {
-      const uint32_t inc = wave_incl_scan(f);
-      const uint32_t total = __shfl(inc, WAVE - 1);
-      wave_sync();
-      sc[l] = inc;
-      wave_sync();
-      for (uint32_t base = 0; base < total; base += HEAD_WIN * WAVE) {
-        uint32_t u[HEAD_WIN], du[HEAD_WIN], pos[HEAD_WIN];
-        int own[HEAD_WIN];
+    unsigned long long hk = best[l];
+    if (!hk) {
+      // the second pass leaves the kept entries alone
 #pragma unroll
-        for (int j = 0; j < HEAD_WIN; ++j) {
-          const uint32_t e = base + j * WAVE + l;
-          own[j] = -1;
-          u[j] = 0;
-          pos[j] = 0;
-          if (e < total) {
-            own[j] = head_owner(sc, e);
-            pos[j] = e - (own[j] ? sc[own[j] - 1] : 0u);
-            u[j] = adj[st[own[j]] + pos[j]];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < HEAD_WIN; ++j)
-          du[j] = own[j] >= 0 && HUB_FALLBACK > 1 ? D.lg[u[j]] : 0u;
-#pragma unroll
-        for (int j = 0; j < HEAD_WIN; ++j)
-          if (own[j] >= 0) atomicMax(&best[own[j]], head_key(du[j], pos[j], u[j]));
-      }
-      wave_sync();
+      for (uint32_t k = 0; k < HUB_FALLBACK; ++k)
+        if (k < f) hk = max(hk, head_key(fd[k], k, fbu[l * HUB_FALLBACK + k]));
     }
     // the non-trimmed rank of v: the tile's prefix, the clear bits of the
     // tile's words before this wave's two, then the lanes below in them
@@ -1364,7 +1352,7 @@ void k_pull_heads(HeadDirs hd, uint64_t n) {
                                   : (uint32_t)__popc(~tw_a) +
                                         (uint32_t)__popc(~tw_b & ((1u << (l - 32)) - 1u));
     const uint64_t rank = nz + wave_sum(before) + below;
-    if (cand) head[rank] = (uint32_t)best[l];
+    if (cand) head[rank] = (uint32_t)hk;
     wave_sync();
   }
 }
