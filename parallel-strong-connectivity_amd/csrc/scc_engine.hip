@@ -234,60 +234,95 @@ struct SelWindow {
   }
   __device__ bool pred(uint64_t t) const { return !(labels[val(t)] & TOP_BIT); }
 };
+// Log-scale degree bins of the pull heads' hub selection, 8 per octave:
+// degrees below 16 get bins 0..15, larger ones 8 e + (the 3 bits after the
+// leading one), e = floor(log2 d) >= 4.
+constexpr uint32_t HUB_BINS = 256;
+__device__ __forceinline__ uint32_t hub_bin(uint32_t d) {
+  if (d < 16) return d;
+  const uint32_t e = 31 - __clz(d);
+  return 8 * e + ((d >> (e - 3)) & 7);
+}
+
 // ------------------------------------------------------------- trimming
 // Trim, pass 1 (src/scc.hpp:295-302): per SEL_TILE tile, the number of
 // vertices with non-zero in- and out-degree.  The same degree reads give
 // the dead-end bitmaps of the first reach: vis_f marks vertices without
 // out-edges, vis_b those without in-edges, trim_bm their union (one word
-// pair per bitmap, wave and 64 consecutive vertices).
+// pair per bitmap, wave and 64 consecutive vertices).  They also give the
+// pull heads' degree bins: lgf[v] / lgb[v] = hub_bin of the out- / in-
+// degree (saturated to 32 bits) and the histograms of the non-zero ones
+// (hist_f / hist_b, zeroed beforehand, one flush per block).  A block takes
+// tiles with the grid's stride.
 __global__ __launch_bounds__(BLOCK) void k_trim_count(
     const uint64_t *__restrict__ off, const uint64_t *__restrict__ ioff, uint64_t n,
     uint64_t words, uint32_t *tile_cnt, uint32_t *vis_f, uint32_t *vis_b,
-    uint32_t *trim_bm, unsigned long long *run_zero) {
+    uint32_t *trim_bm, unsigned long long *run_zero, uint8_t *lgf, uint8_t *lgb,
+    uint32_t *hist_f, uint32_t *hist_b) {
   __shared__ uint32_t wc[WPB];
+  __shared__ uint32_t hf[HUB_BINS], hb[HUB_BINS];
+  for (uint32_t k = threadIdx.x; k < HUB_BINS; k += BLOCK) hf[k] = hb[k] = 0;
   // the run's contains() probe counter starts here
   if (blockIdx.x == 0 && threadIdx.x == 0) *run_zero = 0;
-  const uint64_t base = (uint64_t)blockIdx.x * SEL_TILE;
+  const uint64_t ntiles = (n + SEL_TILE - 1) / SEL_TILE;
   const int l = lane();
-  uint32_t c = 0;
-  // every offset load of the tile first, then the tests
-  uint64_t o0[SEL_ITEMS], o1[SEL_ITEMS], i0[SEL_ITEMS], i1[SEL_ITEMS];
-#pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    o0[j] = o1[j] = i0[j] = i1[j] = 0;
-    if (i < n) {
-      o0[j] = off[i];
-      o1[j] = off[i + 1];
-      i0[j] = ioff[i];
-      i1[j] = ioff[i + 1];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < SEL_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-    const bool zo = i < n && o1[j] == o0[j];
-    const bool zi = i < n && i1[j] == i0[j];
-    c += (uint32_t)__popcll(ballot(i < n && !zo && !zi));
-    const uint64_t mf = ballot(zo), mb = ballot(zi), mt = mf | mb;
-    const uint64_t wd = (i - l) / 32;   // first word of the wave's 64 vertices
-    if (l == 0 && wd < words) {
-      vis_f[wd] = (uint32_t)mf;
-      vis_b[wd] = (uint32_t)mb;
-      trim_bm[wd] = (uint32_t)mt;
-    }
-    if (l == 32 && wd + 1 < words) {
-      vis_f[wd + 1] = (uint32_t)(mf >> 32);
-      vis_b[wd + 1] = (uint32_t)(mb >> 32);
-      trim_bm[wd + 1] = (uint32_t)(mt >> 32);
-    }
-  }
-  if (l == 0) wc[threadIdx.x / WAVE] = c;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < WPB; ++w) t += wc[w];
-    tile_cnt[blockIdx.x] = t;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * SEL_TILE;
+    uint32_t c = 0;
+    // every offset load of the tile first, then the tests
+    uint64_t o0[SEL_ITEMS], o1[SEL_ITEMS], i0[SEL_ITEMS], i1[SEL_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SEL_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+      o0[j] = o1[j] = i0[j] = i1[j] = 0;
+      if (i < n) {
+        o0[j] = off[i];
+        o1[j] = off[i + 1];
+        i0[j] = ioff[i];
+        i1[j] = ioff[i + 1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SEL_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+      const bool zo = i < n && o1[j] == o0[j];
+      const bool zi = i < n && i1[j] == i0[j];
+      c += (uint32_t)__popcll(ballot(i < n && !zo && !zi));
+      const uint64_t mf = ballot(zo), mb = ballot(zi), mt = mf | mb;
+      const uint64_t wd = (i - l) / 32;   // first word of the wave's 64 vertices
+      if (l == 0 && wd < words) {
+        vis_f[wd] = (uint32_t)mf;
+        vis_b[wd] = (uint32_t)mb;
+        trim_bm[wd] = (uint32_t)mt;
+      }
+      if (l == 32 && wd + 1 < words) {
+        vis_f[wd + 1] = (uint32_t)(mf >> 32);
+        vis_b[wd + 1] = (uint32_t)(mb >> 32);
+        trim_bm[wd + 1] = (uint32_t)(mt >> 32);
+      }
+      if (i < n) {
+        const uint32_t df = (uint32_t)min(o1[j] - o0[j], (uint64_t)0xFFFFFFFFu);
+        const uint32_t db = (uint32_t)min(i1[j] - i0[j], (uint64_t)0xFFFFFFFFu);
+        const uint32_t bf = hub_bin(df), bb = hub_bin(db);
+        lgf[i] = (uint8_t)bf;
+        lgb[i] = (uint8_t)bb;
+        if (df) atomicAdd(&hf[bf], 1u);
+        if (db) atomicAdd(&hb[bb], 1u);
+      }
+    }
+    if (l == 0) wc[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < WPB; ++w) t += wc[w];
+      tile_cnt[tile] = t;
+    }
+    __syncthreads();   // wc is written again by the next tile
+  }
+  for (uint32_t k = threadIdx.x; k < HUB_BINS; k += BLOCK) {
+    if (hf[k]) atomicAdd(&hist_f[k], hf[k]);
+    if (hb[k]) atomicAdd(&hist_b[k], hb[k]);
   }
 }
 
@@ -1075,19 +1110,11 @@ constexpr uint32_t HUB_SCAN = 64;
 constexpr uint32_t HUB_FALLBACK = PSC_HUB_FALLBACK;
 constexpr uint32_t HUB_MAX = 2048;
 constexpr uint32_t HUB_SLOTS = 2 * HUB_MAX;   // open addressing, load <= 1/2
-constexpr uint32_t HUB_BINS = 256;            // log-scale degree histogram
 constexpr uint32_t HUB_EMPTY = 0xFFFFFFFFu;
 constexpr int HEAD_BLOCK = 1024;
 // layout of the hub scratch (u32): histogram, threshold, keys, degrees
 constexpr uint32_t HUB_WORDS = HUB_BINS + 2 + 2 * HUB_SLOTS;
 
-// 8 bins per octave: degrees below 16 get bins 0..15, larger ones
-// 8 e + (the 3 bits after the leading one), e = floor(log2 d) >= 4.
-__device__ __forceinline__ uint32_t hub_bin(uint32_t d) {
-  if (d < 16) return d;
-  const uint32_t e = 31 - __clz(d);
-  return 8 * e + ((d >> (e - 3)) & 7);
-}
 __device__ __forceinline__ uint32_t hub_bin_floor(uint32_t b) {   // smallest d in bin b
   if (b < 16) return b;
   const uint32_t e = b / 8;
@@ -1098,16 +1125,13 @@ __device__ __forceinline__ uint32_t hub_hash(uint32_t v) {
 }
 static_assert(HUB_SLOTS == 1u << 12, "hub_hash yields 12 bits");
 
-// lg[v] = the log-scale bin of d = off[v + 1] - off[v] (hub_bin) and the
-// histogram of the non-zero degrees over the HUB_BINS bins (zeroed
-// beforehand).
 // Both searches are set up in the same launches: blockIdx.y selects the
 // search, whose own CSR ranks the heads and whose opposite CSR is pulled.
 struct HeadDir {
   const uint64_t *own_off;    // degrees in the search's direction
   const uint64_t *pull_off;   // the pull direction's CSR
   const uint32_t *pull_adj;
-  uint8_t *lg;                // n log-scale degree bins (hub_bin), scratch
+  uint8_t *lg;                // n log-scale degree bins (hub_bin, k_trim_count), scratch
   uint32_t *hub;              // HUB_WORDS: histogram, threshold, table
   uint32_t *head;             // n heads (out)
 };
@@ -1116,26 +1140,6 @@ struct HeadDirs {
   const uint32_t *trim;                // trimmed vertices
   const unsigned long long *nzoff;     // non-trimmed vertices before each tile
 };
-
-__global__ __launch_bounds__(BLOCK) void k_degrees32(HeadDirs hd, uint64_t n) {
-  const HeadDir &D = hd.d[blockIdx.y];
-  const uint64_t *__restrict__ off = D.own_off;
-  uint32_t *hist = D.hub;
-  uint8_t *lg = D.lg;
-  __shared__ uint32_t h[HUB_BINS];
-  for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK) h[i] = 0;
-  __syncthreads();
-  for (uint64_t v = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; v < n;
-       v += (uint64_t)gridDim.x * BLOCK) {
-    const uint32_t d = (uint32_t)min(off[v + 1] - off[v], (uint64_t)0xFFFFFFFFu);
-    const uint32_t bin = hub_bin(d);
-    lg[v] = (uint8_t)bin;
-    if (d) atomicAdd(&h[bin], 1u);
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < HUB_BINS; i += BLOCK)
-    if (h[i]) atomicAdd(&hist[i], h[i]);
-}
 
 // One block: the hub threshold T = the floor of the lowest bin such that at
 // most HUB_MAX vertices have degree >= T (no vertex qualifies when the top
@@ -2878,9 +2882,12 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     starts_h.assign(2 * (G + 1), 0);
     if (n) {
       unsigned long long *tot = misc.as<unsigned long long>();
-      k_trim_count<<<(unsigned)ntiles, BLOCK, 0, stream>>>(
+      for (int s = 0; s < 2; ++s)
+        k_fill_u32<<<1, BLOCK, 0, stream>>>(hd.d[s].hub, HUB_BINS, 0u);
+      k_trim_count<<<(unsigned)std::min<uint64_t>(ntiles, 4 * (uint64_t)num_cu), BLOCK, 0, stream>>>(
           d_off, d_ioff, n, words, tiles.as<uint32_t>(), dead.as<uint32_t>(),
-          dead.as<uint32_t>() + words, trim_static.as<uint32_t>(), tot + 2);
+          dead.as<uint32_t>() + words, trim_static.as<uint32_t>(), tot + 2, hd.d[0].lg,
+          hd.d[1].lg, hd.d[0].hub, hd.d[1].hub);
       PSC_TRY(scan_tiles(ntiles, tot, trim_off.as<unsigned long long>()));
       k_graph_starts<<<grid_for((G + 1) * (uint64_t)WAVE), BLOCK, 0, stream>>>(
           d_off, d_ioff, n, voff_d.as<unsigned long long>(), G,
@@ -2891,9 +2898,9 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
     const unsigned blocks = (unsigned)std::min<uint64_t>((nn + BLOCK - 1) / BLOCK, 2048);
     const unsigned hblocks = (unsigned)std::max<uint64_t>(
         1, std::min<uint64_t>((nn + HEAD_BLOCK - 1) / HEAD_BLOCK, (uint64_t)num_cu));
-    for (int s = 0; s < 2; ++s)
-      k_fill_u32<<<1, BLOCK, 0, stream>>>(hd.d[s].hub, HUB_BINS, 0u);
-    k_degrees32<<<dim3(blocks, 2), BLOCK, 0, stream>>>(hd, n);
+    if (!n)
+      for (int s = 0; s < 2; ++s)
+        k_fill_u32<<<1, BLOCK, 0, stream>>>(hd.d[s].hub, HUB_BINS, 0u);
     k_hub_select<<<dim3(1, 2), BLOCK, 0, stream>>>(hd);
     k_hub_insert<<<dim3(blocks, 2), BLOCK, 0, stream>>>(hd, n);
     k_pull_heads<<<dim3(hblocks, 2), HEAD_BLOCK, 0, stream>>>(hd, n);
