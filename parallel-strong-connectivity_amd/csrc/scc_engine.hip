@@ -266,28 +266,37 @@ __global__ __launch_bounds__(BLOCK) void k_trim_count(
   if (blockIdx.x == 0 && threadIdx.x == 0) *run_zero = 0;
   const uint64_t ntiles = (n + SEL_TILE - 1) / SEL_TILE;
   const int l = lane();
+  // wave-uniform, so each wave's offsets past its last vertex are scalar loads
+  const uint32_t w0 = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE) * WAVE;
   __syncthreads();
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t base = tile * SEL_TILE;
     uint32_t c = 0;
-    // every offset load of the tile first, then the tests
-    uint64_t o0[SEL_ITEMS], o1[SEL_ITEMS], i0[SEL_ITEMS], i1[SEL_ITEMS];
+    // every offset load of the tile first, then the tests: a lane loads its
+    // vertex's start (vertex n's too, the end of vertex n - 1) and takes its
+    // end from the next lane; the wave's last lane's end is a scalar load
+    uint64_t o0[SEL_ITEMS], i0[SEL_ITEMS], oe[SEL_ITEMS], ie[SEL_ITEMS];
 #pragma unroll
     for (int j = 0; j < SEL_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-      o0[j] = o1[j] = i0[j] = i1[j] = 0;
-      if (i < n) {
+      const uint64_t nx = base + (uint64_t)j * BLOCK + w0 + WAVE;
+      o0[j] = i0[j] = 0;
+      if (i <= n) {
         o0[j] = off[i];
-        o1[j] = off[i + 1];
         i0[j] = ioff[i];
-        i1[j] = ioff[i + 1];
       }
+      oe[j] = nx <= n ? off[nx] : 0;
+      ie[j] = nx <= n ? ioff[nx] : 0;
     }
 #pragma unroll
     for (int j = 0; j < SEL_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
-      const bool zo = i < n && o1[j] == o0[j];
-      const bool zi = i < n && i1[j] == i0[j];
+      // the shuffles run in every lane (an inactive source lane reads 0)
+      const uint64_t on = __shfl_down((unsigned long long)o0[j], 1);
+      const uint64_t in = __shfl_down((unsigned long long)i0[j], 1);
+      const uint64_t o1 = l == WAVE - 1 ? oe[j] : on, i1 = l == WAVE - 1 ? ie[j] : in;
+      const bool zo = i < n && o1 == o0[j];
+      const bool zi = i < n && i1 == i0[j];
       c += (uint32_t)__popcll(ballot(i < n && !zo && !zi));
       const uint64_t mf = ballot(zo), mb = ballot(zi), mt = mf | mb;
       const uint64_t wd = (i - l) / 32;   // first word of the wave's 64 vertices
@@ -302,8 +311,8 @@ __global__ __launch_bounds__(BLOCK) void k_trim_count(
         trim_bm[wd + 1] = (uint32_t)(mt >> 32);
       }
       if (i < n) {
-        const uint32_t df = (uint32_t)min(o1[j] - o0[j], (uint64_t)0xFFFFFFFFu);
-        const uint32_t db = (uint32_t)min(i1[j] - i0[j], (uint64_t)0xFFFFFFFFu);
+        const uint32_t df = (uint32_t)min(o1 - o0[j], (uint64_t)0xFFFFFFFFu);
+        const uint32_t db = (uint32_t)min(i1 - i0[j], (uint64_t)0xFFFFFFFFu);
         const uint32_t bf = hub_bin(df), bb = hub_bin(db);
         lgf[i] = (uint8_t)bf;
         lgb[i] = (uint8_t)bb;
@@ -2884,7 +2893,7 @@ int Engine::init(const psc_graph *dg, bool owned, uint32_t ngraphs,
       unsigned long long *tot = misc.as<unsigned long long>();
       for (int s = 0; s < 2; ++s)
         k_fill_u32<<<1, BLOCK, 0, stream>>>(hd.d[s].hub, HUB_BINS, 0u);
-      k_trim_count<<<(unsigned)std::min<uint64_t>(ntiles, 4 * (uint64_t)num_cu), BLOCK, 0, stream>>>(
+      k_trim_count<<<(unsigned)std::min<uint64_t>(ntiles, 5 * (uint64_t)num_cu), BLOCK, 0, stream>>>(
           d_off, d_ioff, n, words, tiles.as<uint32_t>(), dead.as<uint32_t>(),
           dead.as<uint32_t>() + words, trim_static.as<uint32_t>(), tot + 2, hd.d[0].lg,
           hd.d[1].lg, hd.d[0].hub, hd.d[1].hub);
